@@ -1,0 +1,843 @@
+/*
+ * avd_oracle.c -- CPU restatement of av-denoise's denoising core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this file: it is the
+ * checker the parity tests, the smoke check and the CPU baseline of the benchmark run the HIP
+ * path against.  Every function cites the reference file:line it restates.
+ *
+ * Arithmetic contract
+ *   - Sample codes, block-match SAD/SSD, member positions, group words and the fixed-point
+ *     accumulators are integers and are exact.
+ *   - The NL4D filter is written twice over one template: `_f32` repeats the reference's f32
+ *     operation order (fmaf in the DCT sums, round-half-away in to_fixed), `_f64` evaluates the
+ *     same formulas in double.  The HIP FP32 form is checked bit-exact against `_f32`, and the
+ *     `_f64` form is the accuracy yardstick (PSNR).
+ *   - The NLM window sums are restated in both precisions; they are continuous, so the HIP path
+ *     is checked against them to a stated tolerance.
+ *
+ * Where the reference scores block-match candidates as f32 sums of normalised samples, this
+ * restatement scores the exact integer sums (SSD for NL4D grouping, SAD for motion).  The f32
+ * value is the integer sum over max^2 (or max) up to rounding, so both rank candidates alike
+ * except at f32 near-ties; the integer form is the contract the HIP kernels meet bit for bit.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PATCH 8
+#define MAX_K 8
+#define UNFILLED_D 0xFFFFFFFFFFFFFFFFull
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Plane ingest / egress: engine/kernels.rs:25-63 (ingest_words), :103-158 (egress_words).     */
+/* ------------------------------------------------------------------------------------------ */
+
+/* ring[p*C + c] = ((word[p/spw] >> (p%spw)*bits) & mask) / max, for channels planes. */
+void avdo_ingest_words(const uint32_t *plane0, const uint32_t *plane1, float *slot, uint32_t pixels,
+                       uint32_t channels, uint32_t spw, float maxv)
+{
+    uint32_t bits = 32u / spw, mask = (bits == 32u) ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    for (uint32_t p = 0; p < pixels; ++p) {
+        uint32_t w = p / spw, sh = (p % spw) * bits;
+        slot[(size_t)p * channels] = (float)((plane0[w] >> sh) & mask) / maxv;
+        if (channels > 1) slot[(size_t)p * channels + 1] = (float)((plane1[w] >> sh) & mask) / maxv;
+    }
+}
+
+/* code = u32(clamp(v,0,1)*max + 0.5), packed per word, tail lanes zero. */
+void avdo_egress_words(const float *frame, uint32_t *plane0, uint32_t *plane1, uint32_t pixels,
+                       uint32_t channels, uint32_t spw, float maxv)
+{
+    uint32_t bits = 32u / spw;
+    uint32_t words = (pixels + spw - 1) / spw;
+    for (uint32_t w = 0; w < words; ++w) {
+        uint32_t p0 = 0, p1 = 0;
+        for (uint32_t l = 0; l < spw; ++l) {
+            uint32_t p = w * spw + l;
+            if (p >= pixels) continue;
+            float v0 = frame[(size_t)p * channels];
+            v0 = v0 < 0.f ? 0.f : (v0 > 1.f ? 1.f : v0);
+            volatile float t0 = v0 * maxv;
+            p0 |= ((uint32_t)(t0 + 0.5f)) << (l * bits);
+            if (channels > 1) {
+                float v1 = frame[(size_t)p * channels + 1];
+                v1 = v1 < 0.f ? 0.f : (v1 > 1.f ? 1.f : v1);
+                volatile float t1 = v1 * maxv;
+                p1 |= ((uint32_t)(t1 + 0.5f)) << (l * bits);
+            }
+        }
+        plane0[w] = p0;
+        if (channels > 1) plane1[w] = p1;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* NLMeans window sums: nlmeans/kernels/fused.rs:138-357 (pair), :364-530 (single),            */
+/* helpers.rs:36-72 (line_sum_sq, channel_scale, welsch_weight), accumulate.rs:54-101 (finish). */
+/* ------------------------------------------------------------------------------------------ */
+
+#define NLM_TEMPLATE(R, SFX, EXPF)                                                                  \
+    /* d(t,q): squared difference of centre pixel t and neighbour t+q, both clamped per axis. */   \
+    static inline R nlm_pix_dist##SFX(const float *cf, const float *nf, int W, int H, int C, int tx, \
+                                      int ty, int qx, int qy, R scale)                              \
+    {                                                                                               \
+        int cx = clampi(tx, 0, W - 1), cy = clampi(ty, 0, H - 1);                                   \
+        int nx = clampi(tx + qx, 0, W - 1), ny = clampi(ty + qy, 0, H - 1);                         \
+        const float *a = cf + ((size_t)cy * W + cx) * C, *b = nf + ((size_t)ny * W + nx) * C;       \
+        R s = 0;                                                                                    \
+        for (int c = 0; c < C; ++c) {                                                               \
+            R d = (R)a[c] - (R)b[c];                                                                \
+            s += d * d;                                                                             \
+        }                                                                                           \
+        return s * scale;                                                                           \
+    }                                                                                               \
+    /* Patch distance at pixel (x,y) for offset q, summed row-major over the patch. */              \
+    static R nlm_patch_dist##SFX(const float *cf, const float *nf, int W, int H, int C, int P, int x, \
+                                 int y, int qx, int qy, R scale)                                    \
+    {                                                                                               \
+        R sum = 0;                                                                                  \
+        for (int uy = -P; uy <= P; ++uy)                                                            \
+            for (int ux = -P; ux <= P; ++ux)                                                        \
+                sum += nlm_pix_dist##SFX(cf, nf, W, H, C, x + ux, y + uy, qx, qy, scale);           \
+        return sum;                                                                                 \
+    }                                                                                               \
+    /* Spatial window of frame f against itself, q != 0, offsets from lut (fused.rs:364-530). */    \
+    void avdo_nlm_single##SFX(const float *f, int W, int H, int C, int P, int S, float h2,          \
+                              const float *lut, R *accum, R *wsum, R *maxw)                         \
+    {                                                                                               \
+        R scale = (C == 1) ? (R)3.0 : (R)1.5;                                                       \
+        int side = 2 * S + 1;                                                                       \
+        _Pragma("omp parallel for schedule(dynamic, 4)") for (int y = 0; y < H; ++y)               \
+        {                                                                                           \
+            for (int x = 0; x < W; ++x) {                                                           \
+                R acc[2] = {0, 0}, ws = 0, mw = 0;                                                  \
+                for (int qy = -S; qy <= S; ++qy)                                                    \
+                    for (int qx = -S; qx <= S; ++qx) {                                              \
+                        if (qx == 0 && qy == 0) continue;                                           \
+                        R d = nlm_patch_dist##SFX(f, f, W, H, C, P, x, y, qx, qy, scale);           \
+                        R off = (R)lut[(qy + S) * side + (qx + S)];                                 \
+                        R e = d - off;                                                              \
+                        if (e < 0) e = 0;                                                           \
+                        R w = EXPF(-e * (R)h2);                                                     \
+                        int nx = clampi(x + qx, 0, W - 1), ny = clampi(y + qy, 0, H - 1);           \
+                        const float *nb = f + ((size_t)ny * W + nx) * C;                            \
+                        for (int c = 0; c < C; ++c) acc[c] += (R)nb[c] * w;                         \
+                        ws += w;                                                                    \
+                        if (w > mw) mw = w;                                                         \
+                    }                                                                               \
+                size_t p = (size_t)y * W + x;                                                       \
+                for (int c = 0; c < C; ++c) accum[p * C + c] += acc[c];                             \
+                wsum[p] += ws;                                                                      \
+                if (mw > maxw[p]) maxw[p] = mw;                                                     \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+    /* One temporal pair: fwd frame at +q, bwd frame at -q, whole window incl. q=0               */ \
+    /* (fused.rs:138-347).  conf_fwd/conf_bwd may be NULL (no confidence).                        */ \
+    void avdo_nlm_pair##SFX(const float *cf, const float *ff, const float *bf, int W, int H, int C,  \
+                            int P, int S, float h2, float noise_offset, const float *conf_fwd,      \
+                            const float *conf_bwd, int step, int blocks_x, int blocks_y, R *accum,  \
+                            R *wsum, R *maxw)                                                       \
+    {                                                                                               \
+        R scale = (C == 1) ? (R)3.0 : (R)1.5;                                                       \
+        _Pragma("omp parallel for schedule(dynamic, 4)") for (int y = 0; y < H; ++y)               \
+        {                                                                                           \
+            for (int x = 0; x < W; ++x) {                                                           \
+                R acc[2] = {0, 0}, ws = 0, mw = 0;                                                  \
+                int bcol = x / step, brow = y / step;                                               \
+                if (bcol > blocks_x - 1) bcol = blocks_x - 1;                                       \
+                if (brow > blocks_y - 1) brow = blocks_y - 1;                                       \
+                int bidx = brow * blocks_x + bcol;                                                  \
+                for (int qy = -S; qy <= S; ++qy)                                                    \
+                    for (int qx = -S; qx <= S; ++qx) {                                              \
+                        R df = nlm_patch_dist##SFX(cf, ff, W, H, C, P, x, y, qx, qy, scale);        \
+                        R db = nlm_patch_dist##SFX(cf, bf, W, H, C, P, x, y, -qx, -qy, scale);      \
+                        R ef = df - (R)noise_offset, eb = db - (R)noise_offset;                     \
+                        if (ef < 0) ef = 0;                                                         \
+                        if (eb < 0) eb = 0;                                                         \
+                        R wf = EXPF(-ef * (R)h2), wb = EXPF(-eb * (R)h2);                           \
+                        if (conf_fwd) {                                                             \
+                            wf *= (R)conf_fwd[bidx];                                                \
+                            wb *= (R)conf_bwd[bidx];                                                \
+                        }                                                                           \
+                        int fx = clampi(x + qx, 0, W - 1), fy = clampi(y + qy, 0, H - 1);           \
+                        int bx = clampi(x - qx, 0, W - 1), by = clampi(y - qy, 0, H - 1);           \
+                        const float *fp = ff + ((size_t)fy * W + fx) * C;                           \
+                        const float *bp = bf + ((size_t)by * W + bx) * C;                           \
+                        for (int c = 0; c < C; ++c) acc[c] = acc[c] + (R)fp[c] * wf + (R)bp[c] * wb; \
+                        ws += wf + wb;                                                              \
+                        R m2 = wf > wb ? wf : wb;                                                   \
+                        if (m2 > mw) mw = m2;                                                       \
+                    }                                                                               \
+                size_t p = (size_t)y * W + x;                                                       \
+                for (int c = 0; c < C; ++c) accum[p * C + c] += acc[c];                             \
+                wsum[p] += ws;                                                                      \
+                if (mw > maxw[p]) maxw[p] = mw;                                                     \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+    /* out = (x*sw + accum)/(sw + wsum), sw = wref*maxw; x where the denominator <= 1e-30.       */ \
+    void avdo_nlm_finish##SFX(const float *f, const R *accum, const R *wsum, const R *maxw,         \
+                              float wref, int pixels, int C, float *out)                            \
+    {                                                                                               \
+        for (int p = 0; p < pixels; ++p) {                                                          \
+            R sw = (R)wref * maxw[p];                                                               \
+            R den = sw + wsum[p];                                                                   \
+            for (int c = 0; c < C; ++c) {                                                           \
+                size_t i = (size_t)p * C + c;                                                       \
+                out[i] = (den > (R)1e-30) ? (float)(((R)f[i] * sw + accum[i]) * ((R)1 / den)) : f[i]; \
+            }                                                                                       \
+        }                                                                                           \
+    }
+
+NLM_TEMPLATE(float, _f32, expf)
+NLM_TEMPLATE(double, _f64, exp)
+
+/* ------------------------------------------------------------------------------------------ */
+/* Motion: motion/kernels/pyramid.rs:11-44 (mc_downscale), block_match.rs:117-211 (coarse),    */
+/* :306-369 (fine), :489-535 (write_fine_result, match_confidence), :653-927 (staging, SADs),   */
+/* :940-1000 (argmin), compensate.rs:12-44 (mc_warp).                                          */
+/* Level 0 holds codes; level 1 holds the sum of each 2x2 group of codes, which is the          */
+/* reference's f32 mean times 4*max, so every SAD ranks exactly as the reference's f32 SAD.     */
+/* ------------------------------------------------------------------------------------------ */
+
+void avdo_downscale_sum(const int32_t *src, int sw, int sh, int32_t *dst, int dw, int dh)
+{
+    for (int y = 0; y < dh; ++y)
+        for (int x = 0; x < dw; ++x) {
+            int sx = 2 * x, sy = 2 * y;
+            int sx1 = sx + 1 < sw ? sx + 1 : sx, sy1 = sy + 1 < sh ? sy + 1 : sy;
+            dst[(size_t)y * dw + x] = src[(size_t)sy * sw + sx] + src[(size_t)sy * sw + sx1] +
+                                      src[(size_t)sy1 * sw + sx] + src[(size_t)sy1 * sw + sx1];
+        }
+}
+
+/* Best offset of one block over a (2R+1)^2 window centred on (cdx,cdy).  Lowest SAD, then the
+ * lowest row-major index, and the window centre wins a tie with that pick.  Returns the SAD. */
+static int64_t bm_search(const int32_t *cen, const int32_t *nb, int w, int h, int ox, int oy, int blk,
+                         int R, int cdx, int cdy, int *bdx, int *bdy)
+{
+    int side = 2 * R + 1;
+    int64_t best = INT64_MAX, centre_sad = 0;
+    int bx = cdx, by = cdy;
+    for (int dy = 0; dy < side; ++dy)
+        for (int dx = 0; dx < side; ++dx) {
+            int64_t sad = 0;
+            for (int py = 0; py < blk; ++py)
+                for (int px = 0; px < blk; ++px) {
+                    int cx = clampi(ox + px, 0, w - 1), cy = clampi(oy + py, 0, h - 1);
+                    int nx = clampi(ox + cdx - R + dx + px, 0, w - 1);
+                    int ny = clampi(oy + cdy - R + dy + py, 0, h - 1);
+                    int64_t d = (int64_t)cen[(size_t)cy * w + cx] - nb[(size_t)ny * w + nx];
+                    sad += d < 0 ? -d : d;
+                }
+            if (dy == R && dx == R) centre_sad = sad;
+            if (sad < best) {
+                best = sad;
+                bx = cdx + dx - R;
+                by = cdy + dy - R;
+            }
+        }
+    if (centre_sad <= best) {
+        best = centre_sad;
+        bx = cdx;
+        by = cdy;
+    }
+    *bdx = bx;
+    *bdy = by;
+    return best;
+}
+
+/* Coarse pass on the level-1 sums: one block per (col,row), blksize 8 at step 4 for the default
+ * 16/8 grid, winner scaled by 2 and written to every fine block the coarse block covers. */
+void avdo_block_match_coarse(const int32_t *cen1, const int32_t *nb1, int lw, int lh, int blk, int step,
+                             int R, int level_scale, int fine_blocks_x, int fine_blocks_y, int fine_step,
+                             int32_t *mv)
+{
+    int cbx = (lw + step - 1) / step, cby = (lh + step - 1) / step;
+    if (cbx < 1) cbx = 1;
+    if (cby < 1) cby = 1;
+    for (int br = 0; br < cby; ++br)
+        for (int bc = 0; bc < cbx; ++bc) {
+            int dx, dy;
+            bm_search(cen1, nb1, lw, lh, bc * step, br * step, blk, R, 0, 0, &dx, &dy);
+            int c0 = bc * step * level_scale / fine_step, c1 = (bc + 1) * step * level_scale / fine_step;
+            int r0 = br * step * level_scale / fine_step, r1 = (br + 1) * step * level_scale / fine_step;
+            if (c0 > fine_blocks_x) c0 = fine_blocks_x;
+            if (r0 > fine_blocks_y) r0 = fine_blocks_y;
+            c1 = (bc == cbx - 1) ? fine_blocks_x : (c1 > fine_blocks_x ? fine_blocks_x : c1);
+            r1 = (br == cby - 1) ? fine_blocks_y : (r1 > fine_blocks_y ? fine_blocks_y : r1);
+            for (int fr = r0; fr < r1; ++fr)
+                for (int fc = c0; fc < c1; ++fc) {
+                    mv[((size_t)fr * fine_blocks_x + fc) * 2] = dx * level_scale;
+                    mv[((size_t)fr * fine_blocks_x + fc) * 2 + 1] = dy * level_scale;
+                }
+        }
+}
+
+/* match_confidence (block_match.rs:523-535) on the SAD in normalised units. */
+float avdo_match_confidence(float best_sad, float floor_sad, float thsad)
+{
+    float excess = best_sad - floor_sad;
+    if (excess < 0.f) excess = 0.f;
+    float t2 = thsad * thsad, e2 = excess * excess;
+    float c = (t2 - e2) / (t2 + e2);
+    return c < 0.f ? 0.f : c;
+}
+
+/* Fine pass on level 0: seeded by mv (use_seed) and refined in place; optional confidence from
+ * the SAD over max (normalised units), the derived SAD floor and thsad. */
+void avdo_block_match_fine(const int32_t *cen0, const int32_t *nb0, int w, int h, int blk, int step,
+                           int R, int blocks_x, int blocks_y, int use_seed, int32_t *mv, float *conf,
+                           float maxv, float sad_floor, float thsad)
+{
+    for (int br = 0; br < blocks_y; ++br)
+        for (int bc = 0; bc < blocks_x; ++bc) {
+            size_t b = (size_t)br * blocks_x + bc;
+            int sdx = use_seed ? mv[b * 2] : 0, sdy = use_seed ? mv[b * 2 + 1] : 0;
+            int dx, dy;
+            int64_t sad = bm_search(cen0, nb0, w, h, bc * step, br * step, blk, R, sdx, sdy, &dx, &dy);
+            mv[b * 2] = dx;
+            mv[b * 2 + 1] = dy;
+            if (conf) conf[b] = avdo_match_confidence((float)sad / maxv, sad_floor, thsad);
+        }
+}
+
+/* mc_warp: dst(x,y) = src(clamp(x+mv.x), clamp(y+mv.y)) with block min(x/step, bx-1). */
+void avdo_warp(const float *src, float *dst, const int32_t *mv, int W, int H, int C, int step,
+               int blocks_x, int blocks_y)
+{
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            int bc = x / step, br = y / step;
+            if (bc > blocks_x - 1) bc = blocks_x - 1;
+            if (br > blocks_y - 1) br = blocks_y - 1;
+            const int32_t *v = mv + ((size_t)br * blocks_x + bc) * 2;
+            int sx = clampi(x + v[0], 0, W - 1), sy = clampi(y + v[1], 0, H - 1);
+            for (int c = 0; c < C; ++c) dst[((size_t)y * W + x) * C + c] = src[((size_t)sy * W + sx) * C + c];
+        }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* NL4D collaborative pass: collab/kernels/fused/mod.rs:170-553 (search_group), :565-850       */
+/* (filter_group), centre.rs:37-300, trajectory.rs:18-450, plane_ops.rs:36-77 (shift_insert8),  */
+/* transforms.rs:39-187 (dct8, haar, variance ladder), grid.rs:119-220, aggregate.rs:147-211   */
+/* (scatter_patch, to_fixed), :252-290 (collab_normalise), geometry.rs:6-29, group.rs:9-66.    */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+    int W, H, C;
+    int step;           /* reference-grid stride (STEP = 4 in the reference) */
+    int refs_x, refs_y;
+    int S;              /* spatial_radius */
+    int radius;         /* temporal radius */
+    int grid_frames;    /* collab/mod.rs:45-51 */
+    int refine;
+    int k_max;
+    int blk_step, blksize, blocks_x, blocks_y;
+    float c_min;
+    float lambda[2];    /* per-channel hard-threshold multiplier */
+    float sigma[2];     /* per-channel sigma, normalised */
+    float weight_scale;
+    float accum_scale;
+    float kaiser[8];
+    float profile[8];   /* dct_noise_profile, all ones at rho = 0 (transforms.rs:196-221) */
+} avdo_collab_params;
+
+static inline uint32_t pack_pos_t(uint32_t x, uint32_t y, uint32_t t) { return (t << 26) | (y << 13) | x; }
+
+/* Distance between the 8x8 patch at (ax,ay) of frame a and (bx,by) of frame b over C channels.
+ * Codes: the exact integer SSD.  Floats: the reference's f32 form (distance.rs:160-209) -- each
+ * column sums its squared differences row by row, the 8-lane butterfly of plane_ops.rs:17-23 adds
+ * the columns, and the channel scale multiplies the total. */
+static inline uint64_t dist_u16(const uint16_t *a, const uint16_t *b, int W, int C, int ax, int ay, int bx, int by)
+{
+    uint64_t s = 0;
+    for (int r = 0; r < PATCH; ++r) {
+        const uint16_t *pa = a + ((size_t)(ay + r) * W + ax) * C;
+        const uint16_t *pb = b + ((size_t)(by + r) * W + bx) * C;
+        for (int i = 0; i < PATCH * C; ++i) {
+            int64_t d = (int64_t)pa[i] - (int64_t)pb[i];
+            s += (uint64_t)(d * d);
+        }
+    }
+    return s;
+}
+
+static inline float dist_f32(const float *a, const float *b, int W, int C, int ax, int ay, int bx, int by)
+{
+    float part[8];
+    for (int q = 0; q < PATCH; ++q) {
+        float acc = 0.f;
+        for (int r = 0; r < PATCH; ++r)
+            for (int c = 0; c < C; ++c) {
+                float d = a[((size_t)(ay + r) * W + ax + q) * C + c] - b[((size_t)(by + r) * W + bx + q) * C + c];
+                acc = fmaf(d, d, acc);
+            }
+        part[q] = acc;
+    }
+    float sum = ((part[0] + part[1]) + (part[2] + part[3])) + ((part[4] + part[5]) + (part[6] + part[7]));
+    return sum * (C == 1 ? 3.0f : 1.5f);
+}
+
+/* covering_lo (trajectory.rs:18-22). */
+static inline int covering_lo(int p, int blksize, int step)
+{
+    int over = (p + PATCH > blksize ? p + PATCH : blksize) - blksize;
+    return (over + step - 1) / step;
+}
+
+/* search_group (fused/mod.rs:407-553): the group of one reference as 8 pack_pos_t words and the
+ * group word.  T is the ring sample type, D the distance type, SELF_D pins the self-match below
+ * every distance (centre.rs:274-290) and UNF is the unfilled distance (plane_ops.rs:79). */
+#define SEARCH_TEMPLATE(T, D, SFX, DIST, SELF_D, UNF)                                                  \
+    static void collab_search_one##SFX(const T *ring, size_t slot_len, const avdo_collab_params *pp,      \
+                                       int centre_slot, const uint32_t *nslots, const int32_t *mv,          \
+                                       const float *conf, int ix, int iy, uint32_t *out9)                  \
+    {                                                                                                    \
+        const int W = pp->W, H = pp->H, C = pp->C, S = pp->S;                                            \
+        const int maxx = W - PATCH, maxy = H - PATCH;                                                    \
+        const int rx = (ix * pp->step < maxx) ? ix * pp->step : maxx;                                    \
+        const int ry = (iy * pp->step < maxy) ? iy * pp->step : maxy;                                    \
+        const T *cf = ring + (size_t)centre_slot * slot_len;                                             \
+        /* centre_search, clipped walk, shift_insert8: a tie never displaces an earlier visit. */        \
+        D bd[8];                                                                                         \
+        uint32_t bp[8];                                                                                  \
+        for (int i = 0; i < 8; ++i) {                                                                    \
+            bd[i] = UNF;                                                                                 \
+            bp[i] = 0;                                                                                   \
+        }                                                                                                \
+        int left = clampi(rx - S, 0, maxx), right = clampi(rx + S, 0, maxx);                             \
+        int top = clampi(ry - S, 0, maxy), bottom = clampi(ry + S, 0, maxy);                             \
+        for (int y = top; y <= bottom; ++y)                                                              \
+            for (int x = left; x <= right; ++x) {                                                        \
+                D d = (x == rx && y == ry) ? SELF_D : DIST(cf, cf, W, C, rx, ry, x, y);                  \
+                if (!(d < bd[7])) continue;                                                              \
+                int k = 7;                                                                               \
+                while (k > 0 && d < bd[k - 1]) {                                                         \
+                    bd[k] = bd[k - 1];                                                                   \
+                    bp[k] = bp[k - 1];                                                                   \
+                    --k;                                                                                 \
+                }                                                                                        \
+                bd[k] = d;                                                                               \
+                bp[k] = pack_pos_t((uint32_t)x, (uint32_t)y, 0);                                         \
+            }                                                                                            \
+        uint32_t n_live = (uint32_t)(right - left + 1) * (uint32_t)(bottom - top + 1);                   \
+        uint32_t k_use = 1;                                                                              \
+        while (k_use * 2 <= n_live && k_use * 2 <= (uint32_t)pp->k_max) k_use *= 2;                      \
+        uint32_t members[8];                                                                             \
+        for (int m = 0; m < 8; ++m) members[m] = bp[m];                                                  \
+        int use_grid = 0;                                                                                \
+        if (pp->grid_frames > 1 && pp->k_max == MAX_K) {                                                 \
+            const int gf = pp->grid_frames, volumes = MAX_K / gf, tail = gf - 1;                         \
+            const int covers = (pp->blksize + pp->blk_step - 1) / pp->blk_step;                          \
+            const int nnb = 2 * pp->radius;                                                              \
+            const size_t nblk = (size_t)pp->blocks_x * pp->blocks_y;                                     \
+            uint32_t gpos[8];                                                                            \
+            D gd[8];                                                                                     \
+            for (int v = 0; v < volumes; ++v) {                                                          \
+                int first = v * gf;                                                                      \
+                uint32_t anchor = members[v];                                                            \
+                int ax = (int)(anchor & 0x1FFF), ay = (int)((anchor >> 13) & 0x1FFF);                    \
+                gpos[first] = anchor;                                                                    \
+                gd[first] = 0;                                                                           \
+                for (int j = 0; j < tail; ++j) {                                                         \
+                    gpos[first + 1 + j] = 0;                                                             \
+                    gd[first + 1 + j] = UNF;                                                             \
+                }                                                                                        \
+                int held_n = first + 1;                                                                  \
+                int bx_hi = ax / pp->blk_step, by_hi = ay / pp->blk_step;                                \
+                if (bx_hi > pp->blocks_x - 1) bx_hi = pp->blocks_x - 1;                                  \
+                if (by_hi > pp->blocks_y - 1) by_hi = pp->blocks_y - 1;                                  \
+                int bx_lo = covering_lo(ax, pp->blksize, pp->blk_step);                                  \
+                int by_lo = covering_lo(ay, pp->blksize, pp->blk_step);                                  \
+                if (bx_lo > bx_hi) bx_lo = bx_hi;                                                        \
+                if (by_lo > by_hi) by_lo = by_hi;                                                        \
+                for (int t = 0; t < nnb; ++t) {                                                          \
+                    const T *nf = ring + (size_t)nslots[t] * slot_len;                                   \
+                    uint32_t packed_t = (uint32_t)t + 1;                                                 \
+                    D fd = UNF;                                                                          \
+                    uint32_t fpos = 0;                                                                   \
+                    int sl[16], sr[16], st[16], sb[16];                                                  \
+                    for (int s2 = 0; s2 < covers * covers; ++s2) {                                       \
+                        sl[s2] = 1;                                                                      \
+                        sr[s2] = 0;                                                                      \
+                        st[s2] = 1;                                                                      \
+                        sb[s2] = 0;                                                                      \
+                    }                                                                                    \
+                    for (int iyc = 0; iyc < covers; ++iyc)                                               \
+                        for (int ixc = 0; ixc < covers; ++ixc) {                                         \
+                            int wbx = bx_lo + ixc, wby = by_lo + iyc;                                    \
+                            if (wbx > bx_hi || wby > by_hi) continue;                                    \
+                            int block = wby * pp->blocks_x + wbx;                                        \
+                            float bconf = conf[(size_t)t * nblk + block];                                \
+                            if (!(bconf >= pp->c_min)) continue;                                         \
+                            const int32_t *v2 = mv + ((size_t)t * nblk + block) * 2;                     \
+                            int px = ax + v2[0], py = ay + v2[1];                                        \
+                            int tl = clampi(px - pp->refine, 0, maxx), tr = clampi(px + pp->refine, 0, maxx); \
+                            int tt = clampi(py - pp->refine, 0, maxy), tb = clampi(py + pp->refine, 0, maxy); \
+                            for (int cy = tt; cy <= tb; ++cy)                                            \
+                                for (int cx = tl; cx <= tr; ++cx) {                                      \
+                                    int skipped = 0;                                                     \
+                                    for (int s2 = 0; s2 < covers * covers; ++s2)                         \
+                                        if (cx >= sl[s2] && cx <= sr[s2] && cy >= st[s2] && cy <= sb[s2]) \
+                                            skipped = 1;                                                 \
+                                    uint32_t packed = pack_pos_t((uint32_t)cx, (uint32_t)cy, packed_t);  \
+                                    for (int k = 0; k < held_n; ++k)                                     \
+                                        if (gpos[k] == packed) skipped = 1;                              \
+                                    if (skipped) continue;                                               \
+                                    D d = DIST(cf, nf, W, C, ax, ay, cx, cy);                            \
+                                    if (d < fd) {                                                        \
+                                        fd = d;                                                          \
+                                        fpos = packed;                                                   \
+                                    }                                                                    \
+                                }                                                                        \
+                            int ri = iyc * covers + ixc;                                                 \
+                            sl[ri] = tl;                                                                 \
+                            sr[ri] = tr;                                                                 \
+                            st[ri] = tt;                                                                 \
+                            sb[ri] = tb;                                                                 \
+                        }                                                                                \
+                    /* The frame's match joins the volume's ascending list, earlier frames win. */       \
+                    D cd = fd;                                                                           \
+                    uint32_t cp = fpos;                                                                  \
+                    for (int j = 0; j < tail; ++j) {                                                     \
+                        int si = first + 1 + j;                                                          \
+                        if (cd < gd[si]) {                                                               \
+                            D hd = gd[si];                                                               \
+                            uint32_t hp = gpos[si];                                                      \
+                            gd[si] = cd;                                                                 \
+                            gpos[si] = cp;                                                               \
+                            cd = hd;                                                                     \
+                            cp = hp;                                                                     \
+                        }                                                                                \
+                    }                                                                                    \
+                }                                                                                        \
+            }                                                                                            \
+            use_grid = (k_use == MAX_K);                                                                 \
+            for (int v = 0; v < volumes; ++v) use_grid = use_grid && (gd[v * gf + tail] < UNF);          \
+            if (use_grid) {                                                                              \
+                for (int m = 0; m < 8; ++m) members[m] = gpos[m];                                        \
+                k_use = MAX_K;                                                                           \
+            }                                                                                            \
+        }                                                                                                \
+        for (int m = 0; m < 8; ++m) out9[m] = members[m];                                                \
+        out9[8] = k_use | (use_grid ? 0x100u : 0u);                                                      \
+    }                                                                                                    \
+    void avdo_collab_search##SFX(const T *ring, const avdo_collab_params *pp, int centre_slot,            \
+                                 const uint32_t *nslots, const int32_t *mv, const float *conf,            \
+                                 uint32_t *members)                                                      \
+    {                                                                                                    \
+        size_t slot_len = (size_t)pp->W * pp->H * pp->C;                                                 \
+        _Pragma("omp parallel for schedule(dynamic, 2)") for (int iy = 0; iy < pp->refs_y; ++iy)         \
+        {                                                                                                \
+            for (int ix = 0; ix < pp->refs_x; ++ix)                                                      \
+                collab_search_one##SFX(ring, slot_len, pp, centre_slot, nslots, mv, conf, ix, iy,          \
+                                       members + ((size_t)iy * pp->refs_x + ix) * 9);                    \
+        }                                                                                                \
+    }
+
+/* Codes: the self-match is distance 0 and every other candidate its SSD plus one. */
+static inline uint64_t dist_u16_shift(const uint16_t *a, const uint16_t *b, int W, int C, int ax, int ay, int bx,
+                                      int by)
+{
+    return dist_u16(a, b, W, C, ax, ay, bx, by) + 1;
+}
+
+SEARCH_TEMPLATE(uint16_t, uint64_t, _u16, dist_u16_shift, 0ull, UNFILLED_D)
+SEARCH_TEMPLATE(float, float, _f32, dist_f32, -1.0e38f, 3.0e38f)
+
+/* dct8_basis (transforms.rs:39-43) evaluated in double and rounded to f32 as the kernel taps. */
+static double dct8_basis(int row, int col)
+{
+    double rs = row == 0 ? 1.0 / sqrt(8.0) : 0.5;
+    return rs * cos(M_PI * (2.0 * col + 1.0) * row / 16.0);
+}
+
+#define COLLAB_TEMPLATE(R, SFX, FMA, SQRT, ROUND)                                                   \
+    static void dct8_fwd##SFX(R *l, const R *te, const R *to)                                       \
+    {                                                                                               \
+        R s[4], d[4];                                                                               \
+        for (int k = 0; k < 4; ++k) {                                                               \
+            s[k] = l[k] + l[7 - k];                                                                 \
+            d[k] = l[k] - l[7 - k];                                                                 \
+        }                                                                                           \
+        for (int m = 0; m < 4; ++m) {                                                               \
+            R e = 0, o = 0;                                                                         \
+            for (int k = 0; k < 4; ++k) {                                                           \
+                e = FMA(s[k], te[m * 4 + k], e);                                                    \
+                o = FMA(d[k], to[m * 4 + k], o);                                                    \
+            }                                                                                       \
+            l[2 * m] = e;                                                                           \
+            l[2 * m + 1] = o;                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+    static void dct8_inv##SFX(R *l, const R *te, const R *to)                                       \
+    {                                                                                               \
+        R sn[8];                                                                                    \
+        for (int j = 0; j < 8; ++j) sn[j] = l[j];                                                   \
+        for (int k = 0; k < 4; ++k) {                                                               \
+            R e = 0, o = 0;                                                                         \
+            for (int m = 0; m < 4; ++m) {                                                           \
+                e = FMA(sn[2 * m], te[m * 4 + k], e);                                               \
+                o = FMA(sn[2 * m + 1], to[m * 4 + k], o);                                           \
+            }                                                                                       \
+            l[k] = e + o;                                                                           \
+            l[7 - k] = e - o;                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+    /* In-place Haar butterfly over members a (low) and b (high): (a+b)/sqrt2, (a-b)/sqrt2.      */ \
+    static inline void haar_pair##SFX(R *stk, int a, int b, R f)                                    \
+    {                                                                                               \
+        for (int i = 0; i < 64; ++i) {                                                              \
+            R x = stk[a * 64 + i], y = stk[b * 64 + i];                                             \
+            stk[a * 64 + i] = (x + y) * f;                                                          \
+            stk[b * 64 + i] = (x - y) * f;                                                          \
+        }                                                                                           \
+    }                                                                                               \
+    static inline void ihaar_pair##SFX(R *stk, int a, int b, R f)                                   \
+    {                                                                                               \
+        for (int i = 0; i < 64; ++i) {                                                              \
+            R lo = stk[a * 64 + i], hi = stk[b * 64 + i];                                           \
+            stk[a * 64 + i] = (lo + hi) * f;                                                        \
+            stk[b * 64 + i] = (lo - hi) * f;                                                        \
+        }                                                                                           \
+    }                                                                                               \
+    /* The group's forward Haar, written as in-place butterflies: the reference's reordering     */ \
+    /* (approximations first) only permutes member slots, and with one variance per member slot  */ \
+    /* (every member carries sigma^2) the threshold does not depend on the slot.  Slot 0 holds    */ \
+    /* the group approximation in both layouts.                                                  */ \
+    static void group_fwd##SFX(R *stk, int k_use, int grid, int gf, R f)                            \
+    {                                                                                               \
+        if (grid) {                                                                                 \
+            int vols = MAX_K / gf;                                                                  \
+            for (int v = 0; v < vols; ++v)                                                          \
+                for (int len = gf, st = 1; len >= 2; len /= 2, st *= 2)                             \
+                    for (int p = 0; p < gf; p += 2 * st) haar_pair##SFX(stk, v * gf + p, v * gf + p + st, f); \
+            for (int fr = 0; fr < gf; ++fr)                                                         \
+                for (int len = vols, st = 1; len >= 2; len /= 2, st *= 2)                           \
+                    for (int p = 0; p < vols; p += 2 * st) haar_pair##SFX(stk, fr + p * gf, fr + (p + st) * gf, f); \
+        } else {                                                                                    \
+            for (int len = k_use, st = 1; len >= 2; len /= 2, st *= 2)                              \
+                for (int p = 0; p < k_use; p += 2 * st) haar_pair##SFX(stk, p, p + st, f);          \
+        }                                                                                           \
+    }                                                                                               \
+    static void group_inv##SFX(R *stk, int k_use, int grid, int gf, R f)                            \
+    {                                                                                               \
+        if (grid) {                                                                                 \
+            int vols = MAX_K / gf;                                                                  \
+            int top = 1;                                                                            \
+            while (top * 2 < vols) top *= 2;                                                        \
+            for (int fr = 0; fr < gf; ++fr)                                                         \
+                for (int st = top; st >= 1 && vols >= 2; st /= 2)                                   \
+                    for (int p = 0; p < vols; p += 2 * st) ihaar_pair##SFX(stk, fr + p * gf, fr + (p + st) * gf, f); \
+            int topf = 1;                                                                           \
+            while (topf * 2 < gf) topf *= 2;                                                        \
+            for (int v = 0; v < vols; ++v)                                                          \
+                for (int st = topf; st >= 1 && gf >= 2; st /= 2)                                    \
+                    for (int p = 0; p < gf; p += 2 * st) ihaar_pair##SFX(stk, v * gf + p, v * gf + p + st, f); \
+        } else {                                                                                    \
+            int top = 1;                                                                            \
+            while (top * 2 < k_use) top *= 2;                                                       \
+            for (int st = top; st >= 1 && k_use >= 2; st /= 2)                                      \
+                for (int p = 0; p < k_use; p += 2 * st) ihaar_pair##SFX(stk, p, p + st, f);         \
+        }                                                                                           \
+    }                                                                                               \
+    /* to_fixed / to_fixed_weight (aggregate.rs:147-159). */                                        \
+    static inline int32_t to_fixed##SFX(R v, R scale)                                               \
+    {                                                                                               \
+        R c = v < (R)-5 ? (R)-5 : (v > (R)5 ? (R)5 : v);                                            \
+        return (int32_t)ROUND(c * scale);                                                           \
+    }                                                                                               \
+    static inline int32_t to_fixed_weight##SFX(R w, R scale)                                        \
+    {                                                                                               \
+        R c = w < (R)0 ? (R)0 : (w > (R)1 ? (R)1 : w);                                              \
+        return (int32_t)ROUND(c * scale * (R)5);                                                    \
+    }                                                                                               \
+    /* filter_group for every reference: DCT, group Haar, hard threshold, weight, inverses and    */ \
+    /* the fixed-point scatter into each member's own frame region.                               */ \
+    void avdo_collab_filter##SFX(const float *values, const avdo_collab_params *pp, int centre_slot,   \
+                                 const uint32_t *nslots, const uint32_t *members, int32_t *accum,    \
+                                 int32_t *wsum, float *group_weight)                                \
+    {                                                                                               \
+        const int W = pp->W, H = pp->H, C = pp->C;                                                  \
+        const size_t px = (size_t)W * H;                                                            \
+        R te[16], to[16];                                                                           \
+        for (int m = 0; m < 4; ++m)                                                                 \
+            for (int k = 0; k < 4; ++k) {                                                           \
+                te[m * 4 + k] = (R)(float)dct8_basis(2 * m, k);                                     \
+                to[m * 4 + k] = (R)(float)dct8_basis(2 * m + 1, k);                                 \
+            }                                                                                       \
+        const R f = (R)0.70710678118654752440f;                                                     \
+        const R scale = (R)pp->accum_scale;                                                         \
+        _Pragma("omp parallel for schedule(dynamic, 2)") for (int iy = 0; iy < pp->refs_y; ++iy)     \
+        {                                                                                           \
+            R stk[8 * 64];                                                                          \
+            for (int ix = 0; ix < pp->refs_x; ++ix) {                                               \
+                size_t ref = (size_t)iy * pp->refs_x + ix;                                          \
+                const uint32_t *mem = members + ref * 9;                                            \
+                uint32_t gw = mem[8];                                                               \
+                int k_use = (int)(gw & 0xFF), grid = (gw & 0x100) != 0;                             \
+                int slot_of[8];                                                                     \
+                for (int m = 0; m < 8; ++m) {                                                       \
+                    uint32_t t = mem[m] >> 26;                                                      \
+                    slot_of[m] = t > 0 ? (int)nslots[t - 1] : centre_slot;                          \
+                }                                                                                   \
+                R scaled_weight = 0;                                                                \
+                for (int c = 0; c < C; ++c) {                                                       \
+                    R sig2 = (R)pp->sigma[c] * (R)pp->sigma[c];                                     \
+                    for (int m = 0; m < k_use; ++m) {                                               \
+                        int mx = (int)(mem[m] & 0x1FFF), my = (int)((mem[m] >> 13) & 0x1FFF);       \
+                        const float *src = values + (size_t)slot_of[m] * px * C;                    \
+                        for (int r = 0; r < 8; ++r)                                                 \
+                            for (int q = 0; q < 8; ++q)                                             \
+                                stk[m * 64 + r * 8 + q] = (R)src[((size_t)(my + r) * W + mx + q) * C + c]; \
+                        R line[8];                                                                  \
+                        for (int q = 0; q < 8; ++q) { /* vertical pass, column q */                 \
+                            for (int r = 0; r < 8; ++r) line[r] = stk[m * 64 + r * 8 + q];          \
+                            dct8_fwd##SFX(line, te, to);                                            \
+                            for (int r = 0; r < 8; ++r) stk[m * 64 + r * 8 + q] = line[r];          \
+                        }                                                                           \
+                        for (int r = 0; r < 8; ++r) { /* horizontal pass, vertical frequency r */   \
+                            dct8_fwd##SFX(stk + m * 64 + r * 8, te, to);                            \
+                        }                                                                           \
+                    }                                                                               \
+                    group_fwd##SFX(stk, k_use, grid, pp->grid_frames, f);                           \
+                    R cv[64], thr[64];                                                              \
+                    for (int v = 0; v < 8; ++v)                                                     \
+                        for (int u = 0; u < 8; ++u) {                                               \
+                            cv[v * 8 + u] = sig2 * ((R)pp->profile[u] * (R)pp->profile[v]);         \
+                            thr[v * 8 + u] = (R)pp->lambda[c] * SQRT(cv[v * 8 + u]);                \
+                        }                                                                           \
+                    int kept[64];                                                                   \
+                    for (int i = 0; i < 64; ++i) kept[i] = 0;                                       \
+                    for (int m = 0; m < k_use; ++m)                                                 \
+                        for (int v = 0; v < 8; ++v)                                                 \
+                            for (int u = 0; u < 8; ++u) {                                           \
+                                R *cf = &stk[m * 64 + v * 8 + u];                                   \
+                                int keep = (*cf < 0 ? -*cf : *cf) >= thr[v * 8 + u];               \
+                                if (m == 0 && v == 0 && u == 0) keep = 1;                           \
+                                if (keep) kept[v * 8 + u]++;                                        \
+                                else *cf = 0;                                                       \
+                            }                                                                       \
+                    /* retained_v: lane v (vertical frequency) adds each kept coefficient's        */ \
+                    /* variance, horizontal frequency outermost, then the 8-lane butterfly         */ \
+                    /* (fused/mod.rs:747-781, plane_ops.rs:17-23).                                 */ \
+                    R part[8];                                                                      \
+                    for (int v = 0; v < 8; ++v) {                                                   \
+                        R acc = 0;                                                                  \
+                        for (int u = 0; u < 8; ++u)                                                 \
+                            for (int n = 0; n < kept[v * 8 + u]; ++n) acc += cv[v * 8 + u];         \
+                        part[v] = acc;                                                              \
+                    }                                                                               \
+                    R sum = ((part[0] + part[1]) + (part[2] + part[3])) + ((part[4] + part[5]) + (part[6] + part[7])); \
+                    R weight = (isnan(sum) || isinf(sum)) ? (R)0 : (R)1 / (sum > (R)1e-12f ? sum : (R)1e-12f); \
+                    if (c == 0) {                                                                   \
+                        group_weight[ref] = (float)weight;                                          \
+                        scaled_weight = weight * (R)pp->weight_scale;                               \
+                    }                                                                               \
+                    group_inv##SFX(stk, k_use, grid, pp->grid_frames, f);                           \
+                    for (int m = 0; m < k_use; ++m) {                                               \
+                        for (int r = 0; r < 8; ++r) dct8_inv##SFX(stk + m * 64 + r * 8, te, to);    \
+                        R line[8];                                                                  \
+                        for (int q = 0; q < 8; ++q) {                                               \
+                            for (int r = 0; r < 8; ++r) line[r] = stk[m * 64 + r * 8 + q];          \
+                            dct8_inv##SFX(line, te, to);                                            \
+                            for (int r = 0; r < 8; ++r) stk[m * 64 + r * 8 + q] = line[r];          \
+                        }                                                                           \
+                        int mx = (int)(mem[m] & 0x1FFF), my = (int)((mem[m] >> 13) & 0x1FFF);       \
+                        size_t base = (size_t)slot_of[m] * px;                                      \
+                        for (int r = 0; r < 8; ++r)                                                 \
+                            for (int q = 0; q < 8; ++q) {                                           \
+                                R win = (R)pp->kaiser[r] * (R)pp->kaiser[q];                        \
+                                R w = scaled_weight * win;                                          \
+                                int32_t a = to_fixed##SFX(stk[m * 64 + r * 8 + q] * w, scale);      \
+                                size_t pix = base + (size_t)(my + r) * W + mx + q;                  \
+                                _Pragma("omp atomic") accum[pix * C + c] += a;                      \
+                                if (c == 0) {                                                       \
+                                    int32_t b = to_fixed_weight##SFX(w, scale);                     \
+                                    _Pragma("omp atomic") wsum[pix] += b;                           \
+                                }                                                                   \
+                            }                                                                       \
+                    }                                                                               \
+                }                                                                                   \
+            }                                                                                       \
+        }                                                                                           \
+    }
+
+static inline float fma_f32(float a, float b, float c) { return fmaf(a, b, c); }
+static inline double fma_f64(double a, double b, double c) { return a * b + c; }
+
+COLLAB_TEMPLATE(float, _f32, fma_f32, sqrtf, roundf)
+COLLAB_TEMPLATE(double, _f64, fma_f64, sqrt, round)
+
+/* collab_normalise (aggregate.rs:252-290): out = accum*5/wsum, or accum when wsum is 0; the
+ * region is zeroed afterwards. */
+void avdo_collab_normalise(int32_t *accum, int32_t *wsum, float *out, size_t pixels, int C)
+{
+    for (size_t p = 0; p < pixels; ++p) {
+        int32_t w = wsum[p];
+        for (int c = 0; c < C; ++c) {
+            float a = (float)accum[p * C + c];
+            out[p * C + c] = (w != 0) ? (a * 5.0f) / (float)w : a;
+            accum[p * C + c] = 0;
+        }
+        wsum[p] = 0;
+    }
+}
+
+int avdo_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void avdo_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* Exported single-step helpers, so the oracle's own tests can pin each piece against the
+ * reference's known answers (aggregate.rs:305-330, transforms.rs tests). */
+int32_t avdo_to_fixed_f32(float v, float scale) { return to_fixed_f32(v, scale); }
+int32_t avdo_to_fixed_weight_f32(float w, float scale) { return to_fixed_weight_f32(w, scale); }
+
+static void dct_taps_f32(float *te, float *to)
+{
+    for (int m = 0; m < 4; ++m)
+        for (int k = 0; k < 4; ++k) {
+            te[m * 4 + k] = (float)dct8_basis(2 * m, k);
+            to[m * 4 + k] = (float)dct8_basis(2 * m + 1, k);
+        }
+}
+
+void avdo_dct8_fwd_f32(float *line)
+{
+    float te[16], to[16];
+    dct_taps_f32(te, to);
+    dct8_fwd_f32(line, te, to);
+}
+
+void avdo_dct8_inv_f32(float *line)
+{
+    float te[16], to[16];
+    dct_taps_f32(te, to);
+    dct8_inv_f32(line, te, to);
+}
+
+void avdo_group_fwd_f32(float *stk, int k_use, int grid, int gf) { group_fwd_f32(stk, k_use, grid, gf, 0.70710678118654752440f); }
+void avdo_group_inv_f32(float *stk, int k_use, int grid, int gf) { group_inv_f32(stk, k_use, grid, gf, 0.70710678118654752440f); }

@@ -1,0 +1,675 @@
+"""CPU oracle for av-denoise's denoising core -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` import
+this module, and only as the checker.  The product (``av-denoise_amd``) never imports it.
+
+It wraps ``liboracle.so`` (``avd_oracle.c``) and adds, in plain Python/numpy:
+
+* the host-side closed forms the reference computes once per engine
+  (``nlmeans/params.rs:229-268``, ``noise/correlation.rs:43-88``, ``collab/kernels/aggregate.rs:36-125``,
+  ``collab/geometry.rs:6-29``, ``motion/confidence.rs:17-29``);
+* stream drivers that replay the reference's ring and pass scheduling:
+  NLMeans with repeated edges (``nlmeans/denoiser/ring.rs``, ``nlmeans/engine.rs:201-262``,
+  ``nlmeans/dispatch/mod.rs:653-679``) and NL4D with shifted edges and head/middle/tail passes
+  (``nl4d/denoiser.rs:370-600``, ``nl4d/engine.rs:264-360``, ``nl4d/collab_pass.rs:50-200``).
+
+Parity status of each function is stated in DESIGN.md's oracle table.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+F32P = ctypes.POINTER(ctypes.c_float)
+F64P = ctypes.POINTER(ctypes.c_double)
+U32P = ctypes.POINTER(ctypes.c_uint32)
+I32P = ctypes.POINTER(ctypes.c_int32)
+U16P = ctypes.POINTER(ctypes.c_uint16)
+
+
+class CollabParams(ctypes.Structure):
+    _fields_ = [
+        ("W", ctypes.c_int), ("H", ctypes.c_int), ("C", ctypes.c_int),
+        ("step", ctypes.c_int), ("refs_x", ctypes.c_int), ("refs_y", ctypes.c_int),
+        ("S", ctypes.c_int), ("radius", ctypes.c_int), ("grid_frames", ctypes.c_int),
+        ("refine", ctypes.c_int), ("k_max", ctypes.c_int),
+        ("blk_step", ctypes.c_int), ("blksize", ctypes.c_int),
+        ("blocks_x", ctypes.c_int), ("blocks_y", ctypes.c_int),
+        ("c_min", ctypes.c_float), ("lam", ctypes.c_float * 2), ("sigma", ctypes.c_float * 2),
+        ("weight_scale", ctypes.c_float), ("accum_scale", ctypes.c_float),
+        ("kaiser", ctypes.c_float * 8), ("profile", ctypes.c_float * 8),
+    ]
+
+
+def lib():
+    """The compiled oracle; build it with ``make -C oracle`` (``__graft_entry__.build`` does)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing; run `make -C oracle` first")
+        _LIB = ctypes.CDLL(path)
+        _LIB.avdo_match_confidence.restype = ctypes.c_float
+        _LIB.avdo_max_threads.restype = ctypes.c_int
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def set_threads(n: int) -> None:
+    lib().avdo_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().avdo_max_threads())
+
+
+# ----------------------------------------------------------------------------------------------
+# Sample formats and plane I/O (engine/plane.rs:8-52, engine/kernels.rs:25-158)
+# ----------------------------------------------------------------------------------------------
+
+def samples_per_word(depth: int) -> int:
+    return 4 if depth == 8 else 2
+
+
+def max_value(depth: int) -> float:
+    return float((1 << depth) - 1)
+
+
+def pack_plane(codes: np.ndarray, depth: int) -> np.ndarray:
+    """Pack integer codes (H, W) into little-endian u32 words, tail lanes zero."""
+    flat = codes.reshape(-1).astype(np.uint32)
+    spw = samples_per_word(depth)
+    words = (flat.size + spw - 1) // spw
+    padded = np.zeros(words * spw, np.uint32)
+    padded[: flat.size] = flat
+    padded = padded.reshape(words, spw)
+    bits = 32 // spw
+    out = np.zeros(words, np.uint32)
+    for lane in range(spw):
+        out |= padded[:, lane] << np.uint32(lane * bits)
+    return out
+
+
+def unpack_plane(words: np.ndarray, depth: int, pixels: int) -> np.ndarray:
+    spw = samples_per_word(depth)
+    bits = 32 // spw
+    mask = np.uint32((1 << bits) - 1)
+    lanes = [(words >> np.uint32(l * bits)) & mask for l in range(spw)]
+    return np.stack(lanes, axis=1).reshape(-1)[:pixels].astype(np.int32)
+
+
+def ingest(planes_words, pixels: int, channels: int, depth: int) -> np.ndarray:
+    out = np.zeros(pixels * channels, np.float32)
+    p0 = np.ascontiguousarray(planes_words[0], np.uint32)
+    p1 = np.ascontiguousarray(planes_words[1] if channels > 1 else planes_words[0], np.uint32)
+    lib().avdo_ingest_words(_p(p0, U32P), _p(p1, U32P), _p(out, F32P), ctypes.c_uint32(pixels),
+                            ctypes.c_uint32(channels), ctypes.c_uint32(samples_per_word(depth)),
+                            ctypes.c_float(max_value(depth)))
+    return out
+
+
+def egress(frame: np.ndarray, pixels: int, channels: int, depth: int):
+    spw = samples_per_word(depth)
+    words = (pixels + spw - 1) // spw
+    p0 = np.zeros(words, np.uint32)
+    p1 = np.zeros(words, np.uint32)
+    f = np.ascontiguousarray(frame, np.float32)
+    lib().avdo_egress_words(_p(f, F32P), _p(p0, U32P), _p(p1, U32P), ctypes.c_uint32(pixels),
+                            ctypes.c_uint32(channels), ctypes.c_uint32(spw), ctypes.c_float(max_value(depth)))
+    return [p0, p1][:channels]
+
+
+# ----------------------------------------------------------------------------------------------
+# Closed forms
+# ----------------------------------------------------------------------------------------------
+
+HQ_STRENGTH_LUMA = [0.45, 0.45, 0.42, 0.42, 0.35, 0.35, 0.35, 0.30, 0.30]    # nlmeans/params.rs:13
+HQ_STRENGTH_CHROMA = [1.00, 0.85, 0.70, 0.70, 0.70, 0.70, 0.70, 0.70, 0.70]  # nlmeans/params.rs:16
+
+
+def channel_scale(channels: int) -> float:
+    return 3.0 if channels == 1 else 1.5  # nlmeans/kernels/helpers.rs:51-62
+
+
+def h2_inv_norm(strength: float, patch_radius: int) -> np.float32:
+    """NLM_NORM / (NLM_LEGACY * h^2 * area) in f32 (nlmeans/params.rs:238-243)."""
+    area = np.float32((2 * patch_radius + 1) ** 2)
+    s = np.float32(strength)
+    return np.float32(np.float32(255.0 * 255.0) / (np.float32(3.0) * s * s * area))
+
+
+def noise_offset(sigma: float, channels: int, patch_radius: int) -> np.float32:
+    """2 * scale * sum(sigma^2) * area in f32 (nlmeans/params.rs:250-262)."""
+    area = np.float32((2 * patch_radius + 1) ** 2)
+    s = np.float32(sigma)
+    sum_sq = np.float32(0.0)
+    for _ in range(channels):
+        sum_sq = np.float32(sum_sq + s * s)
+    return np.float32(np.float32(2.0) * np.float32(channel_scale(channels)) * sum_sq * area)
+
+
+def spatial_offset_lut(search_radius: int, offset: float) -> np.ndarray:
+    """noise/correlation.rs:75-88 at rho = 0: the flat offset everywhere but the centre."""
+    side = 2 * search_radius + 1
+    lut = np.full(side * side, np.float32(offset), np.float32)
+    lut[search_radius * side + search_radius] = 0.0
+    return lut
+
+
+def sad_noise_floor(blksize: int, sigma: float) -> np.float32:
+    area = np.float32(blksize * blksize)
+    return np.float32(area * np.float32(2.0) * np.float32(sigma) / np.float32(math.sqrt(math.pi)))
+
+
+def thsad(blksize: int, scale: float) -> np.float32:
+    return np.float32(np.float32(scale) * np.float32(blksize * blksize) * np.float32(0.02))
+
+
+def kaiser_window(beta: float) -> np.ndarray:
+    """aggregate.rs:92-125: I0(beta*sqrt(1-(2i/7-1)^2)) / I0(beta), f64 then f32."""
+    def i0(x):
+        half, term, s = x / 2.0, 1.0, 1.0
+        for k in range(1, 32):
+            term *= half / k
+            s += term * term
+        return s
+    denom = i0(float(np.float32(beta)))
+    out = np.zeros(8, np.float32)
+    for i in range(8):
+        pos = 2.0 * i / 7.0 - 1.0
+        out[i] = np.float32(i0(float(np.float32(beta)) * math.sqrt(1.0 - pos * pos)) / denom)
+    return out
+
+
+def cross_frame_accum_scale(spatial_radius: int, temporal_radius: int, step: int = 4) -> np.float32:
+    """aggregate.rs:36-55: the largest power of two keeping the worst case under i32::MAX/2."""
+    refs_per_axis = ((8 - 1) + 2 * spatial_radius) // step + 1
+    contribs = float(refs_per_axis) * float(refs_per_axis) * 8.0
+    passes = float(4 * temporal_radius + 1)
+    max_raw = contribs * passes * 5.0
+    budget = float(2**31 - 1) / 2.0
+    exponent = math.floor(math.log2(budget / max_raw))
+    return np.float32(2.0 ** exponent)
+
+
+def dct8_basis(row: int, col: int) -> float:
+    """transforms.rs:39-43, in double."""
+    scale = 1.0 / math.sqrt(8.0) if row == 0 else 0.5
+    return scale * math.cos(math.pi * (2.0 * col + 1.0) * row / 16.0)
+
+
+def dct_noise_profile(rho: float) -> np.ndarray:
+    """transforms.rs:196-221: g(u) = sum_ij B_u(i) B_u(j) rho^|i-j|, exactly ones at rho <= 0."""
+    if rho <= 0.0:
+        return np.ones(8, np.float32)
+    r = float(np.float32(rho))
+    out = np.zeros(8, np.float32)
+    for u in range(8):
+        acc = 0.0
+        for i in range(8):
+            for j in range(8):
+                acc += dct8_basis(u, i) * dct8_basis(u, j) * r ** abs(i - j)
+        out[u] = np.float32(acc)
+    return out
+
+
+def weight_scale(sigma: float, g_max: float = 1.0) -> np.float32:
+    norm = np.float32(np.float32(sigma) * np.float32(sigma) * np.float32(g_max) * np.float32(g_max))
+    return norm if (np.isfinite(norm) and norm > np.float32(1e-12)) else np.float32(1e-12)
+
+
+def refs_along(dim: int, step: int = 4) -> int:
+    return (dim - 8 + step - 1) // step + 1
+
+
+def ref_pos(index: int, dim: int, step: int = 4) -> int:
+    return min(index * step, dim - 8)
+
+
+def grid_frames(temporal_radius: int) -> int:
+    return 1 if temporal_radius == 0 else (2 if temporal_radius == 1 else 4)
+
+
+# ----------------------------------------------------------------------------------------------
+# Pass-level restatements (numpy in, numpy out)
+# ----------------------------------------------------------------------------------------------
+
+def nlm_single(frame, W, H, C, P, S, h2, lut, accum, wsum, maxw, precision="f32"):
+    fn = lib().avdo_nlm_single_f32 if precision == "f32" else lib().avdo_nlm_single_f64
+    pt = F32P if precision == "f32" else F64P
+    fn(_p(frame, F32P), W, H, C, P, S, ctypes.c_float(h2), _p(lut, F32P), _p(accum, pt), _p(wsum, pt), _p(maxw, pt))
+
+
+def nlm_pair(centre, fwd, bwd, W, H, C, P, S, h2, offset, conf_f, conf_b, step, bx, by, accum, wsum, maxw,
+             precision="f32"):
+    fn = lib().avdo_nlm_pair_f32 if precision == "f32" else lib().avdo_nlm_pair_f64
+    pt = F32P if precision == "f32" else F64P
+    cf = _p(conf_f, F32P) if conf_f is not None else None
+    cb = _p(conf_b, F32P) if conf_b is not None else None
+    fn(_p(centre, F32P), _p(fwd, F32P), _p(bwd, F32P), W, H, C, P, S, ctypes.c_float(h2), ctypes.c_float(offset),
+       cf, cb, step, bx, by, _p(accum, pt), _p(wsum, pt), _p(maxw, pt))
+
+
+def nlm_finish(frame, accum, wsum, maxw, wref, pixels, C, precision="f32"):
+    out = np.zeros(pixels * C, np.float32)
+    fn = lib().avdo_nlm_finish_f32 if precision == "f32" else lib().avdo_nlm_finish_f64
+    pt = F32P if precision == "f32" else F64P
+    fn(_p(frame, F32P), _p(accum, pt), _p(wsum, pt), _p(maxw, pt), ctypes.c_float(wref), pixels, C, _p(out, F32P))
+    return out
+
+
+def downscale_sum(level0: np.ndarray, w: int, h: int) -> np.ndarray:
+    dw, dh = max(w // 2, 1), max(h // 2, 1)
+    out = np.zeros(dw * dh, np.int32)
+    src = np.ascontiguousarray(level0, np.int32)
+    lib().avdo_downscale_sum(_p(src, I32P), w, h, _p(out, I32P), dw, dh)
+    return out
+
+
+@dataclass
+class MotionGeom:
+    """MotionCtx for an Mvtools mode (motion/context.rs:53-90) with the coarse level of analyse.rs:112-145."""
+    W: int
+    H: int
+    blksize: int = 16
+    overlap: int = 8
+    search_radius: int = 4
+    pyramid_levels: int = 2
+
+    @property
+    def step(self):
+        return self.blksize - self.overlap
+
+    @property
+    def blocks_x(self):
+        return max((self.W + self.step - 1) // self.step, 1)
+
+    @property
+    def blocks_y(self):
+        return max((self.H + self.step - 1) // self.step, 1)
+
+    @property
+    def blocks(self):
+        return self.blocks_x * self.blocks_y
+
+
+def motion_search(geom: MotionGeom, pyr_centre, pyr_nb, maxv, sad_floor, th, want_conf=True):
+    """Coarse (level 1) then seeded fine (level 0) block match of one neighbour against the centre."""
+    mv = np.zeros(geom.blocks * 2, np.int32)
+    conf = np.zeros(geom.blocks, np.float32) if want_conf else None
+    L = lib()
+    use_seed = 0
+    if geom.pyramid_levels >= 2:
+        lw, lh = max(geom.W // 2, 1), max(geom.H // 2, 1)
+        cblk = max(geom.blksize // 2, 2)
+        cstep = max(geom.step // 2, 1)
+        L.avdo_block_match_coarse(_p(pyr_centre[1], I32P), _p(pyr_nb[1], I32P), lw, lh, cblk, cstep,
+                                  geom.search_radius, 2, geom.blocks_x, geom.blocks_y, geom.step, _p(mv, I32P))
+        use_seed = 1
+    L.avdo_block_match_fine(_p(pyr_centre[0], I32P), _p(pyr_nb[0], I32P), geom.W, geom.H, geom.blksize, geom.step,
+                            geom.search_radius, geom.blocks_x, geom.blocks_y, use_seed, _p(mv, I32P),
+                            _p(conf, F32P) if want_conf else None, ctypes.c_float(maxv), ctypes.c_float(sad_floor),
+                            ctypes.c_float(th))
+    return mv, conf
+
+
+def warp(src, mv, W, H, C, geom: MotionGeom):
+    dst = np.zeros_like(src)
+    lib().avdo_warp(_p(src, F32P), _p(dst, F32P), _p(mv, I32P), W, H, C, geom.step, geom.blocks_x, geom.blocks_y)
+    return dst
+
+
+def collab_params(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, geom: MotionGeom, step=4,
+                  profile=None, sigmas=None):
+    p = CollabParams()
+    p.W, p.H, p.C = W, H, C
+    p.step = step
+    p.refs_x, p.refs_y = refs_along(W, step), refs_along(H, step)
+    p.S, p.radius, p.grid_frames, p.refine, p.k_max = S, radius, grid_frames(radius), refine, 8
+    p.blk_step, p.blksize, p.blocks_x, p.blocks_y = geom.step, geom.blksize, geom.blocks_x, geom.blocks_y
+    p.c_min = c_min
+    sig = list(sigmas) if sigmas is not None else [sigma, sigma]
+    prof = dct_noise_profile(0.0) if profile is None else np.asarray(profile, np.float32)
+    for c in range(2):
+        p.lam[c] = float(lam)
+        p.sigma[c] = float(sig[min(c, len(sig) - 1)])
+    p.weight_scale = float(weight_scale(sig[0], float(prof.max())))
+    for i in range(8):
+        p.profile[i] = float(prof[i])
+    p.accum_scale = float(cross_frame_accum_scale(S, radius, step))
+    k = kaiser_window(kaiser_beta)
+    for i in range(8):
+        p.kaiser[i] = float(k[i])
+    return p
+
+
+def collab_search(ring, params: CollabParams, centre_slot, nslots, mv, conf):
+    """search_group over every reference; ``ring`` is u16 codes (exact SSD) or f32 values."""
+    refs = params.refs_x * params.refs_y
+    members = np.zeros(refs * 9, np.uint32)
+    ns = np.ascontiguousarray(nslots, np.uint32) if len(nslots) else np.zeros(1, np.uint32)
+    mvv = np.ascontiguousarray(mv, np.int32) if mv is not None and mv.size else np.zeros(2, np.int32)
+    cf = np.ascontiguousarray(conf, np.float32) if conf is not None and conf.size else np.zeros(1, np.float32)
+    if ring.dtype == np.uint16:
+        fn, rp = lib().avdo_collab_search_u16, _p(ring, U16P)
+    else:
+        fn, rp = lib().avdo_collab_search_f32, _p(np.ascontiguousarray(ring, np.float32), F32P)
+    fn(rp, ctypes.byref(params), centre_slot, _p(ns, U32P), _p(mvv, I32P), _p(cf, F32P), _p(members, U32P))
+    return members
+
+
+def collab_filter(values_ring, params: CollabParams, centre_slot, nslots, members, accum, wsum, precision="f32"):
+    refs = params.refs_x * params.refs_y
+    gw = np.zeros(refs, np.float32)
+    ns = np.ascontiguousarray(nslots, np.uint32) if len(nslots) else np.zeros(1, np.uint32)
+    fn = lib().avdo_collab_filter_f32 if precision == "f32" else lib().avdo_collab_filter_f64
+    fn(_p(values_ring, F32P), ctypes.byref(params), centre_slot, _p(ns, U32P), _p(members, U32P), _p(accum, I32P),
+       _p(wsum, I32P), _p(gw, F32P))
+    return gw
+
+
+def collab_normalise(accum_region, wsum_region, pixels, C):
+    out = np.zeros(pixels * C, np.float32)
+    lib().avdo_collab_normalise(_p(accum_region, I32P), _p(wsum_region, I32P), _p(out, F32P),
+                                ctypes.c_size_t(pixels), C)
+    return out
+
+
+# ----------------------------------------------------------------------------------------------
+# Stream drivers
+# ----------------------------------------------------------------------------------------------
+
+@dataclass
+class NlmSpec:
+    """Resolved NLM parameters of one pass (nlmeans/options.rs:106-137, nlmeans/params.rs:172-262)."""
+    temporal_radius: int = 0
+    search_radius: int = 2
+    patch_radius: int = 4
+    strength: float = 1.2
+    self_weight: float = 1.0
+    hq: bool = False
+    sigma: float | None = None            # HQ pinned sigma (normalised)
+    auto_strength: bool = True
+    noise_floor: bool = True
+    temporal_confidence: bool = True
+    thsad_scale: float = 1.0
+    motion: MotionGeom | None = None      # Mvtools block geometry (W/H filled per pass)
+
+    def h2(self):
+        strength = self.strength
+        if self.hq and self.auto_strength and self.sigma is not None:
+            strength = float(np.float32(np.float32(self.strength) * np.float32(self.sigma) * np.float32(255.0)))
+        return h2_inv_norm(strength, self.patch_radius)
+
+    def offset(self, channels):
+        if self.hq and self.noise_floor and self.sigma is not None:
+            return noise_offset(self.sigma, channels, self.patch_radius)
+        return np.float32(0.0)
+
+
+class _Ring:
+    """A ring of 2r+1 ingested slots with their motion pyramids (nlmeans/denoiser/ring.rs)."""
+
+    def __init__(self, W, H, C, depth, total):
+        self.W, self.H, self.C, self.depth, self.total = W, H, C, depth, total
+        self.values = np.zeros((total, H * W * C), np.float32)
+        self.codes = np.zeros((total, H * W * C), np.uint16)
+        self.pyr = [None] * total
+        self.slot_frames = [0] * total
+        self.head = 0
+        self.loaded = 0
+        self.real = 0
+
+    def phys(self, logical):
+        start = self.head % self.total
+        return (start + (logical % self.total)) % self.total
+
+    def _build_pyramid(self, slot):
+        l0 = self.codes[slot].reshape(self.H * self.W, self.C)[:, 0].astype(np.int32)
+        self.pyr[slot] = [l0, downscale_sum(l0, self.W, self.H)]
+
+    def write(self, words):
+        slot = self.head % self.total
+        px = self.W * self.H
+        self.values[slot] = ingest(words, px, self.C, self.depth)
+        codes = [unpack_plane(w, self.depth, px) for w in words[: self.C]]
+        self.codes[slot] = np.stack(codes, axis=1).reshape(-1).astype(np.uint16)
+        self._build_pyramid(slot)
+        self.slot_frames[slot] = self.real
+        self.head += 1
+        if self.loaded < self.total:
+            self.loaded += 1
+        self.real += 1
+
+    def duplicate_last(self):
+        last = (self.head - 1) % self.total
+        nxt = self.head % self.total
+        self.values[nxt] = self.values[last]
+        self.codes[nxt] = self.codes[last]
+        self.pyr[nxt] = self.pyr[last]
+        self.slot_frames[nxt] = self.slot_frames[last]
+        self.head += 1
+
+    def ready(self):
+        return self.loaded >= self.total
+
+
+class NlmPass:
+    """One NLMeans plane group over a stream (Repeat edges)."""
+
+    def __init__(self, spec: NlmSpec, W, H, C, depth_in, depth_out, precision="f32"):
+        self.s, self.W, self.H, self.C = spec, W, H, C
+        self.depth_out = depth_out
+        self.precision = precision
+        self.r = spec.temporal_radius
+        self.ring = _Ring(W, H, C, depth_in, 2 * self.r + 1)
+        self.out = []
+        if spec.motion is not None and self.r > 0:
+            spec.motion = MotionGeom(W, H, spec.motion.blksize, spec.motion.overlap, spec.motion.search_radius,
+                                     spec.motion.pyramid_levels)
+        self.maxv = max_value(depth_in)
+
+    def push(self, words):
+        self.ring.write(words)
+        if self.ring.real == 1 and self.r > 0:
+            for _ in range(self.r):
+                self.ring.duplicate_last()
+                self.ring.loaded += 1
+        if self.ring.ready():
+            self.out.append(self._denoise())
+        return len(self.out)
+
+    def finish(self):
+        target = min(self.ring.real, self.r)
+        produced = 0
+        while produced < target:
+            self.ring.duplicate_last()
+            if self.ring.loaded < self.ring.total:
+                self.ring.loaded += 1
+            if self.ring.ready():
+                self.out.append(self._denoise())
+                produced += 1
+        self.ring = _Ring(self.W, self.H, self.C, self.ring.depth, self.ring.total)
+        return target
+
+    def _denoise(self):
+        s, W, H, C, r = self.s, self.W, self.H, self.C, self.r
+        px = W * H
+        dt = np.float32 if self.precision == "f32" else np.float64
+        accum = np.zeros(px * C, dt)
+        wsum = np.zeros(px, dt)
+        maxw = np.zeros(px, dt)
+        h2 = float(s.h2())
+        off = float(s.offset(C))
+        centre = self.ring.phys(r)
+        cf = self.ring.values[centre]
+        frames = {}
+        confs = {}
+        geom = s.motion
+        if r > 0 and geom is not None:
+            sigma = s.sigma if s.sigma is not None else 0.0
+            floor = float(sad_noise_floor(geom.blksize, sigma))
+            th = float(thsad(geom.blksize, s.thsad_scale))
+            nidx = 0
+            for k in list(range(-r, 0)) + list(range(1, r + 1)):
+                slot = self.ring.phys(r + k)
+                mv, conf = motion_search(geom, self.ring.pyr[centre], self.ring.pyr[slot], self.maxv, floor, th)
+                frames[k] = warp(self.ring.values[slot], mv, W, H, C, geom)
+                confs[k] = conf if (s.hq and s.temporal_confidence) else None
+                nidx += 1
+        else:
+            for k in range(-r, r + 1):
+                if k:
+                    frames[k] = self.ring.values[self.ring.phys(r + k)]
+                    confs[k] = None
+        for qk in range(-r, 0):
+            fwd, bwd = frames[qk], frames[-qk]
+            gb = geom if geom is not None else MotionGeom(W, H)
+            nlm_pair(cf, fwd, bwd, W, H, C, s.patch_radius, s.search_radius, h2, off, confs[qk], confs[-qk],
+                     gb.step, gb.blocks_x, gb.blocks_y, accum, wsum, maxw, self.precision)
+        lut = spatial_offset_lut(s.search_radius, off)
+        nlm_single(cf, W, H, C, s.patch_radius, s.search_radius, h2, lut, accum, wsum, maxw, self.precision)
+        res = nlm_finish(cf, accum, wsum, maxw, s.self_weight, px, C, self.precision)
+        return res
+
+
+@dataclass
+class Nl4dSpec:
+    """Resolved NL4D parameters of one pass (nl4d/options.rs:163-185, nl4d/params.rs:27-198)."""
+    temporal_radius: int = 2
+    refine: int = 2
+    spatial_radius: int = 9
+    lambda_ht: float = 4.158
+    c_min: float = 0.05
+    kaiser_beta: float = 2.0
+    sigma: float = 0.02
+    thsad_scale: float = 1.0
+    motion: MotionGeom = field(default_factory=lambda: MotionGeom(0, 0))
+    step: int = 4
+
+
+class Nl4dPass:
+    """One NL4D plane group over a stream (Shifted edges, head/middle/tail passes)."""
+
+    def __init__(self, spec: Nl4dSpec, W, H, C, depth_in, depth_out, precision="f32"):
+        self.s, self.W, self.H, self.C = spec, W, H, C
+        self.depth_out = depth_out
+        self.precision = precision
+        self.r = spec.temporal_radius
+        self.N = 2 * self.r + 1
+        self.ring = _Ring(W, H, C, depth_in, self.N)
+        self.geom = MotionGeom(W, H, spec.motion.blksize, spec.motion.overlap, spec.motion.search_radius,
+                               spec.motion.pyramid_levels)
+        self.maxv = max_value(depth_in)
+        self.params = collab_params(W, H, C, self.r, spec.spatial_radius, spec.refine, spec.c_min, spec.lambda_ht,
+                                    spec.sigma, spec.kaiser_beta, self.geom, spec.step)
+        px = W * H
+        self.accum = np.zeros(self.N * px * C, np.int32)
+        self.wsum = np.zeros(self.N * px, np.int32)
+        self.passes = 0
+        self.out = []
+        self.last_members = None
+
+    # nl4d/collab_pass.rs:50-200 and nlmeans/denoiser/machinery.rs:45-101
+    def _run_pass(self, centre_logical, clear):
+        px = self.W * self.H
+        C = self.C
+        centre = self.ring.phys(centre_logical)
+        if clear == "whole":
+            self.accum[:] = 0
+            self.wsum[:] = 0
+        elif clear == "newest":
+            newest = (centre + self.r) % self.N
+            self.accum[newest * px * C:(newest + 1) * px * C] = 0
+            self.wsum[newest * px:(newest + 1) * px] = 0
+        nslots = [self.ring.phys(l) for l in range(self.N) if l != centre_logical]
+        sig = float(self.s.sigma)
+        floor = float(sad_noise_floor(self.geom.blksize, sig))
+        th = float(thsad(self.geom.blksize, self.s.thsad_scale))
+        mvs, confs = [], []
+        for slot in nslots:
+            mv, conf = motion_search(self.geom, self.ring.pyr[centre], self.ring.pyr[slot], self.maxv, floor, th)
+            mvs.append(mv)
+            confs.append(conf)
+        mv = np.concatenate(mvs) if mvs else np.zeros(2, np.int32)
+        conf = np.concatenate(confs) if confs else np.zeros(1, np.float32)
+        members = collab_search(self.ring.codes, self.params, centre, nslots, mv, conf)
+        self.last_members = members
+        self.last_fields = (mv, conf)
+        collab_filter(self.ring.values, self.params, centre, nslots, members, self.accum, self.wsum, self.precision)
+        self.passes += 1
+
+    def _read_region(self, slot):
+        px = self.W * self.H
+        C = self.C
+        a = self.accum[slot * px * C:(slot + 1) * px * C]
+        w = self.wsum[slot * px:(slot + 1) * px]
+        out = collab_normalise(a, w, px, C)
+        self.accum[slot * px * C:(slot + 1) * px * C] = 0
+        self.wsum[slot * px:(slot + 1) * px] = 0
+        return out
+
+    def push(self, words):
+        self.ring.write(words)
+        if not self.ring.ready():
+            return len(self.out)
+        r = self.r
+        clear = "whole" if self.passes == 0 else "newest"
+        if self.passes == 0:
+            for c in range(r):
+                self._run_pass(c, clear)
+                clear = "none"
+        self._run_pass(r, clear)
+        if self.passes > r:
+            centre = self.ring.phys(r)
+            slot = (centre + self.N - r) % self.N
+            self.out.append(self._read_region(slot))
+        return len(self.out)
+
+    def finish(self):
+        real = self.ring.real
+        emit = min(real, 2 * self.r)
+        if emit == 0:
+            return 0
+        short = real < self.N
+        if short:
+            while not self.ring.ready():
+                self.ring.duplicate_last()
+                self.ring.loaded += 1
+            centres, last_real = list(range(emit)), emit - 1
+        else:
+            centres, last_real = list(range(self.r + 1, 2 * self.r + 1)), 2 * self.r
+        clear = "whole" if (short or self.passes == 0) else "none"
+        for c in centres:
+            self._run_pass(c, clear)
+            clear = "none"
+        first = last_real + 1 - emit
+        for logical in range(first, last_real + 1):
+            self.out.append(self._read_region(self.ring.phys(logical)))
+        self.ring = _Ring(self.W, self.H, self.C, self.ring.depth, self.N)
+        self.passes = 0
+        return emit
+
+
+def run_stream(pass_obj, frames_words):
+    """Push every frame, then finish; returns the egressed output words per emitted frame."""
+    outputs = []
+    for words in frames_words:
+        pass_obj.push(words)
+        while pass_obj.out:
+            outputs.append(pass_obj.out.pop(0))
+    pass_obj.finish()
+    while pass_obj.out:
+        outputs.append(pass_obj.out.pop(0))
+    px = pass_obj.W * pass_obj.H
+    return [egress(f, px, pass_obj.C, pass_obj.depth_out) for f in outputs], outputs
