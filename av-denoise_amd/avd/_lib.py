@@ -1,0 +1,201 @@
+"""ctypes bindings of ``include/avd/avd.h``: the structs, the error classes and the loader.
+
+The loader fails loudly when ``libavd.so`` is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libavd.so")
+
+u32 = ctypes.c_uint32
+f32 = ctypes.c_float
+
+
+class SampleFormat(ctypes.Structure):
+    """SampleFormat (engine/plane.rs:8-52): kind 0 U8, 1 U16 with ``depth`` bits."""
+    _fields_ = [("kind", u32), ("depth", u32)]
+
+    @classmethod
+    def u8(cls):
+        return cls(0, 8)
+
+    @classmethod
+    def u16(cls, depth: int):
+        return cls(1, depth)
+
+    @classmethod
+    def for_depth(cls, depth: int):
+        return cls.u8() if depth == 8 else cls.u16(depth)
+
+
+class Geometry(ctypes.Structure):
+    """Geometry (engine/frame.rs:73-97).  chroma: 0 Gray, 1 4:2:0, 2 4:2:2, 3 4:4:4."""
+    _fields_ = [("width", u32), ("height", u32), ("chroma", u32), ("input", SampleFormat), ("output", SampleFormat),
+                ("chroma_siting", u32)]
+
+
+class DevicePlane(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("bytes", ctypes.c_uint64), ("width", u32), ("height", u32)]
+
+
+class MotionSearch(ctypes.Structure):
+    """MotionSearch (motion/search.rs:5-40).  estimation: 0 Auto, 1 Direct, 2 Chained."""
+    _fields_ = [("blksize", u32), ("overlap", u32), ("search_radius", u32), ("max_search_escalation", u32),
+                ("pyramid_levels", u32), ("estimation", u32), ("refine_radius", u32)]
+
+
+class NlmeansOptions(ctypes.Structure):
+    """NlmeansAlgorithm with NlmeansOptions and HqParams (nlmeans/options.rs:6-106)."""
+    _fields_ = [("hq", u32), ("temporal", u32), ("temporal_radius", u32),
+                ("has_search_radius", u32), ("search_radius", u32),
+                ("has_patch_radius", u32), ("patch_radius", u32),
+                ("has_strength", u32), ("strength", f32),
+                ("has_self_weight", u32), ("self_weight", f32),
+                ("motion_compensation", u32), ("motion", MotionSearch),
+                ("auto_strength", u32), ("noise_floor", u32), ("has_sigma", u32), ("sigma", f32),
+                ("temporal_confidence", u32), ("thsad_scale", f32), ("sigma_scale", f32)]
+
+
+class Nl4dOptions(ctypes.Structure):
+    """Nl4dOptions (nl4d/options.rs:11-94) plus the grid stride and group size."""
+    _fields_ = [("motion", MotionSearch), ("has_sigma", u32), ("sigma", f32), ("sigma_scale", f32),
+                ("thsad_scale", f32), ("refine", u32), ("spatial_radius", u32), ("has_lambda_ht", u32),
+                ("lambda_ht", f32), ("lambda_ht_scale", f32), ("c_min", f32), ("kaiser_beta", f32),
+                ("field_lambda", f32), ("noise_map", u32), ("pooled_threshold", u32), ("grain_data", u32),
+                ("temporal_radius", u32), ("step", u32), ("group_size", u32)]
+
+
+class EngineSpec(ctypes.Structure):
+    """EngineSpec (engine/spec.rs:146-157).  kind 0 NLMeans, 1 NL4D."""
+    _fields_ = [("kind", u32), ("geometry", Geometry), ("clean_luma", u32), ("clean_chroma", u32),
+                ("nlm_luma", NlmeansOptions), ("nlm_chroma", NlmeansOptions),
+                ("nl4d_luma", Nl4dOptions), ("nl4d_chroma", Nl4dOptions), ("stream", ctypes.c_void_p)]
+
+
+class WindowSpan(ctypes.Structure):
+    _fields_ = [("behind", ctypes.c_uint64), ("ahead", ctypes.c_uint64), ("edges", u32)]
+
+
+class CollabPass(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("channels", u32), ("step", u32), ("spatial_radius", u32),
+                ("temporal_radius", u32), ("refine", u32), ("group_size", u32), ("blk_step", u32),
+                ("blksize", u32), ("blocks_x", u32), ("blocks_y", u32), ("centre_slot", u32), ("slots", u32),
+                ("c_min", f32), ("lam", f32 * 2), ("sigma", f32 * 2), ("weight_scale", f32),
+                ("accum_scale", f32), ("kaiser", f32 * 8), ("profile", f32 * 8)]
+
+
+class MotionPass(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("channels", u32), ("blksize", u32), ("overlap", u32),
+                ("search_radius", u32), ("pyramid_levels", u32), ("centre_slot", u32), ("neighbours", u32),
+                ("max_value", f32), ("sad_floor", f32), ("thsad", f32)]
+
+
+class NlmWindow(ctypes.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("channels", u32), ("patch_radius", u32), ("search_radius", u32),
+                ("h2_inv_norm", f32), ("noise_offset", f32), ("step", u32), ("blocks_x", u32), ("blocks_y", u32)]
+
+
+# ---- errors (error.rs:3-20) --------------------------------------------------------------------
+
+class Error(RuntimeError):
+    status = -1
+
+
+class InvalidOptions(Error):
+    status = 1
+
+
+class InvalidGeometry(Error):
+    status = 2
+
+
+class PlaneMismatch(Error):
+    status = 3
+
+
+class OutputsPending(Error):
+    status = 4
+
+
+class NothingToEmit(Error):
+    status = 5
+
+
+class ContextAfterPush(Error):
+    status = 6
+
+
+class NeedsReset(Error):
+    status = 7
+
+
+class Gpu(Error):
+    status = 8
+
+
+class NullArgument(Error):
+    status = 9
+
+
+_BY_STATUS = {c.status: c for c in (InvalidOptions, InvalidGeometry, PlaneMismatch, OutputsPending, NothingToEmit,
+                                     ContextAfterPush, NeedsReset, Gpu, NullArgument)}
+
+# Every exported symbol and its ctypes signature (restype, argtypes).
+P = ctypes.c_void_p
+SIGNATURES = {
+    "avd_abi_version": (u32, []),
+    "avd_last_error_message": (ctypes.c_char_p, []),
+    "avd_motion_search_default": (None, [ctypes.POINTER(MotionSearch)]),
+    "avd_nlmeans_options_default": (None, [ctypes.POINTER(NlmeansOptions)]),
+    "avd_nlmeans_options_for_preset": (ctypes.c_int, [u32, ctypes.POINTER(NlmeansOptions)]),
+    "avd_nl4d_options_default": (None, [ctypes.POINTER(Nl4dOptions)]),
+    "avd_nl4d_options_for_preset": (ctypes.c_int, [u32, ctypes.POINTER(Nl4dOptions)]),
+    "avd_plane_bytes": (ctypes.c_uint64, [SampleFormat, u32, u32]),
+    "avd_engine_create": (ctypes.c_int, [ctypes.POINTER(EngineSpec), ctypes.POINTER(P)]),
+    "avd_engine_destroy": (None, [P]),
+    "avd_engine_push": (ctypes.c_int, [P, ctypes.POINTER(DevicePlane), u32, ctypes.POINTER(u32)]),
+    "avd_engine_push_context": (ctypes.c_int, [P, ctypes.POINTER(DevicePlane), u32]),
+    "avd_engine_emit_into": (ctypes.c_int, [P, ctypes.POINTER(DevicePlane), u32]),
+    "avd_engine_finish": (ctypes.c_int, [P, ctypes.POINTER(u32)]),
+    "avd_engine_reset": (ctypes.c_int, [P]),
+    "avd_engine_window_span": (ctypes.c_int, [P, ctypes.POINTER(WindowSpan)]),
+    "avd_engine_max_held_frames": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64)]),
+    "avd_engine_synchronize": (ctypes.c_int, [P]),
+    "avd_engine_fail_for_test": (ctypes.c_int, [P]),
+    "avd_site_ingest": (ctypes.c_int, [P, P, P, P, u32, u32, SampleFormat, P]),
+    "avd_site_egress": (ctypes.c_int, [P, P, P, u32, u32, SampleFormat, P]),
+    "avd_site_collab_search": (ctypes.c_int, [ctypes.POINTER(CollabPass), P, P, P, P, P, P, P]),
+    "avd_site_collab_filter": (ctypes.c_int, [ctypes.POINTER(CollabPass), P, P, P, P, P, P, P]),
+    "avd_site_collab_normalise": (ctypes.c_int, [P, P, P, u32, u32, P]),
+    "avd_site_motion": (ctypes.c_int, [ctypes.POINTER(MotionPass), P, P, P, P, P]),
+    "avd_site_nlm_single": (ctypes.c_int, [ctypes.POINTER(NlmWindow), P, P, P, P, P, P]),
+    "avd_site_nlm_pair": (ctypes.c_int, [ctypes.POINTER(NlmWindow), P, P, P, P, P, P, P, P, P]),
+    "avd_site_nlm_finish": (ctypes.c_int, [P, P, P, P, f32, u32, u32, P, P]),
+}
+
+_LIB = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads ``libavd.so``; raises when it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python av-denoise_amd/avd/build.py` "
+                               "(or __graft_entry__.build()); there is no CPU fallback")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = load().avd_last_error_message().decode("utf-8", "replace")
+        raise _BY_STATUS.get(status, Error)(msg)

@@ -1,0 +1,340 @@
+// The extern "C" boundary: status codes, thread-local error messages, defaults and presets, the
+// engine entry points and the per-stage launch sites.  No exception crosses this file.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "engine.h"
+
+struct avd_engine {
+    avd::Engine impl;
+    explicit avd_engine(const avd_engine_spec &s) : impl(s) {}
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+avd_status fail(avd_status s, const std::string &m)
+{
+    g_error = m;
+    return s;
+}
+
+template <typename F>
+avd_status guard(F &&f)
+{
+    try {
+        g_error.clear();
+        f();
+        return AVD_OK;
+    } catch (const avd::Failure &e) {
+        return fail(e.status, e.message);
+    } catch (const std::bad_alloc &) {
+        return fail(AVD_ERR_GPU, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(AVD_ERR_GPU, e.what());
+    }
+}
+
+avd_status hip_status(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return AVD_OK;
+    if (e == hipErrorInvalidValue) return fail(AVD_ERR_INVALID_OPTIONS, std::string(what) + ": unsupported parameters");
+    return fail(AVD_ERR_GPU, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+avd_status check_format(avd_sample_format f)
+{
+    return guard([&] { avd::validate_format(f); });
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t avd_abi_version(void) { return AVD_ABI_VERSION; }
+
+const char *avd_last_error_message(void) { return g_error.c_str(); }
+
+// MotionSearch::default (motion/search.rs:29-40).
+void avd_motion_search_default(avd_motion_search *out)
+{
+    if (!out) return;
+    std::memset(out, 0, sizeof(*out));
+    out->blksize = 16;
+    out->overlap = 8;
+    out->search_radius = 4;
+    out->max_search_escalation = 4;
+    out->pyramid_levels = 2;
+    out->estimation = 0;  // Auto
+    out->refine_radius = 2;
+}
+
+// NlmeansOptions / HqParams defaults (nlmeans/options.rs:6-27, nlmeans/params.rs:139-152).
+void avd_nlmeans_options_default(avd_nlmeans_options *out)
+{
+    if (!out) return;
+    std::memset(out, 0, sizeof(*out));
+    avd_motion_search_default(&out->motion);
+    out->auto_strength = 1;
+    out->noise_floor = 1;
+    out->temporal_confidence = 1;
+    out->thsad_scale = 1.0f;
+    out->sigma_scale = 1.0f;
+}
+
+// nlmeans_variant_for / nlmeans_temporal_radius_for / nlmeans_search_radius_for
+// (nlmeans/options.rs:58-82) and search_escalation_for (options.rs:32-38).
+avd_status avd_nlmeans_options_for_preset(uint32_t preset, avd_nlmeans_options *out)
+{
+    if (!out) return fail(AVD_ERR_NULL_ARGUMENT, "out is null");
+    if (preset > 4) return fail(AVD_ERR_INVALID_OPTIONS, "unknown preset");
+    static const uint32_t radius[5] = {0, 1, 2, 4, 8};
+    static const uint32_t search[5] = {2, 2, 2, 4, 4};
+    avd_nlmeans_options_default(out);
+    out->hq = preset == 0 ? 0 : 1;
+    out->temporal_radius = radius[preset];
+    out->temporal = radius[preset] > 0 ? 1 : 0;
+    out->has_search_radius = 1;
+    out->search_radius = search[preset];
+    if (preset >= 3) out->motion.max_search_escalation = 8;
+    return AVD_OK;
+}
+
+// Nl4dOptions::default (nl4d/options.rs:68-94, nl4d/params.rs:107-129).
+void avd_nl4d_options_default(avd_nl4d_options *out)
+{
+    if (!out) return;
+    std::memset(out, 0, sizeof(*out));
+    avd_motion_search_default(&out->motion);
+    out->sigma_scale = 1.0f;
+    out->thsad_scale = 1.0f;
+    out->refine = 2;
+    out->spatial_radius = 9;
+    out->lambda_ht_scale = 1.0f;
+    out->c_min = 0.05f;
+    out->kaiser_beta = 2.0f;
+    out->field_lambda = 1.0f;
+    out->noise_map = 1;
+    out->pooled_threshold = 1;
+    out->grain_data = 0;
+    out->temporal_radius = 2;
+    out->step = 4;
+    out->group_size = 8;
+}
+
+// nl4d runs the base and slow presets only; slow raises the search escalation ceiling to 8
+// (av-denoise/src/bin/cli/nl4d.rs:231-275).
+avd_status avd_nl4d_options_for_preset(uint32_t preset, avd_nl4d_options *out)
+{
+    if (!out) return fail(AVD_ERR_NULL_ARGUMENT, "out is null");
+    if (preset != 2 && preset != 3) return fail(AVD_ERR_INVALID_OPTIONS, "nl4d only runs the base and slow presets");
+    avd_nl4d_options_default(out);
+    if (preset == 3) out->motion.max_search_escalation = 8;
+    return AVD_OK;
+}
+
+uint64_t avd_plane_bytes(avd_sample_format format, uint32_t width, uint32_t height)
+{
+    return avd::plane_bytes(format, (uint64_t)width * height);
+}
+
+avd_status avd_engine_create(const avd_engine_spec *spec, avd_engine **out)
+{
+    if (!spec || !out) return fail(AVD_ERR_NULL_ARGUMENT, "spec or out is null");
+    *out = nullptr;
+    return guard([&] { *out = new avd_engine(*spec); });
+}
+
+void avd_engine_destroy(avd_engine *engine) { delete engine; }
+
+avd_status avd_engine_push(avd_engine *e, const avd_device_plane *planes, uint32_t count, uint32_t *ready)
+{
+    if (!e || !planes || !ready) return fail(AVD_ERR_NULL_ARGUMENT, "engine, planes or ready is null");
+    return guard([&] { *ready = e->impl.push(planes, count); });
+}
+
+avd_status avd_engine_push_context(avd_engine *e, const avd_device_plane *planes, uint32_t count)
+{
+    if (!e || !planes) return fail(AVD_ERR_NULL_ARGUMENT, "engine or planes is null");
+    return guard([&] { e->impl.push_context(planes, count); });
+}
+
+avd_status avd_engine_emit_into(avd_engine *e, const avd_device_plane *planes, uint32_t count)
+{
+    if (!e || !planes) return fail(AVD_ERR_NULL_ARGUMENT, "engine or planes is null");
+    return guard([&] { e->impl.emit(planes, count); });
+}
+
+avd_status avd_engine_finish(avd_engine *e, uint32_t *tail)
+{
+    if (!e || !tail) return fail(AVD_ERR_NULL_ARGUMENT, "engine or tail is null");
+    return guard([&] { *tail = e->impl.finish(); });
+}
+
+avd_status avd_engine_reset(avd_engine *e)
+{
+    if (!e) return fail(AVD_ERR_NULL_ARGUMENT, "engine is null");
+    return guard([&] { e->impl.reset(); });
+}
+
+avd_status avd_engine_window_span(const avd_engine *e, avd_window_span *out)
+{
+    if (!e || !out) return fail(AVD_ERR_NULL_ARGUMENT, "engine or out is null");
+    return guard([&] { *out = e->impl.window_span(); });
+}
+
+avd_status avd_engine_max_held_frames(const avd_engine *e, uint64_t *out)
+{
+    if (!e || !out) return fail(AVD_ERR_NULL_ARGUMENT, "engine or out is null");
+    return guard([&] { *out = e->impl.max_held_frames(); });
+}
+
+avd_status avd_engine_synchronize(avd_engine *e)
+{
+    if (!e) return fail(AVD_ERR_NULL_ARGUMENT, "engine is null");
+    return guard([&] { e->impl.synchronize(); });
+}
+
+avd_status avd_engine_fail_for_test(avd_engine *e)
+{
+    if (!e) return fail(AVD_ERR_NULL_ARGUMENT, "engine is null");
+    return guard([&] { e->impl.fail_for_test(); });
+}
+
+// ---- launch sites ----------------------------------------------------------------------------
+
+avd_status avd_site_ingest(const void *plane0, const void *plane1, float *slot, uint16_t *codes, uint32_t pixels,
+                           uint32_t channels, avd_sample_format format, void *stream)
+{
+    if (!plane0 || !slot || (channels == 2 && !plane1)) return fail(AVD_ERR_NULL_ARGUMENT, "null plane or slot");
+    if (channels < 1 || channels > 2) return fail(AVD_ERR_INVALID_GEOMETRY, "channels must be 1 or 2");
+    avd_status st = check_format(format);
+    if (st != AVD_OK) return st;
+    return hip_status(avd::launch_ingest(static_cast<const uint32_t *>(plane0),
+                                         static_cast<const uint32_t *>(plane1 ? plane1 : plane0), slot, codes, pixels,
+                                         (int)channels, (int)avd::samples_per_word(format), avd::max_value(format),
+                                         static_cast<hipStream_t>(stream)),
+                      "ingest");
+}
+
+avd_status avd_site_egress(const float *frame, void *plane0, void *plane1, uint32_t pixels, uint32_t channels,
+                           avd_sample_format format, void *stream)
+{
+    if (!frame || !plane0 || (channels == 2 && !plane1)) return fail(AVD_ERR_NULL_ARGUMENT, "null frame or plane");
+    if (channels < 1 || channels > 2) return fail(AVD_ERR_INVALID_GEOMETRY, "channels must be 1 or 2");
+    avd_status st = check_format(format);
+    if (st != AVD_OK) return st;
+    return hip_status(avd::launch_egress(frame, static_cast<uint32_t *>(plane0),
+                                         static_cast<uint32_t *>(plane1 ? plane1 : plane0), pixels, (int)channels,
+                                         (int)avd::samples_per_word(format), avd::max_value(format),
+                                         static_cast<hipStream_t>(stream)),
+                      "egress");
+}
+
+avd_status avd_site_collab_search(const avd_collab_pass *pass, const uint16_t *ring_codes, const float *ring_values,
+                                  const uint32_t *neighbour_slots, const int32_t *mv_field, const float *confidence,
+                                  uint32_t *members, void *stream)
+{
+    if (!pass || !members || (!ring_codes && !ring_values)) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (pass->temporal_radius > 0 && (!neighbour_slots || !mv_field || !confidence))
+        return fail(AVD_ERR_NULL_ARGUMENT, "a temporal pass needs neighbour slots, motion and confidence");
+    return hip_status(avd::launch_collab_search(*pass, ring_codes, ring_values, neighbour_slots, mv_field, confidence,
+                                                members, static_cast<hipStream_t>(stream)),
+                      "collab_search");
+}
+
+avd_status avd_site_collab_filter(const avd_collab_pass *pass, const float *ring_values,
+                                  const uint32_t *neighbour_slots, const uint32_t *members, int32_t *accum,
+                                  int32_t *wsum, float *group_weight, void *stream)
+{
+    if (!pass || !ring_values || !members || !accum || !wsum || !group_weight)
+        return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    return hip_status(avd::launch_collab_filter(*pass, ring_values, neighbour_slots, members, accum, wsum,
+                                                group_weight, static_cast<hipStream_t>(stream)),
+                      "collab_filter");
+}
+
+avd_status avd_site_collab_normalise(int32_t *accum_region, int32_t *wsum_region, float *out, uint32_t pixels,
+                                     uint32_t channels, void *stream)
+{
+    if (!accum_region || !wsum_region || !out) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (channels < 1 || channels > 2) return fail(AVD_ERR_INVALID_GEOMETRY, "channels must be 1 or 2");
+    return hip_status(avd::launch_collab_normalise(accum_region, wsum_region, out, pixels, (int)channels,
+                                                   static_cast<hipStream_t>(stream)),
+                      "collab_normalise");
+}
+
+avd_status avd_site_motion(const avd_motion_pass *p, const uint16_t *ring_codes, const uint32_t *neighbour_slots,
+                           int32_t *mv_field, float *confidence, void *stream)
+{
+    if (!p || !ring_codes || !neighbour_slots || !mv_field) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (p->overlap >= p->blksize || p->width == 0 || p->height == 0 || p->channels < 1 || p->channels > 2)
+        return fail(AVD_ERR_INVALID_OPTIONS, "invalid motion geometry");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    avd::MotionGeom g{(int)p->width, (int)p->height, (int)p->channels, (int)p->blksize, (int)(p->blksize - p->overlap),
+                      (int)p->search_radius, (int)p->pyramid_levels, 0, 0};
+    g.blocks_x = std::max((int)avd::div_up(p->width, g.step), 1);
+    g.blocks_y = std::max((int)avd::div_up(p->height, g.step), 1);
+    // Level-1 sums of every slot the pass reads, in scratch owned by this call.
+    const size_t l1 = (size_t)std::max(p->width / 2, 1u) * std::max(p->height / 2, 1u);
+    uint32_t slots = p->centre_slot + 1;
+    std::vector<uint32_t> host(p->neighbours);
+    hipError_t e = hipMemcpyAsync(host.data(), neighbour_slots, p->neighbours * sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_status(e, "read neighbour slots");
+    for (uint32_t n : host) slots = std::max(slots, n + 1);
+    int32_t *pyr = nullptr;
+    if (p->pyramid_levels >= 2) {
+        e = hipMalloc(&pyr, slots * l1 * sizeof(int32_t));
+        if (e != hipSuccess) return hip_status(e, "hipMalloc");
+        const size_t slot_len = (size_t)p->width * p->height * p->channels;
+        for (uint32_t sl = 0; sl < slots && e == hipSuccess; ++sl)
+            e = avd::launch_downscale(ring_codes + sl * slot_len, p->width, p->height, p->channels, pyr + sl * l1, s);
+    }
+    if (e == hipSuccess)
+        e = avd::launch_motion(g, ring_codes, pyr, (int)p->centre_slot, neighbour_slots, (int)p->neighbours, mv_field,
+                               confidence, p->max_value, p->sad_floor, p->thsad, s);
+    if (pyr) {
+        hipError_t e2 = hipStreamSynchronize(s);
+        (void)hipFree(pyr);
+        if (e == hipSuccess) e = e2;
+    }
+    return hip_status(e, "motion");
+}
+
+avd_status avd_site_nlm_single(const avd_nlm_window *w, const float *frame, const float *lut, float *accum,
+                               float *wsum, float *maxw, void *stream)
+{
+    if (!w || !frame || !lut || !accum || !wsum || !maxw) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    avd::NlmWindow k{(int)w->width, (int)w->height, (int)w->channels, (int)w->patch_radius, (int)w->search_radius,
+                     w->h2_inv_norm * 1.4426950408889634f, w->noise_offset, 1, 1, 1};
+    return hip_status(avd::launch_nlm_single(k, frame, lut, accum, wsum, maxw, static_cast<hipStream_t>(stream)),
+                      "nlm_single");
+}
+
+avd_status avd_site_nlm_pair(const avd_nlm_window *w, const float *centre, const float *fwd, const float *bwd,
+                             const float *conf_fwd, const float *conf_bwd, float *accum, float *wsum, float *maxw,
+                             void *stream)
+{
+    if (!w || !centre || !fwd || !bwd || !accum || !wsum || !maxw) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    avd::NlmWindow k{(int)w->width, (int)w->height, (int)w->channels, (int)w->patch_radius, (int)w->search_radius,
+                     w->h2_inv_norm * 1.4426950408889634f, w->noise_offset, (int)w->step, (int)w->blocks_x,
+                     (int)w->blocks_y};
+    return hip_status(avd::launch_nlm_pair(k, centre, fwd, bwd, conf_fwd, conf_bwd, accum, wsum, maxw,
+                                           static_cast<hipStream_t>(stream)),
+                      "nlm_pair");
+}
+
+avd_status avd_site_nlm_finish(const float *frame, const float *accum, const float *wsum, const float *maxw,
+                               float self_weight, uint32_t pixels, uint32_t channels, float *out, void *stream)
+{
+    if (!frame || !accum || !wsum || !maxw || !out) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    return hip_status(avd::launch_nlm_finish(frame, accum, wsum, maxw, self_weight, pixels, (int)channels, out,
+                                             static_cast<hipStream_t>(stream)),
+                      "nlm_finish");
+}
+
+}  // extern "C"

@@ -1,0 +1,647 @@
+// NL4D grouping, collaborative filter and aggregation for gfx950.
+//
+// collab_search follows search_group (collab/kernels/fused/mod.rs:407-553): the centre search
+// (centre.rs:37-300) scores the clipped (2S+1)^2 rectangle around each reference and keeps the
+// best eight by distance, a tie never displacing an earlier row-major visit (plane_ops.rs:36-77),
+// the self-match pinned to slot 0; the trajectory search (trajectory.rs:142-450) follows each
+// volume anchor into every neighbour frame through the refine rectangles of its covering motion
+// blocks, skipping blocks under c_min, positions inside a rectangle already searched for that
+// frame and positions the group already holds, keeps the best per frame and the best grid_frames-1
+// frames per volume.  The group word is the member count with 0x100 set for a volume grid.
+//
+// Distances: with u16 codes the exact integer SSD (packed 16-bit differences into v_dot2_i32_i16),
+// with f32 values the reference's F32Search form (per-column sums of squared differences, the
+// 8-lane butterfly over columns, times the channel scale).
+//
+// collab_filter follows filter_group (fused/mod.rs:565-850): per channel a vertical then a
+// horizontal 8-point DCT (transforms.rs:45-111, fused multiply-adds with f32 taps), the group
+// Haar (stack of k_use members or time-by-volume grid, grid.rs:119-220), hard threshold at
+// lambda*sqrt(sigma^2 * prof[u] * prof[v]) with the group DC always kept, the inverse-variance
+// group weight from channel 0, the inverses, and the fixed-point scatter of every member into its
+// own frame's region with the Kaiser taper (aggregate.rs:147-211).
+//
+// MI355X layout: one wavefront per reference.  In the search the 64 lanes split the candidate
+// positions and a wave-wide (distance, visit-order) minimum picks the members; in the filter lane
+// (m, q) = (lane / 8, lane % 8) owns column q of member m, so the 8x8 transposes stay inside 8-lane
+// groups (through LDS) and the Haar butterflies between members are lane-xor shuffles.  The group
+// Haar is written as in-place butterflies: the reference's reordering of approximations only
+// permutes member slots, every member carries the same variance, and slot 0 holds the group
+// approximation in both layouts, so every kept coefficient and every output value is identical.
+#include "common.h"
+
+namespace avd {
+
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kMaxS = 16;
+constexpr uint64_t kNoKey = ~0ull;
+
+typedef short short2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ uint32_t pack_pos_t(uint32_t x, uint32_t y, uint32_t t) { return (t << 26) | (y << 13) | x; }
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
+{
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint64_t o = shfl_u64(v, (int)(threadIdx.x & 63) ^ off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// Orders a non-negative f32 distance as an unsigned integer; 0 stays for the pinned self-match.
+__device__ __forceinline__ uint32_t float_key(float d) { return __float_as_uint(d) + 1u; }
+
+// ---------------------------------------------------------------------------------------------
+// Distances.  `ring` is the slot's base; (ax, ay) the reference/anchor patch and (bx, by) the
+// candidate, both in frame `a` / `b`.
+// ---------------------------------------------------------------------------------------------
+
+template <int C>
+__device__ __forceinline__ uint32_t ssd_codes(const uint16_t *__restrict__ a, const uint16_t *__restrict__ b, int W,
+                                              int ax, int ay, int bx, int by)
+{
+    int acc = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const uint16_t *pa = a + ((size_t)(ay + r) * W + ax) * C;
+        const uint16_t *pb = b + ((size_t)(by + r) * W + bx) * C;
+#pragma unroll
+        for (int i = 0; i < 8 * C; i += 2) {
+            short2_t va = {(short)pa[i], (short)pa[i + 1]};
+            short2_t vb = {(short)pb[i], (short)pb[i + 1]};
+            short2_t d = va - vb;
+            acc = __builtin_amdgcn_sdot2(d, d, acc, false);
+        }
+    }
+    return (uint32_t)acc;
+}
+
+template <int C>
+__device__ __forceinline__ float dist_values(const float *__restrict__ a, const float *__restrict__ b, int W, int ax,
+                                             int ay, int bx, int by)
+{
+    float part[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const float d = a[((size_t)(ay + r) * W + ax + q) * C + c] - b[((size_t)(by + r) * W + bx + q) * C + c];
+                acc = __builtin_fmaf(d, d, acc);
+            }
+        part[q] = acc;
+    }
+    const float sum = ((part[0] + part[1]) + (part[2] + part[3])) + ((part[4] + part[5]) + (part[6] + part[7]));
+    return sum * (C == 1 ? 3.0f : 1.5f);
+}
+
+// Distance key of a candidate: codes give the SSD plus one (the self-match is 0); values give the
+// f32 distance's bits plus one.  Both order exactly as the reference's distances.
+template <int C, bool CODES>
+__device__ __forceinline__ uint32_t dist_key(const uint16_t *ca, const uint16_t *cb, const float *fa, const float *fb,
+                                             int W, int ax, int ay, int bx, int by)
+{
+    if (CODES) return ssd_codes<C>(ca, cb, W, ax, ay, bx, by) + 1u;
+    return float_key(dist_values<C>(fa, fb, W, ax, ay, bx, by));
+}
+
+// ---------------------------------------------------------------------------------------------
+// collab_search
+// ---------------------------------------------------------------------------------------------
+
+// Inserts `k` into the ascending 8-entry list `a`, dropping the largest (keys are unique).
+__device__ __forceinline__ void insert8(uint64_t *a, uint64_t k)
+{
+#pragma unroll
+    for (int i = 7; i > 0; --i) a[i] = (k < a[i - 1]) ? a[i - 1] : ((k < a[i]) ? k : a[i]);
+    a[0] = (k < a[0]) ? k : a[0];
+}
+
+// Makes this wave's LDS writes visible to its other lanes before they read.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int C, bool CODES>
+__global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p, const uint16_t *__restrict__ codes,
+                                                               const float *__restrict__ values,
+                                                               const uint32_t *__restrict__ nslots,
+                                                               const int32_t *__restrict__ mv,
+                                                               const float *__restrict__ conf,
+                                                               uint32_t *__restrict__ members)
+{
+    __shared__ int s_rect[kWaves][16][5];  // left, right, top, bottom, positions (0 when skipped)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int W = p.width, H = p.height;
+    const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
+    const int refs = refs_x * refs_y;
+    int ref = blockIdx.x * kWaves + wave;
+    const bool live = ref < refs;
+    if (!live) ref = refs - 1;
+    const int iy = ref / refs_x, ix = ref - iy * refs_x;
+    const int maxx = W - 8, maxy = H - 8;
+    const int rx = min(ix * (int)p.step, maxx), ry = min(iy * (int)p.step, maxy);
+    const size_t slot_len = (size_t)W * H * C;
+    const uint16_t *cc = CODES ? codes + (size_t)p.centre_slot * slot_len : nullptr;
+    const float *cv = CODES ? nullptr : values + (size_t)p.centre_slot * slot_len;
+    const int S = p.spatial_radius;
+
+    // ---- centre search: each lane keeps the best eight (distance, visit order) keys of its share
+    // of the clipped rectangle; eight rounds of a wave-wide minimum merge them in order.
+    const int left = clampi(rx - S, 0, maxx), right = clampi(rx + S, 0, maxx);
+    const int top = clampi(ry - S, 0, maxy), bottom = clampi(ry + S, 0, maxy);
+    const int rw = right - left + 1, rh = bottom - top + 1;
+    const int n_live = rw * rh;
+    uint64_t best8[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) best8[i] = kNoKey;
+    for (int idx = lane; idx < n_live; idx += 64) {
+        const int y = top + idx / rw, x = left + idx % rw;
+        const uint32_t d = (x == rx && y == ry) ? 0u : dist_key<C, CODES>(cc, cc, cv, cv, W, rx, ry, x, y);
+        const uint64_t key = ((uint64_t)d << 16) | (uint64_t)idx;
+        if (key < best8[7]) insert8(best8, key);
+    }
+    uint32_t mem[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const uint64_t best = wave_min_u64(best8[0]);
+        if (best == kNoKey) {
+            mem[m] = 0;  // an unfilled slot keeps position word 0
+        } else {
+            const int idx = (int)(best & 0xFFFF);
+            mem[m] = pack_pos_t((uint32_t)(left + idx % rw), (uint32_t)(top + idx / rw), 0);
+            if (best8[0] == best) {
+#pragma unroll
+                for (int i = 0; i < 7; ++i) best8[i] = best8[i + 1];
+                best8[7] = kNoKey;
+            }
+        }
+    }
+    uint32_t k_use = 1;
+    while (k_use * 2 <= (uint32_t)n_live && k_use * 2 <= p.group_size) k_use *= 2;
+
+    // ---- trajectory search into the neighbour frames.
+    const int radius = p.temporal_radius;
+    const int gf = radius == 0 ? 1 : (radius == 1 ? 2 : 4);
+    bool use_grid = false;
+    if (gf > 1 && p.group_size == 8) {
+        const int volumes = 8 / gf, tail = gf - 1;
+        const int bstep = p.blk_step, bsz = p.blksize, bx_n = p.blocks_x, by_n = p.blocks_y;
+        const int covers = (bsz + bstep - 1) / bstep;
+        const int nrect = covers * covers;
+        const size_t nblk = (size_t)bx_n * by_n;
+        const int refine = p.refine;
+        uint32_t gpos[8];
+        uint64_t gd[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            gpos[i] = 0;
+            gd[i] = kNoKey;
+        }
+        for (int v = 0; v < volumes; ++v) {
+            const int first = v * gf;
+            uint32_t anchor = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (i == v) anchor = mem[i];
+            const int ax = (int)(anchor & 0x1FFF), ay = (int)((anchor >> 13) & 0x1FFF);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i == first) {
+                    gpos[i] = anchor;
+                    gd[i] = 0;
+                } else if (i > first && i < first + gf) {
+                    gpos[i] = 0;
+                    gd[i] = kNoKey;
+                }
+            }
+            const int held_n = first + 1;
+            const int bx_hi = min(ax / bstep, bx_n - 1), by_hi = min(ay / bstep, by_n - 1);
+            const int ovx = max(ax + 8, bsz) - bsz, ovy = max(ay + 8, bsz) - bsz;
+            const int bx_lo = min((ovx + bstep - 1) / bstep, bx_hi), by_lo = min((ovy + bstep - 1) / bstep, by_hi);
+            for (int t = 0; t < 2 * radius; ++t) {
+                const uint16_t *nc = CODES ? codes + (size_t)nslots[t] * slot_len : nullptr;
+                const float *nv = CODES ? nullptr : values + (size_t)nslots[t] * slot_len;
+                // Lane b describes rectangle b of this frame, in the reference's (iy, ix) walk order.
+                if (lane < nrect) {
+                    const int wbx = bx_lo + lane % covers, wby = by_lo + lane / covers;
+                    int l = 1, r = 0, tp = 1, bt = 0;
+                    if (wbx <= bx_hi && wby <= by_hi) {
+                        const int block = wby * bx_n + wbx;
+                        if (conf[(size_t)t * nblk + block] >= p.c_min) {
+                            const int32_t *v2 = mv + ((size_t)t * nblk + block) * 2;
+                            const int px = ax + v2[0], py = ay + v2[1];
+                            l = clampi(px - refine, 0, maxx);
+                            r = clampi(px + refine, 0, maxx);
+                            tp = clampi(py - refine, 0, maxy);
+                            bt = clampi(py + refine, 0, maxy);
+                        }
+                    }
+                    s_rect[wave][lane][0] = l;
+                    s_rect[wave][lane][1] = r;
+                    s_rect[wave][lane][2] = tp;
+                    s_rect[wave][lane][3] = bt;
+                    s_rect[wave][lane][4] = (r - l + 1) * (bt - tp + 1);
+                }
+                wave_sync();
+                // Lanes split each rectangle's positions; the walk index orders the first visit.
+                uint64_t best = kNoKey;
+                int walk = 0;
+                for (int b = 0; b < nrect; ++b) {
+                    const int n = s_rect[wave][b][4];
+                    if (n <= 0) continue;
+                    const int l = s_rect[wave][b][0], tp = s_rect[wave][b][2];
+                    const int w = s_rect[wave][b][1] - l + 1;
+                    for (int i = lane; i < n; i += 64) {
+                        const int cx = l + i % w, cy = tp + i / w;
+                        bool skipped = false;
+                        for (int s2 = 0; s2 < b; ++s2)
+                            if (s_rect[wave][s2][4] > 0 && cx >= s_rect[wave][s2][0] && cx <= s_rect[wave][s2][1] &&
+                                cy >= s_rect[wave][s2][2] && cy <= s_rect[wave][s2][3])
+                                skipped = true;
+                        const uint32_t packed = pack_pos_t((uint32_t)cx, (uint32_t)cy, (uint32_t)t + 1);
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                            if (k < held_n && gpos[k] == packed) skipped = true;
+                        if (skipped) continue;
+                        const uint32_t d = dist_key<C, CODES>(cc, nc, cv, nv, W, ax, ay, cx, cy);
+                        const uint64_t key = ((uint64_t)d << 32) | (uint32_t)(walk + i);
+                        best = key < best ? key : best;
+                    }
+                    walk += n;
+                }
+                best = wave_min_u64(best);
+                uint64_t cd = kNoKey;
+                uint32_t cp = 0;
+                if (best != kNoKey) {
+                    int i = (int)(best & 0xFFFFFFFFu);
+                    for (int b = 0; b < nrect; ++b) {
+                        const int n = s_rect[wave][b][4];
+                        if (n <= 0) continue;
+                        if (i < n) {
+                            const int l = s_rect[wave][b][0], tp = s_rect[wave][b][2];
+                            const int w = s_rect[wave][b][1] - l + 1;
+                            cp = pack_pos_t((uint32_t)(l + i % w), (uint32_t)(tp + i / w), (uint32_t)t + 1);
+                            break;
+                        }
+                        i -= n;
+                    }
+                    cd = best >> 32;
+                }
+                wave_sync();
+                // The frame's match joins the volume's ascending list; earlier frames win ties.
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if (i > first && i < first + gf && cd < gd[i]) {
+                        const uint64_t hd = gd[i];
+                        const uint32_t hp = gpos[i];
+                        gd[i] = cd;
+                        gpos[i] = cp;
+                        cd = hd;
+                        cp = hp;
+                    }
+                }
+            }
+        }
+        use_grid = (k_use == 8);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (i % gf == tail) use_grid = use_grid && (gd[i] != kNoKey);
+        if (use_grid) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) mem[m] = gpos[m];
+            k_use = 8;
+        }
+    }
+    if (live && lane < 9) {
+        uint32_t word = k_use | (use_grid ? 0x100u : 0u);
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            if (lane == m) word = mem[m];
+        members[(size_t)ref * 9 + lane] = word;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// collab_filter
+// ---------------------------------------------------------------------------------------------
+
+struct DctTaps {
+    float e[16], o[16];
+};
+
+__constant__ DctTaps c_taps;
+
+__device__ __forceinline__ void dct8_fwd(float *l)
+{
+    float s[4], d[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        s[k] = l[k] + l[7 - k];
+        d[k] = l[k] - l[7 - k];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        float e = 0.0f, o = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            e = __builtin_fmaf(s[k], c_taps.e[m * 4 + k], e);
+            o = __builtin_fmaf(d[k], c_taps.o[m * 4 + k], o);
+        }
+        l[2 * m] = e;
+        l[2 * m + 1] = o;
+    }
+}
+
+__device__ __forceinline__ void dct8_inv(float *l)
+{
+    float sn[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sn[j] = l[j];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float e = 0.0f, o = 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            e = __builtin_fmaf(sn[2 * m], c_taps.e[m * 4 + k], e);
+            o = __builtin_fmaf(sn[2 * m + 1], c_taps.o[m * 4 + k], o);
+        }
+        l[k] = e + o;
+        l[7 - k] = e - o;
+    }
+}
+
+// One Haar butterfly level between member lanes `xor_lanes` apart.  `active` lanes take part; a
+// low lane keeps (low + high) * f, a high lane (low - high) * f, operands in the reference's order.
+__device__ __forceinline__ void haar_level(float *v, int xor_lanes, bool active, bool low)
+{
+    const float f = 0.70710678118654752440f;
+    const int partner = (int)(threadIdx.x & 63) ^ xor_lanes;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const float other = __shfl(v[u], partner);
+        if (active) v[u] = low ? (v[u] + other) * f : (other - v[u]) * f;
+    }
+}
+
+// Forward (inverse = false) or inverse group transform over the 8 member lanes.
+__device__ void group_transform(float *v, int m, int k_use, bool grid, int gf, bool inverse)
+{
+    int lv_xor[6];
+    bool lv_active[6], lv_low[6];
+    int n = 0;
+    if (grid) {
+        const int vols = 8 / gf, f = m % gf, vol = m / gf;
+        for (int st = 1; st < gf; st *= 2) {
+            lv_xor[n] = st * 8;
+            lv_active[n] = (f % st) == 0;
+            lv_low[n] = ((f / st) % 2) == 0;
+            ++n;
+        }
+        for (int st = 1; st < vols; st *= 2) {
+            lv_xor[n] = st * gf * 8;
+            lv_active[n] = (vol % st) == 0;
+            lv_low[n] = ((vol / st) % 2) == 0;
+            ++n;
+        }
+    } else {
+        for (int st = 1; st < k_use; st *= 2) {
+            lv_xor[n] = st * 8;
+            lv_active[n] = (m % st) == 0 && m < k_use;
+            lv_low[n] = ((m / st) % 2) == 0;
+            ++n;
+        }
+    }
+    if (!inverse) {
+        for (int i = 0; i < n; ++i) haar_level(v, lv_xor[i], lv_active[i], lv_low[i]);
+    } else {
+        for (int i = n - 1; i >= 0; --i) haar_level(v, lv_xor[i], lv_active[i], lv_low[i]);
+    }
+}
+
+__device__ __forceinline__ int32_t to_fixed(float v, float scale)
+{
+    const float c = fminf(fmaxf(v, -5.0f), 5.0f);
+    return (int32_t)roundf(c * scale);
+}
+
+__device__ __forceinline__ int32_t to_fixed_weight(float w, float scale)
+{
+    const float c = fminf(fmaxf(w, 0.0f), 1.0f);
+    return (int32_t)roundf(c * scale * 5.0f);
+}
+
+template <int C>
+__global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p, const float *__restrict__ values,
+                                                               const uint32_t *__restrict__ nslots,
+                                                               const uint32_t *__restrict__ members,
+                                                               int32_t *__restrict__ accum, int32_t *__restrict__ wsum,
+                                                               float *__restrict__ group_weight)
+{
+    __shared__ float s_t[kWaves][8][8][9];  // [wave][member][row][col], padded
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane >> 3, q = lane & 7;
+    const int W = p.width, H = p.height;
+    const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
+    const int refs = refs_x * refs_y;
+    int ref = blockIdx.x * kWaves + wave;
+    const bool live = ref < refs;
+    if (!live) ref = refs - 1;
+    const uint32_t *mw = members + (size_t)ref * 9;
+    const uint32_t gword = mw[8];
+    const int k_use = (int)(gword & 0xFF);
+    const bool grid = (gword & 0x100u) != 0;
+    const int gf = p.temporal_radius == 0 ? 1 : (p.temporal_radius == 1 ? 2 : 4);
+    const uint32_t pos = mw[m];
+    const uint32_t t = pos >> 26;
+    const int slot = t > 0 ? (int)nslots[t - 1] : (int)p.centre_slot;
+    const int mx = (int)(pos & 0x1FFF), my = (int)((pos >> 13) & 0x1FFF);
+    const size_t px = (size_t)W * H;
+    const float *src = values + (size_t)slot * px * C;
+    const float scale = p.accum_scale;
+    const bool in_group = m < k_use;
+    float scaled_weight = 0.0f;
+
+    for (int c = 0; c < C; ++c) {
+        float col[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) col[r] = src[((size_t)(my + r) * W + mx + q) * C + c];
+        dct8_fwd(col);  // vertical: col[v]
+#pragma unroll
+        for (int v = 0; v < 8; ++v) s_t[wave][m][v][q] = col[v];
+        __syncthreads();
+        float row[8];  // lane (m, v = q): coefficients (v, u) for u = 0..7
+#pragma unroll
+        for (int u = 0; u < 8; ++u) row[u] = s_t[wave][m][q][u];
+        __syncthreads();
+        dct8_fwd(row);
+        const int v = q;
+        group_transform(row, m, k_use, grid, gf, false);
+
+        const float sig2 = p.sigma[c] * p.sigma[c];
+        const float lam = p.lambda[c];
+        uint32_t counts = 0;  // kept coefficients per u, one nibble each
+        float cvs[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            cvs[u] = sig2 * (p.profile[u] * p.profile[v]);
+            const float thr = lam * sqrtf(cvs[u]);
+            bool keep = fabsf(row[u]) >= thr;
+            if (m == 0 && v == 0 && u == 0) keep = true;
+            keep = keep && in_group;
+            if (keep) counts += 1u << (4 * u);
+            else if (in_group) row[u] = 0.0f;
+        }
+        counts += (uint32_t)__shfl_xor((int)counts, 8);
+        counts += (uint32_t)__shfl_xor((int)counts, 16);
+        counts += (uint32_t)__shfl_xor((int)counts, 32);
+        float part = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int n = (counts >> (4 * u)) & 15;
+            for (int i = 0; i < n; ++i) part += cvs[u];
+        }
+        float sum = part;
+        sum += __shfl_xor(sum, 1);
+        sum += __shfl_xor(sum, 2);
+        sum += __shfl_xor(sum, 4);
+        const float weight = (isnan(sum) || isinf(sum)) ? 0.0f : 1.0f / fmaxf(sum, 1e-12f);
+        if (c == 0) {
+            if (live && lane == 0) group_weight[ref] = weight;
+            scaled_weight = weight * p.weight_scale;
+        }
+
+        group_transform(row, m, k_use, grid, gf, true);
+        dct8_inv(row);  // lane (m, v): values at columns k for vertical frequency v
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s_t[wave][m][v][k] = row[k];
+        __syncthreads();
+#pragma unroll
+        for (int vv = 0; vv < 8; ++vv) col[vv] = s_t[wave][m][vv][q];
+        __syncthreads();
+        dct8_inv(col);  // rows of column q
+
+        if (live && in_group) {
+            const size_t base = (size_t)slot * px;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const float win = p.kaiser[r] * p.kaiser[q];
+                const float w = scaled_weight * win;
+                const size_t pix = base + (size_t)(my + r) * W + mx + q;
+                atomicAdd(&accum[pix * C + c], to_fixed(col[r] * w, scale));
+                if (c == 0) atomicAdd(&wsum[pix], to_fixed_weight(w, scale));
+            }
+        }
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_normalise(int32_t *__restrict__ accum, int32_t *__restrict__ wsum,
+                                                   float *__restrict__ out, uint32_t pixels)
+{
+    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < pixels; p += gridDim.x * 256) {
+        const int32_t w = wsum[p];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const float a = (float)accum[(size_t)p * C + c];
+            out[(size_t)p * C + c] = (w != 0) ? (a * 5.0f) / (float)w : a;
+            accum[(size_t)p * C + c] = 0;
+        }
+        wsum[p] = 0;
+    }
+}
+
+double dct8_basis(int row, int col)
+{
+    const double rs = row == 0 ? 1.0 / sqrt(8.0) : 0.5;
+    return rs * cos(M_PI * (2.0 * col + 1.0) * row / 16.0);
+}
+
+hipError_t upload_taps(hipStream_t s)
+{
+    static bool done = false;  // per process; the taps are constants
+    if (done) return hipSuccess;
+    DctTaps t;
+    for (int m = 0; m < 4; ++m)
+        for (int k = 0; k < 4; ++k) {
+            t.e[m * 4 + k] = (float)dct8_basis(2 * m, k);
+            t.o[m * 4 + k] = (float)dct8_basis(2 * m + 1, k);
+        }
+    hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_taps), &t, sizeof(t), 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) done = true;
+    return e;
+}
+
+inline int ref_count(const avd_collab_pass &p)
+{
+    const int rx = ((int)p.width - 8 + (int)p.step - 1) / (int)p.step + 1;
+    const int ry = ((int)p.height - 8 + (int)p.step - 1) / (int)p.step + 1;
+    return rx * ry;
+}
+
+bool pass_ok(const avd_collab_pass &p)
+{
+    return p.width >= 8 && p.height >= 8 && p.width < 8192 && p.height < 8192 && (p.channels == 1 || p.channels == 2) &&
+           p.step >= 1 && p.step <= 8 && p.spatial_radius >= 1 && p.spatial_radius <= kMaxS && p.group_size == 8 &&
+           p.temporal_radius <= 8 && p.refine <= 4 && p.blk_step >= 1 && p.blksize >= 1 &&
+           (p.blksize + p.blk_step - 1) / p.blk_step <= 4;
+}
+
+}  // namespace
+
+hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
+                                const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
+                                hipStream_t s)
+{
+    if (!pass_ok(p)) return hipErrorInvalidValue;
+    const unsigned g = div_up(ref_count(p), kWaves);
+    if (codes) {
+        if (p.channels == 1) k_collab_search<1, true><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members);
+        else k_collab_search<2, true><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members);
+    } else {
+        if (p.channels == 1) k_collab_search<1, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members);
+        else k_collab_search<2, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
+                                const uint32_t *members, int32_t *accum, int32_t *wsum, float *group_weight,
+                                hipStream_t s)
+{
+    if (!pass_ok(p)) return hipErrorInvalidValue;
+    hipError_t e = upload_taps(s);
+    if (e != hipSuccess) return e;
+    const unsigned g = div_up(ref_count(p), kWaves);
+    if (p.channels == 1) k_collab_filter<1><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, accum, wsum, group_weight);
+    else k_collab_filter<2><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, accum, wsum, group_weight);
+    return hipGetLastError();
+}
+
+hipError_t launch_collab_normalise(int32_t *accum, int32_t *wsum, float *out, uint32_t pixels, int C, hipStream_t s)
+{
+    unsigned g = div_up(pixels, 256);
+    g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
+    if (C == 1) k_normalise<1><<<g, 256, 0, s>>>(accum, wsum, out, pixels);
+    else k_normalise<2><<<g, 256, 0, s>>>(accum, wsum, out, pixels);
+    return hipGetLastError();
+}
+
+}  // namespace avd

@@ -1,0 +1,65 @@
+// Shared declarations of the HIP kernels' host launchers and the small device helpers they use.
+// Every launcher enqueues on `stream` and returns the launch's hipError_t.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "avd/avd.h"
+
+namespace avd {
+
+constexpr int kPatch = 8;
+constexpr int kMaxK = 8;
+
+inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// ---- io.hip -----------------------------------------------------------------------------
+hipError_t launch_ingest(const uint32_t *p0, const uint32_t *p1, float *slot, uint16_t *codes, uint32_t pixels,
+                         int channels, int samples_per_word, float max_value, hipStream_t s);
+hipError_t launch_egress(const float *frame, uint32_t *p0, uint32_t *p1, uint32_t pixels, int channels,
+                         int samples_per_word, float max_value, hipStream_t s);
+hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, uint64_t words, hipStream_t s);
+hipError_t launch_fill_u32(void *dst, uint32_t value, uint64_t words, hipStream_t s);
+
+// ---- nlm.hip ----------------------------------------------------------------------------
+struct NlmWindow {
+    int W, H, C, P, S;
+    float h2_log2e;      // h2_inv_norm * log2(e): the weight is exp2(-max(d - off, 0) * h2_log2e)
+    float offset;        // pair: the flat noise offset
+    int step, blocks_x, blocks_y;
+};
+hipError_t launch_nlm_single(const NlmWindow &w, const float *frame, const float *lut, float *accum, float *wsum,
+                             float *maxw, hipStream_t s);
+hipError_t launch_nlm_pair(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd,
+                           const float *conf_f, const float *conf_b, float *accum, float *wsum, float *maxw,
+                           hipStream_t s);
+hipError_t launch_nlm_finish(const float *frame, const float *accum, const float *wsum, const float *maxw,
+                             float self_weight, uint32_t pixels, int C, float *out, hipStream_t s);
+bool nlm_supported(int C, int P, int S);
+
+// ---- motion.hip -------------------------------------------------------------------------
+struct MotionGeom {
+    int W, H, C;
+    int blksize, step, radius, levels;
+    int blocks_x, blocks_y;
+};
+// Level-1 sums of lane 0 of one u16 ring slot.
+hipError_t launch_downscale(const uint16_t *codes_slot, int W, int H, int C, int32_t *level1, hipStream_t s);
+// Coarse + fine search of `neighbours` slots against `centre`; pyr1 holds one level-1 plane per slot.
+hipError_t launch_motion(const MotionGeom &g, const uint16_t *codes, const int32_t *pyr1, int centre_slot,
+                         const uint32_t *nslots_dev, int neighbours, int32_t *mv, float *conf, float max_value,
+                         float sad_floor, float thsad, hipStream_t s);
+hipError_t launch_warp(const float *src, float *dst, const int32_t *mv, int W, int H, int C, int step,
+                       int blocks_x, int blocks_y, hipStream_t s);
+
+// ---- collab.hip -------------------------------------------------------------------------
+hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
+                                const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
+                                hipStream_t s);
+hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
+                                const uint32_t *members, int32_t *accum, int32_t *wsum, float *group_weight,
+                                hipStream_t s);
+hipError_t launch_collab_normalise(int32_t *accum, int32_t *wsum, float *out, uint32_t pixels, int C, hipStream_t s);
+
+}  // namespace avd

@@ -1,0 +1,203 @@
+// Host-side engine types (see engine.cpp).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace avd {
+
+// An error carrying the C ABI status it maps to.
+struct Failure {
+    avd_status status;
+    std::string message;
+    Failure(avd_status s, std::string m) : status(s), message(std::move(m)) {}
+};
+
+void check_hip(hipError_t e, const char *what);
+
+// Owned device memory.
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer();
+    void allocate(size_t n);
+    void release();
+    template <typename T>
+    T *as() const
+    {
+        return static_cast<T *>(ptr);
+    }
+};
+
+uint32_t samples_per_word(avd_sample_format f);
+float max_value(avd_sample_format f);
+uint64_t plane_bytes(avd_sample_format f, uint64_t pixels);
+void validate_format(avd_sample_format f);
+
+float h2_inv_norm(float strength, uint32_t patch_radius);
+float noise_offset(float sigma, int channels, uint32_t patch_radius);
+float sad_noise_floor(uint32_t blksize, float sigma);
+float thsad(uint32_t blksize, float scale);
+void kaiser_window(float beta, float out[8]);
+float cross_frame_accum_scale(uint32_t spatial_radius, uint32_t temporal_radius, uint32_t step);
+float weight_scale(float sigma, float g_max);
+
+// The 2r+1 ingested frames of one plane group: f32 values, their u16 codes and, for motion, the
+// level-1 sums of lane 0.
+struct Ring {
+    uint32_t W = 0, H = 0, N = 0;
+    int C = 1;
+    size_t px = 0, l1_px = 0;
+    DeviceBuffer values, codes, pyr1;
+    std::vector<uint32_t> slot_frames;
+    uint32_t head = 0, loaded = 0, real = 0;
+
+    void init(uint32_t W, uint32_t H, int C, uint32_t slots, bool pyramid);
+    void reset();
+    uint32_t phys(int logical) const;
+    bool ready() const { return loaded >= N; }
+    void write(const avd_device_plane *planes, avd_sample_format f, hipStream_t s);
+    void duplicate_last(hipStream_t s);
+    float *value_slot(uint32_t s) const { return values.as<float>() + (size_t)s * px * C; }
+    uint16_t *code_slot(uint32_t s) const { return codes.as<uint16_t>() + (size_t)s * px * C; }
+    int32_t *l1_slot(uint32_t s) const { return pyr1.as<int32_t>() + (size_t)s * l1_px; }
+    // Device table of neighbour slots for every (ring start, centre) pair: entry
+    // (start * N + centre) holds the N-1 physical slots of the other logical positions, oldest
+    // first, so a pass binds its neighbours without a host-to-device copy.
+    DeviceBuffer neighbour_table;
+    const uint32_t *neighbours_of(uint32_t centre_logical) const
+    {
+        return neighbour_table.as<uint32_t>() + ((size_t)(head % N) * N + centre_logical) * (N > 1 ? N - 1 : 1);
+    }
+    const float *value_ptr() const { return values.as<float>(); }
+    const uint16_t *code_ptr() const { return codes.as<uint16_t>(); }
+    const int32_t *l1_ptr() const { return pyr1.as<int32_t>(); }
+};
+
+// One plane group's denoiser (the reference's PassEngine + FramePass).
+class Pass {
+  public:
+    virtual ~Pass() = default;
+    virtual uint32_t push(const avd_device_plane *planes) = 0;
+    virtual void push_context(const avd_device_plane *planes) = 0;
+    virtual void emit(const avd_device_plane *planes) = 0;
+    virtual uint32_t finish() = 0;
+    virtual void reset_stream() = 0;
+    virtual size_t max_held() const = 0;
+    virtual bool has_pending() const = 0;
+    uint32_t emitted_frame() const { return emitted_frame_; }
+
+  protected:
+    uint32_t emitted_frame_ = 0;
+};
+
+class NlmPass : public Pass {
+  public:
+    NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, avd_sample_format in, avd_sample_format out,
+            hipStream_t s);
+    uint32_t push(const avd_device_plane *planes) override;
+    void push_context(const avd_device_plane *planes) override;
+    void emit(const avd_device_plane *planes) override;
+    uint32_t finish() override;
+    void reset_stream() override;
+    size_t max_held() const override;
+    bool has_pending() const override;
+
+  private:
+    void ingest(const avd_device_plane *planes);
+    void denoise();
+    void egress(const float *frame, const avd_device_plane *planes);
+
+    uint32_t W_, H_;
+    int C_;
+    avd_sample_format in_, out_;
+    hipStream_t stream_;
+    uint32_t r_ = 0, S_ = 2, P_ = 4;
+    bool hq_ = false, has_sigma_ = false, motion_ = false, confidence_ = false, pushed_ = false;
+    float sigma_ = 0.0f, self_weight_ = 1.0f, h2_ = 0.0f, offset_ = 0.0f, thsad_scale_ = 1.0f;
+    MotionGeom mg_{};
+    Ring ring_;
+    DeviceBuffer accum_, wsum_, maxw_, lut_, comp_, mv_, conf_;
+    DeviceBuffer outputs_[2];
+    bool ready_ = false;
+    int ready_output_ = 0, next_output_ = 0;
+    uint32_t tail_remaining_ = 0;
+};
+
+class Nl4dPass : public Pass {
+  public:
+    Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd_sample_format in, avd_sample_format out,
+             hipStream_t s);
+    uint32_t push(const avd_device_plane *planes) override;
+    void push_context(const avd_device_plane *planes) override;
+    void emit(const avd_device_plane *planes) override;
+    uint32_t finish() override;
+    void reset_stream() override;
+    size_t max_held() const override;
+    bool has_pending() const override;
+
+  private:
+    enum class Clear { Whole, Newest, None };
+    uint32_t submit();
+    void run_pass(uint32_t centre_logical, Clear clear);
+    void zero_region(uint32_t slot);
+
+    uint32_t W_, H_;
+    int C_;
+    avd_sample_format in_, out_;
+    hipStream_t stream_;
+    uint32_t r_ = 1, N_ = 3;
+    float sigma_ = 0.0f, thsad_scale_ = 1.0f;
+    MotionGeom mg_{};
+    avd_collab_pass pass_{};
+    size_t refs_ = 0;
+    Ring ring_;
+    DeviceBuffer accum_, wsum_, members_, group_weight_, output_, mv_, conf_;
+    std::vector<uint32_t> pending_, tail_;
+    std::vector<bool> zeroed_;
+    uint32_t passes_ = 0;
+    bool continuation_ = false, tail_emitting_ = false;
+};
+
+class Engine {
+  public:
+    explicit Engine(const avd_engine_spec &spec);
+    uint32_t push(const avd_device_plane *planes, uint32_t count);
+    void push_context(const avd_device_plane *planes, uint32_t count);
+    void emit(const avd_device_plane *planes, uint32_t count);
+    uint32_t finish();
+    void reset();
+    void synchronize();
+    size_t max_held_frames() const;
+    avd_window_span window_span() const;
+    void fail_for_test();
+
+  private:
+    Pass &lead();
+    const Pass &lead() const;
+    void check_usable() const;
+    void check_planes_for(const avd_device_plane *planes, uint32_t count, avd_sample_format f) const;
+    void store_uncleaned(const avd_device_plane *planes, uint32_t frame);
+    void load_uncleaned(const avd_device_plane *planes, uint32_t frame);
+    void restart();
+    template <typename F>
+    auto guarded(F &&f) -> decltype(f());
+
+    avd_engine_spec spec_;
+    hipStream_t stream_ = nullptr;
+    std::unique_ptr<Pass> luma_, chroma_;
+    uint32_t planes_ = 1, chroma_w_ = 0, chroma_h_ = 0;
+    DeviceBuffer passthrough_;
+    uint32_t pt_slots_ = 0, pt_planes_ = 0;
+    uint64_t pt_words_ = 0;
+    uint32_t next_frame_ = 0, tail_left_ = 0;
+    bool pushed_ = false, poisoned_ = false;
+};
+
+}  // namespace avd

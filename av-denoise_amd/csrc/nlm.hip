@@ -1,0 +1,316 @@
+// NLMeans window kernels for gfx950.
+//
+// Semantics follow nlmeans/kernels/fused.rs: nlm_fused_single_window (:364-530) walks every
+// offset q != 0 of the (2s+1)^2 window of a frame against itself; nlm_fused_pair_accumulate_window
+// (:138-347) walks every q of one temporal pair, the forward frame at +q and the backward frame at
+// -q.  Per offset the per-pixel distance is sum_c (x - y)^2 * scale (3 luma, 1.5 chroma) with
+// clamp-to-edge reads, the patch distance is its (2P+1)^2 box sum, the weight is
+// exp(-max(D - offset, 0) * h2) (times the block confidence on a pair), and each pixel gathers
+// sum(w * y), sum(w) and max(w) in registers, added to the accumulators once.  nlm_finish
+// (accumulate.rs:54-101) then blends in the centre with self_weight * max_w.
+//
+// MI355X layout.  A wavefront owns 64 adjacent distance columns: lane l holds the distance of
+// image column X0 - P + l, and after the horizontal box sum lane l (l >= 2P) owns output column
+// X0 - 2P + l.  The horizontal sum is a chain of 2P DPP wave_shr:1 adds (each a single v_add_f32
+// with a DPP source), so it needs no LDS and no barrier.  Each lane walks ROWS output rows of its
+// column: the centre values of its ROWS + 2P distance rows stay in registers for the whole
+// window, and the vertical box sum adds 2P+1 of the row sums kept in registers.  The neighbour
+// frame(s) of the workgroup are staged once in LDS with their (P+s) halo, so the offset loop runs
+// with no barrier at all.  exp is v_exp_f32 (exp2) with log2(e) folded into h2 on the host.
+#include "common.h"
+
+namespace avd {
+
+namespace {
+
+constexpr int kRows = 16;
+constexpr int kMaxP = 8;
+constexpr int kMaxS = 8;
+
+__device__ __forceinline__ float wave_shr1(float v)
+{
+    // DPP wave_shr:1: lane l reads lane l-1; lane 0 reads 0.
+    return __builtin_amdgcn_update_dpp(0.0f, v, 0x138, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+
+// Loads a clamped (rows x cols) window of an interleaved frame into LDS.
+template <int C, int WV>
+__device__ void stage_tile(float *tile, const float *__restrict__ frame, int W, int H, int x0, int y0, int cols,
+                           int rows)
+{
+    const int n = cols * rows;
+    for (int i = threadIdx.x; i < n; i += WV * 64) {
+        const int ty = i / cols, tx = i - ty * cols;
+        const int sx = clampi(x0 + tx, W - 1), sy = clampi(y0 + ty, H - 1);
+        const float *src = frame + ((size_t)sy * W + sx) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) tile[(size_t)i * C + c] = src[c];
+    }
+}
+
+template <int C, int P, int WV, bool PAIR, bool CONF>
+__global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__restrict__ centre,
+                                                     const float *__restrict__ fwd, const float *__restrict__ bwd,
+                                                     const float *__restrict__ lut, const float *__restrict__ conf_f,
+                                                     const float *__restrict__ conf_b, float *__restrict__ accum,
+                                                     float *__restrict__ wsum, float *__restrict__ maxw)
+{
+    extern __shared__ float lds[];
+    constexpr int DR = kRows + 2 * P;  // distance rows per wave
+    constexpr int OW = 64 - 2 * P;     // output columns per wave
+    const int S = w.S, W = w.W, H = w.H;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int X0 = blockIdx.x * OW;
+    const int Y0 = blockIdx.y * (WV * kRows);
+
+    // Neighbour tiles: columns X0-P-S .. X0-P+63+S, rows Y0-P-S .. Y0+WV*kRows+P-1+S.
+    const int tcols = 64 + 2 * S, trows = WV * kRows + 2 * P + 2 * S;
+    const int tx0 = X0 - P - S, ty0 = Y0 - P - S;
+    float *tf = lds;
+    float *tb = lds + (size_t)tcols * trows * C;
+    stage_tile<C, WV>(tf, fwd, W, H, tx0, ty0, tcols, trows);
+    if (PAIR) stage_tile<C, WV>(tb, bwd, W, H, tx0, ty0, tcols, trows);
+
+    // Centre values of this lane's distance column, one per distance row, in registers.
+    const int xd = X0 - P + lane;
+    const int ybase = Y0 + wave * kRows - P;
+    float cen[DR][C];
+#pragma unroll
+    for (int r = 0; r < DR; ++r) {
+        const float *src = centre + ((size_t)clampi(ybase + r, H - 1) * W + clampi(xd, W - 1)) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) cen[r][c] = src[c];
+    }
+    __syncthreads();
+
+    const float scale = (C == 1) ? 3.0f : 1.5f;
+    const float h2l = w.h2_log2e;
+    const int ox = X0 - 2 * P + lane;  // output column of this lane (valid for lane >= 2P)
+    float conf_fv[kRows], conf_bv[kRows];
+    if (CONF) {
+        const int bcol = min(clampi(ox, W - 1) / w.step, w.blocks_x - 1);
+#pragma unroll
+        for (int j = 0; j < kRows; ++j) {
+            const int oy = clampi(Y0 + wave * kRows + j, H - 1);
+            const int bidx = min(oy / w.step, w.blocks_y - 1) * w.blocks_x + bcol;
+            conf_fv[j] = conf_f[bidx];
+            conf_bv[j] = conf_b[bidx];
+        }
+    }
+
+    float acc[kRows][C], ws[kRows], mw[kRows];
+#pragma unroll
+    for (int j = 0; j < kRows; ++j) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) acc[j][c] = 0.0f;
+        ws[j] = 0.0f;
+        mw[j] = 0.0f;
+    }
+
+    const int side = 2 * S + 1;
+    // Tile coordinates of this lane's distance position at offset 0.
+    const int trow0 = wave * kRows + S;  // + r + qy
+    const int tcol0 = lane + S;          // + qx
+    // Gather column of the output pixel: lane - P + S (+/- qx).  Lanes below 2P own no output and
+    // read a clamped column so every LDS read stays inside the tile.
+    const int gcol0 = max(lane, P) - P + S;
+    for (int qy = -S; qy <= S; ++qy) {
+        for (int qx = -S; qx <= S; ++qx) {
+            if (!PAIR && qx == 0 && qy == 0) continue;
+            const float off = PAIR ? w.offset : lut[(qy + S) * side + (qx + S)];
+            float hf[DR], hb[DR];
+#pragma unroll
+            for (int r = 0; r < DR; ++r) {
+                const float *nf = tf + ((size_t)(trow0 + r + qy) * tcols + (tcol0 + qx)) * C;
+                float df = 0.0f;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const float e = cen[r][c] - nf[c];
+                    df += e * e;
+                }
+                df *= scale;
+                float a = df;
+#pragma unroll
+                for (int k = 0; k < 2 * P; ++k) a = df + wave_shr1(a);
+                hf[r] = a;
+                if (PAIR) {
+                    const float *nb = tb + ((size_t)(trow0 + r - qy) * tcols + (tcol0 - qx)) * C;
+                    float db = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        const float e = cen[r][c] - nb[c];
+                        db += e * e;
+                    }
+                    db *= scale;
+                    float b = db;
+#pragma unroll
+                    for (int k = 0; k < 2 * P; ++k) b = db + wave_shr1(b);
+                    hb[r] = b;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kRows; ++j) {
+                float Df = 0.0f;
+#pragma unroll
+                for (int k = 0; k <= 2 * P; ++k) Df += hf[j + k];
+                float wf = __builtin_amdgcn_exp2f(-fmaxf(Df - off, 0.0f) * h2l);
+                if (CONF) wf *= conf_fv[j];
+                // The gathered neighbour sits at the output pixel plus q: distance row j+P, column lane-P.
+                const float *gf = tf + ((size_t)(trow0 + j + P + qy) * tcols + (gcol0 + qx)) * C;
+                if (PAIR) {
+                    float Db = 0.0f;
+#pragma unroll
+                    for (int k = 0; k <= 2 * P; ++k) Db += hb[j + k];
+                    float wb = __builtin_amdgcn_exp2f(-fmaxf(Db - off, 0.0f) * h2l);
+                    if (CONF) wb *= conf_bv[j];
+                    const float *gb = tb + ((size_t)(trow0 + j + P - qy) * tcols + (gcol0 - qx)) * C;
+#pragma unroll
+                    for (int c = 0; c < C; ++c) acc[j][c] = acc[j][c] + gf[c] * wf + gb[c] * wb;
+                    ws[j] += wf + wb;
+                    mw[j] = fmaxf(mw[j], fmaxf(wf, wb));
+                } else {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) acc[j][c] += gf[c] * wf;
+                    ws[j] += wf;
+                    mw[j] = fmaxf(mw[j], wf);
+                }
+            }
+        }
+    }
+
+    if (lane < 2 * P || ox >= W) return;
+#pragma unroll
+    for (int j = 0; j < kRows; ++j) {
+        const int oy = Y0 + wave * kRows + j;
+        if (oy >= H) break;
+        const size_t p = (size_t)oy * W + ox;
+#pragma unroll
+        for (int c = 0; c < C; ++c) accum[p * C + c] += acc[j][c];
+        wsum[p] += ws[j];
+        maxw[p] = fmaxf(maxw[p], mw[j]);
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void k_finish(const float *__restrict__ frame, const float *__restrict__ accum,
+                                                const float *__restrict__ wsum, const float *__restrict__ maxw,
+                                                float self_weight, uint32_t pixels, float *__restrict__ out)
+{
+    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < pixels; p += gridDim.x * 256) {
+        const float sw = self_weight * maxw[p];
+        const float den = sw + wsum[p];
+        if (den > 1e-30f) {
+            const float inv = 1.0f / den;
+#pragma unroll
+            for (int c = 0; c < C; ++c) out[(size_t)p * C + c] = (frame[(size_t)p * C + c] * sw + accum[(size_t)p * C + c]) * inv;
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; ++c) out[(size_t)p * C + c] = frame[(size_t)p * C + c];
+        }
+    }
+}
+
+// Waves per workgroup: pairs of two-channel frames stage two tiles of twice the width, so they
+// take half the rows to keep the LDS footprint near that of a luma pass.
+constexpr int waves_for(int C, bool pair) { return (pair && C == 2) ? 2 : 4; }
+
+size_t tile_bytes(int C, int P, int S, bool pair)
+{
+    const int wv = waves_for(C, pair);
+    const size_t one = (size_t)(64 + 2 * S) * (wv * kRows + 2 * P + 2 * S) * C * sizeof(float);
+    return pair ? 2 * one : one;
+}
+
+int device_lds_limit()
+{
+    static int limit = -1;
+    if (limit < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 64 * 1024;
+        limit = v;
+    }
+    return limit;
+}
+
+template <int C, int P>
+hipError_t dispatch(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd, const float *lut,
+                    const float *cf, const float *cb, float *accum, float *wsum, float *maxw, bool pair, bool conf,
+                    hipStream_t s)
+{
+    constexpr int WS = waves_for(C, false), WP = waves_for(C, true);
+    const size_t lds = tile_bytes(C, P, w.S, pair);
+    if (!pair) {
+        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WS * kRows));
+        k_nlm<C, P, WS, false, false><<<grid, WS * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw);
+    } else {
+        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WP * kRows));
+        if (conf)
+            k_nlm<C, P, WP, true, true><<<grid, WP * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw);
+        else
+            k_nlm<C, P, WP, true, false><<<grid, WP * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw);
+    }
+    return hipGetLastError();
+}
+
+template <int C>
+hipError_t dispatch_p(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd, const float *lut,
+                      const float *cf, const float *cb, float *accum, float *wsum, float *maxw, bool pair, bool conf,
+                      hipStream_t s)
+{
+    switch (w.P) {
+    case 0: return dispatch<C, 0>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 1: return dispatch<C, 1>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 2: return dispatch<C, 2>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 3: return dispatch<C, 3>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 4: return dispatch<C, 4>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 5: return dispatch<C, 5>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 6: return dispatch<C, 6>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 7: return dispatch<C, 7>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 8: return dispatch<C, 8>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+bool nlm_supported(int C, int P, int S)
+{
+    return (C == 1 || C == 2) && P >= 0 && P <= kMaxP && S >= 0 && S <= kMaxS &&
+           tile_bytes(C, P, S, true) <= (size_t)device_lds_limit() &&
+           tile_bytes(C, P, S, false) <= (size_t)device_lds_limit();
+}
+
+hipError_t launch_nlm_single(const NlmWindow &w, const float *frame, const float *lut, float *accum, float *wsum,
+                             float *maxw, hipStream_t s)
+{
+    if (!nlm_supported(w.C, w.P, w.S)) return hipErrorInvalidValue;
+    if (w.C == 1) return dispatch_p<1>(w, frame, frame, frame, lut, nullptr, nullptr, accum, wsum, maxw, false, false, s);
+    return dispatch_p<2>(w, frame, frame, frame, lut, nullptr, nullptr, accum, wsum, maxw, false, false, s);
+}
+
+hipError_t launch_nlm_pair(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd,
+                           const float *conf_f, const float *conf_b, float *accum, float *wsum, float *maxw,
+                           hipStream_t s)
+{
+    if (!nlm_supported(w.C, w.P, w.S)) return hipErrorInvalidValue;
+    const bool conf = conf_f != nullptr && conf_b != nullptr;
+    if (w.C == 1) return dispatch_p<1>(w, centre, fwd, bwd, nullptr, conf_f, conf_b, accum, wsum, maxw, true, conf, s);
+    return dispatch_p<2>(w, centre, fwd, bwd, nullptr, conf_f, conf_b, accum, wsum, maxw, true, conf, s);
+}
+
+hipError_t launch_nlm_finish(const float *frame, const float *accum, const float *wsum, const float *maxw,
+                             float self_weight, uint32_t pixels, int C, float *out, hipStream_t s)
+{
+    unsigned g = div_up(pixels, 256);
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    if (C == 1) k_finish<1><<<g, 256, 0, s>>>(frame, accum, wsum, maxw, self_weight, pixels, out);
+    else k_finish<2><<<g, 256, 0, s>>>(frame, accum, wsum, maxw, self_weight, pixels, out);
+    return hipGetLastError();
+}
+
+}  // namespace avd

@@ -1,0 +1,293 @@
+/*
+ * avd.h -- C ABI of the MI355X-native av-denoise denoising core.
+ *
+ * Every entry point is plain C: pointers, sizes and POD structs, no C++ or torch types.  A
+ * device plane is a pointer into HIP device memory; the caller owns it and lends it for the call.
+ *
+ * Reference interfaces each declaration replaces (paths relative to the reference checkout):
+ *   avd_engine_*        Engine trait          av-denoise-core/src/engine/mod.rs:168-211
+ *   avd_engine_create   build_engine          av-denoise-core/src/tune/setup.rs:79-92
+ *   avd_engine_spec     EngineSpec            av-denoise-core/src/engine/spec.rs:146-157
+ *   avd_geometry        Geometry              av-denoise-core/src/engine/frame.rs:73-97
+ *   avd_sample_format   SampleFormat          av-denoise-core/src/engine/plane.rs:8-52
+ *   avd_device_plane    DevicePlane           av-denoise-core/src/engine/plane.rs:54-86
+ *   avd_nlmeans_options NlmeansAlgorithm      av-denoise-core/src/nlmeans/options.rs:6-106
+ *   avd_nl4d_options    Nl4dOptions           av-denoise-core/src/nl4d/options.rs:11-94
+ *   avd_motion_search   MotionSearch          av-denoise-core/src/motion/search.rs:5-40
+ *   avd_status          Error                 av-denoise-core/src/error.rs:3-20
+ *   avd_preset_*        Preset ladder         av-denoise-core/src/options.rs:16-38,
+ *                                             av-denoise-core/src/nlmeans/options.rs:58-82
+ *   avd_site_*          launch sites          av-denoise-core/src/tune/site.rs:56-157 (the
+ *                       kernel-level boundary: collab_search / collab_filter of the SplitSearch
+ *                       profile, fused/split.rs:34-281, and the motion snapshot of
+ *                       nl4d/snapshot.rs), exposed so a caller can run one pass of one stage.
+ *
+ * Threading: an engine is driven by one thread at a time (the reference's Engine is Send, not
+ * Sync).  Engines on different threads are independent.  Work runs in order on the engine's HIP
+ * stream.  Any HIP failure poisons the engine until avd_engine_reset.
+ */
+#ifndef AVD_AVD_H
+#define AVD_AVD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVD_ABI_VERSION 1u
+
+/* Error classes, one per reference Error variant (error.rs:3-20), plus HIP failures. */
+typedef enum avd_status {
+    AVD_OK = 0,
+    AVD_ERR_INVALID_OPTIONS = 1,
+    AVD_ERR_INVALID_GEOMETRY = 2,
+    AVD_ERR_PLANE_MISMATCH = 3,
+    AVD_ERR_OUTPUTS_PENDING = 4,
+    AVD_ERR_NOTHING_TO_EMIT = 5,
+    AVD_ERR_CONTEXT_AFTER_PUSH = 6,
+    AVD_ERR_NEEDS_RESET = 7,
+    AVD_ERR_GPU = 8,
+    AVD_ERR_NULL_ARGUMENT = 9
+} avd_status;
+
+/* SampleFormat: U8, or U16 holding `depth` significant bits (9..16). */
+typedef enum avd_format_kind { AVD_FORMAT_U8 = 0, AVD_FORMAT_U16 = 1, AVD_FORMAT_F32 = 2 } avd_format_kind;
+
+typedef struct avd_sample_format {
+    uint32_t kind;  /* avd_format_kind */
+    uint32_t depth; /* 8 for U8, 9..16 for U16, ignored for F32 */
+} avd_sample_format;
+
+typedef enum avd_chroma_layout {
+    AVD_CHROMA_GRAY = 0,
+    AVD_CHROMA_420 = 1,
+    AVD_CHROMA_422 = 2,
+    AVD_CHROMA_444 = 3
+} avd_chroma_layout;
+
+typedef struct avd_geometry {
+    uint32_t width;  /* luma plane */
+    uint32_t height;
+    uint32_t chroma; /* avd_chroma_layout */
+    avd_sample_format input;
+    avd_sample_format output;
+    uint32_t chroma_siting; /* carried for callers, never changes the output */
+} avd_geometry;
+
+/* A tightly packed plane of whole u32 words in device memory (DevicePlane). */
+typedef struct avd_device_plane {
+    void *data;      /* hipDeviceptr_t */
+    uint64_t bytes;  /* capacity of `data`; must be >= the plane's word-rounded size */
+    uint32_t width;
+    uint32_t height;
+} avd_device_plane;
+
+/* MotionSearch / MotionCompensationMode::Mvtools.  estimation: 0 Auto, 1 Direct, 2 Chained. */
+typedef struct avd_motion_search {
+    uint32_t blksize;
+    uint32_t overlap;
+    uint32_t search_radius;
+    uint32_t max_search_escalation;
+    uint32_t pyramid_levels;
+    uint32_t estimation;
+    uint32_t refine_radius; /* for Chained */
+} avd_motion_search;
+
+/* NlmeansAlgorithm: Fast or Hq, with NlmeansOptions (+ HqParams).  `has_*` flags stand in for
+ * the reference's Option<> fields. */
+typedef struct avd_nlmeans_options {
+    uint32_t hq;                 /* 0 Fast, 1 Hq */
+    uint32_t temporal;           /* DenoisingMode: 0 Spacial, 1 Temporal */
+    uint32_t temporal_radius;
+    uint32_t has_search_radius, search_radius;
+    uint32_t has_patch_radius, patch_radius;
+    uint32_t has_strength;
+    float strength;
+    uint32_t has_self_weight;
+    float self_weight;
+    uint32_t motion_compensation; /* 0 None, 1 Mvtools */
+    avd_motion_search motion;
+    /* HqParams */
+    uint32_t auto_strength;
+    uint32_t noise_floor;
+    uint32_t has_sigma;
+    float sigma;                 /* normalised, 0..1 */
+    uint32_t temporal_confidence;
+    float thsad_scale;
+    float sigma_scale;
+} avd_nlmeans_options;
+
+/* Nl4dOptions.  `step` and `group_size` are this build's reference-grid stride and group size;
+ * the reference fixes them at 4 and 8 (collab/mod.rs:16-20). */
+typedef struct avd_nl4d_options {
+    avd_motion_search motion;
+    uint32_t has_sigma;
+    float sigma;
+    float sigma_scale;
+    float thsad_scale;
+    uint32_t refine;
+    uint32_t spatial_radius;
+    uint32_t has_lambda_ht;
+    float lambda_ht;
+    float lambda_ht_scale;
+    float c_min;
+    float kaiser_beta;
+    float field_lambda;
+    uint32_t noise_map;
+    uint32_t pooled_threshold;
+    uint32_t grain_data;
+    uint32_t temporal_radius;
+    uint32_t step;
+    uint32_t group_size;
+} avd_nl4d_options;
+
+typedef enum avd_engine_kind { AVD_ENGINE_NLMEANS = 0, AVD_ENGINE_NL4D = 1 } avd_engine_kind;
+
+/* EngineSpec: which engine, the geometry, and the options of each plane group it cleans. */
+typedef struct avd_engine_spec {
+    uint32_t kind; /* avd_engine_kind */
+    avd_geometry geometry;
+    uint32_t clean_luma;
+    uint32_t clean_chroma;
+    avd_nlmeans_options nlm_luma;
+    avd_nlmeans_options nlm_chroma;
+    avd_nl4d_options nl4d_luma;
+    avd_nl4d_options nl4d_chroma;
+    void *stream; /* hipStream_t to run on; NULL runs on the default stream */
+} avd_engine_spec;
+
+typedef struct avd_engine avd_engine;
+
+/* Window span (WindowSpan, engine/mod.rs:41-60).  edges: 0 Repeat, 1 Shifted. */
+typedef struct avd_window_span {
+    uint64_t behind;
+    uint64_t ahead;
+    uint32_t edges;
+} avd_window_span;
+
+/* ---- library ---------------------------------------------------------------------------- */
+
+uint32_t avd_abi_version(void);
+/* The message of the last error on this thread (empty when none). */
+const char *avd_last_error_message(void);
+
+/* ---- option defaults and presets (options.rs:16-38, nlmeans/options.rs:58-82,
+ *      nl4d/options.rs:68-94) -------------------------------------------------------------- */
+
+/* preset: 0 veryfast, 1 fast, 2 base, 3 slow, 4 veryslow. */
+void avd_motion_search_default(avd_motion_search *out);
+void avd_nlmeans_options_default(avd_nlmeans_options *out);
+avd_status avd_nlmeans_options_for_preset(uint32_t preset, avd_nlmeans_options *out);
+void avd_nl4d_options_default(avd_nl4d_options *out);
+avd_status avd_nl4d_options_for_preset(uint32_t preset, avd_nl4d_options *out);
+/* Bytes a plane of width x height samples occupies in `format` (SampleFormat::plane_bytes). */
+uint64_t avd_plane_bytes(avd_sample_format format, uint32_t width, uint32_t height);
+
+/* ---- engine (Engine trait) ------------------------------------------------------------- */
+
+avd_status avd_engine_create(const avd_engine_spec *spec, avd_engine **out);
+void avd_engine_destroy(avd_engine *engine);
+/* Ingests one frame (all planes, luma first); *ready = frames ready to emit. */
+avd_status avd_engine_push(avd_engine *engine, const avd_device_plane *planes, uint32_t count, uint32_t *ready);
+/* Ingests one frame as context only (Engine::push_context). */
+avd_status avd_engine_push_context(avd_engine *engine, const avd_device_plane *planes, uint32_t count);
+/* Writes the oldest ready frame into planes (Engine::emit_into). */
+avd_status avd_engine_emit_into(avd_engine *engine, const avd_device_plane *planes, uint32_t count);
+/* Ends the stream; *tail = tail frames to emit (Engine::finish). */
+avd_status avd_engine_finish(avd_engine *engine, uint32_t *tail);
+/* Abandons the current stream and clears a poisoned engine (Engine::reset). */
+avd_status avd_engine_reset(avd_engine *engine);
+avd_status avd_engine_window_span(const avd_engine *engine, avd_window_span *out);
+avd_status avd_engine_max_held_frames(const avd_engine *engine, uint64_t *out);
+/* Waits for every queued launch of the engine. */
+avd_status avd_engine_synchronize(avd_engine *engine);
+/* Poisons the engine as a GPU failure would (fail_through_guard_for_test, nlmeans/engine.rs:195). */
+avd_status avd_engine_fail_for_test(avd_engine *engine);
+
+/* ---- launch sites: one stage of one pass over caller-owned device buffers ----------------- */
+
+/* Plane ingest (engine/kernels.rs:25-63): u32 words of `channels` planes into an interleaved f32
+ * slot and its u16 code mirror (codes may be NULL). */
+avd_status avd_site_ingest(const void *plane0, const void *plane1, float *slot, uint16_t *codes, uint32_t pixels,
+                           uint32_t channels, avd_sample_format format, void *stream);
+/* Plane egress (engine/kernels.rs:103-158). */
+avd_status avd_site_egress(const float *frame, void *plane0, void *plane1, uint32_t pixels, uint32_t channels,
+                           avd_sample_format format, void *stream);
+
+/* Parameters of one NL4D grouping pass (collab_fused's bindings, collab/kernels/fused/mod.rs:170-211). */
+typedef struct avd_collab_pass {
+    uint32_t width, height, channels;
+    uint32_t step;             /* reference-grid stride */
+    uint32_t spatial_radius;
+    uint32_t temporal_radius;  /* 0 for a single-frame ring */
+    uint32_t refine;
+    uint32_t group_size;       /* k_max */
+    uint32_t blk_step, blksize, blocks_x, blocks_y;
+    uint32_t centre_slot;
+    uint32_t slots;            /* frames in the ring */
+    float c_min;
+    float lambda[2];
+    float sigma[2];
+    float weight_scale;
+    float accum_scale;
+    float kaiser[8];
+    float profile[8];
+} avd_collab_pass;
+
+/* search_group over every reference (collab_search, fused/split.rs:34-170): writes 9 words per
+ * reference (8 pack_pos_t members and the group word).  `ring_codes` (u16 codes, exact SSD) or,
+ * when NULL, `ring_values` (f32, the reference's F32Search distance) is searched.  mv_field holds
+ * 2*radius slices of blocks_x*blocks_y (dx,dy) pairs, confidence the matching floats. */
+avd_status avd_site_collab_search(const avd_collab_pass *pass, const uint16_t *ring_codes, const float *ring_values,
+                                  const uint32_t *neighbour_slots, const int32_t *mv_field, const float *confidence,
+                                  uint32_t *members, void *stream);
+/* filter_group over every reference (collab_filter, fused/split.rs:181-281): transforms, hard
+ * threshold and the fixed-point scatter into accum (slots*px*channels i32) and wsum (slots*px i32). */
+avd_status avd_site_collab_filter(const avd_collab_pass *pass, const float *ring_values,
+                                  const uint32_t *neighbour_slots, const uint32_t *members, int32_t *accum,
+                                  int32_t *wsum, float *group_weight, void *stream);
+/* collab_normalise (aggregate.rs:252-290) of one region, which it leaves zeroed. */
+avd_status avd_site_collab_normalise(int32_t *accum_region, int32_t *wsum_region, float *out, uint32_t pixels,
+                                     uint32_t channels, void *stream);
+
+/* Motion of every neighbour slot against the centre (mc_block_match_coarse + mc_block_match_fine
+ * with confidence, motion/analyse.rs:175-267) on the lane-0 codes of an interleaved u16 ring.
+ * mv_field/confidence as in avd_site_collab_search.  pyramid_levels 1 or 2. */
+typedef struct avd_motion_pass {
+    uint32_t width, height, channels;
+    uint32_t blksize, overlap, search_radius, pyramid_levels;
+    uint32_t centre_slot;
+    uint32_t neighbours;
+    float max_value;
+    float sad_floor;
+    float thsad;
+} avd_motion_pass;
+
+avd_status avd_site_motion(const avd_motion_pass *pass, const uint16_t *ring_codes, const uint32_t *neighbour_slots,
+                           int32_t *mv_field, float *confidence, void *stream);
+
+/* Parameters of one NLM window launch (nlm_fused_single_window / nlm_fused_pair_accumulate_window,
+ * nlmeans/kernels/fused.rs:138-530). */
+typedef struct avd_nlm_window {
+    uint32_t width, height, channels;
+    uint32_t patch_radius, search_radius;
+    float h2_inv_norm;
+    float noise_offset;        /* pair: flat offset; single: offsets from `lut` */
+    uint32_t step, blocks_x, blocks_y; /* confidence block grid */
+} avd_nlm_window;
+
+avd_status avd_site_nlm_single(const avd_nlm_window *w, const float *frame, const float *lut, float *accum,
+                               float *wsum, float *maxw, void *stream);
+avd_status avd_site_nlm_pair(const avd_nlm_window *w, const float *centre, const float *fwd, const float *bwd,
+                             const float *conf_fwd, const float *conf_bwd, float *accum, float *wsum, float *maxw,
+                             void *stream);
+avd_status avd_site_nlm_finish(const float *frame, const float *accum, const float *wsum, const float *maxw,
+                               float self_weight, uint32_t pixels, uint32_t channels, float *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AVD_AVD_H */

@@ -1,0 +1,315 @@
+"""GPU parity: the HIP path through the C ABI against the oracle on the same seeded inputs.
+
+Bar: integer and index work (codes, block-match vectors, group members, fixed-point
+accumulators, NL4D output words) bit-exact; NLMeans sums within 1e-4 relative of the FP64
+oracle and outputs within 1 LSB of the FP32 oracle.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import avd
+from avd import _lib as L
+
+import oracle as O
+import refsetups as R
+from gpu_helpers import (collab_pass, dev, gray_geometry, host, ptr, require_gpu, run_engine, site_collab_filter,
+                         site_collab_search, words_dev, words_host)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _gpu():
+    require_gpu()
+
+
+# ---- plane I/O ----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("depth,C,pixels", [(8, 1, 1001), (10, 1, 777), (12, 2, 515), (8, 2, 64), (16, 1, 33)])
+def test_ingest_egress_match_the_oracle(depth, C, pixels):
+    rng = np.random.default_rng(depth * 10 + C)
+    codes = [rng.integers(0, 1 << depth, pixels) for _ in range(C)]
+    words = [O.pack_plane(c, depth) for c in codes]
+    fmt = L.SampleFormat.for_depth(depth)
+    planes = [dev(w.view(np.uint8)) for w in words]
+    slot = torch.zeros(pixels * C, dtype=torch.float32, device="cuda")
+    cod = torch.zeros(pixels * C, dtype=torch.int16, device="cuda")
+    L.check(avd.load().avd_site_ingest(ptr(planes[0]), ptr(planes[-1]), ptr(slot), ptr(cod), pixels, C, fmt, None))
+    want = O.ingest(words, pixels, C, depth)
+    assert np.array_equal(host(slot), want)
+    assert np.array_equal(host(cod).view(np.uint16).astype(np.int64), np.stack(codes, 1).reshape(-1))
+    outs = [torch.zeros_like(p) for p in planes]
+    L.check(avd.load().avd_site_egress(ptr(slot), ptr(outs[0]), ptr(outs[-1]), pixels, C, fmt, None))
+    for o, w in zip(outs, words):
+        assert np.array_equal(words_host(o), w)
+
+
+# ---- NLMeans window kernels ----------------------------------------------------------------
+
+def _window(W, H, C, P, S, h2, off, step=8, bx=1, by=1):
+    w = L.NlmWindow()
+    w.width, w.height, w.channels, w.patch_radius, w.search_radius = W, H, C, P, S
+    w.h2_inv_norm, w.noise_offset, w.step, w.blocks_x, w.blocks_y = h2, off, step, bx, by
+    return w
+
+
+@pytest.mark.parametrize("W,H,C,P,S", [(70, 45, 1, 2, 3), (64, 40, 2, 3, 2), (37, 29, 1, 0, 1), (90, 70, 1, 4, 2)])
+def test_nlm_single_matches_the_oracle(W, H, C, P, S):
+    rng = np.random.default_rng(W + C)
+    f = rng.uniform(0, 1, W * H * C).astype(np.float32)
+    h2 = float(O.h2_inv_norm(1.2, P))
+    off = 0.01
+    lut = O.spatial_offset_lut(S, off)
+    acc = [np.zeros(W * H * C), np.zeros(W * H), np.zeros(W * H)]
+    O.nlm_single(f, W, H, C, P, S, h2, lut, *acc, precision="f64")
+    d_acc = torch.zeros(W * H * C, device="cuda")
+    d_ws = torch.zeros(W * H, device="cuda")
+    d_mw = torch.zeros(W * H, device="cuda")
+    d_f, d_lut = dev(f), dev(lut)  # held for the launch
+    L.check(avd.load().avd_site_nlm_single(ctypes.byref(_window(W, H, C, P, S, h2, off)), ptr(d_f), ptr(d_lut),
+                                           ptr(d_acc), ptr(d_ws), ptr(d_mw), None))
+    for got, want in zip((host(d_acc), host(d_ws), host(d_mw)), acc):
+        assert np.allclose(got, want, rtol=1e-4, atol=1e-6), np.max(np.abs(got - want))
+
+
+@pytest.mark.parametrize("conf", [False, True])
+@pytest.mark.parametrize("C", [1, 2])
+def test_nlm_pair_matches_the_oracle(conf, C):
+    W, H, P, S = 66, 50, 3, 3
+    rng = np.random.default_rng(3 + C)
+    frames = [rng.uniform(0, 1, W * H * C).astype(np.float32) for _ in range(3)]
+    h2 = float(O.h2_inv_norm(0.8, P))
+    off = 0.02
+    bx, by = (W + 7) // 8, (H + 7) // 8
+    cf = rng.uniform(0, 1, bx * by).astype(np.float32) if conf else None
+    cb = rng.uniform(0, 1, bx * by).astype(np.float32) if conf else None
+    acc = [np.zeros(W * H * C), np.zeros(W * H), np.zeros(W * H)]
+    O.nlm_pair(frames[0], frames[1], frames[2], W, H, C, P, S, h2, off, cf, cb, 8, bx, by, *acc, precision="f64")
+    d = [torch.zeros(W * H * C, device="cuda"), torch.zeros(W * H, device="cuda"), torch.zeros(W * H, device="cuda")]
+    keep = [dev(x) for x in frames] + ([dev(cf), dev(cb)] if conf else [None, None])  # held for the launch
+    L.check(avd.load().avd_site_nlm_pair(ctypes.byref(_window(W, H, C, P, S, h2, off, 8, bx, by)), ptr(keep[0]),
+                                         ptr(keep[1]), ptr(keep[2]), ptr(keep[3]), ptr(keep[4]), ptr(d[0]), ptr(d[1]),
+                                         ptr(d[2]), None))
+    for got, want in zip((host(x) for x in d), acc):
+        assert np.allclose(got, want, rtol=1e-4, atol=1e-6), np.max(np.abs(got - want))
+
+
+# ---- motion ------------------------------------------------------------------------------
+
+def _moving_codes(W, H, depth, frames, seed):
+    clip = avd.synth.clip(frames, W, H, depth, seed, chroma="gray", sigma_luma=4.0)
+    return [f[0].numpy().astype(np.int32) for f in clip]
+
+
+@pytest.mark.parametrize("levels,R", [(2, 4), (1, 3), (2, 8)])
+def test_motion_matches_the_oracle(levels, R):
+    W, H, depth = 96, 72, 10
+    codes = _moving_codes(W, H, depth, 5, 11)
+    ring = np.stack([c.reshape(-1) for c in codes]).astype(np.uint16)
+    geom = O.MotionGeom(W, H, 16, 8, R, levels)
+    centre, slots = 2, [0, 1, 3, 4]
+    sigma = 4.0 / 1023
+    floor, th = float(O.sad_noise_floor(16, sigma)), float(O.thsad(16, 1.0))
+    mp = L.MotionPass()
+    mp.width, mp.height, mp.channels, mp.blksize, mp.overlap = W, H, 1, 16, 8
+    mp.search_radius, mp.pyramid_levels, mp.centre_slot, mp.neighbours = R, levels, centre, 4
+    mp.max_value, mp.sad_floor, mp.thsad = 1023.0, floor, th
+    blocks = geom.blocks
+    mv = torch.zeros(4 * blocks * 2, dtype=torch.int32, device="cuda")
+    conf = torch.zeros(4 * blocks, dtype=torch.float32, device="cuda")
+    ns = dev(np.asarray(slots, np.uint32).view(np.int32))
+    d_ring = dev(ring.view(np.int16))  # held for the launch
+    L.check(avd.load().avd_site_motion(ctypes.byref(mp), ptr(d_ring), ptr(ns), ptr(mv), ptr(conf), None))
+    got_mv, got_conf = host(mv), host(conf)
+    pyr = [[c.reshape(-1), O.downscale_sum(c.reshape(-1), W, H)] for c in codes]
+    for i, s in enumerate(slots):
+        want_mv, want_conf = O.motion_search(geom, pyr[centre], pyr[s], 1023.0, floor, th)
+        assert np.array_equal(got_mv[i * blocks * 2:(i + 1) * blocks * 2], want_mv), i
+        assert np.array_equal(got_conf[i * blocks:(i + 1) * blocks], want_conf), i
+    assert np.any(got_mv != 0)
+
+
+# ---- NL4D grouping and filter ---------------------------------------------------------------
+
+def test_collab_matches_the_recorded_reference_setups():
+    """The reference's recorded f32 setups (collab/tests/fused/recorded.rs) through the HIP sites."""
+    W = H = 64
+    mv, conf, slots, bx, by = R.cross_frame_fields(W, H, 2)
+    ring = R.unique_frame(W, H * 5).reshape(-1)
+    geom = O.MotionGeom(W, H, 16, 8)
+    p = collab_pass(W, H, 1, 2, 4, 2, 0.5, 5.3, 0.02, 0.0, 2, 5, geom)
+    op = O.collab_params(W, H, 1, 2, 4, 2, 0.5, 5.3, 0.02, 0.0, geom)
+    want = O.collab_search(ring, op, 2, slots, mv, conf)
+    got = site_collab_search(p, values=dev(ring), nslots=slots, mv=mv, conf=conf)
+    assert np.array_equal(got, want)
+    acc, ws, gw = site_collab_filter(p, dev(ring), slots, got, 5)
+    wacc = np.zeros(5 * W * H, np.int32)
+    wws = np.zeros(5 * W * H, np.int32)
+    wgw = O.collab_filter(ring, op, 2, slots, want, wacc, wws, "f32")
+    assert np.array_equal(acc, wacc) and np.array_equal(ws, wws) and np.array_equal(gw, wgw)
+    d = R.digest(acc, ws, gw)
+    assert d["covered"] == 16453
+
+
+@pytest.mark.parametrize("C,radius,S", [(1, 2, 9), (2, 1, 6), (1, 3, 12), (1, 0, 5)])
+def test_collab_codes_search_and_filter_are_bit_exact(C, radius, S):
+    W, H, depth = 80, 56, 10
+    frames = 2 * radius + 1
+    clip = avd.synth.clip(frames, W, H, depth, 5 + C, chroma="444" if C == 2 else "gray", sigma_luma=12.0)
+    planes = [[f[0].numpy()] if C == 1 else [f[1].numpy(), f[2].numpy()] for f in clip]
+    codes = np.stack([np.stack([p.reshape(-1) for p in fr], 1).reshape(-1) for fr in planes]).astype(np.uint16)
+    values = (codes.astype(np.float32) / np.float32(1023.0)).astype(np.float32)
+    geom = O.MotionGeom(W, H, 16, 8, 4, 2)
+    centre = radius
+    slots = [s for s in range(frames) if s != centre]
+    pyr = [[codes[s].reshape(-1, C)[:, 0].astype(np.int32),
+            O.downscale_sum(codes[s].reshape(-1, C)[:, 0].astype(np.int32), W, H)] for s in range(frames)]
+    sigma = 12.0 / 1023
+    floor, th = float(O.sad_noise_floor(16, sigma)), float(O.thsad(16, 1.0))
+    mvs, confs = [], []
+    for s in slots:
+        m, c = O.motion_search(geom, pyr[centre], pyr[s], 1023.0, floor, th)
+        mvs.append(m)
+        confs.append(c)
+    mv = np.concatenate(mvs) if mvs else np.zeros(2, np.int32)
+    conf = np.concatenate(confs) if confs else np.ones(1, np.float32)
+    lam = 2.7
+    p = collab_pass(W, H, C, radius, S, 2, 0.05, lam, sigma, 2.0, centre, frames, geom)
+    op = O.collab_params(W, H, C, radius, S, 2, 0.05, lam, sigma, 2.0, geom)
+    want = O.collab_search(codes, op, centre, slots, mv, conf)
+    got = site_collab_search(p, codes=dev(codes.view(np.int16)), nslots=slots, mv=mv, conf=conf)
+    assert np.array_equal(got, want)
+    acc, ws, gw = site_collab_filter(p, dev(values), slots, got, frames)
+    wacc = np.zeros(frames * W * H * C, np.int32)
+    wws = np.zeros(frames * W * H, np.int32)
+    wgw = O.collab_filter(values, op, centre, slots, want, wacc, wws, "f32")
+    assert np.array_equal(acc, wacc)
+    assert np.array_equal(ws, wws)
+    assert np.array_equal(gw, wgw)
+    if radius >= 2:
+        assert np.any(got.reshape(-1, 9)[:, 8] & 0x100), "some groups build a volume grid"
+
+
+# ---- engines ------------------------------------------------------------------------------
+
+def _nl4d_spec(r, S, sigma, lam=2.7):
+    return O.Nl4dSpec(temporal_radius=r, refine=2, spatial_radius=S, lambda_ht=lam, c_min=0.05, kaiser_beta=2.0,
+                      sigma=sigma, motion=O.MotionGeom(0, 0, 16, 8, 4, 2))
+
+
+@pytest.mark.parametrize("frames,r", [(7, 1), (9, 2), (2, 2), (5, 3)])
+def test_nl4d_engine_stream_is_bit_exact(frames, r):
+    W, H, depth = 72, 56, 10
+    S = 7
+    sigma = 16.0 / 1023
+    clip = avd.synth.clip(frames, W, H, depth, 100 + r, chroma="gray", sigma_luma=16.0)
+    codes = [f[0].numpy() for f in clip]
+    opt = avd.nl4d_options(temporal_radius=r, spatial_radius=S, sigma=sigma, lambda_ht=2.7, pooled_threshold=0,
+                           noise_map=0, motion={"estimation": 1})
+    eng = avd.build_engine("nl4d", gray_geometry(W, H, depth), luma=opt)
+    got = run_engine(eng, [[words_dev(c, depth)] for c in codes], depth, [(W, H)])
+    p = O.Nl4dPass(_nl4d_spec(r, S, sigma), W, H, 1, depth, depth)
+    want, _ = O.run_stream(p, [[O.pack_plane(c, depth)] for c in codes])
+    assert len(got) == len(want) == frames
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert np.array_equal(g[0], w[0]), f"frame {i}"
+
+
+def test_nl4d_engine_420_luma_and_chroma_are_bit_exact():
+    W, H, depth, r, S = 64, 48, 10, 2, 6
+    sigma = 16.0 / 1023
+    frames = 6
+    clip = avd.synth.clip(frames, W, H, depth, 77, chroma="420", sigma_luma=16.0, sigma_chroma=8.0)
+    geom = L.Geometry(W, H, 1, L.SampleFormat.u16(10), L.SampleFormat.u16(10), 0)
+    opt = avd.nl4d_options(temporal_radius=r, spatial_radius=S, sigma=sigma, lambda_ht=2.7, pooled_threshold=0,
+                           noise_map=0, motion={"estimation": 1})
+    eng = avd.build_engine("nl4d", geom, luma=opt, chroma=opt)
+    dims = [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
+    got = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f] for f in clip], depth, dims)
+    pl = O.Nl4dPass(_nl4d_spec(r, S, sigma), W, H, 1, depth, depth)
+    pc = O.Nl4dPass(_nl4d_spec(r, S, sigma), W // 2, H // 2, 2, depth, depth)
+    want_l, _ = O.run_stream(pl, [[O.pack_plane(f[0].numpy(), depth)] for f in clip])
+    want_c, _ = O.run_stream(pc, [[O.pack_plane(f[1].numpy(), depth), O.pack_plane(f[2].numpy(), depth)] for f in clip])
+    assert len(got) == frames
+    for i in range(frames):
+        assert np.array_equal(got[i][0], want_l[i][0]), f"luma {i}"
+        assert np.array_equal(got[i][1], want_c[i][0]), f"U {i}"
+        assert np.array_equal(got[i][2], want_c[i][1]), f"V {i}"
+
+
+def test_nl4d_chroma_passes_through_when_only_luma_is_cleaned():
+    W, H, depth = 48, 40, 8
+    clip = avd.synth.clip(5, W, H, depth, 3, chroma="420", sigma_luma=6.0)
+    geom = L.Geometry(W, H, 1, L.SampleFormat.u8(), L.SampleFormat.u8(), 0)
+    opt = avd.nl4d_options(temporal_radius=1, spatial_radius=5, sigma=6 / 255, pooled_threshold=0, noise_map=0,
+                           motion={"estimation": 1})
+    eng = avd.build_engine("nl4d", geom, luma=opt)
+    dims = [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
+    got = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f] for f in clip], depth, dims)
+    for i, f in enumerate(clip):
+        assert np.array_equal(got[i][1], O.pack_plane(f[1].numpy(), depth))
+        assert np.array_equal(got[i][2], O.pack_plane(f[2].numpy(), depth))
+
+
+def _max_lsb(a_words, b_words, depth, pixels):
+    a = O.unpack_plane(a_words, depth, pixels)
+    b = O.unpack_plane(b_words, depth, pixels)
+    return int(np.max(np.abs(a - b)))
+
+
+@pytest.mark.parametrize("case", ["spatial8", "temporal10", "hq_motion10"])
+def test_nlm_engine_stream_within_one_lsb(case):
+    if case == "spatial8":
+        W, H, depth, frames = 80, 48, 8, 3
+        opt = avd.nlmeans_options(search_radius=7, patch_radius=2)
+        spec = O.NlmSpec(0, 7, 2, 1.2, 1.0)
+        sig = 10.0
+    elif case == "temporal10":
+        W, H, depth, frames = 72, 40, 10, 5
+        opt = avd.nlmeans_options(temporal=1, temporal_radius=2, search_radius=4, patch_radius=3)
+        spec = O.NlmSpec(2, 4, 3, 1.2, 1.0)
+        sig = 12.0
+    else:
+        W, H, depth, frames = 80, 64, 10, 6
+        sigma = 12.0 / 1023
+        opt = avd.nlmeans_options(hq=1, temporal=1, temporal_radius=2, search_radius=3, patch_radius=2, sigma=sigma,
+                                  motion_compensation=1, motion={"estimation": 1})
+        spec = O.NlmSpec(2, 3, 2, O.HQ_STRENGTH_LUMA[2], 1.0, hq=True, sigma=sigma,
+                         motion=O.MotionGeom(0, 0, 16, 8, 4, 2))
+        sig = 12.0
+    clip = avd.synth.clip(frames, W, H, depth, 9, chroma="gray", sigma_luma=sig)
+    codes = [f[0].numpy() for f in clip]
+    eng = avd.build_engine("nlmeans", gray_geometry(W, H, depth), luma=opt)
+    got = run_engine(eng, [[words_dev(c, depth)] for c in codes], depth, [(W, H)])
+    want, _ = O.run_stream(O.NlmPass(spec, W, H, 1, depth, depth), [[O.pack_plane(c, depth)] for c in codes])
+    assert len(got) == len(want) == frames
+    for i in range(frames):
+        assert _max_lsb(got[i][0], want[i][0], depth, W * H) <= 1, i
+
+
+def test_engine_protocol_errors():
+    W, H, depth = 32, 24, 8
+    opt = avd.nlmeans_options()
+    eng = avd.build_engine("nlmeans", gray_geometry(W, H, depth), luma=opt)
+    plane = words_dev(np.full((H, W), 100), depth)
+    out = torch.zeros_like(plane)
+    with pytest.raises(avd.NothingToEmit):
+        eng.emit_into([out])
+    assert eng.push([plane]) == 1
+    with pytest.raises(avd.OutputsPending):
+        eng.push([plane])
+    eng.emit_into([out])
+    with pytest.raises(avd.ContextAfterPush):
+        eng.push_context([plane])
+    with pytest.raises(avd.PlaneMismatch):
+        eng.push([torch.zeros(8, dtype=torch.uint8, device="cuda")])
+    eng.fail_for_test()
+    with pytest.raises(avd.NeedsReset):
+        eng.push([plane])
+    eng.reset()
+    assert eng.push([plane]) == 1
+    eng.emit_into([out])
+    assert np.array_equal(words_host(out), words_host(plane))
