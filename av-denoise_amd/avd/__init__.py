@@ -162,6 +162,18 @@ class Engine:
         check(self._lib.avd_engine_max_held_frames(self._h, ctypes.byref(v)))
         return v.value
 
+    def sample_device_time(self, every: int) -> None:
+        """Samples device time of every frame (1) or of none (0) (Engine::sample_device_time)."""
+        check(self._lib.avd_engine_sample_device_time(self._h, every))
+
+    def timing(self) -> dict:
+        """Per-stage device milliseconds, launches and work since sampling began (Engine::timing)."""
+        from ._lib import Timing
+
+        t = Timing()
+        check(self._lib.avd_engine_timing(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
     def fail_for_test(self) -> None:
         check(self._lib.avd_engine_fail_for_test(self._h))
 

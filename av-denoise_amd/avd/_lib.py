@@ -79,6 +79,20 @@ class WindowSpan(ctypes.Structure):
     _fields_ = [("behind", ctypes.c_uint64), ("ahead", ctypes.c_uint64), ("edges", u32)]
 
 
+STAGES = ["ingest", "motion", "collab_search", "collab_filter", "normalise", "egress", "nlm_pair", "nlm_single",
+          "nlm_finish", "warp"]
+
+
+class Timing(ctypes.Structure):
+    """avd_timing: device milliseconds, launches and algorithmic work per stage."""
+    _fields_ = [("milliseconds", ctypes.c_double * len(STAGES)), ("launches", ctypes.c_uint64 * len(STAGES)),
+                ("work", ctypes.c_uint64 * len(STAGES))]
+
+    def as_dict(self):
+        return {name: {"ms": self.milliseconds[i], "launches": int(self.launches[i]), "work": int(self.work[i])}
+                for i, name in enumerate(STAGES)}
+
+
 class CollabPass(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("channels", u32), ("step", u32), ("spatial_radius", u32),
                 ("temporal_radius", u32), ("refine", u32), ("group_size", u32), ("blk_step", u32),
@@ -165,6 +179,8 @@ SIGNATURES = {
     "avd_engine_max_held_frames": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64)]),
     "avd_engine_synchronize": (ctypes.c_int, [P]),
     "avd_engine_fail_for_test": (ctypes.c_int, [P]),
+    "avd_engine_sample_device_time": (ctypes.c_int, [P, u32]),
+    "avd_engine_timing": (ctypes.c_int, [P, ctypes.POINTER(Timing)]),
     "avd_site_ingest": (ctypes.c_int, [P, P, P, P, u32, u32, SampleFormat, P]),
     "avd_site_egress": (ctypes.c_int, [P, P, P, u32, u32, SampleFormat, P]),
     "avd_site_collab_search": (ctypes.c_int, [ctypes.POINTER(CollabPass), P, P, P, P, P, P, P]),

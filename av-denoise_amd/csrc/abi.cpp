@@ -203,6 +203,18 @@ avd_status avd_engine_fail_for_test(avd_engine *e)
     return guard([&] { e->impl.fail_for_test(); });
 }
 
+avd_status avd_engine_sample_device_time(avd_engine *e, uint32_t every)
+{
+    if (!e) return fail(AVD_ERR_NULL_ARGUMENT, "engine is null");
+    return guard([&] { e->impl.sample_device_time(every); });
+}
+
+avd_status avd_engine_timing(avd_engine *e, avd_timing *out)
+{
+    if (!e || !out) return fail(AVD_ERR_NULL_ARGUMENT, "engine or out is null");
+    return guard([&] { *out = e->impl.timing(); });
+}
+
 // ---- launch sites ----------------------------------------------------------------------------
 
 avd_status avd_site_ingest(const void *plane0, const void *plane1, float *slot, uint16_t *codes, uint32_t pixels,

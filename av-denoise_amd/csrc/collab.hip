@@ -144,7 +144,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
                                                                const uint32_t *__restrict__ nslots,
                                                                const int32_t *__restrict__ mv,
                                                                const float *__restrict__ conf,
-                                                               uint32_t *__restrict__ members)
+                                                               uint32_t *__restrict__ members,
+                                                               unsigned long long *__restrict__ scored_count)
 {
     __shared__ int s_rect[kWaves][16][5];  // left, right, top, bottom, positions (0 when skipped)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -171,9 +172,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
     uint64_t best8[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) best8[i] = kNoKey;
+    uint32_t scored = 0;  // candidate patches this lane scores, for the stage's work count
     for (int idx = lane; idx < n_live; idx += 64) {
         const int y = top + idx / rw, x = left + idx % rw;
-        const uint32_t d = (x == rx && y == ry) ? 0u : dist_key<C, CODES>(cc, cc, cv, cv, W, rx, ry, x, y);
+        const bool self = x == rx && y == ry;
+        scored += self ? 0u : 1u;
+        const uint32_t d = self ? 0u : dist_key<C, CODES>(cc, cc, cv, cv, W, rx, ry, x, y);
         const uint64_t key = ((uint64_t)d << 16) | (uint64_t)idx;
         if (key < best8[7]) insert8(best8, key);
     }
@@ -280,6 +284,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
                         for (int k = 0; k < 8; ++k)
                             if (k < held_n && gpos[k] == packed) skipped = true;
                         if (skipped) continue;
+                        ++scored;
                         const uint32_t d = dist_key<C, CODES>(cc, nc, cv, nv, W, ax, ay, cx, cy);
                         const uint64_t key = ((uint64_t)d << 32) | (uint32_t)(walk + i);
                         best = key < best ? key : best;
@@ -328,6 +333,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
             for (int m = 0; m < 8; ++m) mem[m] = gpos[m];
             k_use = 8;
         }
+    }
+    if (scored_count) {
+        uint32_t total = live ? scored : 0u;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) total += (uint32_t)__shfl_xor((int)total, off);
+        if (lane == 0) atomicAdd(scored_count, (unsigned long long)total * C);
     }
     if (live && lane < 9) {
         uint32_t word = k_use | (use_grid ? 0x100u : 0u);
@@ -608,16 +619,16 @@ bool pass_ok(const avd_collab_pass &p)
 
 hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
                                 const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
-                                hipStream_t s)
+                                hipStream_t s, unsigned long long *scored_count)
 {
     if (!pass_ok(p)) return hipErrorInvalidValue;
     const unsigned g = div_up(ref_count(p), kWaves);
     if (codes) {
-        if (p.channels == 1) k_collab_search<1, true><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members);
-        else k_collab_search<2, true><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members);
+        if (p.channels == 1) k_collab_search<1, true><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
+        else k_collab_search<2, true><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
     } else {
-        if (p.channels == 1) k_collab_search<1, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members);
-        else k_collab_search<2, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members);
+        if (p.channels == 1) k_collab_search<1, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
+        else k_collab_search<2, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
     }
     return hipGetLastError();
 }

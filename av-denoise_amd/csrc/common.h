@@ -54,9 +54,10 @@ hipError_t launch_warp(const float *src, float *dst, const int32_t *mv, int W, i
                        int blocks_x, int blocks_y, hipStream_t s);
 
 // ---- collab.hip -------------------------------------------------------------------------
+// `scored_count`, when non-null, gains the candidate patches the launch scores times the channels.
 hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
                                 const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
-                                hipStream_t s);
+                                hipStream_t s, unsigned long long *scored_count = nullptr);
 hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
                                 const uint32_t *members, int32_t *accum, int32_t *wsum, float *group_weight,
                                 hipStream_t s);

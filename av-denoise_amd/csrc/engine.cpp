@@ -157,6 +157,88 @@ float weight_scale(float sigma, float g_max)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Stage timing
+// ---------------------------------------------------------------------------------------------
+
+StageTimer::~StageTimer()
+{
+    for (auto &p : pending_) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+}
+
+void StageTimer::set_sampling(bool on, hipStream_t s)
+{
+    harvest();
+    on_ = on;
+    for (int i = 0; i < AVD_STAGE_COUNT; ++i) {
+        ms_[i] = 0.0;
+        launches_[i] = 0;
+        work_[i] = 0;
+    }
+    if (on) {
+        if (!counter_.ptr) counter_.allocate(sizeof(unsigned long long) * 2);
+        check_hip(launch_fill_u32(counter_.ptr, 0, 2, s), "zero counter");
+    }
+}
+
+int StageTimer::begin(int stage, hipStream_t s)
+{
+    if (!on_) return -1;
+    Pending p{stage, nullptr, nullptr, 0};
+    check_hip(hipEventCreate(&p.start), "hipEventCreate");
+    check_hip(hipEventCreate(&p.stop), "hipEventCreate");
+    check_hip(hipEventRecord(p.start, s), "hipEventRecord");
+    pending_.push_back(p);
+    return (int)pending_.size() - 1;
+}
+
+void StageTimer::end(int handle, uint64_t work, hipStream_t s)
+{
+    if (handle < 0) return;
+    Pending &p = pending_[(size_t)handle];
+    p.work = work;
+    check_hip(hipEventRecord(p.stop, s), "hipEventRecord");
+}
+
+void StageTimer::harvest()
+{
+    for (auto &p : pending_) {
+        check_hip(hipEventSynchronize(p.stop), "hipEventSynchronize");
+        float ms = 0.0f;
+        check_hip(hipEventElapsedTime(&ms, p.start, p.stop), "hipEventElapsedTime");
+        ms_[p.stage] += ms;
+        launches_[p.stage] += 1;
+        work_[p.stage] += p.work;
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    pending_.clear();
+}
+
+void StageTimer::fold_counter(hipStream_t s)
+{
+    if (!on_ || !counter_.ptr) return;
+    unsigned long long v = 0;
+    check_hip(hipMemcpyAsync(&v, counter_.ptr, sizeof(v), hipMemcpyDeviceToHost, s), "read counter");
+    check_hip(hipStreamSynchronize(s), "read counter");
+    work_[AVD_STAGE_COLLAB_SEARCH] += v;
+    check_hip(launch_fill_u32(counter_.ptr, 0, 2, s), "zero counter");
+}
+
+avd_timing StageTimer::totals() const
+{
+    avd_timing t;
+    for (int i = 0; i < AVD_STAGE_COUNT; ++i) {
+        t.milliseconds[i] = ms_[i];
+        t.launches[i] = launches_[i];
+        t.work[i] = work_[i];
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Frame ring shared by both passes (nlmeans/denoiser/ring.rs)
 // ---------------------------------------------------------------------------------------------
 
@@ -324,7 +406,10 @@ void NlmPass::reset_stream()
 
 void NlmPass::ingest(const avd_device_plane *planes)
 {
-    ring_.write(planes, in_, stream_);
+    timed(AVD_STAGE_INGEST, ring_.px, stream_, "ingest", [&] {
+        ring_.write(planes, in_, stream_);
+        return hipSuccess;
+    });
     if (ring_.real == 1 && r_ > 0) {  // prime_leading_edge_if_first (ring.rs:106-125)
         for (uint32_t i = 0; i < r_; ++i) {
             ring_.duplicate_last(stream_);
@@ -364,17 +449,19 @@ void NlmPass::denoise()
         for (int k = -(int)r_; k <= (int)r_; ++k)
             if (k != 0) slots.push_back(ring_.phys((int)r_ + k));
         const float mx = max_value(in_);
-        check_hip(launch_motion(mg_, ring_.code_ptr(), ring_.l1_ptr(), (int)centre, ring_.neighbours_of(r_),
-                                (int)slots.size(), mv_.as<int32_t>(), conf_.as<float>(), mx,
-                                sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), stream_),
-                  "motion");
         const size_t blocks = (size_t)mg_.blocks_x * mg_.blocks_y;
+        timed(AVD_STAGE_MOTION, blocks * slots.size(), stream_, "motion", [&] {
+            return launch_motion(mg_, ring_.code_ptr(), ring_.l1_ptr(), (int)centre, ring_.neighbours_of(r_),
+                                 (int)slots.size(), mv_.as<int32_t>(), conf_.as<float>(), mx,
+                                 sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), stream_);
+        });
         for (size_t i = 0; i < slots.size(); ++i) {
             const int k = (int)i < (int)r_ ? (int)i - (int)r_ : (int)i - (int)r_ + 1;
             float *dst = comp_.as<float>() + (size_t)slots[i] * px * C;
-            check_hip(launch_warp(ring_.value_slot(slots[i]), dst, mv_.as<int32_t>() + i * blocks * 2, W_, H_, C,
-                                  mg_.step, mg_.blocks_x, mg_.blocks_y, stream_),
-                      "warp");
+            timed(AVD_STAGE_WARP, px, stream_, "warp", [&] {
+                return launch_warp(ring_.value_slot(slots[i]), dst, mv_.as<int32_t>() + i * blocks * 2, W_, H_, C,
+                                   mg_.step, mg_.blocks_x, mg_.blocks_y, stream_);
+            });
             nb[k + r_] = dst;
         }
     }
@@ -393,17 +480,21 @@ void NlmPass::denoise()
         const int idx = k < 0 ? k + (int)r_ : k + (int)r_ - 1;
         return conf_.as<float>() + (size_t)idx * blocks;
     };
+    const uint64_t window = (uint64_t)(2 * S_ + 1) * (2 * S_ + 1);
     for (int qk = -(int)r_; qk < 0; ++qk)
-        check_hip(launch_nlm_pair(w, cf, nb[qk + r_], nb[-qk + r_], conf_of(qk), conf_of(-qk), accum_.as<float>(),
-                                  wsum_.as<float>(), maxw_.as<float>(), stream_),
-                  "nlm pair");
-    check_hip(launch_nlm_single(w, cf, lut_.as<float>(), accum_.as<float>(), wsum_.as<float>(), maxw_.as<float>(),
-                                stream_),
-              "nlm single");
+        timed(AVD_STAGE_NLM_PAIR, px * window * 2, stream_, "nlm pair", [&] {
+            return launch_nlm_pair(w, cf, nb[qk + r_], nb[-qk + r_], conf_of(qk), conf_of(-qk), accum_.as<float>(),
+                                   wsum_.as<float>(), maxw_.as<float>(), stream_);
+        });
+    timed(AVD_STAGE_NLM_SINGLE, px * (window - 1), stream_, "nlm single", [&] {
+        return launch_nlm_single(w, cf, lut_.as<float>(), accum_.as<float>(), wsum_.as<float>(), maxw_.as<float>(),
+                                 stream_);
+    });
     float *out = outputs_[next_output_].as<float>();
-    check_hip(launch_nlm_finish(cf, accum_.as<float>(), wsum_.as<float>(), maxw_.as<float>(), self_weight_,
-                                (uint32_t)px, C, out, stream_),
-              "nlm finish");
+    timed(AVD_STAGE_NLM_FINISH, px, stream_, "nlm finish", [&] {
+        return launch_nlm_finish(cf, accum_.as<float>(), wsum_.as<float>(), maxw_.as<float>(), self_weight_,
+                                 (uint32_t)px, C, out, stream_);
+    });
     ready_output_ = next_output_;
     next_output_ = (next_output_ + 1) % 2;
 }
@@ -412,8 +503,9 @@ void NlmPass::egress(const float *frame, const avd_device_plane *planes)
 {
     uint32_t *p0 = static_cast<uint32_t *>(planes[0].data);
     uint32_t *p1 = C_ > 1 ? static_cast<uint32_t *>(planes[1].data) : p0;
-    check_hip(launch_egress(frame, p0, p1, W_ * H_, C_, (int)samples_per_word(out_), max_value(out_), stream_),
-              "egress");
+    timed(AVD_STAGE_EGRESS, (uint64_t)W_ * H_, stream_, "egress", [&] {
+        return launch_egress(frame, p0, p1, W_ * H_, C_, (int)samples_per_word(out_), max_value(out_), stream_);
+    });
 }
 
 void NlmPass::emit(const avd_device_plane *planes)
@@ -561,18 +653,23 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
     }
     std::fill(zeroed_.begin(), zeroed_.end(), false);
     const uint32_t *nslots = ring_.neighbours_of(centre_logical);
-    check_hip(launch_motion(mg_, ring_.code_ptr(), ring_.l1_ptr(), (int)centre, nslots, (int)(N_ - 1),
-                            mv_.as<int32_t>(), conf_.as<float>(), max_value(in_),
-                            sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), stream_),
-              "motion");
+    const uint64_t blocks = (uint64_t)mg_.blocks_x * mg_.blocks_y * (N_ - 1);
+    timed(AVD_STAGE_MOTION, blocks, stream_, "motion", [&] {
+        return launch_motion(mg_, ring_.code_ptr(), ring_.l1_ptr(), (int)centre, nslots, (int)(N_ - 1),
+                             mv_.as<int32_t>(), conf_.as<float>(), max_value(in_),
+                             sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), stream_);
+    });
     avd_collab_pass p = pass_;
     p.centre_slot = centre;
-    check_hip(launch_collab_search(p, ring_.code_ptr(), ring_.value_ptr(), nslots, mv_.as<int32_t>(),
-                                   conf_.as<float>(), members_.as<uint32_t>(), stream_),
-              "collab search");
-    check_hip(launch_collab_filter(p, ring_.value_ptr(), nslots, members_.as<uint32_t>(),
-                                   accum_.as<int32_t>(), wsum_.as<int32_t>(), group_weight_.as<float>(), stream_),
-              "collab filter");
+    unsigned long long *counter = timer_ ? timer_->scored_counter() : nullptr;
+    timed(AVD_STAGE_COLLAB_SEARCH, 0, stream_, "collab search", [&] {
+        return launch_collab_search(p, ring_.code_ptr(), ring_.value_ptr(), nslots, mv_.as<int32_t>(),
+                                    conf_.as<float>(), members_.as<uint32_t>(), stream_, counter);
+    });
+    timed(AVD_STAGE_COLLAB_FILTER, refs_, stream_, "collab filter", [&] {
+        return launch_collab_filter(p, ring_.value_ptr(), nslots, members_.as<uint32_t>(), accum_.as<int32_t>(),
+                                    wsum_.as<int32_t>(), group_weight_.as<float>(), stream_);
+    });
     ++passes_;
 }
 
@@ -597,7 +694,10 @@ uint32_t Nl4dPass::submit()
 
 uint32_t Nl4dPass::push(const avd_device_plane *planes)
 {
-    ring_.write(planes, in_, stream_);
+    timed(AVD_STAGE_INGEST, ring_.px, stream_, "ingest", [&] {
+        ring_.write(planes, in_, stream_);
+        return hipSuccess;
+    });
     submit();
     return (uint32_t)pending_.size();
 }
@@ -621,15 +721,17 @@ void Nl4dPass::emit(const avd_device_plane *planes)
     emitted_frame_ = ring_.slot_frames[slot];
     const size_t px = (size_t)W_ * H_;
     // read_region + normalise_region (nl4d/denoiser.rs:402-410, :720-760): the region is left zeroed.
-    check_hip(launch_collab_normalise(accum_.as<int32_t>() + slot * px * C_, wsum_.as<int32_t>() + slot * px,
-                                      output_.as<float>(), (uint32_t)px, C_, stream_),
-              "normalise");
+    timed(AVD_STAGE_NORMALISE, px, stream_, "normalise", [&] {
+        return launch_collab_normalise(accum_.as<int32_t>() + slot * px * C_, wsum_.as<int32_t>() + slot * px,
+                                       output_.as<float>(), (uint32_t)px, C_, stream_);
+    });
     zeroed_[slot] = true;
     uint32_t *p0 = static_cast<uint32_t *>(planes[0].data);
     uint32_t *p1 = C_ > 1 ? static_cast<uint32_t *>(planes[1].data) : p0;
-    check_hip(launch_egress(output_.as<float>(), p0, p1, (uint32_t)px, C_, (int)samples_per_word(out_), max_value(out_),
-                            stream_),
-              "egress");
+    timed(AVD_STAGE_EGRESS, px, stream_, "egress", [&] {
+        return launch_egress(output_.as<float>(), p0, p1, (uint32_t)px, C_, (int)samples_per_word(out_),
+                             max_value(out_), stream_);
+    });
     if (tail_emitting_ && tail_.empty() && pending_.empty()) {
         tail_emitting_ = false;
         reset_stream();
@@ -711,6 +813,8 @@ Engine::Engine(const avd_engine_spec &spec) : spec_(spec)
         if (spec.clean_chroma)
             chroma_.reset(new Nl4dPass(spec.nl4d_chroma, chroma_w_, chroma_h_, 2, g.input, g.output, stream_));
     }
+    if (luma_) luma_->set_timer(&timer_);
+    if (chroma_) chroma_->set_timer(&timer_);
     if (luma_ && chroma_ && luma_->max_held() != chroma_->max_held())
         throw Failure(AVD_ERR_INVALID_OPTIONS, "the luma and chroma passes read different window spans");
     if (passes_through) {
@@ -874,6 +978,23 @@ avd_window_span Engine::window_span() const
     w.behind = w.ahead = max_held_frames();
     w.edges = spec_.kind == AVD_ENGINE_NL4D ? 1u : 0u;
     return w;
+}
+
+void Engine::sample_device_time(uint32_t every)
+{
+    guarded([&] {
+        timer_.set_sampling(every != 0, stream_);
+        return 0;
+    });
+}
+
+avd_timing Engine::timing()
+{
+    return guarded([&] {
+        timer_.fold_counter(stream_);
+        timer_.harvest();
+        return timer_.totals();
+    });
 }
 
 void Engine::fail_for_test()

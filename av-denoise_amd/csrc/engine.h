@@ -80,6 +80,37 @@ struct Ring {
     const int32_t *l1_ptr() const { return pyr1.as<int32_t>(); }
 };
 
+// Device time per stage, bracketed by HIP events on the engine's stream while sampling is on
+// (the reference's DeviceTimer, engine/timing.rs:13-200).  Passes share their engine's timer.
+class StageTimer {
+  public:
+    ~StageTimer();
+    void set_sampling(bool on, hipStream_t s);
+    bool on() const { return on_; }
+    // Records the start of a launch of `stage` and returns its handle (or -1 when not sampling).
+    int begin(int stage, hipStream_t s);
+    void end(int handle, uint64_t work, hipStream_t s);
+    void add_work(int stage, uint64_t work) { work_[stage] += work; }
+    // Waits for every pending launch and folds it into the totals.
+    void harvest();
+    avd_timing totals() const;
+    unsigned long long *scored_counter() const { return on_ ? counter_.as<unsigned long long>() : nullptr; }
+    void fold_counter(hipStream_t s);
+
+  private:
+    struct Pending {
+        int stage;
+        hipEvent_t start, stop;
+        uint64_t work;
+    };
+    bool on_ = false;
+    std::vector<Pending> pending_;
+    double ms_[AVD_STAGE_COUNT] = {};
+    uint64_t launches_[AVD_STAGE_COUNT] = {};
+    uint64_t work_[AVD_STAGE_COUNT] = {};
+    DeviceBuffer counter_;
+};
+
 // One plane group's denoiser (the reference's PassEngine + FramePass).
 class Pass {
   public:
@@ -92,9 +123,19 @@ class Pass {
     virtual size_t max_held() const = 0;
     virtual bool has_pending() const = 0;
     uint32_t emitted_frame() const { return emitted_frame_; }
+    void set_timer(StageTimer *t) { timer_ = t; }
 
   protected:
+    // Runs `launch` (returning hipError_t) as one timed launch of `stage`.
+    template <typename F>
+    void timed(int stage, uint64_t work, hipStream_t s, const char *what, F &&launch)
+    {
+        const int h = timer_ ? timer_->begin(stage, s) : -1;
+        check_hip(launch(), what);
+        if (timer_) timer_->end(h, work, s);
+    }
     uint32_t emitted_frame_ = 0;
+    StageTimer *timer_ = nullptr;
 };
 
 class NlmPass : public Pass {
@@ -177,6 +218,8 @@ class Engine {
     size_t max_held_frames() const;
     avd_window_span window_span() const;
     void fail_for_test();
+    void sample_device_time(uint32_t every);
+    avd_timing timing();
 
   private:
     Pass &lead();
@@ -191,6 +234,7 @@ class Engine {
 
     avd_engine_spec spec_;
     hipStream_t stream_ = nullptr;
+    StageTimer timer_;
     std::unique_ptr<Pass> luma_, chroma_;
     uint32_t planes_ = 1, chroma_w_ = 0, chroma_h_ = 0;
     DeviceBuffer passthrough_;

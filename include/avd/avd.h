@@ -206,6 +206,36 @@ avd_status avd_engine_synchronize(avd_engine *engine);
 /* Poisons the engine as a GPU failure would (fail_through_guard_for_test, nlmeans/engine.rs:195). */
 avd_status avd_engine_fail_for_test(avd_engine *engine);
 
+/* Device time per stage (EngineTiming / DeviceTimer, engine/timing.rs:13-200): with sampling on,
+ * every launch of a stage is bracketed by HIP events on the engine's stream.  `work` counts the
+ * stage's algorithmic units: candidate patches scored times channels for AVD_STAGE_COLLAB_SEARCH, references
+ * filtered for AVD_STAGE_COLLAB_FILTER, blocks x neighbours for AVD_STAGE_MOTION, pixels for the
+ * rest. */
+enum {
+    AVD_STAGE_INGEST = 0,
+    AVD_STAGE_MOTION = 1,
+    AVD_STAGE_COLLAB_SEARCH = 2,
+    AVD_STAGE_COLLAB_FILTER = 3,
+    AVD_STAGE_NORMALISE = 4,
+    AVD_STAGE_EGRESS = 5,
+    AVD_STAGE_NLM_PAIR = 6,
+    AVD_STAGE_NLM_SINGLE = 7,
+    AVD_STAGE_NLM_FINISH = 8,
+    AVD_STAGE_WARP = 9,
+    AVD_STAGE_COUNT = 10
+};
+
+typedef struct avd_timing {
+    double milliseconds[AVD_STAGE_COUNT];
+    uint64_t launches[AVD_STAGE_COUNT];
+    uint64_t work[AVD_STAGE_COUNT];
+} avd_timing;
+
+/* Samples device time of every frame (every = 1) or of none (every = 0) (sample_device_time). */
+avd_status avd_engine_sample_device_time(avd_engine *engine, uint32_t every);
+/* Waits for the sampled launches and returns the totals since sampling began (Engine::timing). */
+avd_status avd_engine_timing(avd_engine *engine, avd_timing *out);
+
 /* ---- launch sites: one stage of one pass over caller-owned device buffers ----------------- */
 
 /* Plane ingest (engine/kernels.rs:25-63): u32 words of `channels` planes into an interleaved f32
