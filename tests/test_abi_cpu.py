@@ -1,0 +1,94 @@
+"""The C ABI without a GPU: the library loads, exports every declared symbol, and validates
+options, geometry and presets before touching the device."""
+import re
+import os
+
+import pytest
+
+import avd
+from avd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "avd", "avd.h")).read()
+    return sorted(set(re.findall(r"\b(avd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    lib = avd.load()
+    names = declared_symbols()
+    assert len(names) >= 30
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in L.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_abi_version_and_defaults():
+    lib = avd.load()
+    assert lib.avd_abi_version() == 1
+    m = avd.motion_search()
+    assert (m.blksize, m.overlap, m.search_radius, m.pyramid_levels) == (16, 8, 4, 2)
+    o = avd.nl4d_options()
+    assert (o.refine, o.spatial_radius, o.temporal_radius, o.step, o.group_size) == (2, 9, 2, 4, 8)
+    assert abs(o.c_min - 0.05) < 1e-7 and abs(o.kaiser_beta - 2.0) < 1e-7
+    assert o.noise_map == 1 and o.pooled_threshold == 1 and o.has_sigma == 0
+    n = avd.nlmeans_options()
+    assert n.hq == 0 and n.auto_strength == 1 and n.temporal_confidence == 1
+
+
+@pytest.mark.parametrize("preset,radius,search,hq", [("veryfast", 0, 2, 0), ("fast", 1, 2, 1), ("base", 2, 2, 1),
+                                                      ("slow", 4, 4, 1), ("veryslow", 8, 4, 1)])
+def test_nlmeans_preset_ladder(preset, radius, search, hq):
+    o = avd.nlmeans_options_for_preset(preset)
+    assert (o.temporal_radius, o.search_radius, o.hq) == (radius, search, hq)
+    assert o.temporal == (1 if radius else 0)
+    assert o.motion.max_search_escalation == (8 if preset in ("slow", "veryslow") else 4)
+
+
+def test_nl4d_presets():
+    assert avd.nl4d_options_for_preset("base").motion.max_search_escalation == 4
+    assert avd.nl4d_options_for_preset("slow").motion.max_search_escalation == 8
+    for bad in ("veryfast", "fast", "veryslow"):
+        with pytest.raises(avd.InvalidOptions):
+            avd.nl4d_options_for_preset(bad)
+
+
+def test_plane_bytes():
+    assert avd.plane_bytes(avd.SampleFormat.u8(), 5, 3) == 16
+    assert avd.plane_bytes(avd.SampleFormat.u16(10), 5, 3) == 32
+    assert avd.plane_bytes(avd.SampleFormat.u16(10), 3840, 2160) == 3840 * 2160 * 2
+
+
+def geom(chroma=0, fmt_in=None, fmt_out=None, w=64, h=48):
+    f = fmt_in or avd.SampleFormat.u16(10)
+    return avd.Geometry(w, h, chroma, f, fmt_out or f, 0)
+
+
+@pytest.mark.parametrize("kind,luma,chroma,g,err", [
+    ("nl4d", dict(temporal_radius=0, sigma=0.02), None, geom(), avd.InvalidOptions),
+    ("nl4d", dict(refine=5, sigma=0.02), None, geom(), avd.InvalidOptions),
+    ("nl4d", dict(spatial_radius=17, sigma=0.02), None, geom(), avd.InvalidOptions),
+    ("nl4d", dict(c_min=1.0, sigma=0.02), None, geom(), avd.InvalidOptions),
+    ("nl4d", dict(kaiser_beta=9.0, sigma=0.02), None, geom(), avd.InvalidOptions),
+    ("nl4d", dict(lambda_ht_scale=20.0, sigma=0.02), None, geom(), avd.InvalidOptions),
+    ("nl4d", dict(pooled_threshold=0, noise_map=0), None, geom(), avd.InvalidOptions),  # no sigma
+    ("nl4d", dict(sigma=0.02, pooled_threshold=0), dict(sigma=0.02), geom(0), avd.InvalidGeometry),  # gray chroma
+    ("nl4d", dict(sigma=0.02, pooled_threshold=0), None, geom(0, avd.SampleFormat.u16(16)), avd.InvalidGeometry),
+    ("nl4d", dict(sigma=0.02, pooled_threshold=0), None, geom(0, w=7, h=7), avd.InvalidGeometry),
+    ("nlmeans", dict(patch_radius=17), None, geom(), avd.InvalidOptions),
+    ("nlmeans", dict(search_radius=9), None, geom(), avd.InvalidOptions),
+    ("nlmeans", dict(hq=1), None, geom(), avd.InvalidOptions),  # HQ needs a fixed sigma here
+    ("nlmeans", dict(), None, geom(0, avd.SampleFormat(2, 0)), avd.InvalidGeometry),  # F32 planes
+    ("nlmeans", dict(), None, geom(1, avd.SampleFormat.u8(), avd.SampleFormat.u16(10)), avd.InvalidGeometry),
+])
+def test_engine_validation_errors(kind, luma, chroma, g, err):
+    mk = avd.nl4d_options if kind == "nl4d" else avd.nlmeans_options
+    with pytest.raises(err):
+        avd.build_engine(kind, g, luma=mk(**luma), chroma=mk(**chroma) if chroma is not None else None)
+
+
+def test_a_spec_must_clean_something():
+    with pytest.raises(avd.InvalidOptions):
+        avd.build_engine("nlmeans", geom(1), luma=None, chroma=None)
