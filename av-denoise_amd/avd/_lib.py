@@ -98,7 +98,8 @@ class CollabPass(ctypes.Structure):
                 ("temporal_radius", u32), ("refine", u32), ("group_size", u32), ("blk_step", u32),
                 ("blksize", u32), ("blocks_x", u32), ("blocks_y", u32), ("centre_slot", u32), ("slots", u32),
                 ("c_min", f32), ("lam", f32 * 2), ("sigma", f32 * 2), ("weight_scale", f32),
-                ("accum_scale", f32), ("kaiser", f32 * 8), ("profile", f32 * 8)]
+                ("accum_scale", f32), ("kaiser", f32 * 8), ("profile", f32 * 8),
+                ("code_bits", u32)]
 
 
 class MotionPass(ctypes.Structure):

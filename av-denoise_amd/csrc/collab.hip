@@ -9,7 +9,7 @@
 // frame and positions the group already holds, keeps the best per frame and the best grid_frames-1
 // frames per volume.  The group word is the member count with 0x100 set for a volume grid.
 //
-// Distances: with u16 codes the exact integer SSD (packed 16-bit differences into v_dot2_i32_i16),
+// Distances: with u16 codes the exact integer SSD (packed 16-bit products, v_dot2_u32_u16),
 // with f32 values the reference's F32Search form (per-column sums of squared differences, the
 // 8-lane butterfly over columns, times the channel scale).
 //
@@ -350,6 +350,557 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
 }
 
 // ---------------------------------------------------------------------------------------------
+// collab_search on u16 codes: exact integer SSD with column sliding.
+//
+// Every lane holds the reference (anchor) patch in registers.  A lane owns one candidate column x
+// and a run of k consecutive candidate rows y0..y0+k-1: it streams the k+7 frame rows of that
+// column once, adds each row's packed cross products with the eight patch rows into the eight
+// open candidates (v_dot2_u32_u16) and its energy into a window sum, and hands every finished
+// candidate's SSD = |P|^2 + |B|^2 - 2<P, B> to `emit`.  Each frame row is loaded once per column
+// instead of once per candidate.
+// ---------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ short2_t as_s2(uint32_t v) { return __builtin_bit_cast(short2_t, v); }
+
+// Eight pixels starting at absolute code index `a` of the ring, as 4*C packed 16-bit pairs.
+// One-channel rows may start on an odd code: two aligned loads and a byte funnel shift.
+template <int C>
+__device__ __forceinline__ void load_row(const uint16_t *__restrict__ ring, size_t a, uint32_t (&out)[4 * C])
+{
+    if (C == 1) {
+        const uint32_t *p = reinterpret_cast<const uint32_t *>(ring + (a & ~(size_t)1));
+        const uint32_t d0 = p[0], d1 = p[1], d2 = p[2], d3 = p[3];
+        const uint32_t e = ring[a + 7];
+        const uint32_t sh = (uint32_t)(a & 1) * 2u;
+        out[0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
+        out[1] = __builtin_amdgcn_alignbyte(d2, d1, sh);
+        out[2] = __builtin_amdgcn_alignbyte(d3, d2, sh);
+        out[3] = __builtin_amdgcn_alignbyte(e, d3, sh);
+    } else {
+        const uint32_t *p = reinterpret_cast<const uint32_t *>(ring + a);
+#pragma unroll
+        for (int i = 0; i < 4 * C; ++i) out[i] = p[i];
+    }
+}
+
+typedef unsigned short ushort2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ ushort2_t as_u2(uint32_t v) { return __builtin_bit_cast(ushort2_t, v); }
+
+// Sum over the row of products of packed code pairs (v_dot2_u32_u16).
+template <int C>
+__device__ __forceinline__ uint32_t row_dot(const uint32_t (&a)[4 * C], const uint32_t (&b)[4 * C], uint32_t acc)
+{
+#pragma unroll
+    for (int i = 0; i < 4 * C; ++i) acc = __builtin_amdgcn_udot2(as_u2(a[i]), as_u2(b[i]), acc, false);
+    return acc;
+}
+
+// The 8x8 patch at (x, y) of the frame starting at code `frame`; every lane holds a copy (in
+// vector registers: scalar registers cannot hold it beside the kernel's other uniform state).
+template <int C>
+__device__ __forceinline__ void load_patch(const uint16_t *__restrict__ ring, size_t frame, int W, int x, int y,
+                                           uint32_t (&P)[8][4 * C])
+{
+#pragma unroll
+    for (int r = 0; r < 8; ++r) load_row<C>(ring, frame + ((size_t)(y + r) * W + x) * C, P[r]);
+}
+
+// Candidates (x, y0 + j) for j < K against patch P, whose energy sum(P^2) is A2; emit(j, ssd)
+// for j < kk (kk <= K).  SSD = A2 + B2 - 2 * <P, B>, exact in u32 for codes of at most 12 bits
+// (2 * 64 * C * 4095^2 < 2^32): the cross products accumulate per candidate with
+// v_dot2_u32_u16, and B2 is the window sum of per-row energies.  Candidate j uses frame row
+// y0 + t with patch row t - j, and its sums sit in slot j % 8; unrolled, so every (row, patch
+// row) pair is a compile-time slot.  Rows past the frame (a lane with kk < K) are clamped and
+// their sums dropped.
+template <int C, int K, typename Emit>
+__device__ __forceinline__ void slide_fixed(const uint16_t *__restrict__ ring, size_t frame, int W, int H, int x,
+                                            int y0, int kk, const uint32_t (&P)[8][4 * C], uint32_t A2, Emit &&emit)
+{
+    uint32_t acc[8], energy[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = 0;
+    uint32_t window = 0;
+    const size_t col = frame + (size_t)x * C;
+#pragma unroll
+    for (int t = 0; t < K + 7; ++t) {
+        uint32_t row[4 * C];
+        load_row<C>(ring, col + (size_t)min(y0 + t, H - 1) * W * C, row);
+        // Pair index outermost: consecutive dot ops feed different sums (a dependent dot op
+        // waits a cycle).
+        uint32_t e = 0;
+#pragma unroll
+        for (int i = 0; i < 4 * C; ++i) {
+            e = __builtin_amdgcn_udot2(as_u2(row[i]), as_u2(row[i]), e, false);
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (t - r >= 0 && t - r < K)
+                    acc[(t - r) & 7] = __builtin_amdgcn_udot2(as_u2(P[r][i]), as_u2(row[i]), acc[(t - r) & 7], false);
+        }
+        energy[t & 7] = e;
+        window += e;
+        if (t >= 7) {
+            if (t - 7 < kk) emit(t - 7, A2 + window - 2u * acc[(t - 7) & 7]);
+            acc[(t - 7) & 7] = 0;
+            window -= energy[(t - 7) & 7];
+        }
+    }
+}
+
+// The same for a run length k known only at run time (k up to 33 with spatial_radius 16).
+template <int C, typename Emit>
+__device__ __forceinline__ void slide_column(const uint16_t *__restrict__ ring, size_t frame, int W, int H, int x,
+                                             int y0, int k, int kk, const uint32_t (&P)[8][4 * C], uint32_t A2,
+                                             Emit &&emit)
+{
+    uint32_t acc[8], energy[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = 0;
+    uint32_t window = 0;
+    const size_t col = frame + (size_t)x * C;
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+        uint32_t row[4 * C];
+        load_row<C>(ring, col + (size_t)min(y0 + t, H - 1) * W * C, row);
+        energy[t] = row_dot<C>(row, row, 0u);
+        window += energy[t];
+#pragma unroll
+        for (int r = 0; r <= t; ++r)
+            if (t - r < k) acc[(t - r) & 7] = row_dot<C>(P[r], row, acc[(t - r) & 7]);
+    }
+    for (int t0 = 7; t0 < k + 7; t0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int t = t0 + u;  // t % 8 == (7 + u) % 8
+            if (t < k + 7) {
+                uint32_t row[4 * C];
+                load_row<C>(ring, col + (size_t)min(y0 + t, H - 1) * W * C, row);
+                uint32_t e = 0;
+#pragma unroll
+                for (int i = 0; i < 4 * C; ++i) {
+                    e = __builtin_amdgcn_udot2(as_u2(row[i]), as_u2(row[i]), e, false);
+#pragma unroll
+                    for (int r = 0; r < 8; ++r)
+                        if (t - r < k)
+                            acc[(7 + u - r) & 7] =
+                                __builtin_amdgcn_udot2(as_u2(P[r][i]), as_u2(row[i]), acc[(7 + u - r) & 7], false);
+                }
+                energy[(7 + u) & 7] = e;
+                window += e;
+                if (t - 7 < kk) emit(t - 7, A2 + window - 2u * acc[u]);
+                acc[u] = 0;
+                window -= energy[u];
+            }
+        }
+    }
+}
+
+// Dispatches a wave-uniform run length to its unrolled form.
+template <int C, typename Emit>
+__device__ __forceinline__ void slide(const uint16_t *__restrict__ ring, size_t frame, int W, int H, int x, int y0,
+                                      int k, int kk, const uint32_t (&P)[8][4 * C], uint32_t A2, Emit &&emit)
+{
+    switch (k) {
+#define AVD_SLIDE_CASE(n) \
+    case n: slide_fixed<C, n>(ring, frame, W, H, x, y0, kk, P, A2, emit); break;
+        AVD_SLIDE_CASE(1) AVD_SLIDE_CASE(2) AVD_SLIDE_CASE(3) AVD_SLIDE_CASE(4) AVD_SLIDE_CASE(5) AVD_SLIDE_CASE(6)
+        AVD_SLIDE_CASE(7) AVD_SLIDE_CASE(8) AVD_SLIDE_CASE(9) AVD_SLIDE_CASE(10) AVD_SLIDE_CASE(11)
+        AVD_SLIDE_CASE(12) AVD_SLIDE_CASE(13) AVD_SLIDE_CASE(14) AVD_SLIDE_CASE(15) AVD_SLIDE_CASE(16)
+        AVD_SLIDE_CASE(17)
+#undef AVD_SLIDE_CASE
+    default: slide_column<C>(ring, frame, W, H, x, y0, k, kk, P, A2, emit); break;
+    }
+}
+
+template <int C>
+__device__ __forceinline__ uint32_t patch_energy(const uint32_t (&P)[8][4 * C])
+{
+    uint32_t e[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) e[r] = row_dot<C>(P[r], P[r], 0u);
+    return ((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7]));
+}
+
+__device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Inserts `k` into the ascending list `a` of eight distinct keys, dropping the largest: slot i
+// becomes clamp(k, a[i-1], a[i]), one v_med3_u32 per slot.
+__device__ __forceinline__ void insert8_u32(uint32_t *a, uint32_t k)
+{
+#pragma unroll
+    for (int i = 7; i > 0; --i) a[i] = med3u(a[i - 1], k, a[i]);
+    a[0] = min(a[0], k);
+}
+
+__device__ __forceinline__ int wave_inclusive_scan(int x)
+{
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(x, off);
+        if (lane >= off) x += o;
+    }
+    return x;
+}
+
+// blockIdx -> work block, so that each XCD (blocks are dealt round-robin over eight) walks a
+// contiguous run of references and keeps their shared search windows in its own L2.
+__device__ __forceinline__ int xcd_block()
+{
+    const int n = (int)gridDim.x, b = (int)blockIdx.x;
+    const int xcd = b & 7, i = b >> 3, q = n >> 3, rem = n & 7;
+    return xcd * q + min(xcd, rem) + i;
+}
+
+constexpr int kMaxEntries = 16 * 16;  // neighbour frames x covering blocks per frame
+
+// Reference `ref`'s top-left corner on the STEP grid, the last position clamped to dim - 8.
+struct RefPos {
+    int ref, rx, ry;
+    bool live;
+};
+
+__device__ __forceinline__ RefPos ref_of_wave(const avd_collab_pass &p, int wave)
+{
+    const int W = p.width, H = p.height;
+    const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
+    const int refs = refs_x * refs_y;
+    RefPos r;
+    r.ref = xcd_block() * kWaves + wave;
+    r.live = r.ref < refs;
+    if (!r.live) r.ref = refs - 1;
+    const int iy = r.ref / refs_x, ix = r.ref - iy * refs_x;
+    r.rx = min(ix * (int)p.step, W - 8);
+    r.ry = min(iy * (int)p.step, H - 8);
+    return r;
+}
+
+// Adds this workgroup's scored candidates (times the channels) to one of kScoreSlots counters, so
+// the atomics of many workgroups do not queue on one address.  Every thread of the block calls it.
+__device__ __forceinline__ void add_scored(unsigned long long *scored_count, uint32_t scored, bool live, int C)
+{
+    __shared__ uint32_t s_sum[kWaves];
+    if (!scored_count) return;
+    uint32_t total = live ? scored : 0u;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) total += (uint32_t)__shfl_xor((int)total, off);
+    if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long sum = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) sum += s_sum[w];
+        atomicAdd(scored_count + (size_t)(blockIdx.x % kScoreSlots) * kScoreStride, sum * (unsigned long long)C);
+    }
+}
+
+// Centre search (centre.rs:37-300) of every reference: the best eight of the clipped rectangle by
+// (distance, row-major visit), the self-match pinned first.  Writes the eight positions and the
+// stack word k_use; the trajectory kernel may then replace them with a volume grid.
+// NARROW: codes of at most 10 bits and spatial_radius <= 15, so a lane's keys are
+// (distance << 5 | candidate) in 32 bits; otherwise (distance << 32 | visit index) in 64.
+template <int C, bool NARROW>
+__global__ __launch_bounds__(kWaves * 64) void k_centre_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
+                                                              uint32_t *__restrict__ members,
+                                                              unsigned long long *__restrict__ scored_count)
+{
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const RefPos rp = ref_of_wave(p, wave);
+    const int W = p.width, H = p.height, maxx = W - 8, maxy = H - 8;
+    const int rx = rp.rx, ry = rp.ry;
+    const size_t cframe = (size_t)p.centre_slot * ((size_t)W * H * C);
+    const int S = p.spatial_radius;
+
+    uint32_t P[8][4 * C];
+    load_patch<C>(ring, cframe, W, rx, ry, P);
+    const uint32_t A2 = patch_energy<C>(P);
+
+    const int left = clampi(rx - S, 0, maxx), right = clampi(rx + S, 0, maxx);
+    const int top = clampi(ry - S, 0, maxy), bottom = clampi(ry + S, 0, maxy);
+    const int rw = right - left + 1, rh = bottom - top + 1;
+    const int n_live = rw * rh;
+    const int per_col = max(1, 64 / rw);
+    const int k = (rh + per_col - 1) / per_col;
+    const int tasks = rw * ((rh + k - 1) / k);  // <= 64 for rw <= 33
+    const int col = lane % rw, y0 = top + (lane / rw) * k;
+    uint32_t L[8];
+    uint64_t LW[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        L[i] = ~0u;
+        LW[i] = kNoKey;
+    }
+    if (lane < tasks) {
+        const int x = left + col;
+        const int kk = min(k, bottom - y0 + 1);
+        slide<C>(ring, cframe, W, H, x, y0, k, kk, P, A2, [&](int j, uint32_t ssd) {
+            const bool self = x == rx && y0 + j == ry;
+            const uint32_t d = self ? 0u : ssd + 1u;
+            if (NARROW) insert8_u32(L, (d << 5) | (uint32_t)j);
+            else insert8(LW, ((uint64_t)d << 32) | (uint32_t)((y0 + j - top) * rw + col));
+        });
+    }
+    uint32_t word = 0;  // lane m < 8 keeps member m, lane 8 the group word
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        uint64_t head;
+        if (NARROW)
+            head = L[0] == ~0u ? kNoKey
+                               : (((uint64_t)(L[0] >> 5) << 32) | (uint32_t)((y0 + (int)(L[0] & 31) - top) * rw + col));
+        else
+            head = LW[0];
+        const uint64_t best = wave_min_u64(head);
+        if (best != kNoKey) {  // an unfilled slot keeps position word 0
+            const int idx = (int)(best & 0xFFFFFFFFu);
+            if (lane == m) word = pack_pos_t((uint32_t)(left + idx % rw), (uint32_t)(top + idx / rw), 0);
+            if (head == best) {
+#pragma unroll
+                for (int i = 0; i < 7; ++i) {
+                    L[i] = L[i + 1];
+                    LW[i] = LW[i + 1];
+                }
+                L[7] = ~0u;
+                LW[7] = kNoKey;
+            }
+        }
+    }
+    uint32_t k_use = 1;
+    while (k_use * 2 <= (uint32_t)n_live && k_use * 2 <= p.group_size) k_use *= 2;
+    if (lane == 8) word = k_use;
+    if (rp.live && lane < 9) members[(size_t)rp.ref * 9 + lane] = word;
+    add_scored(scored_count, lane == 0 ? (uint32_t)(n_live - 1) : 0u, rp.live, C);  // the self-match is not scored
+}
+
+// Trajectory search (trajectory.rs:142-450) and the grid decision (fused/mod.rs:407-553) of every
+// reference whose centre search is in `members`: each volume anchor is followed into every
+// neighbour frame through the refine rectangles of its covering motion blocks, skipping blocks
+// under c_min, positions inside a rectangle already searched for that frame and positions the
+// group already holds; the best per frame joins the volume.  When every volume fills and the
+// stack holds eight, the members become the volume grid.  Needs temporal_radius >= 1.
+//
+// K = 2 * refine + 1.  The rectangles of every volume are built up front (entry (v, t, b), b the
+// covering block in row-major order, padded to a power-of-two square); per volume the non-empty
+// ones are compacted and lane task k * K + col slides column col of rectangle k.  Each frame's
+// best (distance, walk index) is an LDS atomic min; lane t decodes frame t, and the frames join the
+// volume through the reference's ordered insertion.
+constexpr int kMaxRects = 512;  // volumes * neighbour frames * padded covering blocks
+
+__device__ __forceinline__ uint32_t lane_mask_lt(uint64_t bal)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+}
+
+template <int C, int K>
+__global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
+                                                                  const uint32_t *__restrict__ nslots,
+                                                                  const int32_t *__restrict__ mv,
+                                                                  const float *__restrict__ conf,
+                                                                  uint32_t *__restrict__ members,
+                                                                  unsigned long long *__restrict__ scored_count)
+{
+    __shared__ uint32_t s_rect[kWaves][kMaxRects][2];  // (left | right << 16), (top | bottom << 16)
+    __shared__ uint8_t s_list[kWaves][256];             // non-empty entries of the current volume
+    __shared__ uint32_t s_slot[kWaves][16];
+    __shared__ unsigned long long s_fbest[kWaves][16];
+    constexpr int refine = (K - 1) / 2;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const RefPos rp = ref_of_wave(p, wave);
+    const int W = p.width, H = p.height, maxx = W - 8, maxy = H - 8;
+    const size_t slot_len = (size_t)W * H * C;
+    const size_t cframe = (size_t)p.centre_slot * slot_len;
+    const uint32_t mine = members[(size_t)rp.ref * 9 + min(lane, 8)];
+    const uint32_t k_use = (uint32_t)__shfl((int)mine, 8);
+    uint32_t anchors[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) anchors[v] = (uint32_t)__shfl((int)mine, v);
+
+    const int radius = p.temporal_radius;
+    const int gf = radius == 1 ? 2 : 4;
+    const int volumes = 8 / gf, tail = gf - 1;
+    const int bstep = p.blk_step, bsz = p.blksize, bx_n = p.blocks_x, by_n = p.blocks_y;
+    const int covers = (bsz + bstep - 1) / bstep;
+    const int cshift = covers <= 1 ? 0 : (covers <= 2 ? 1 : 2);
+    const int nshift = 2 * cshift, cmask = (1 << cshift) - 1;
+    const int frames_n = 2 * radius;
+    const int per_vol = frames_n << nshift;
+    const size_t nblk = (size_t)bx_n * by_n;
+    if (lane < frames_n) s_slot[wave][lane] = nslots[lane];
+
+    // ---- every volume's rectangles.
+    for (int idx = lane; idx < volumes * per_vol; idx += 64) {
+        const int v = idx / per_vol, e = idx - v * per_vol;
+        const int t = e >> nshift, b = e & ((1 << nshift) - 1);
+        uint32_t anchor = anchors[0];
+#pragma unroll
+        for (int i = 1; i < 4; ++i)
+            if (v == i) anchor = anchors[i];
+        const int ax = (int)(anchor & 0x1FFF), ay = (int)((anchor >> 13) & 0x1FFF);
+        const int bx_hi = min(ax / bstep, bx_n - 1), by_hi = min(ay / bstep, by_n - 1);
+        const int ovx = max(ax + 8, bsz) - bsz, ovy = max(ay + 8, bsz) - bsz;
+        const int bx_lo = min((ovx + bstep - 1) / bstep, bx_hi), by_lo = min((ovy + bstep - 1) / bstep, by_hi);
+        const int wbx = bx_lo + (b & cmask), wby = by_lo + (b >> cshift);
+        uint32_t xr = 1u, yr = 1u;  // left 1 > right 0: empty
+        if (wbx <= bx_hi && wby <= by_hi) {
+            const size_t blk = (size_t)t * nblk + (size_t)wby * bx_n + wbx;
+            const float c = conf[blk];
+            const int32_t dx = mv[blk * 2], dy = mv[blk * 2 + 1];
+            if (c >= p.c_min) {
+                const int px = ax + dx, py = ay + dy;
+                xr = (uint32_t)clampi(px - refine, 0, maxx) | ((uint32_t)clampi(px + refine, 0, maxx) << 16);
+                yr = (uint32_t)clampi(py - refine, 0, maxy) | ((uint32_t)clampi(py + refine, 0, maxy) << 16);
+            }
+        }
+        s_rect[wave][idx][0] = xr;
+        s_rect[wave][idx][1] = yr;
+    }
+    wave_sync();
+
+    uint32_t scored = 0;
+    uint32_t gpos[8];
+    uint64_t gd[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        gpos[i] = 0;
+        gd[i] = kNoKey;
+    }
+    uint32_t P[8][4 * C];
+    for (int v = 0; v < volumes; ++v) {
+        const int first = v * gf;
+        uint32_t anchor = anchors[0];
+#pragma unroll
+        for (int i = 1; i < 4; ++i)
+            if (v == i) anchor = anchors[i];
+        load_patch<C>(ring, cframe, W, (int)(anchor & 0x1FFF), (int)((anchor >> 13) & 0x1FFF), P);
+        const uint32_t A2 = patch_energy<C>(P);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i == first) {
+                gpos[i] = anchor;
+                gd[i] = 0;
+            } else if (i > first && i < first + gf) {
+                gpos[i] = 0;
+                gd[i] = kNoKey;
+            }
+        }
+        const int held_n = first + 1;
+        const uint32_t(*rects)[2] = s_rect[wave] + v * per_vol;
+        // Compact the non-empty rectangles, in entry order.
+        int n_valid = 0;
+        for (int base = 0; base < per_vol; base += 64) {
+            const int e = base + lane;
+            const bool valid = e < per_vol && (rects[e][0] & 0xFFFFu) <= (rects[e][0] >> 16);
+            const uint64_t bal = __ballot(valid);
+            if (valid) s_list[wave][n_valid + lane_mask_lt(bal)] = (uint8_t)e;
+            n_valid += __popcll(bal);
+        }
+        if (lane < frames_n) s_fbest[wave][lane] = kNoKey;
+        wave_sync();
+        for (int task = lane; task < n_valid * K; task += 64) {
+            const int k = task / K, col = task - k * K;
+            const int e = s_list[wave][k];
+            const int t = e >> nshift, b = e & ((1 << nshift) - 1);
+            const int l = (int)(rects[e][0] & 0xFFFFu), r = (int)(rects[e][0] >> 16);
+            const int tp = (int)(rects[e][1] & 0xFFFFu), bt = (int)(rects[e][1] >> 16);
+            const int cx = l + col;
+            if (cx > r) continue;  // a rectangle clipped narrower than K
+            const int w = r - l + 1, h = bt - tp + 1;
+            // Positions inside an earlier rectangle of this frame, or already held by the group,
+            // are skipped; `walk` counts the positions of the earlier rectangles.
+            uint32_t skip = 0;
+            int walk = 0;
+            for (int b2 = 0; b2 < b; ++b2) {
+                const uint32_t x2 = rects[(t << nshift) + b2][0], y2 = rects[(t << nshift) + b2][1];
+                const int l2 = (int)(x2 & 0xFFFFu), r2 = (int)(x2 >> 16);
+                if (l2 > r2) continue;
+                const int tp2 = (int)(y2 & 0xFFFFu), bt2 = (int)(y2 >> 16);
+                walk += (r2 - l2 + 1) * (bt2 - tp2 + 1);
+                if (cx >= l2 && cx <= r2) {
+                    const int ylo = max(tp2, tp), yhi = min(bt2, bt);
+                    if (ylo <= yhi) skip |= ((2u << (yhi - ylo)) - 1u) << (ylo - tp);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i < held_n) {
+                    const uint32_t g = gpos[i];
+                    const int gy = (int)((g >> 13) & 0x1FFF);
+                    if ((int)(g >> 26) == t + 1 && (int)(g & 0x1FFF) == cx && gy >= tp && gy <= bt)
+                        skip |= 1u << (gy - tp);
+                }
+            }
+            scored += (uint32_t)__popc(((1u << h) - 1u) & ~skip);
+            uint64_t best = kNoKey;
+            // Every lane runs K rows of candidates; a rectangle clipped shorter drops the rest.
+            slide_fixed<C, K>(ring, (size_t)s_slot[wave][t] * slot_len, W, H, cx, tp, h, P, A2,
+                              [&](int j, uint32_t ssd) {
+                                  const uint64_t key = ((uint64_t)(ssd + 1u) << 32) | (uint32_t)(walk + j * w + col);
+                                  if (!((skip >> j) & 1u) && key < best) best = key;
+                              });
+            if (best != kNoKey) atomicMin(&s_fbest[wave][t], (unsigned long long)best);
+        }
+        wave_sync();
+        // Lane t decodes frame t's match.
+        uint64_t cd = kNoKey;
+        uint32_t cp = 0;
+        if (lane < frames_n) {
+            const uint64_t best = s_fbest[wave][lane];
+            if (best != kNoKey) {
+                int i = (int)(best & 0xFFFFFFFFu);
+                for (int b = 0; b < (1 << nshift); ++b) {
+                    const uint32_t x2 = rects[(lane << nshift) + b][0], y2 = rects[(lane << nshift) + b][1];
+                    const int l2 = (int)(x2 & 0xFFFFu), r2 = (int)(x2 >> 16);
+                    if (l2 > r2) continue;
+                    const int tp2 = (int)(y2 & 0xFFFFu), bt2 = (int)(y2 >> 16);
+                    const int w2 = r2 - l2 + 1, n2 = w2 * (bt2 - tp2 + 1);
+                    if (i < n2) {
+                        cp = pack_pos_t((uint32_t)(l2 + i % w2), (uint32_t)(tp2 + i / w2), (uint32_t)lane + 1);
+                        break;
+                    }
+                    i -= n2;
+                }
+                cd = best >> 32;
+            }
+        }
+        // Frames join the volume's ascending list in frame order; a later frame never displaces an
+        // equal one, and a displaced entry moves on past equal ones (the reference's ordered
+        // insertion, which is not a stable sort when distances tie).
+        for (int u = 0; u < frames_n; ++u) {
+            uint64_t d = shfl_u64(cd, u);
+            uint32_t q = (uint32_t)__shfl((int)cp, u);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i > first && i < first + gf && d < gd[i]) {
+                    const uint64_t hd = gd[i];
+                    const uint32_t hp = gpos[i];
+                    gd[i] = d;
+                    gpos[i] = q;
+                    d = hd;
+                    q = hp;
+                }
+            }
+        }
+        wave_sync();
+    }
+    bool use_grid = k_use == 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (i % gf == tail) use_grid = use_grid && (gd[i] != kNoKey);
+    if (use_grid && rp.live && lane < 9) {
+        uint32_t word = 8u | 0x100u;
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            if (lane == m) word = gpos[m];
+        members[(size_t)rp.ref * 9 + lane] = word;
+    }
+    add_scored(scored_count, scored, rp.live, C);
+}
+
+// ---------------------------------------------------------------------------------------------
 // collab_filter
 // ---------------------------------------------------------------------------------------------
 
@@ -611,7 +1162,7 @@ bool pass_ok(const avd_collab_pass &p)
 {
     return p.width >= 8 && p.height >= 8 && p.width < 8192 && p.height < 8192 && (p.channels == 1 || p.channels == 2) &&
            p.step >= 1 && p.step <= 8 && p.spatial_radius >= 1 && p.spatial_radius <= kMaxS && p.group_size == 8 &&
-           p.temporal_radius <= 8 && p.refine <= 4 && p.blk_step >= 1 && p.blksize >= 1 &&
+           p.temporal_radius <= 8 && p.refine >= 1 && p.refine <= 4 && p.blk_step >= 1 && p.blksize >= 1 &&
            (p.blksize + p.blk_step - 1) / p.blk_step <= 4;
 }
 
@@ -624,8 +1175,31 @@ hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes,
     if (!pass_ok(p)) return hipErrorInvalidValue;
     const unsigned g = div_up(ref_count(p), kWaves);
     if (codes) {
-        if (p.channels == 1) k_collab_search<1, true><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
-        else k_collab_search<2, true><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
+        // 32-bit keys hold (SSD + 1) << 5: SSD < 64 * C * 1023^2 + 1 < 2^27 for codes of at most
+        // 10 bits, and a lane's run of candidates stays below 32 while rw <= 32.
+        const bool narrow = p.code_bits >= 1 && p.code_bits <= 10 && p.spatial_radius <= 15;
+        const dim3 blk(kWaves * 64);
+        if (p.channels == 1) {
+            if (narrow) k_centre_codes<1, true><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+            else k_centre_codes<1, false><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+        } else {
+            if (narrow) k_centre_codes<2, true><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+            else k_centre_codes<2, false><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+        }
+        if (p.temporal_radius >= 1 && p.group_size == 8) {
+#define AVD_TRAJ(Cn, Kn) k_trajectory_codes<Cn, Kn><<<g, blk, 0, s>>>(p, codes, nslots, mv, conf, members, scored_count)
+            switch (p.refine * 2 + 1 + (p.channels == 2 ? 100 : 0)) {
+            case 3: AVD_TRAJ(1, 3); break;
+            case 5: AVD_TRAJ(1, 5); break;
+            case 7: AVD_TRAJ(1, 7); break;
+            case 9: AVD_TRAJ(1, 9); break;
+            case 103: AVD_TRAJ(2, 3); break;
+            case 105: AVD_TRAJ(2, 5); break;
+            case 107: AVD_TRAJ(2, 7); break;
+            default: AVD_TRAJ(2, 9); break;
+            }
+#undef AVD_TRAJ
+        }
     } else {
         if (p.channels == 1) k_collab_search<1, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
         else k_collab_search<2, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);

@@ -54,7 +54,9 @@ hipError_t launch_warp(const float *src, float *dst, const int32_t *mv, int W, i
                        int blocks_x, int blocks_y, hipStream_t s);
 
 // ---- collab.hip -------------------------------------------------------------------------
-// `scored_count`, when non-null, gains the candidate patches the launch scores times the channels.
+// `scored_count`, when non-null, points at kScoreSlots counters kScoreStride apart; their sum gains
+// the candidate patches the launch scores times the channels.
+constexpr int kScoreSlots = 64, kScoreStride = 16;
 hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
                                 const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
                                 hipStream_t s, unsigned long long *scored_count = nullptr);

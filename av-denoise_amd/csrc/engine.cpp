@@ -178,8 +178,8 @@ void StageTimer::set_sampling(bool on, hipStream_t s)
         work_[i] = 0;
     }
     if (on) {
-        if (!counter_.ptr) counter_.allocate(sizeof(unsigned long long) * 2);
-        check_hip(launch_fill_u32(counter_.ptr, 0, 2, s), "zero counter");
+        if (!counter_.ptr) counter_.allocate(sizeof(unsigned long long) * kScoreSlots * kScoreStride);
+        check_hip(launch_fill_u32(counter_.ptr, 0, 2 * kScoreSlots * kScoreStride, s), "zero counter");
     }
 }
 
@@ -220,11 +220,12 @@ void StageTimer::harvest()
 void StageTimer::fold_counter(hipStream_t s)
 {
     if (!on_ || !counter_.ptr) return;
-    unsigned long long v = 0;
-    check_hip(hipMemcpyAsync(&v, counter_.ptr, sizeof(v), hipMemcpyDeviceToHost, s), "read counter");
+    std::vector<unsigned long long> v((size_t)kScoreSlots * kScoreStride);
+    check_hip(hipMemcpyAsync(v.data(), counter_.ptr, v.size() * sizeof(v[0]), hipMemcpyDeviceToHost, s),
+              "read counter");
     check_hip(hipStreamSynchronize(s), "read counter");
-    work_[AVD_STAGE_COLLAB_SEARCH] += v;
-    check_hip(launch_fill_u32(counter_.ptr, 0, 2, s), "zero counter");
+    for (int i = 0; i < kScoreSlots; ++i) work_[AVD_STAGE_COLLAB_SEARCH] += v[(size_t)i * kScoreStride];
+    check_hip(launch_fill_u32(counter_.ptr, 0, 2 * kScoreSlots * kScoreStride, s), "zero counter");
 }
 
 avd_timing StageTimer::totals() const
@@ -607,6 +608,7 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     pass_.weight_scale = weight_scale(sigma_, 1.0f);
     pass_.accum_scale = cross_frame_accum_scale(o.spatial_radius, r_, o.step);
     kaiser_window(o.kaiser_beta, pass_.kaiser);
+    pass_.code_bits = in.kind == AVD_FORMAT_U8 ? 8u : in.depth;
 
     ring_.init(W, H, C, N_, true);
     const size_t px = (size_t)W * H;

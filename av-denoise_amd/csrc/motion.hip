@@ -11,13 +11,14 @@
 // exact SAD does.  The confidence reads the level-0 SAD divided by max, one f32 division.
 #include "common.h"
 
+#include <algorithm>
+
 namespace avd {
 
 namespace {
 
 constexpr int kMaxR = 8;
 constexpr int kMaxBlk = 32;
-constexpr int kThreads = 128;
 
 __device__ __forceinline__ int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
 
@@ -40,120 +41,186 @@ struct Level {
     const uint16_t *codes;  // level 0 when non-null
     const int32_t *plane;   // level 1 otherwise
     int w, h, C;
-    __device__ __forceinline__ int32_t at(int x, int y) const
+    __device__ __forceinline__ uint32_t at(int x, int y) const
     {
         x = clampi(x, w - 1);
         y = clampi(y, h - 1);
-        return codes ? (int32_t)codes[((size_t)y * w + x) * C] : plane[(size_t)y * w + x];
+        return codes ? (uint32_t)codes[((size_t)y * w + x) * C] : (uint32_t)plane[(size_t)y * w + x];
     }
 };
 
-// One window search for this workgroup's block: stages the block and the window in LDS, scores
-// every offset, and returns (thread 0 only) the winner relative to (cdx, cdy) and its SAD.
-__device__ void search_window(const Level &cen, const Level &nb, int ox, int oy, int blk, int R, int cdx, int cdy,
-                              int32_t *s_blk, int32_t *s_win, int32_t *s_sad, int *bdx, int *bdy, int32_t *bsad)
+// |a - b| + acc on unsigned words in one instruction.
+__device__ __forceinline__ uint32_t sad_u32(uint32_t a, uint32_t b, uint32_t acc)
 {
-    const int side = 2 * R + 1;
-    const int ww = blk + 2 * R;
-    for (int i = threadIdx.x; i < blk * blk; i += kThreads) {
+    uint32_t r;
+    asm("v_sad_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(acc));
+    return r;
+}
+
+constexpr int kBmThreads = 512;
+constexpr int kBmLdsBudget = 64 * 1024;  // window bytes staged per batch of neighbours
+
+// Block-match of one block against every neighbour frame: one workgroup per block.  The block is
+// staged in LDS once; the neighbours go in batches whose (blk + 2R)^2 windows, centred on each
+// neighbour's seed, fit the LDS budget.  Thread i scores candidate (neighbour i / side^2, offset
+// i % side^2) over the blk^2 pixels with v_sad_u32 (the block row is a broadcast read, the window
+// row the thread's own).  Each neighbour's minimum (SAD << 9 | offset) falls out of an LDS
+// atomic min, so the lowest SAD and then the lowest row-major offset wins; the window centre then
+// wins a tie with that pick.  Writes through `finish(n, dx, dy, sad)` (thread n of the block).
+// `nb` is the neighbours' level with its base at slot 0; neighbour n sits nslots[n] levels on.
+template <typename Seed, typename Finish>
+__device__ __forceinline__ void match_block(const Level &cen, const Level &nb, size_t level_len,
+                                            const uint32_t *__restrict__ nslots, int neighbours, int ox, int oy,
+                                            int blk, int R, Seed &&seed, Finish &&finish, uint32_t *lds)
+{
+    const int side = 2 * R + 1, side2 = side * side, ww = blk + 2 * R;
+    const int win = ww * ww;
+    int batch = kBmLdsBudget / (win * 4);
+    batch = max(1, min(batch, min(neighbours, kBmThreads / side2 > 0 ? kBmThreads / side2 : 1)));
+    batch = max(1, min(batch, 16));
+    uint32_t *s_blk = lds;                       // blk * blk
+    uint32_t *s_key = s_blk + blk * blk;          // 16
+    uint32_t *s_csad = s_key + 16;                // 16
+    int *s_seed = (int *)(s_csad + 16);           // 32
+    uint32_t *s_slot = (uint32_t *)(s_seed + 32); // 16
+    uint32_t *s_win = s_slot + 16;                // batch * win
+    for (int i = threadIdx.x; i < blk * blk; i += kBmThreads) {
         const int py = i / blk, px = i - py * blk;
         s_blk[i] = cen.at(ox + px, oy + py);
     }
-    const int wx0 = ox + cdx - R, wy0 = oy + cdy - R;
-    for (int i = threadIdx.x; i < ww * ww; i += kThreads) {
-        const int wy = i / ww, wx = i - wy * ww;
-        s_win[i] = nb.at(wx0 + wx, wy0 + wy);
-    }
-    __syncthreads();
-    for (int c = threadIdx.x; c < side * side; c += kThreads) {
-        const int dy = c / side, dx = c - dy * side;
-        int32_t sad = 0;
-        for (int py = 0; py < blk; ++py) {
-            const int32_t *a = s_blk + py * blk;
-            const int32_t *b = s_win + (py + dy) * ww + dx;
-            for (int px = 0; px < blk; ++px) sad += abs(a[px] - b[px]);
+    for (int n0 = 0; n0 < neighbours; n0 += batch) {
+        const int nb_n = min(batch, neighbours - n0);
+        if ((int)threadIdx.x < nb_n) {
+            int sdx, sdy;
+            seed(n0 + threadIdx.x, sdx, sdy);
+            s_seed[2 * threadIdx.x] = sdx;
+            s_seed[2 * threadIdx.x + 1] = sdy;
+            s_slot[threadIdx.x] = nslots[n0 + threadIdx.x];
+            s_key[threadIdx.x] = ~0u;
         }
-        s_sad[c] = sad;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int32_t best = s_sad[0];
-        int bi = 0;
-        for (int c = 1; c < side * side; ++c)
-            if (s_sad[c] < best) {
-                best = s_sad[c];
-                bi = c;
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb_n * win; i += kBmThreads) {
+            const int nl = i / win, j = i - nl * win;
+            const int wy = j / ww, wx = j - wy * ww;
+            Level l = nb;
+            if (l.codes) l.codes += (size_t)s_slot[nl] * level_len;
+            else l.plane += (size_t)s_slot[nl] * level_len;
+            s_win[i] = l.at(ox + s_seed[2 * nl] - R + wx, oy + s_seed[2 * nl + 1] - R + wy);
+        }
+        __syncthreads();
+        for (int task = threadIdx.x; task < nb_n * side2; task += kBmThreads) {
+            const int nl = task / side2, c = task - nl * side2;
+            const int dy = c / side, dx = c - dy * side;
+            const uint32_t *w = s_win + nl * win + dy * ww + dx;
+            uint32_t sad = 0;
+            for (int py = 0; py < blk; ++py) {
+                const uint32_t *brow = s_blk + py * blk;
+                const uint32_t *wrow = w + py * ww;
+                if (blk == 16) {
+#pragma unroll
+                    for (int px = 0; px < 16; ++px) sad = sad_u32(brow[px], wrow[px], sad);
+                } else if (blk == 8) {
+#pragma unroll
+                    for (int px = 0; px < 8; ++px) sad = sad_u32(brow[px], wrow[px], sad);
+                } else {
+                    for (int px = 0; px < blk; ++px) sad = sad_u32(brow[px], wrow[px], sad);
+                }
             }
-        const int32_t centre_sad = s_sad[R * side + R];
-        if (centre_sad <= best) {
-            *bdx = cdx;
-            *bdy = cdy;
-            *bsad = centre_sad;
-        } else {
-            *bdx = cdx + (bi % side) - R;
-            *bdy = cdy + (bi / side) - R;
-            *bsad = best;
+            atomicMin(&s_key[nl], (sad << 9) | (uint32_t)c);
+            if (c == R * side + R) s_csad[nl] = sad;
         }
+        __syncthreads();
+        if ((int)threadIdx.x < nb_n) {
+            const int nl = threadIdx.x;
+            const uint32_t key = s_key[nl];
+            const uint32_t best = key >> 9;
+            const int bi = (int)(key & 511u);
+            const int sdx = s_seed[2 * nl], sdy = s_seed[2 * nl + 1];
+            if (s_csad[nl] <= best) finish(n0 + nl, sdx, sdy, s_csad[nl]);
+            else finish(n0 + nl, sdx + (bi % side) - R, sdy + (bi / side) - R, best);
+        }
+        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_bm_coarse(const int32_t *__restrict__ pyr1, int lw, int lh,
-                                                        int blk, int step, int R, int centre_slot,
-                                                        const uint32_t *__restrict__ nslots,
-                                                        int32_t *__restrict__ mv, int fine_bx, int fine_by,
-                                                        int fine_step)
+size_t match_lds_bytes(int blk, int R, int neighbours)
 {
-    __shared__ int32_t s_blk[kMaxBlk * kMaxBlk];
-    __shared__ int32_t s_win[(kMaxBlk + 2 * kMaxR) * (kMaxBlk + 2 * kMaxR)];
-    __shared__ int32_t s_sad[(2 * kMaxR + 1) * (2 * kMaxR + 1)];
-    const int bc = blockIdx.x, br = blockIdx.y, n = blockIdx.z;
+    const int side2 = (2 * R + 1) * (2 * R + 1), ww = blk + 2 * R, win = ww * ww;
+    int batch = kBmLdsBudget / (win * 4);
+    batch = std::max(1, std::min(batch, std::min(neighbours, kBmThreads / side2 > 0 ? kBmThreads / side2 : 1)));
+    batch = std::max(1, std::min(batch, 16));
+    return (size_t)(blk * blk + 16 + 16 + 32 + 16) * 4 + (size_t)batch * win * 4;
+}
+
+// Coarse pass (block_match.rs:117-211) on level 1 around a zero seed; the winner times 2 seeds every
+// fine block this coarse block covers.
+__global__ __launch_bounds__(kBmThreads) void k_bm_coarse(const int32_t *__restrict__ pyr1, int lw, int lh,
+                                                          int blk, int step, int R, int centre_slot,
+                                                          const uint32_t *__restrict__ nslots, int neighbours,
+                                                          int32_t *__restrict__ mv, int fine_bx, int fine_by,
+                                                          int fine_step)
+{
+    extern __shared__ uint32_t lds[];
+    const int bc = blockIdx.x, br = blockIdx.y;
     const size_t lp = (size_t)lw * lh;
     const Level cen{nullptr, pyr1 + (size_t)centre_slot * lp, lw, lh, 1};
-    const Level nb{nullptr, pyr1 + (size_t)nslots[n] * lp, lw, lh, 1};
-    int dx = 0, dy = 0;
-    int32_t sad = 0;
-    search_window(cen, nb, bc * step, br * step, blk, R, 0, 0, s_blk, s_win, s_sad, &dx, &dy, &sad);
-    if (threadIdx.x != 0) return;
-    int c0 = min(bc * step * 2 / fine_step, fine_bx), r0 = min(br * step * 2 / fine_step, fine_by);
-    int c1 = (bc == (int)gridDim.x - 1) ? fine_bx : min((bc + 1) * step * 2 / fine_step, fine_bx);
-    int r1 = (br == (int)gridDim.y - 1) ? fine_by : min((br + 1) * step * 2 / fine_step, fine_by);
-    int32_t *m = mv + (size_t)n * fine_bx * fine_by * 2;
-    for (int fr = r0; fr < r1; ++fr)
-        for (int fc = c0; fc < c1; ++fc) {
-            m[((size_t)fr * fine_bx + fc) * 2] = dx * 2;
-            m[((size_t)fr * fine_bx + fc) * 2 + 1] = dy * 2;
-        }
+    const Level nb{nullptr, pyr1, lw, lh, 1};
+    const int c0 = min(bc * step * 2 / fine_step, fine_bx), r0 = min(br * step * 2 / fine_step, fine_by);
+    const int c1 = (bc == (int)gridDim.x - 1) ? fine_bx : min((bc + 1) * step * 2 / fine_step, fine_bx);
+    const int r1 = (br == (int)gridDim.y - 1) ? fine_by : min((br + 1) * step * 2 / fine_step, fine_by);
+    match_block(
+        cen, nb, lp, nslots, neighbours, bc * step, br * step, blk, R,
+        [](int, int &sdx, int &sdy) {
+            sdx = 0;
+            sdy = 0;
+        },
+        [&](int n, int dx, int dy, uint32_t) {
+            int32_t *m = mv + (size_t)n * fine_bx * fine_by * 2;
+            for (int fr = r0; fr < r1; ++fr)
+                for (int fc = c0; fc < c1; ++fc) {
+                    m[((size_t)fr * fine_bx + fc) * 2] = dx * 2;
+                    m[((size_t)fr * fine_bx + fc) * 2 + 1] = dy * 2;
+                }
+        },
+        lds);
 }
 
-__global__ __launch_bounds__(kThreads) void k_bm_fine(const uint16_t *__restrict__ codes, int W, int H, int C,
-                                                      int blk, int step, int R, int use_seed, int centre_slot,
-                                                      const uint32_t *__restrict__ nslots, int32_t *__restrict__ mv,
-                                                      float *__restrict__ conf, float max_value, float sad_floor,
-                                                      float thsad)
+// Fine pass (block_match.rs:306-369) on level 0 around the seed, then the confidence
+// (:489-535): the level-0 SAD over max, one f32 division.
+__global__ __launch_bounds__(kBmThreads) void k_bm_fine(const uint16_t *__restrict__ codes, int W, int H, int C,
+                                                        int blk, int step, int R, int use_seed, int centre_slot,
+                                                        const uint32_t *__restrict__ nslots, int neighbours,
+                                                        int32_t *__restrict__ mv, float *__restrict__ conf,
+                                                        float max_value, float sad_floor, float thsad)
 {
-    __shared__ int32_t s_blk[kMaxBlk * kMaxBlk];
-    __shared__ int32_t s_win[(kMaxBlk + 2 * kMaxR) * (kMaxBlk + 2 * kMaxR)];
-    __shared__ int32_t s_sad[(2 * kMaxR + 1) * (2 * kMaxR + 1)];
-    const int bc = blockIdx.x, br = blockIdx.y, n = blockIdx.z;
+    extern __shared__ uint32_t lds[];
+    const int bc = blockIdx.x, br = blockIdx.y;
     const int bx = gridDim.x, by = gridDim.y;
     const size_t sp = (size_t)W * H * C;
     const Level cen{codes + (size_t)centre_slot * sp, nullptr, W, H, C};
-    const Level nb{codes + (size_t)nslots[n] * sp, nullptr, W, H, C};
-    const size_t b = (size_t)n * bx * by + (size_t)br * bx + bc;
-    const int sdx = use_seed ? mv[b * 2] : 0, sdy = use_seed ? mv[b * 2 + 1] : 0;
-    int dx = 0, dy = 0;
-    int32_t sad = 0;
-    search_window(cen, nb, bc * step, br * step, blk, R, sdx, sdy, s_blk, s_win, s_sad, &dx, &dy, &sad);
-    if (threadIdx.x != 0) return;
-    mv[b * 2] = dx;
-    mv[b * 2 + 1] = dy;
-    if (conf) {
-        float excess = (float)sad / max_value - sad_floor;
-        if (excess < 0.0f) excess = 0.0f;
-        const float t2 = thsad * thsad, e2 = excess * excess;
-        const float c = (t2 - e2) / (t2 + e2);
-        conf[b] = c < 0.0f ? 0.0f : c;
-    }
+    const Level nb{codes, nullptr, W, H, C};
+    const size_t blkid = (size_t)br * bx + bc;
+    const size_t per_n = (size_t)bx * by;
+    match_block(
+        cen, nb, sp, nslots, neighbours, bc * step, br * step, blk, R,
+        [&](int n, int &sdx, int &sdy) {
+            const size_t b = (size_t)n * per_n + blkid;
+            sdx = use_seed ? mv[b * 2] : 0;
+            sdy = use_seed ? mv[b * 2 + 1] : 0;
+        },
+        [&](int n, int dx, int dy, uint32_t sad) {
+            const size_t b = (size_t)n * per_n + blkid;
+            mv[b * 2] = dx;
+            mv[b * 2 + 1] = dy;
+            if (conf) {
+                float excess = (float)sad / max_value - sad_floor;
+                if (excess < 0.0f) excess = 0.0f;
+                const float t2 = thsad * thsad, e2 = excess * excess;
+                const float c = (t2 - e2) / (t2 + e2);
+                conf[b] = c < 0.0f ? 0.0f : c;
+            }
+        },
+        lds);
 }
 
 __global__ __launch_bounds__(256) void k_warp(const float *__restrict__ src, float *__restrict__ dst,
@@ -190,22 +257,24 @@ hipError_t launch_motion(const MotionGeom &g, const uint16_t *codes, const int32
                          float sad_floor, float thsad, hipStream_t s)
 {
     if (neighbours <= 0) return hipSuccess;
-    if (g.radius > kMaxR || g.blksize > kMaxBlk || g.blksize < 2 || g.levels < 1 || g.levels > 2)
+    if (g.radius > kMaxR || g.blksize > kMaxBlk || g.blksize < 2 || g.levels < 1 || g.levels > 2 || neighbours > 16)
         return hipErrorInvalidValue;
     int use_seed = 0;
     if (g.levels == 2) {
         const int lw = max(g.W / 2, 1), lh = max(g.H / 2, 1);
         const int cblk = max(g.blksize / 2, 2), cstep = max(g.step / 2, 1);
-        const dim3 grid(max((int)div_up(lw, cstep), 1), max((int)div_up(lh, cstep), 1), neighbours);
-        k_bm_coarse<<<grid, kThreads, 0, s>>>(pyr1, lw, lh, cblk, cstep, g.radius, centre_slot, nslots_dev, mv,
-                                              g.blocks_x, g.blocks_y, g.step);
+        const dim3 grid(max((int)div_up(lw, cstep), 1), max((int)div_up(lh, cstep), 1));
+        k_bm_coarse<<<grid, kBmThreads, match_lds_bytes(cblk, g.radius, neighbours), s>>>(
+            pyr1, lw, lh, cblk, cstep, g.radius, centre_slot, nslots_dev, neighbours, mv, g.blocks_x, g.blocks_y,
+            g.step);
         use_seed = 1;
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    const dim3 grid(g.blocks_x, g.blocks_y, neighbours);
-    k_bm_fine<<<grid, kThreads, 0, s>>>(codes, g.W, g.H, g.C, g.blksize, g.step, g.radius, use_seed, centre_slot,
-                                        nslots_dev, mv, conf, max_value, sad_floor, thsad);
+    const dim3 grid(g.blocks_x, g.blocks_y);
+    k_bm_fine<<<grid, kBmThreads, match_lds_bytes(g.blksize, g.radius, neighbours), s>>>(
+        codes, g.W, g.H, g.C, g.blksize, g.step, g.radius, use_seed, centre_slot, nslots_dev, neighbours, mv, conf,
+        max_value, sad_floor, thsad);
     return hipGetLastError();
 }
 

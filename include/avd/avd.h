@@ -264,6 +264,7 @@ typedef struct avd_collab_pass {
     float accum_scale;
     float kaiser[8];
     float profile[8];
+    uint32_t code_bits;        /* every u16 code is below 2^code_bits; 0 when unknown (up to 16) */
 } avd_collab_pass;
 
 /* search_group over every reference (collab_search, fused/split.rs:34-170): writes 9 words per

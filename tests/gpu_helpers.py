@@ -42,8 +42,10 @@ def words_host(t: torch.Tensor) -> np.ndarray:
     return host(t).view(np.uint32)
 
 
-def collab_pass(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, centre, slots, geom, profile=None):
+def collab_pass(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, centre, slots, geom, profile=None,
+                code_bits=0):
     p = L.CollabPass()
+    p.code_bits = code_bits
     p.width, p.height, p.channels = W, H, C
     p.step, p.spatial_radius, p.temporal_radius, p.refine, p.group_size = 4, S, radius, refine, 8
     p.blk_step, p.blksize, p.blocks_x, p.blocks_y = geom.step, geom.blksize, geom.blocks_x, geom.blocks_y
@@ -112,3 +114,15 @@ def run_engine(engine, frames_planes, depth_out, dims):
         engine.emit_into(bufs)
         outs.append([words_host(b) for b in bufs])
     return outs
+
+
+def assert_members_equal(got: np.ndarray, want: np.ndarray, what: str = "") -> None:
+    """Member words match; on a mismatch names the first differing reference and its two groups."""
+    g, w = got.reshape(-1, 9), want.reshape(-1, 9)
+    bad = np.nonzero(np.any(g != w, axis=1))[0]
+    if len(bad):
+        i = int(bad[0])
+
+        def fmt(row):
+            return [f"({v & 0x1FFF},{(v >> 13) & 0x1FFF},t{v >> 26})" for v in row[:8]] + [hex(int(row[8]))]
+        raise AssertionError(f"{what}: {len(bad)} references differ; first {i}: got {fmt(g[i])} want {fmt(w[i])}")

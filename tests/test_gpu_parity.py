@@ -15,8 +15,8 @@ from avd import _lib as L
 
 import oracle as O
 import refsetups as R
-from gpu_helpers import (collab_pass, dev, gray_geometry, host, ptr, require_gpu, run_engine, site_collab_filter,
-                         site_collab_search, words_dev, words_host)
+from gpu_helpers import (assert_members_equal, collab_pass, dev, gray_geometry, host, ptr, require_gpu, run_engine,
+                         site_collab_filter, site_collab_search, words_dev, words_host)
 
 pytestmark = pytest.mark.gpu
 
@@ -154,6 +154,70 @@ def test_collab_matches_the_recorded_reference_setups():
     assert d["covered"] == 16453
 
 
+def _codes_ring(planes, C):
+    """Frames of (H, W) code planes as one interleaved u16 ring of shape (frames, H*W*C)."""
+    return np.stack([np.stack([p.reshape(-1) for p in fr], 1).reshape(-1) for fr in planes]).astype(np.uint16)
+
+
+def _oracle_fields(codes, W, H, C, centre, slots, sigma, max_value, geom):
+    pyr = {s: [codes[s].reshape(-1, C)[:, 0].astype(np.int32),
+               O.downscale_sum(codes[s].reshape(-1, C)[:, 0].astype(np.int32), W, H)] for s in [centre] + slots}
+    floor, th = float(O.sad_noise_floor(16, sigma)), float(O.thsad(16, 1.0))
+    mvs, confs = [], []
+    for s in slots:
+        m, c = O.motion_search(geom, pyr[centre], pyr[s], max_value, floor, th)
+        mvs.append(m)
+        confs.append(c)
+    mv = np.concatenate(mvs) if mvs else np.zeros(2, np.int32)
+    conf = np.concatenate(confs) if confs else np.ones(1, np.float32)
+    return mv, conf
+
+
+@pytest.mark.parametrize("case", [
+    # (name, W, H, C, depth, radius, S, code_bits, content)
+    ("odd width, narrow keys", 81, 45, 1, 10, 2, 9, 10, "scene"),
+    ("odd width, wide keys", 81, 45, 1, 10, 2, 9, 0, "scene"),
+    ("odd width chroma", 43, 37, 2, 10, 1, 7, 10, "scene"),
+    ("radius 16 (run length 33)", 72, 64, 1, 10, 2, 16, 10, "scene"),
+    ("radius 16 chroma", 70, 50, 2, 8, 1, 16, 8, "scene"),
+    ("12-bit", 64, 48, 1, 12, 3, 12, 12, "scene"),
+    ("12-bit chroma", 48, 40, 2, 12, 2, 6, 12, "scene"),
+    ("flat with steps: ties", 64, 40, 1, 10, 2, 8, 10, "flat"),
+    ("tiny frame", 9, 8, 1, 10, 1, 3, 10, "scene"),
+    ("narrow frame", 8, 40, 2, 10, 2, 5, 10, "scene"),
+    ("8-bit full range noise", 56, 48, 1, 8, 3, 12, 8, "noise"),
+])
+def test_codes_search_edge_cases_are_bit_exact(case):
+    """Member lists of the integer search against the oracle over the shapes and contents that
+    exercise its paths: odd rows (unaligned loads), 32- and 64-bit keys, run lengths past the
+    unrolled forms, clipped rectangles, and equal distances resolved by visit order."""
+    name, W, H, C, depth, radius, S, code_bits, content = case
+    print(f"search case: {name}", flush=True)
+    frames = 2 * radius + 1
+    rng = np.random.default_rng(W * 1000 + H + depth)
+    top = (1 << depth) - 1
+    if content == "scene":
+        clip = avd.synth.clip(frames, W, H, depth, 31 + W, chroma="444" if C == 2 else "gray", sigma_luma=12.0)
+        planes = [[f[0].numpy()] if C == 1 else [f[1].numpy(), f[2].numpy()] for f in clip]
+    elif content == "flat":
+        base = (np.arange(W)[None, :] // 16 * 37 + np.arange(H)[:, None] // 8 * 11) % top
+        planes = [[np.broadcast_to(base, (H, W)).astype(np.int64)] for _ in range(frames)]
+    else:
+        planes = [[rng.integers(0, top + 1, (H, W)) for _ in range(C)] for _ in range(frames)]
+    codes = _codes_ring(planes, C)
+    geom = O.MotionGeom(W, H, 16, 8, 4, 2)
+    centre = radius
+    slots = [s for s in range(frames) if s != centre]
+    sigma = 12.0 / 1023
+    mv, conf = _oracle_fields(codes, W, H, C, centre, slots, sigma, float(top), geom)
+    p = collab_pass(W, H, C, radius, S, 2, 0.05, 2.7, sigma, 2.0, centre, frames, geom, code_bits=code_bits)
+    op = O.collab_params(W, H, C, radius, S, 2, 0.05, 2.7, sigma, 2.0, geom)
+    want = O.collab_search(codes, op, centre, slots, mv, conf)
+    d_codes = dev(codes.view(np.int16))
+    got = site_collab_search(p, codes=d_codes, nslots=slots, mv=mv, conf=conf)
+    assert_members_equal(got, want, name)
+
+
 @pytest.mark.parametrize("C,radius,S", [(1, 2, 9), (2, 1, 6), (1, 3, 12), (1, 0, 5)])
 def test_collab_codes_search_and_filter_are_bit_exact(C, radius, S):
     W, H, depth = 80, 56, 10
@@ -177,10 +241,11 @@ def test_collab_codes_search_and_filter_are_bit_exact(C, radius, S):
     mv = np.concatenate(mvs) if mvs else np.zeros(2, np.int32)
     conf = np.concatenate(confs) if confs else np.ones(1, np.float32)
     lam = 2.7
-    p = collab_pass(W, H, C, radius, S, 2, 0.05, lam, sigma, 2.0, centre, frames, geom)
+    p = collab_pass(W, H, C, radius, S, 2, 0.05, lam, sigma, 2.0, centre, frames, geom, code_bits=depth)
     op = O.collab_params(W, H, C, radius, S, 2, 0.05, lam, sigma, 2.0, geom)
     want = O.collab_search(codes, op, centre, slots, mv, conf)
-    got = site_collab_search(p, codes=dev(codes.view(np.int16)), nslots=slots, mv=mv, conf=conf)
+    d_codes = dev(codes.view(np.int16))
+    got = site_collab_search(p, codes=d_codes, nslots=slots, mv=mv, conf=conf)
     assert np.array_equal(got, want)
     acc, ws, gw = site_collab_filter(p, dev(values), slots, got, frames)
     wacc = np.zeros(frames * W * H * C, np.int32)
