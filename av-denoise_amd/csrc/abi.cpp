@@ -258,22 +258,25 @@ avd_status avd_site_collab_search(const avd_collab_pass *pass, const uint16_t *r
 }
 
 avd_status avd_site_collab_filter(const avd_collab_pass *pass, const float *ring_values,
-                                  const uint32_t *neighbour_slots, const uint32_t *members, int32_t *accum,
-                                  int32_t *wsum, float *group_weight, void *stream)
+                                  const uint32_t *neighbour_slots, const uint32_t *members, uint64_t *aggregate,
+                                  int32_t *accum_second, float *group_weight, void *stream)
 {
-    if (!pass || !ring_values || !members || !accum || !wsum || !group_weight)
+    if (!pass || !ring_values || !members || !aggregate || !group_weight || (pass->channels == 2 && !accum_second))
         return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
-    return hip_status(avd::launch_collab_filter(*pass, ring_values, neighbour_slots, members, accum, wsum,
+    return hip_status(avd::launch_collab_filter(*pass, ring_values, neighbour_slots, members,
+                                                reinterpret_cast<unsigned long long *>(aggregate), accum_second,
                                                 group_weight, static_cast<hipStream_t>(stream)),
                       "collab_filter");
 }
 
-avd_status avd_site_collab_normalise(int32_t *accum_region, int32_t *wsum_region, float *out, uint32_t pixels,
-                                     uint32_t channels, void *stream)
+avd_status avd_site_collab_normalise(uint64_t *aggregate_region, int32_t *accum_second_region, float *out,
+                                     uint32_t pixels, uint32_t channels, void *stream)
 {
-    if (!accum_region || !wsum_region || !out) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (!aggregate_region || !out || (channels == 2 && !accum_second_region))
+        return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
     if (channels < 1 || channels > 2) return fail(AVD_ERR_INVALID_GEOMETRY, "channels must be 1 or 2");
-    return hip_status(avd::launch_collab_normalise(accum_region, wsum_region, out, pixels, (int)channels,
+    return hip_status(avd::launch_collab_normalise(reinterpret_cast<unsigned long long *>(aggregate_region),
+                                                   accum_second_region, out, pixels, (int)channels,
                                                    static_cast<hipStream_t>(stream)),
                       "collab_normalise");
 }

@@ -707,6 +707,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     __shared__ uint8_t s_list[kWaves][256];             // non-empty entries of the current volume
     __shared__ uint32_t s_slot[kWaves][16];
     __shared__ unsigned long long s_fbest[kWaves][16];
+    __shared__ uint32_t s_patch[kWaves][4][8][4 * C];  // every volume's anchor patch
     constexpr int refine = (K - 1) / 2;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const RefPos rp = ref_of_wave(p, wave);
@@ -730,6 +731,17 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     const int per_vol = frames_n << nshift;
     const size_t nblk = (size_t)bx_n * by_n;
     if (lane < frames_n) s_slot[wave][lane] = nslots[lane];
+    if (lane < volumes * 8) {  // lane (v, r) stages row r of volume v's anchor patch
+        const int v = lane >> 3, r = lane & 7;
+        uint32_t anchor = anchors[0];
+#pragma unroll
+        for (int i = 1; i < 4; ++i)
+            if (v == i) anchor = anchors[i];
+        uint32_t row[4 * C];
+        load_row<C>(ring, cframe + ((size_t)(((anchor >> 13) & 0x1FFF) + r) * W + (anchor & 0x1FFF)) * C, row);
+#pragma unroll
+        for (int i = 0; i < 4 * C; ++i) s_patch[wave][v][r][i] = row[i];
+    }
 
     // ---- every volume's rectangles.
     for (int idx = lane; idx < volumes * per_vol; idx += 64) {
@@ -760,13 +772,13 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     }
     wave_sync();
 
+    // The volume lists: positions and distances (SSD + 1 fits 32 bits; ~0u marks an empty slot).
     uint32_t scored = 0;
-    uint32_t gpos[8];
-    uint64_t gd[8];
+    uint32_t gpos[8], gd[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         gpos[i] = 0;
-        gd[i] = kNoKey;
+        gd[i] = ~0u;
     }
     uint32_t P[8][4 * C];
     for (int v = 0; v < volumes; ++v) {
@@ -775,7 +787,10 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
 #pragma unroll
         for (int i = 1; i < 4; ++i)
             if (v == i) anchor = anchors[i];
-        load_patch<C>(ring, cframe, W, (int)(anchor & 0x1FFF), (int)((anchor >> 13) & 0x1FFF), P);
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int i = 0; i < 4 * C; ++i) P[r][i] = s_patch[wave][v][r][i];
         const uint32_t A2 = patch_energy<C>(P);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -784,7 +799,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                 gd[i] = 0;
             } else if (i > first && i < first + gf) {
                 gpos[i] = 0;
-                gd[i] = kNoKey;
+                gd[i] = ~0u;
             }
         }
         const int held_n = first + 1;
@@ -845,7 +860,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
         }
         wave_sync();
         // Lane t decodes frame t's match.
-        uint64_t cd = kNoKey;
+        uint32_t cd = ~0u;
         uint32_t cp = 0;
         if (lane < frames_n) {
             const uint64_t best = s_fbest[wave][lane];
@@ -863,20 +878,19 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                     }
                     i -= n2;
                 }
-                cd = best >> 32;
+                cd = (uint32_t)(best >> 32);
             }
         }
         // Frames join the volume's ascending list in frame order; a later frame never displaces an
         // equal one, and a displaced entry moves on past equal ones (the reference's ordered
         // insertion, which is not a stable sort when distances tie).
         for (int u = 0; u < frames_n; ++u) {
-            uint64_t d = shfl_u64(cd, u);
+            uint32_t d = (uint32_t)__shfl((int)cd, u);
             uint32_t q = (uint32_t)__shfl((int)cp, u);
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 if (i > first && i < first + gf && d < gd[i]) {
-                    const uint64_t hd = gd[i];
-                    const uint32_t hp = gpos[i];
+                    const uint32_t hd = gd[i], hp = gpos[i];
                     gd[i] = d;
                     gpos[i] = q;
                     d = hd;
@@ -889,7 +903,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     bool use_grid = k_use == 8;
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-        if (i % gf == tail) use_grid = use_grid && (gd[i] != kNoKey);
+        if (i % gf == tail) use_grid = use_grid && (gd[i] != ~0u);
     if (use_grid && rp.live && lane < 9) {
         uint32_t word = 8u | 0x100u;
 #pragma unroll
@@ -949,52 +963,67 @@ __device__ __forceinline__ void dct8_inv(float *l)
     }
 }
 
-// One Haar butterfly level between member lanes `xor_lanes` apart.  `active` lanes take part; a
-// low lane keeps (low + high) * f, a high lane (low - high) * f, operands in the reference's order.
-__device__ __forceinline__ void haar_level(float *v, int xor_lanes, bool active, bool low)
+// One Haar butterfly level between member lanes XOR (8, 16 or 32) apart.  `active` lanes take
+// part; a low lane keeps (low + high) * f, a high lane (low - high) * f, operands in the
+// reference's order.
+template <int XOR>
+__device__ __forceinline__ void haar_level(float (&v)[8], bool active, bool low)
 {
     const float f = 0.70710678118654752440f;
-    const int partner = (int)(threadIdx.x & 63) ^ xor_lanes;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-        const float other = __shfl(v[u], partner);
+        const float other = __shfl_xor(v[u], XOR);
         if (active) v[u] = low ? (v[u] + other) * f : (other - v[u]) * f;
     }
 }
 
-// Forward (inverse = false) or inverse group transform over the 8 member lanes.
-__device__ void group_transform(float *v, int m, int k_use, bool grid, int gf, bool inverse)
+// The group Haar over the member lanes (grid.rs:119-220, stack_haar_*, fused/mod.rs:854-892).
+// Level i pairs members 2^i apart (lanes 8 << i apart) in every layout: a stack of k_use members
+// runs log2(k_use) levels; a grid of gf frames per volume runs the time levels (pairs inside a
+// volume) and then the volume levels (pairs of volumes), three in all for eight members.
+struct HaarPlan {
+    int levels;
+    bool act[3], low[3];
+};
+
+__device__ __forceinline__ HaarPlan haar_plan(int m, int k_use, bool grid, int gf)
 {
-    int lv_xor[6];
-    bool lv_active[6], lv_low[6];
-    int n = 0;
+    HaarPlan h;
     if (grid) {
-        const int vols = 8 / gf, f = m % gf, vol = m / gf;
-        for (int st = 1; st < gf; st *= 2) {
-            lv_xor[n] = st * 8;
-            lv_active[n] = (f % st) == 0;
-            lv_low[n] = ((f / st) % 2) == 0;
-            ++n;
-        }
-        for (int st = 1; st < vols; st *= 2) {
-            lv_xor[n] = st * gf * 8;
-            lv_active[n] = (vol % st) == 0;
-            lv_low[n] = ((vol / st) % 2) == 0;
-            ++n;
+        h.levels = 3;
+        const int f = m % gf, vol = m / gf;
+        if (gf == 4) {  // time levels 1, 2; one volume level
+            h.act[0] = true, h.low[0] = (f & 1) == 0;
+            h.act[1] = (f & 1) == 0, h.low[1] = (f & 2) == 0;
+            h.act[2] = true, h.low[2] = (vol & 1) == 0;
+        } else {  // gf == 2: one time level; volume levels 1, 2
+            h.act[0] = true, h.low[0] = f == 0;
+            h.act[1] = true, h.low[1] = (vol & 1) == 0;
+            h.act[2] = (vol & 1) == 0, h.low[2] = (vol & 2) == 0;
         }
     } else {
-        for (int st = 1; st < k_use; st *= 2) {
-            lv_xor[n] = st * 8;
-            lv_active[n] = (m % st) == 0 && m < k_use;
-            lv_low[n] = ((m / st) % 2) == 0;
-            ++n;
+        h.levels = k_use >= 8 ? 3 : (k_use >= 4 ? 2 : (k_use >= 2 ? 1 : 0));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            h.act[i] = (m & ((1 << i) - 1)) == 0 && m < k_use;
+            h.low[i] = ((m >> i) & 1) == 0;
         }
     }
-    if (!inverse) {
-        for (int i = 0; i < n; ++i) haar_level(v, lv_xor[i], lv_active[i], lv_low[i]);
-    } else {
-        for (int i = n - 1; i >= 0; --i) haar_level(v, lv_xor[i], lv_active[i], lv_low[i]);
-    }
+    return h;
+}
+
+__device__ __forceinline__ void group_forward(float (&v)[8], const HaarPlan &h)
+{
+    if (h.levels > 0) haar_level<8>(v, h.act[0], h.low[0]);
+    if (h.levels > 1) haar_level<16>(v, h.act[1], h.low[1]);
+    if (h.levels > 2) haar_level<32>(v, h.act[2], h.low[2]);
+}
+
+__device__ __forceinline__ void group_inverse(float (&v)[8], const HaarPlan &h)
+{
+    if (h.levels > 2) haar_level<32>(v, h.act[2], h.low[2]);
+    if (h.levels > 1) haar_level<16>(v, h.act[1], h.low[1]);
+    if (h.levels > 0) haar_level<8>(v, h.act[0], h.low[0]);
 }
 
 __device__ __forceinline__ int32_t to_fixed(float v, float scale)
@@ -1009,20 +1038,36 @@ __device__ __forceinline__ int32_t to_fixed_weight(float w, float scale)
     return (int32_t)roundf(c * scale * 5.0f);
 }
 
+// One pixel's (weight sum, first-channel sum) pair as one 64-bit addend: the weight in the high
+// word, the signed sum sign-extended below it.  Both sums stay within i32 (the accumulation scale
+// is chosen for that), so the total's low word is the sum and its high word the weight minus the
+// borrow of a negative sum.
+__device__ __forceinline__ unsigned long long pack_aggregate(int32_t accum, int32_t weight)
+{
+    return ((unsigned long long)(uint32_t)weight << 32) + (unsigned long long)(long long)accum;
+}
+
+__device__ __forceinline__ void unpack_aggregate(unsigned long long v, int32_t &accum, int32_t &weight)
+{
+    accum = (int32_t)(uint32_t)v;
+    weight = (int32_t)(uint32_t)(v >> 32) + (accum < 0 ? 1 : 0);
+}
+
 template <int C>
 __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p, const float *__restrict__ values,
                                                                const uint32_t *__restrict__ nslots,
                                                                const uint32_t *__restrict__ members,
-                                                               int32_t *__restrict__ accum, int32_t *__restrict__ wsum,
+                                                               unsigned long long *__restrict__ aggregate,
+                                                               int32_t *__restrict__ accum_second,
                                                                float *__restrict__ group_weight)
 {
     __shared__ float s_t[kWaves][8][8][9];  // [wave][member][row][col], padded
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int m = lane >> 3, q = lane & 7;
     const int W = p.width, H = p.height;
     const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
     const int refs = refs_x * refs_y;
-    int ref = blockIdx.x * kWaves + wave;
+    int ref = xcd_block() * kWaves + wave;
     const bool live = ref < refs;
     if (!live) ref = refs - 1;
     const uint32_t *mw = members + (size_t)ref * 9;
@@ -1030,6 +1075,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
     const int k_use = (int)(gword & 0xFF);
     const bool grid = (gword & 0x100u) != 0;
     const int gf = p.temporal_radius == 0 ? 1 : (p.temporal_radius == 1 ? 2 : 4);
+    const HaarPlan plan = haar_plan(m, k_use, grid, gf);
     const uint32_t pos = mw[m];
     const uint32_t t = pos >> 26;
     const int slot = t > 0 ? (int)nslots[t - 1] : (int)p.centre_slot;
@@ -1038,34 +1084,36 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
     const float *src = values + (size_t)slot * px * C;
     const float scale = p.accum_scale;
     const bool in_group = m < k_use;
+    const int v = q;  // after the first transpose lane (m, q) holds vertical frequency v = q
     float scaled_weight = 0.0f;
 
+#pragma unroll
     for (int c = 0; c < C; ++c) {
         float col[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) col[r] = src[((size_t)(my + r) * W + mx + q) * C + c];
         dct8_fwd(col);  // vertical: col[v]
 #pragma unroll
-        for (int v = 0; v < 8; ++v) s_t[wave][m][v][q] = col[v];
-        __syncthreads();
-        float row[8];  // lane (m, v = q): coefficients (v, u) for u = 0..7
+        for (int vv = 0; vv < 8; ++vv) s_t[wave][m][vv][q] = col[vv];
+        wave_sync();
+        float row[8];  // lane (m, v): coefficients (v, u) for u = 0..7
 #pragma unroll
         for (int u = 0; u < 8; ++u) row[u] = s_t[wave][m][q][u];
-        __syncthreads();
+        wave_sync();
         dct8_fwd(row);
-        const int v = q;
-        group_transform(row, m, k_use, grid, gf, false);
+        group_forward(row, plan);
 
         const float sig2 = p.sigma[c] * p.sigma[c];
         const float lam = p.lambda[c];
+        const float pv = p.profile[v];
         uint32_t counts = 0;  // kept coefficients per u, one nibble each
         float cvs[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            cvs[u] = sig2 * (p.profile[u] * p.profile[v]);
+            cvs[u] = sig2 * (p.profile[u] * pv);
             const float thr = lam * sqrtf(cvs[u]);
             bool keep = fabsf(row[u]) >= thr;
-            if (m == 0 && v == 0 && u == 0) keep = true;
+            if (u == 0 && m == 0 && v == 0) keep = true;
             keep = keep && in_group;
             if (keep) counts += 1u << (4 * u);
             else if (in_group) row[u] = 0.0f;
@@ -1073,6 +1121,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
         counts += (uint32_t)__shfl_xor((int)counts, 8);
         counts += (uint32_t)__shfl_xor((int)counts, 16);
         counts += (uint32_t)__shfl_xor((int)counts, 32);
+        // Retained variance: each lane (m, v) adds, for every u, the variance of (u, v) once per
+        // member that kept it; the eight v lanes then sum by butterfly.
         float part = 0.0f;
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -1089,43 +1139,48 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
             scaled_weight = weight * p.weight_scale;
         }
 
-        group_transform(row, m, k_use, grid, gf, true);
+        group_inverse(row, plan);
         dct8_inv(row);  // lane (m, v): values at columns k for vertical frequency v
 #pragma unroll
         for (int k = 0; k < 8; ++k) s_t[wave][m][v][k] = row[k];
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int vv = 0; vv < 8; ++vv) col[vv] = s_t[wave][m][vv][q];
-        __syncthreads();
+        wave_sync();
         dct8_inv(col);  // rows of column q
 
         if (live && in_group) {
             const size_t base = (size_t)slot * px;
+            const float kq = p.kaiser[q];
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                const float win = p.kaiser[r] * p.kaiser[q];
-                const float w = scaled_weight * win;
+                const float w = scaled_weight * (p.kaiser[r] * kq);
                 const size_t pix = base + (size_t)(my + r) * W + mx + q;
-                atomicAdd(&accum[pix * C + c], to_fixed(col[r] * w, scale));
-                if (c == 0) atomicAdd(&wsum[pix], to_fixed_weight(w, scale));
+                const int32_t a = to_fixed(col[r] * w, scale);
+                if (c == 0) atomicAdd(&aggregate[pix], pack_aggregate(a, to_fixed_weight(w, scale)));
+                else atomicAdd(&accum_second[pix], a);
             }
         }
     }
 }
 
+// collab_normalise (aggregate.rs:252-290): out = accum * 5 / wsum per channel (accum when the
+// weight is zero), then the region is zeroed for its next use.
 template <int C>
-__global__ __launch_bounds__(256) void k_normalise(int32_t *__restrict__ accum, int32_t *__restrict__ wsum,
-                                                   float *__restrict__ out, uint32_t pixels)
+__global__ __launch_bounds__(256) void k_normalise(unsigned long long *__restrict__ aggregate,
+                                                   int32_t *__restrict__ accum_second, float *__restrict__ out,
+                                                   uint32_t pixels)
 {
-    for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < pixels; p += gridDim.x * 256) {
-        const int32_t w = wsum[p];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const float a = (float)accum[(size_t)p * C + c];
-            out[(size_t)p * C + c] = (w != 0) ? (a * 5.0f) / (float)w : a;
-            accum[(size_t)p * C + c] = 0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < pixels; i += gridDim.x * 256) {
+        int32_t a, w;
+        unpack_aggregate(aggregate[i], a, w);
+        out[(size_t)i * C] = (w != 0) ? ((float)a * 5.0f) / (float)w : (float)a;
+        aggregate[i] = 0;
+        if (C == 2) {
+            const float a2 = (float)accum_second[i];
+            out[(size_t)i * C + 1] = (w != 0) ? (a2 * 5.0f) / (float)w : a2;
+            accum_second[i] = 0;
         }
-        wsum[p] = 0;
     }
 }
 
@@ -1208,24 +1263,28 @@ hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes,
 }
 
 hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
-                                const uint32_t *members, int32_t *accum, int32_t *wsum, float *group_weight,
-                                hipStream_t s)
+                                const uint32_t *members, unsigned long long *aggregate, int32_t *accum_second,
+                                float *group_weight, hipStream_t s)
 {
-    if (!pass_ok(p)) return hipErrorInvalidValue;
+    if (!pass_ok(p) || (p.channels == 2 && !accum_second)) return hipErrorInvalidValue;
     hipError_t e = upload_taps(s);
     if (e != hipSuccess) return e;
     const unsigned g = div_up(ref_count(p), kWaves);
-    if (p.channels == 1) k_collab_filter<1><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, accum, wsum, group_weight);
-    else k_collab_filter<2><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, accum, wsum, group_weight);
+    if (p.channels == 1)
+        k_collab_filter<1><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second, group_weight);
+    else
+        k_collab_filter<2><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second, group_weight);
     return hipGetLastError();
 }
 
-hipError_t launch_collab_normalise(int32_t *accum, int32_t *wsum, float *out, uint32_t pixels, int C, hipStream_t s)
+hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,
+                                   int C, hipStream_t s)
 {
+    if (C == 2 && !accum_second) return hipErrorInvalidValue;
     unsigned g = div_up(pixels, 256);
     g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
-    if (C == 1) k_normalise<1><<<g, 256, 0, s>>>(accum, wsum, out, pixels);
-    else k_normalise<2><<<g, 256, 0, s>>>(accum, wsum, out, pixels);
+    if (C == 1) k_normalise<1><<<g, 256, 0, s>>>(aggregate, accum_second, out, pixels);
+    else k_normalise<2><<<g, 256, 0, s>>>(aggregate, accum_second, out, pixels);
     return hipGetLastError();
 }
 

@@ -60,9 +60,12 @@ constexpr int kScoreSlots = 64, kScoreStride = 16;
 hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
                                 const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
                                 hipStream_t s, unsigned long long *scored_count = nullptr);
+// Accumulators of the fixed-point scatter: `aggregate` holds per pixel (weight sum << 32) plus the
+// first channel's sum, one 64-bit atomic add per pixel; `accum_second` the second channel's sum.
 hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
-                                const uint32_t *members, int32_t *accum, int32_t *wsum, float *group_weight,
-                                hipStream_t s);
-hipError_t launch_collab_normalise(int32_t *accum, int32_t *wsum, float *out, uint32_t pixels, int C, hipStream_t s);
+                                const uint32_t *members, unsigned long long *aggregate, int32_t *accum_second,
+                                float *group_weight, hipStream_t s);
+hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,
+                                   int C, hipStream_t s);
 
 }  // namespace avd

@@ -140,6 +140,24 @@ void kaiser_window(float beta, float out[8])
     }
 }
 
+// Algorithmic operations of one motion launch: 3 per pixel of every scored block offset (subtract,
+// absolute value, add), the coarse level on half-size blocks of the 2x2 level.
+uint64_t motion_ops(const MotionGeom &g, int neighbours)
+{
+    const uint64_t side2 = (uint64_t)(2 * g.radius + 1) * (2 * g.radius + 1);
+    uint64_t ops = (uint64_t)g.blocks_x * g.blocks_y * side2 * (uint64_t)g.blksize * g.blksize * 3;
+    if (g.levels == 2) {
+        const int lw = std::max((int)g.W / 2, 1), lh = std::max((int)g.H / 2, 1);
+        const int cblk = std::max(g.blksize / 2, 2), cstep = std::max(g.step / 2, 1);
+        const uint64_t cb = (uint64_t)std::max((lw + cstep - 1) / cstep, 1) * std::max((lh + cstep - 1) / cstep, 1);
+        ops += cb * side2 * (uint64_t)cblk * cblk * 3;
+    }
+    return ops * (uint64_t)neighbours;
+}
+
+// Per-candidate NLM operations (5C + 2(2P + 1) + 7).
+uint64_t nlm_candidate_ops(int C, int P) { return (uint64_t)(5 * C + 2 * (2 * P + 1) + 7); }
+
 float cross_frame_accum_scale(uint32_t spatial_radius, uint32_t temporal_radius, uint32_t step)
 {
     const uint32_t per_axis = ((8 - 1) + 2 * spatial_radius) / step + 1;
@@ -224,7 +242,8 @@ void StageTimer::fold_counter(hipStream_t s)
     check_hip(hipMemcpyAsync(v.data(), counter_.ptr, v.size() * sizeof(v[0]), hipMemcpyDeviceToHost, s),
               "read counter");
     check_hip(hipStreamSynchronize(s), "read counter");
-    for (int i = 0; i < kScoreSlots; ++i) work_[AVD_STAGE_COLLAB_SEARCH] += v[(size_t)i * kScoreStride];
+    // 192 operations per candidate patch and channel: 64 samples x (subtract, multiply, add).
+    for (int i = 0; i < kScoreSlots; ++i) work_[AVD_STAGE_COLLAB_SEARCH] += 192 * v[(size_t)i * kScoreStride];
     check_hip(launch_fill_u32(counter_.ptr, 0, 2 * kScoreSlots * kScoreStride, s), "zero counter");
 }
 
@@ -451,7 +470,7 @@ void NlmPass::denoise()
             if (k != 0) slots.push_back(ring_.phys((int)r_ + k));
         const float mx = max_value(in_);
         const size_t blocks = (size_t)mg_.blocks_x * mg_.blocks_y;
-        timed(AVD_STAGE_MOTION, blocks * slots.size(), stream_, "motion", [&] {
+        timed(AVD_STAGE_MOTION, motion_ops(mg_, (int)slots.size()), stream_, "motion", [&] {
             return launch_motion(mg_, ring_.code_ptr(), ring_.l1_ptr(), (int)centre, ring_.neighbours_of(r_),
                                  (int)slots.size(), mv_.as<int32_t>(), conf_.as<float>(), mx,
                                  sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), stream_);
@@ -483,11 +502,11 @@ void NlmPass::denoise()
     };
     const uint64_t window = (uint64_t)(2 * S_ + 1) * (2 * S_ + 1);
     for (int qk = -(int)r_; qk < 0; ++qk)
-        timed(AVD_STAGE_NLM_PAIR, px * window * 2, stream_, "nlm pair", [&] {
+        timed(AVD_STAGE_NLM_PAIR, px * window * 2 * nlm_candidate_ops(C, P_), stream_, "nlm pair", [&] {
             return launch_nlm_pair(w, cf, nb[qk + r_], nb[-qk + r_], conf_of(qk), conf_of(-qk), accum_.as<float>(),
                                    wsum_.as<float>(), maxw_.as<float>(), stream_);
         });
-    timed(AVD_STAGE_NLM_SINGLE, px * (window - 1), stream_, "nlm single", [&] {
+    timed(AVD_STAGE_NLM_SINGLE, px * (window - 1) * nlm_candidate_ops(C, P_), stream_, "nlm single", [&] {
         return launch_nlm_single(w, cf, lut_.as<float>(), accum_.as<float>(), wsum_.as<float>(), maxw_.as<float>(),
                                  stream_);
     });
@@ -613,8 +632,8 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     ring_.init(W, H, C, N_, true);
     const size_t px = (size_t)W * H;
     refs_ = (size_t)((W - 8 + o.step - 1) / o.step + 1) * ((H - 8 + o.step - 1) / o.step + 1);
-    accum_.allocate(N_ * px * C * sizeof(int32_t));
-    wsum_.allocate(N_ * px * sizeof(int32_t));
+    aggregate_.allocate(N_ * px * sizeof(unsigned long long));
+    if (C == 2) accum_second_.allocate(N_ * px * sizeof(int32_t));
     members_.allocate(refs_ * 9 * sizeof(uint32_t));
     group_weight_.allocate(refs_ * sizeof(float));
     output_.allocate(px * C * sizeof(float));
@@ -639,8 +658,8 @@ void Nl4dPass::reset_stream()
 void Nl4dPass::zero_region(uint32_t slot)
 {
     const size_t px = (size_t)W_ * H_;
-    check_hip(launch_fill_u32(accum_.as<int32_t>() + slot * px * C_, 0, px * C_, stream_), "zero region");
-    check_hip(launch_fill_u32(wsum_.as<int32_t>() + slot * px, 0, px, stream_), "zero region");
+    check_hip(launch_fill_u32(aggregate_.as<uint32_t>() + slot * px * 2, 0, px * 2, stream_), "zero region");
+    if (C_ == 2) check_hip(launch_fill_u32(accum_second_.as<uint32_t>() + slot * px, 0, px, stream_), "zero region");
 }
 
 // run_pass + scatter_pass (nl4d/collab_pass.rs:50-200) for the pass centred on logical `centre`.
@@ -655,8 +674,7 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
     }
     std::fill(zeroed_.begin(), zeroed_.end(), false);
     const uint32_t *nslots = ring_.neighbours_of(centre_logical);
-    const uint64_t blocks = (uint64_t)mg_.blocks_x * mg_.blocks_y * (N_ - 1);
-    timed(AVD_STAGE_MOTION, blocks, stream_, "motion", [&] {
+    timed(AVD_STAGE_MOTION, motion_ops(mg_, (int)(N_ - 1)), stream_, "motion", [&] {
         return launch_motion(mg_, ring_.code_ptr(), ring_.l1_ptr(), (int)centre, nslots, (int)(N_ - 1),
                              mv_.as<int32_t>(), conf_.as<float>(), max_value(in_),
                              sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), stream_);
@@ -668,9 +686,11 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
         return launch_collab_search(p, ring_.code_ptr(), ring_.value_ptr(), nslots, mv_.as<int32_t>(),
                                     conf_.as<float>(), members_.as<uint32_t>(), stream_, counter);
     });
-    timed(AVD_STAGE_COLLAB_FILTER, refs_, stream_, "collab filter", [&] {
-        return launch_collab_filter(p, ring_.value_ptr(), nslots, members_.as<uint32_t>(), accum_.as<int32_t>(),
-                                    wsum_.as<int32_t>(), group_weight_.as<float>(), stream_);
+    const uint64_t filter_ops = (uint64_t)refs_ * 8 * ((uint64_t)C_ * (4096 + 576 + 256) + 64 * (C_ + 1));
+    timed(AVD_STAGE_COLLAB_FILTER, filter_ops, stream_, "collab filter", [&] {
+        return launch_collab_filter(p, ring_.value_ptr(), nslots, members_.as<uint32_t>(),
+                                    aggregate_.as<unsigned long long>(), accum_second_.as<int32_t>(),
+                                    group_weight_.as<float>(), stream_);
     });
     ++passes_;
 }
@@ -724,7 +744,8 @@ void Nl4dPass::emit(const avd_device_plane *planes)
     const size_t px = (size_t)W_ * H_;
     // read_region + normalise_region (nl4d/denoiser.rs:402-410, :720-760): the region is left zeroed.
     timed(AVD_STAGE_NORMALISE, px, stream_, "normalise", [&] {
-        return launch_collab_normalise(accum_.as<int32_t>() + slot * px * C_, wsum_.as<int32_t>() + slot * px,
+        return launch_collab_normalise(aggregate_.as<unsigned long long>() + slot * px,
+                                       C_ == 2 ? accum_second_.as<int32_t>() + slot * px : nullptr,
                                        output_.as<float>(), (uint32_t)px, C_, stream_);
     });
     zeroed_[slot] = true;

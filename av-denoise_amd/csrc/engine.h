@@ -199,7 +199,9 @@ class Nl4dPass : public Pass {
     avd_collab_pass pass_{};
     size_t refs_ = 0;
     Ring ring_;
-    DeviceBuffer accum_, wsum_, members_, group_weight_, output_, mv_, conf_;
+    // Per ring slot: aggregate_ holds (weight sum << 32) + first-channel sum per pixel, accum_second_
+    // the second channel's sum (chroma passes only).
+    DeviceBuffer aggregate_, accum_second_, members_, group_weight_, output_, mv_, conf_;
     std::vector<uint32_t> pending_, tail_;
     std::vector<bool> zeroed_;
     uint32_t passes_ = 0;

@@ -143,6 +143,23 @@ __device__ __forceinline__ void match_block(const Level &cen, const Level &nb, s
     }
 }
 
+constexpr int kRowsLdsBudget = 48 * 1024;
+
+__host__ __device__ inline int match_rows_batch(int blk, int R, int neighbours)
+{
+    const int ww = blk + 2 * R;
+    int batch = kRowsLdsBudget / (ww * ww * 4);
+    batch = batch < 1 ? 1 : batch;
+    batch = batch < neighbours ? batch : neighbours;
+    return batch < 16 ? batch : 16;
+}
+
+size_t match_rows_lds_bytes(int blk, int R, int neighbours)
+{
+    const int ww = blk + 2 * R;
+    return (size_t)(blk * blk + 16 + 16 + 32 + 16) * 4 + (size_t)match_rows_batch(blk, R, neighbours) * ww * ww * 4;
+}
+
 size_t match_lds_bytes(int blk, int R, int neighbours)
 {
     const int side2 = (2 * R + 1) * (2 * R + 1), ww = blk + 2 * R, win = ww * ww;
@@ -150,6 +167,89 @@ size_t match_lds_bytes(int blk, int R, int neighbours)
     batch = std::max(1, std::min(batch, std::min(neighbours, kBmThreads / side2 > 0 ? kBmThreads / side2 : 1)));
     batch = std::max(1, std::min(batch, 16));
     return (size_t)(blk * blk + 16 + 16 + 32 + 16) * 4 + (size_t)batch * win * 4;
+}
+
+// The same search specialised for a compile-time block size and radius: one wave per block, lane
+// (neighbour, dy) scores the whole row of offsets (dx = 0..2R) for its dy.  Per block row it holds
+// the block row and the window row (BLK + 2R samples) in registers, so each window row is read from
+// LDS once for 2R + 1 candidates instead of once per candidate.
+template <int BLK, int R, typename Seed, typename Finish>
+__device__ __forceinline__ void match_rows(const Level &cen, const Level &nb, size_t level_len,
+                                           const uint32_t *__restrict__ nslots, int neighbours, int ox, int oy,
+                                           Seed &&seed, Finish &&finish, uint32_t *lds)
+{
+    constexpr int SIDE = 2 * R + 1, WW = BLK + 2 * R, WIN = WW * WW;
+    const int lane = threadIdx.x;
+    const int batch = match_rows_batch(BLK, R, neighbours);
+    uint32_t *s_blk = lds;                        // BLK * BLK
+    uint32_t *s_key = s_blk + BLK * BLK;          // 16
+    uint32_t *s_csad = s_key + 16;                // 16
+    int *s_seed = (int *)(s_csad + 16);           // 32
+    uint32_t *s_slot = (uint32_t *)(s_seed + 32); // 16
+    uint32_t *s_win = s_slot + 16;                // batch * WIN, rows 16-byte aligned
+    for (int i = lane; i < BLK * BLK; i += 64) {
+        const int py = i / BLK, px = i - py * BLK;
+        s_blk[i] = cen.at(ox + px, oy + py);
+    }
+    for (int n0 = 0; n0 < neighbours; n0 += batch) {
+        const int nb_n = min(batch, neighbours - n0);
+        if (lane < nb_n) {
+            int sdx, sdy;
+            seed(n0 + lane, sdx, sdy);
+            s_seed[2 * lane] = sdx;
+            s_seed[2 * lane + 1] = sdy;
+            s_slot[lane] = nslots[n0 + lane];
+            s_key[lane] = ~0u;
+        }
+        __syncthreads();
+        for (int i = lane; i < nb_n * WIN; i += 64) {
+            const int nl = i / WIN, j = i - nl * WIN;
+            const int wy = j / WW, wx = j - wy * WW;
+            Level l = nb;
+            if (l.codes) l.codes += (size_t)s_slot[nl] * level_len;
+            else l.plane += (size_t)s_slot[nl] * level_len;
+            s_win[i] = l.at(ox + s_seed[2 * nl] - R + wx, oy + s_seed[2 * nl + 1] - R + wy);
+        }
+        __syncthreads();
+        for (int task = lane; task < nb_n * SIDE; task += 64) {
+            const int nl = task / SIDE, dy = task - nl * SIDE;
+            uint32_t sad[SIDE];
+#pragma unroll
+            for (int j = 0; j < SIDE; ++j) sad[j] = 0;
+            const uint32_t *wbase = s_win + nl * WIN + dy * WW;
+#pragma unroll 2
+            for (int py = 0; py < BLK; ++py) {
+                // Rows start on 16-byte boundaries: BLK and WW are multiples of 4 words, and the
+                // window area starts (BLK^2 + 80) words into the 16-byte aligned LDS block.
+                const uint32_t *brow = (const uint32_t *)__builtin_assume_aligned(s_blk + py * BLK, 16);
+                const uint32_t *wrow = (const uint32_t *)__builtin_assume_aligned(wbase + py * WW, 16);
+                uint32_t b[BLK], w[WW];
+#pragma unroll
+                for (int x = 0; x < BLK; ++x) b[x] = brow[x];
+#pragma unroll
+                for (int x = 0; x < WW; ++x) w[x] = wrow[x];
+#pragma unroll
+                for (int j = 0; j < SIDE; ++j)
+#pragma unroll
+                    for (int x = 0; x < BLK; ++x) sad[j] = sad_u32(b[x], w[x + j], sad[j]);
+            }
+            uint32_t best = ~0u;
+#pragma unroll
+            for (int j = 0; j < SIDE; ++j) best = min(best, (sad[j] << 9) | (uint32_t)(dy * SIDE + j));
+            atomicMin(&s_key[nl], best);
+            if (dy == R) s_csad[nl] = sad[R];
+        }
+        __syncthreads();
+        if (lane < nb_n) {
+            const uint32_t key = s_key[lane];
+            const uint32_t best = key >> 9;
+            const int bi = (int)(key & 511u);
+            const int sdx = s_seed[2 * lane], sdy = s_seed[2 * lane + 1];
+            if (s_csad[lane] <= best) finish(n0 + lane, sdx, sdy, s_csad[lane]);
+            else finish(n0 + lane, sdx + (bi % SIDE) - R, sdy + (bi / SIDE) - R, best);
+        }
+        __syncthreads();
+    }
 }
 
 // Coarse pass (block_match.rs:117-211) on level 1 around a zero seed; the winner times 2 seeds every
@@ -223,6 +323,88 @@ __global__ __launch_bounds__(kBmThreads) void k_bm_fine(const uint16_t *__restri
         lds);
 }
 
+template <int BLK, int R>
+__global__ __launch_bounds__(64) void k_bm_coarse_rows(const int32_t *__restrict__ pyr1, int lw, int lh, int step,
+                                                       int centre_slot, const uint32_t *__restrict__ nslots,
+                                                       int neighbours, int32_t *__restrict__ mv, int fine_bx,
+                                                       int fine_by, int fine_step)
+{
+    extern __shared__ __align__(16) uint32_t lds_rows[];
+    const int bc = blockIdx.x, br = blockIdx.y;
+    const size_t lp = (size_t)lw * lh;
+    const Level cen{nullptr, pyr1 + (size_t)centre_slot * lp, lw, lh, 1};
+    const Level nb{nullptr, pyr1, lw, lh, 1};
+    const int c0 = min(bc * step * 2 / fine_step, fine_bx), r0 = min(br * step * 2 / fine_step, fine_by);
+    const int c1 = (bc == (int)gridDim.x - 1) ? fine_bx : min((bc + 1) * step * 2 / fine_step, fine_bx);
+    const int r1 = (br == (int)gridDim.y - 1) ? fine_by : min((br + 1) * step * 2 / fine_step, fine_by);
+    match_rows<BLK, R>(
+        cen, nb, lp, nslots, neighbours, bc * step, br * step,
+        [](int, int &sdx, int &sdy) {
+            sdx = 0;
+            sdy = 0;
+        },
+        [&](int n, int dx, int dy, uint32_t) {
+            int32_t *m = mv + (size_t)n * fine_bx * fine_by * 2;
+            for (int fr = r0; fr < r1; ++fr)
+                for (int fc = c0; fc < c1; ++fc) {
+                    m[((size_t)fr * fine_bx + fc) * 2] = dx * 2;
+                    m[((size_t)fr * fine_bx + fc) * 2 + 1] = dy * 2;
+                }
+        },
+        lds_rows);
+}
+
+template <int BLK, int R>
+__global__ __launch_bounds__(64) void k_bm_fine_rows(const uint16_t *__restrict__ codes, int W, int H, int C, int step,
+                                                     int use_seed, int centre_slot,
+                                                     const uint32_t *__restrict__ nslots, int neighbours,
+                                                     int32_t *__restrict__ mv, float *__restrict__ conf,
+                                                     float max_value, float sad_floor, float thsad)
+{
+    extern __shared__ __align__(16) uint32_t lds_rows[];
+    const int bc = blockIdx.x, br = blockIdx.y;
+    const int bx = gridDim.x, by = gridDim.y;
+    const size_t sp = (size_t)W * H * C;
+    const Level cen{codes + (size_t)centre_slot * sp, nullptr, W, H, C};
+    const Level nb{codes, nullptr, W, H, C};
+    const size_t blkid = (size_t)br * bx + bc;
+    const size_t per_n = (size_t)bx * by;
+    match_rows<BLK, R>(
+        cen, nb, sp, nslots, neighbours, bc * step, br * step,
+        [&](int n, int &sdx, int &sdy) {
+            const size_t b = (size_t)n * per_n + blkid;
+            sdx = use_seed ? mv[b * 2] : 0;
+            sdy = use_seed ? mv[b * 2 + 1] : 0;
+        },
+        [&](int n, int dx, int dy, uint32_t sad) {
+            const size_t b = (size_t)n * per_n + blkid;
+            mv[b * 2] = dx;
+            mv[b * 2 + 1] = dy;
+            if (conf) {
+                float excess = (float)sad / max_value - sad_floor;
+                if (excess < 0.0f) excess = 0.0f;
+                const float t2 = thsad * thsad, e2 = excess * excess;
+                const float c = (t2 - e2) / (t2 + e2);
+                conf[b] = c < 0.0f ? 0.0f : c;
+            }
+        },
+        lds_rows);
+}
+
+// Specialised forms exist for these block sizes and radii.
+inline bool rows_form(int blk, int R) { return (blk == 8 || blk == 16 || blk == 32) && (R == 2 || R == 4 || R == 8); }
+
+#define AVD_ROWS_DISPATCH(BLKV, RV, CALL)                                                                   \
+    do {                                                                                                    \
+        if ((BLKV) == 8) {                                                                                  \
+            if ((RV) == 2) CALL(8, 2); else if ((RV) == 4) CALL(8, 4); else CALL(8, 8);                     \
+        } else if ((BLKV) == 16) {                                                                          \
+            if ((RV) == 2) CALL(16, 2); else if ((RV) == 4) CALL(16, 4); else CALL(16, 8);                  \
+        } else {                                                                                            \
+            if ((RV) == 2) CALL(32, 2); else if ((RV) == 4) CALL(32, 4); else CALL(32, 8);                  \
+        }                                                                                                   \
+    } while (0)
+
 __global__ __launch_bounds__(256) void k_warp(const float *__restrict__ src, float *__restrict__ dst,
                                               const int32_t *__restrict__ mv, int W, int H, int C, int step,
                                               int blocks_x, int blocks_y)
@@ -264,17 +446,34 @@ hipError_t launch_motion(const MotionGeom &g, const uint16_t *codes, const int32
         const int lw = max(g.W / 2, 1), lh = max(g.H / 2, 1);
         const int cblk = max(g.blksize / 2, 2), cstep = max(g.step / 2, 1);
         const dim3 grid(max((int)div_up(lw, cstep), 1), max((int)div_up(lh, cstep), 1));
-        k_bm_coarse<<<grid, kBmThreads, match_lds_bytes(cblk, g.radius, neighbours), s>>>(
-            pyr1, lw, lh, cblk, cstep, g.radius, centre_slot, nslots_dev, neighbours, mv, g.blocks_x, g.blocks_y,
-            g.step);
+        if (rows_form(cblk, g.radius)) {
+#define AVD_COARSE(B, RR)                                                                                    \
+    k_bm_coarse_rows<B, RR><<<grid, 64, match_rows_lds_bytes(B, RR, neighbours), s>>>(                       \
+        pyr1, lw, lh, cstep, centre_slot, nslots_dev, neighbours, mv, g.blocks_x, g.blocks_y, g.step)
+            AVD_ROWS_DISPATCH(cblk, g.radius, AVD_COARSE);
+#undef AVD_COARSE
+        } else {
+            k_bm_coarse<<<grid, kBmThreads, match_lds_bytes(cblk, g.radius, neighbours), s>>>(
+                pyr1, lw, lh, cblk, cstep, g.radius, centre_slot, nslots_dev, neighbours, mv, g.blocks_x,
+                g.blocks_y, g.step);
+        }
         use_seed = 1;
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     const dim3 grid(g.blocks_x, g.blocks_y);
-    k_bm_fine<<<grid, kBmThreads, match_lds_bytes(g.blksize, g.radius, neighbours), s>>>(
-        codes, g.W, g.H, g.C, g.blksize, g.step, g.radius, use_seed, centre_slot, nslots_dev, neighbours, mv, conf,
-        max_value, sad_floor, thsad);
+    if (rows_form(g.blksize, g.radius)) {
+#define AVD_FINE(B, RR)                                                                                      \
+    k_bm_fine_rows<B, RR><<<grid, 64, match_rows_lds_bytes(B, RR, neighbours), s>>>(                         \
+        codes, g.W, g.H, g.C, g.step, use_seed, centre_slot, nslots_dev, neighbours, mv, conf, max_value,    \
+        sad_floor, thsad)
+        AVD_ROWS_DISPATCH(g.blksize, g.radius, AVD_FINE);
+#undef AVD_FINE
+    } else {
+        k_bm_fine<<<grid, kBmThreads, match_lds_bytes(g.blksize, g.radius, neighbours), s>>>(
+            codes, g.W, g.H, g.C, g.blksize, g.step, g.radius, use_seed, centre_slot, nslots_dev, neighbours, mv,
+            conf, max_value, sad_floor, thsad);
+    }
     return hipGetLastError();
 }
 

@@ -25,8 +25,15 @@ W, H, DEPTH, RADIUS, SPATIAL, REFINE = 3840, 2160, 10, 3, 12, 2
 SIGMA_Y, SIGMA_C = 16.0, 8.0     # noise of the clip, in 10-bit codes
 LAMBDA = 2.7
 CLIP_FRAMES = 16
-FP32_VECTOR_PEAK_TOPS = 157.3    # MI355X published FP32 vector peak, TFLOP/s
-OPS_PER_PIXEL_CANDIDATE = 3      # subtract, multiply, accumulate per sample per candidate
+# MI355X published FP32 rate, 157.3 TFLOP/s; the grouping, motion and filter kernels are VALU work
+# (v_dot2 / v_sad integer ops and f32 FMAs), priced against it.
+VALU_PEAK_TOPS = 157.3
+OP_STAGES = ["collab_search", "motion", "collab_filter"]
+STAGE_KERNELS = {
+    "collab_search": "k_centre_codes + k_trajectory_codes",
+    "motion": "k_bm_coarse + k_bm_fine",
+    "collab_filter": "k_collab_filter",
+}
 
 
 def parse():
@@ -124,17 +131,27 @@ def main():
     assert emitted == args.steps, (emitted, args.steps)
     fps = args.steps / elapsed
 
-    # Per-stage device time over a few more steps, with HIP events on the engine's stream.
+    # Per-stage device time over a few more steps, with HIP events on the engine's stream; the
+    # engine counts each stage's algorithmic operations (search: 192 per candidate patch and
+    # channel; motion: 3 per pixel of every scored block offset; filter: 8 members x (C x 4928 +
+    # 64 x (C + 1)) per reference).
     eng.sample_device_time(1)
     run_steps(eng, frames, outs, warm + args.steps, args.sample_steps)
     stages = eng.timing()
     eng.sample_device_time(0)
-    dominant = max(stages, key=lambda k: stages[k]["ms"])
-    search = stages["collab_search"]
-    search_ms = search["ms"] / max(search["launches"], 1)
-    # Work counts candidate patches scored times channels; each costs 64 samples x 3 operations.
-    ops_per_launch = search["work"] / max(search["launches"], 1) * 64 * OPS_PER_PIXEL_CANDIDATE
-    achieved_tops = ops_per_launch / (search_ms * 1e-3) / 1e12
+    rooflines = {}
+    for name in OP_STAGES:
+        st = stages[name]
+        if not st["launches"] or st["ms"] <= 0:
+            continue
+        avg_ms = st["ms"] / st["launches"]
+        ops = st["work"] / st["launches"]
+        tops = ops / (avg_ms * 1e-3) / 1e12
+        rooflines[name] = {"kernels": STAGE_KERNELS[name], "avg_launch_ms": round(avg_ms, 4),
+                           "ops_per_launch": int(ops), "achieved": round(tops, 3),
+                           "frac": round(tops / VALU_PEAK_TOPS, 4)}
+    dominant = max(rooflines, key=lambda k: stages[k]["ms"])
+    dom = rooflines[dominant]
 
     result = {
         "metric": METRIC,
@@ -158,16 +175,17 @@ def main():
             "mpix_per_s": round(fps * W * H / 1e6, 3),
         },
         "roofline": {
-            "kernel": "collab_search",
+            "kernel": f"{dominant} ({dom['kernels']})",
             "bound": "mfma",
-            "achieved": round(achieved_tops, 3),
-            "peak": FP32_VECTOR_PEAK_TOPS,
+            "achieved": dom["achieved"],
+            "peak": VALU_PEAK_TOPS,
             "unit": "TFLOP/s",
-            "frac": round(achieved_tops / FP32_VECTOR_PEAK_TOPS, 4),
+            "frac": dom["frac"],
             "traffic": None,
-            "avg_launch_ms": round(search_ms, 4),
-            "candidate_channels_per_launch": int(search["work"] / max(search["launches"], 1)),
+            "avg_launch_ms": dom["avg_launch_ms"],
+            "ops_per_launch": dom["ops_per_launch"],
         },
+        "stage_rooflines": rooflines,
         "stages_ms_per_frame": {k: round(v["ms"] / max(args.sample_steps, 1), 4) for k, v in stages.items()
                                 if v["launches"]},
         "dominant_stage": dominant,

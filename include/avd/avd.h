@@ -208,9 +208,14 @@ avd_status avd_engine_fail_for_test(avd_engine *engine);
 
 /* Device time per stage (EngineTiming / DeviceTimer, engine/timing.rs:13-200): with sampling on,
  * every launch of a stage is bracketed by HIP events on the engine's stream.  `work` counts the
- * stage's algorithmic units: candidate patches scored times channels for AVD_STAGE_COLLAB_SEARCH, references
- * filtered for AVD_STAGE_COLLAB_FILTER, blocks x neighbours for AVD_STAGE_MOTION, pixels for the
- * rest. */
+ * stage's algorithmic operations: AVD_STAGE_COLLAB_SEARCH 192 per candidate patch scored per
+ * channel (64 samples x subtract, multiply, add); AVD_STAGE_MOTION 3 per pixel of every scored
+ * block offset (subtract, absolute value, add) on both pyramid levels; AVD_STAGE_COLLAB_FILTER per
+ * reference 8 members x (C x (4096 + 576 + 256) + 64 x (C + 1)) (forward and inverse 2-D DCT as
+ * 8x8x8 products, three Haar levels of 3 operations, 4 per threshold, one fixed-point add per
+ * accumulator); AVD_STAGE_NLM_PAIR and AVD_STAGE_NLM_SINGLE (5C + 2(2P + 1) + 7) per candidate
+ * pixel (squared differences, channel scale, separable box sum, weight and accumulation);
+ * pixels for the rest. */
 enum {
     AVD_STAGE_INGEST = 0,
     AVD_STAGE_MOTION = 1,
@@ -275,13 +280,19 @@ avd_status avd_site_collab_search(const avd_collab_pass *pass, const uint16_t *r
                                   const uint32_t *neighbour_slots, const int32_t *mv_field, const float *confidence,
                                   uint32_t *members, void *stream);
 /* filter_group over every reference (collab_filter, fused/split.rs:181-281): transforms, hard
- * threshold and the fixed-point scatter into accum (slots*px*channels i32) and wsum (slots*px i32). */
+ * threshold and the fixed-point scatter (aggregate.rs:147-211).  The reference keeps an i32 sum per
+ * pixel and channel and an i32 weight sum per pixel; here `aggregate` (slots*px u64) holds, per
+ * pixel, weight_sum * 2^32 + first_channel_sum (the sum sign-extended), so one 64-bit atomic add
+ * scatters both; both sums stay within i32 by the accumulation scale, so the low word is the sum
+ * and the high word plus (sum < 0) the weight sum.  `accum_second` (slots*px i32) holds the second
+ * channel's sum when channels == 2 and may be NULL otherwise. */
 avd_status avd_site_collab_filter(const avd_collab_pass *pass, const float *ring_values,
-                                  const uint32_t *neighbour_slots, const uint32_t *members, int32_t *accum,
-                                  int32_t *wsum, float *group_weight, void *stream);
-/* collab_normalise (aggregate.rs:252-290) of one region, which it leaves zeroed. */
-avd_status avd_site_collab_normalise(int32_t *accum_region, int32_t *wsum_region, float *out, uint32_t pixels,
-                                     uint32_t channels, void *stream);
+                                  const uint32_t *neighbour_slots, const uint32_t *members, uint64_t *aggregate,
+                                  int32_t *accum_second, float *group_weight, void *stream);
+/* collab_normalise (aggregate.rs:252-290) of one region (aggregate and, for two channels,
+ * accum_second), which it leaves zeroed. */
+avd_status avd_site_collab_normalise(uint64_t *aggregate_region, int32_t *accum_second_region, float *out,
+                                     uint32_t pixels, uint32_t channels, void *stream);
 
 /* Motion of every neighbour slot against the centre (mc_block_match_coarse + mc_block_match_fine
  * with confidence, motion/analyse.rs:175-267) on the lane-0 codes of an interleaved u16 ring.

@@ -74,17 +74,27 @@ def site_collab_search(p, codes=None, values=None, nslots=None, mv=None, conf=No
     return host(members).view(np.uint32)
 
 
+def unpack_aggregate(agg: np.ndarray, second: np.ndarray | None, C: int):
+    """The filter's packed accumulators as the reference's (accum [px*C] i32, wsum [px] i32)."""
+    a = (agg & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+    w = ((agg >> 32) & 0xFFFFFFFF).astype(np.uint32).view(np.int32) + (a < 0).astype(np.int32)
+    accum = a if C == 1 else np.stack([a, second], 1).reshape(-1)
+    return accum, w
+
+
 def site_collab_filter(p, values, nslots, members, frames):
     refs = O.refs_along(p.width) * O.refs_along(p.height)
     px = p.width * p.height
-    accum = torch.zeros(frames * px * p.channels, dtype=torch.int32, device="cuda")
-    wsum = torch.zeros(frames * px, dtype=torch.int32, device="cuda")
+    agg = torch.zeros(frames * px, dtype=torch.int64, device="cuda")
+    second = torch.zeros(frames * px, dtype=torch.int32, device="cuda") if p.channels == 2 else None
     gw = torch.zeros(refs, dtype=torch.float32, device="cuda")
     ns = dev(np.asarray(nslots if nslots else [0], np.uint32).view(np.int32))
     md = dev(members.view(np.int32))
-    L.check(avd.load().avd_site_collab_filter(ctypes.byref(p), ptr(values), ptr(ns), ptr(md), ptr(accum), ptr(wsum),
-                                              ptr(gw), None))
-    return host(accum), host(wsum), host(gw)
+    L.check(avd.load().avd_site_collab_filter(ctypes.byref(p), ptr(values), ptr(ns), ptr(md), ptr(agg),
+                                              ptr(second), ptr(gw), None))
+    accum, wsum = unpack_aggregate(host(agg).view(np.uint64), host(second) if second is not None else None,
+                                   p.channels)
+    return accum, wsum, host(gw)
 
 
 def gray_geometry(W, H, depth_in, depth_out=None):
