@@ -99,7 +99,7 @@ class CollabPass(ctypes.Structure):
                 ("blksize", u32), ("blocks_x", u32), ("blocks_y", u32), ("centre_slot", u32), ("slots", u32),
                 ("c_min", f32), ("lam", f32 * 2), ("sigma", f32 * 2), ("weight_scale", f32),
                 ("accum_scale", f32), ("kaiser", f32 * 8), ("profile", f32 * 8),
-                ("code_bits", u32)]
+                ("code_bits", u32), ("pool_ratio", f32)]
 
 
 class MotionPass(ctypes.Structure):

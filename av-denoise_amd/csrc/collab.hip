@@ -1106,28 +1106,73 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
         const float sig2 = p.sigma[c] * p.sigma[c];
         const float lam = p.lambda[c];
         const float pv = p.profile[v];
-        uint32_t counts = 0;  // kept coefficients per u, one nibble each
-        float cvs[8];
+        float part = 0.0f;  // retained variance of vertical frequency v (every member lane agrees)
+        if (p.pool_ratio > 0.0f) {
+            // pooled_threshold (fused/pooled.rs:24-147): a coefficient's energy is
+            // c^2 / max(sigma^2 prof[v], 1e-20) / max(prof[u], 1e-20); the spatial DC counts as zero
+            // while it stands in as a neighbour; the pooled sum (self + below + above, then left,
+            // then right) must reach threshold^2 times the in-range neighbour count; the spatial DC
+            // takes its own test at lambda and the group DC is always kept.
+            const float lv = sig2 * pv;
+            const float li = 1.0f / fmaxf(lv, 1e-20f);
+            float e[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            cvs[u] = sig2 * (p.profile[u] * pv);
-            const float thr = lam * sqrtf(cvs[u]);
-            bool keep = fabsf(row[u]) >= thr;
-            if (u == 0 && m == 0 && v == 0) keep = true;
-            keep = keep && in_group;
-            if (keep) counts += 1u << (4 * u);
-            else if (in_group) row[u] = 0.0f;
-        }
-        counts += (uint32_t)__shfl_xor((int)counts, 8);
-        counts += (uint32_t)__shfl_xor((int)counts, 16);
-        counts += (uint32_t)__shfl_xor((int)counts, 32);
-        // Retained variance: each lane (m, v) adds, for every u, the variance of (u, v) once per
-        // member that kept it; the eight v lanes then sum by butterfly.
-        float part = 0.0f;
+            for (int u = 0; u < 8; ++u) e[u] = row[u] * row[u] * li * (1.0f / fmaxf(p.profile[u], 1e-20f));
+            if (v == 0) e[0] = 0.0f;
+            const bool has_up = v > 0, has_down = v < 7;
+            const float vertical = 1.0f + (has_up ? 1.0f : 0.0f) + (has_down ? 1.0f : 0.0f);
+            const float threshold = lam * p.pool_ratio;
+            const float bar = threshold * threshold;
+            const float bar_first = bar * (vertical + (v == 1 ? 0.0f : 1.0f));
+            const float bar_second = bar * (vertical + (v == 0 ? 1.0f : 2.0f));
+            const float bar_middle = bar * (vertical + 2.0f), bar_last = bar * (vertical + 1.0f);
+            float kept_profile = 0.0f;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int n = (counts >> (4 * u)) & 15;
-            for (int i = 0; i < n; ++i) part += cvs[u];
+            for (int u = 0; u < 8; ++u) {
+                const float above = __shfl_up(e[u], 1), below = __shfl_down(e[u], 1);
+                float sum = e[u] + (has_down ? below : 0.0f) + (has_up ? above : 0.0f);
+                if (u > 0) sum += e[u - 1];
+                if (u < 7) sum += e[u + 1];
+                const float bar_u = u == 0 ? bar_first : (u == 1 ? bar_second : (u == 7 ? bar_last : bar_middle));
+                bool keep = sum >= bar_u;
+                if (u == 0) {
+                    const bool own = fabsf(row[0]) >= lam * sqrtf(lv * p.profile[0]);
+                    if (v == 0) keep = own;
+                    if (m == 0) keep = keep || v == 0;
+                }
+                kept_profile += keep ? p.profile[u] : 0.0f;
+                if (in_group && !keep) row[u] = 0.0f;
+            }
+            // The reference's lane v adds each member's kept profile times its variance, in member
+            // order; lane (j, v) holds member j's.
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float kp = __shfl(kept_profile, (j << 3) | v);
+                if (j < k_use) part += kp * lv;
+            }
+        } else {
+            uint32_t counts = 0;  // kept coefficients per u, one nibble each
+            float cvs[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                cvs[u] = sig2 * (p.profile[u] * pv);
+                const float thr = lam * sqrtf(cvs[u]);
+                bool keep = fabsf(row[u]) >= thr;
+                if (u == 0 && m == 0 && v == 0) keep = true;
+                keep = keep && in_group;
+                if (keep) counts += 1u << (4 * u);
+                else if (in_group) row[u] = 0.0f;
+            }
+            counts += (uint32_t)__shfl_xor((int)counts, 8);
+            counts += (uint32_t)__shfl_xor((int)counts, 16);
+            counts += (uint32_t)__shfl_xor((int)counts, 32);
+            // Retained variance: each lane (m, v) adds, for every u, the variance of (u, v) once per
+            // member that kept it; the eight v lanes then sum by butterfly.
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int n = (counts >> (4 * u)) & 15;
+                for (int i = 0; i < n; ++i) part += cvs[u];
+            }
         }
         float sum = part;
         sum += __shfl_xor(sum, 1);

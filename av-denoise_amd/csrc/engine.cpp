@@ -590,7 +590,6 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
         throw Failure(AVD_ERR_INVALID_OPTIONS, "search escalation is not built yet");
     if (!o.has_sigma)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "noise estimation is not built yet; pass a fixed sigma");
-    if (o.pooled_threshold) throw Failure(AVD_ERR_INVALID_OPTIONS, "the pooled threshold is not built yet");
     if (o.grain_data) throw Failure(AVD_ERR_INVALID_OPTIONS, "grain measurement is not built yet");
     if (o.step != 4 || o.group_size != 8)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "this build runs the reference grid: step 4, group size 8");
@@ -628,6 +627,8 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     pass_.accum_scale = cross_frame_accum_scale(o.spatial_radius, r_, o.step);
     kaiser_window(o.kaiser_beta, pass_.kaiser);
     pass_.code_bits = in.kind == AVD_FORMAT_U8 ? 8u : in.depth;
+    // nl4d_pool_ratio (nl4d/options.rs:109-117): the calibrated 2.42 over the plane's default lambda.
+    pass_.pool_ratio = o.pooled_threshold ? 2.42f / (C == 1 ? 4.158f : 3.234f) : 0.0f;
 
     ring_.init(W, H, C, N_, true);
     const size_t px = (size_t)W * H;

@@ -270,6 +270,8 @@ typedef struct avd_collab_pass {
     float kaiser[8];
     float profile[8];
     uint32_t code_bits;        /* every u16 code is below 2^code_bits; 0 when unknown (up to 16) */
+    float pool_ratio;          /* > 0: pooled threshold at lambda * pool_ratio (fused/pooled.rs);
+                                  0: the per-coefficient hard threshold */
 } avd_collab_pass;
 
 /* search_group over every reference (collab_search, fused/split.rs:34-170): writes 9 words per

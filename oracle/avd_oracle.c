@@ -349,6 +349,7 @@ typedef struct {
     float accum_scale;
     float kaiser[8];
     float profile[8];   /* dct_noise_profile, all ones at rho = 0 (transforms.rs:196-221) */
+    float pool_ratio;   /* > 0: pooled threshold at lambda * pool_ratio (nl4d/options.rs:109-117) */
 } avdo_collab_params;
 
 static inline uint32_t pack_pos_t(uint32_t x, uint32_t y, uint32_t t) { return (t << 26) | (y << 13) | x; }
@@ -661,6 +662,71 @@ static double dct8_basis(int row, int col)
         R c = w < (R)0 ? (R)0 : (w > (R)1 ? (R)1 : w);                                              \
         return (int32_t)ROUND(c * scale * (R)5);                                                    \
     }                                                                                               \
+    /* pooled_threshold (collab/kernels/fused/pooled.rs:24-147) over one group in the layout      */ \
+    /* stk[j * 64 + v * 8 + u]: lane v walks each member row j left to right; a coefficient's      */ \
+    /* energy is c^2 / max(var_j * prof[v], 1e-20) / max(prof[u], 1e-20), the spatial DC counts as  */ \
+    /* zero while it stands in as a neighbour, the pooled sum (self + down + up, then left, then   */ \
+    /* right) meets threshold^2 times the in-range neighbour count, the spatial DC takes its own   */ \
+    /* test at dc_lambda and the group DC is always kept.  retained[v] gains, per member in order, */ \
+    /* the member's kept profile sum times its lane variance.                                     */ \
+    static void pooled##SFX(R *stk, const R *var, const float *prof, int k_use, R threshold, R dc_lambda,   \
+                            R *retained)                                                            \
+    {                                                                                               \
+        const R bar = threshold * threshold;                                                        \
+        R orig[8 * 64]; /* every lane reads the coefficients before any is dropped */               \
+        for (int i = 0; i < 8 * 64; ++i) orig[i] = stk[i];                                          \
+        for (int v = 0; v < 8; ++v) {                                                               \
+            const int dc_lane = v == 0, has_up = v > 0, has_down = v < 7;                           \
+            const R vertical = (R)1 + (has_up ? (R)1 : (R)0) + (has_down ? (R)1 : (R)0);           \
+            const R bars[4] = {bar * (vertical + (v == 1 ? (R)0 : (R)1)), bar * (vertical + (dc_lane ? (R)1 : (R)2)), \
+                               bar * (vertical + (R)2), bar * (vertical + (R)1)};                  \
+            R acc = 0;                                                                              \
+            for (int j = 0; j < 8; ++j) {                                                           \
+                R e[3][8]; /* lanes v-1, v, v+1 */                                                  \
+                for (int d = -1; d <= 1; ++d) {                                                     \
+                    const int vv = v + d;                                                           \
+                    for (int u = 0; u < 8; ++u) {                                                   \
+                        R en = 0;                                                                   \
+                        if (vv >= 0 && vv < 8) {                                                    \
+                            R lv = var[j] * (R)prof[vv];                                            \
+                            R li = (R)1 / (lv > (R)1e-20f ? lv : (R)1e-20f);                        \
+                            R pi = (R)1 / ((R)prof[u] > (R)1e-20f ? (R)prof[u] : (R)1e-20f);        \
+                            R cf = orig[j * 64 + vv * 8 + u];                                       \
+                            en = cf * cf * li * pi;                                                 \
+                            if (vv == 0 && u == 0) en = 0;                                          \
+                        }                                                                           \
+                        e[d + 1][u] = en;                                                           \
+                    }                                                                               \
+                }                                                                                   \
+                const R lane_variance = var[j] * (R)prof[v];                                        \
+                R kept_profile = 0;                                                                 \
+                for (int u = 0; u < 8; ++u) {                                                       \
+                    R up = has_up ? e[0][u] : (R)0, down = has_down ? e[2][u] : (R)0;              \
+                    R sum = e[1][u] + down + up;                                                    \
+                    if (u > 0) sum += e[1][u - 1];                                                  \
+                    if (u < 7) sum += e[1][u + 1];                                                  \
+                    const R cb = u == 0 ? bars[0] : (u == 1 ? bars[1] : (u == 7 ? bars[3] : bars[2])); \
+                    int keep = sum >= cb;                                                           \
+                    R *cf = &stk[j * 64 + v * 8 + u];                                               \
+                    if (u == 0) {                                                                   \
+                        R dcv = lane_variance * (R)prof[0];                                         \
+                        int own = (*cf < 0 ? -*cf : *cf) >= dc_lambda * SQRT(dcv);                  \
+                        if (dc_lane) keep = own;                                                    \
+                        if (j == 0) keep = keep || dc_lane;                                         \
+                    }                                                                               \
+                    kept_profile += keep ? (R)prof[u] : (R)0;                                       \
+                    if (j < k_use && !keep) *cf = 0;                                                \
+                }                                                                                   \
+                if (j < k_use) acc += kept_profile * lane_variance;                                 \
+            }                                                                                       \
+            retained[v] = acc;                                                                      \
+        }                                                                                           \
+    }                                                                                               \
+    void avdo_pooled_threshold##SFX(R *stk, const R *var, const float *prof, int k_use, R threshold,  \
+                                    R dc_lambda, R *retained)                                       \
+    {                                                                                               \
+        pooled##SFX(stk, var, prof, k_use, threshold, dc_lambda, retained);                         \
+    }                                                                                               \
     /* filter_group for every reference: DCT, group Haar, hard threshold, weight, inverses and    */ \
     /* the fixed-point scatter into each member's own frame region.                               */ \
     void avdo_collab_filter##SFX(const float *values, const avdo_collab_params *pp, int centre_slot,   \
@@ -710,6 +776,13 @@ static double dct8_basis(int row, int col)
                         }                                                                           \
                     }                                                                               \
                     group_fwd##SFX(stk, k_use, grid, pp->grid_frames, f);                           \
+                    R part[8];                                                                      \
+                    if (pp->pool_ratio > 0.0f) {                                                    \
+                        R var[8];                                                                   \
+                        for (int j = 0; j < 8; ++j) var[j] = sig2;                                  \
+                        const R lam_c = (R)pp->lambda[c];                                           \
+                        pooled##SFX(stk, var, pp->profile, k_use, lam_c * (R)pp->pool_ratio, lam_c, part); \
+                    } else {                                                                        \
                     R cv[64], thr[64];                                                              \
                     for (int v = 0; v < 8; ++v)                                                     \
                         for (int u = 0; u < 8; ++u) {                                               \
@@ -730,12 +803,12 @@ static double dct8_basis(int row, int col)
                     /* retained_v: lane v (vertical frequency) adds each kept coefficient's        */ \
                     /* variance, horizontal frequency outermost, then the 8-lane butterfly         */ \
                     /* (fused/mod.rs:747-781, plane_ops.rs:17-23).                                 */ \
-                    R part[8];                                                                      \
                     for (int v = 0; v < 8; ++v) {                                                   \
                         R acc = 0;                                                                  \
                         for (int u = 0; u < 8; ++u)                                                 \
                             for (int n = 0; n < kept[v * 8 + u]; ++n) acc += cv[v * 8 + u];         \
                         part[v] = acc;                                                              \
+                    }                                                                               \
                     }                                                                               \
                     R sum = ((part[0] + part[1]) + (part[2] + part[3])) + ((part[4] + part[5]) + (part[6] + part[7])); \
                     R weight = (isnan(sum) || isinf(sum)) ? (R)0 : (R)1 / (sum > (R)1e-12f ? sum : (R)1e-12f); \

@@ -45,6 +45,7 @@ class CollabParams(ctypes.Structure):
         ("c_min", ctypes.c_float), ("lam", ctypes.c_float * 2), ("sigma", ctypes.c_float * 2),
         ("weight_scale", ctypes.c_float), ("accum_scale", ctypes.c_float),
         ("kaiser", ctypes.c_float * 8), ("profile", ctypes.c_float * 8),
+        ("pool_ratio", ctypes.c_float),
     ]
 
 
@@ -329,9 +330,24 @@ def warp(src, mv, W, H, C, geom: MotionGeom):
     return dst
 
 
+POOLED_THRESHOLD = 2.42  # nl4d/options.rs:109
+
+
+def default_lambda_ht(C: int) -> float:
+    """nl4d_default_lambda_ht (nl4d/options.rs:101-106): luma 4.158, chroma 3.234."""
+    return 4.158 if C == 1 else 3.234
+
+
+def pool_ratio(C: int) -> float:
+    """nl4d_pool_ratio (nl4d/options.rs:115-117): the calibrated pooled threshold over the plane's
+    default lambda, in f32."""
+    return float(np.float32(POOLED_THRESHOLD) / np.float32(default_lambda_ht(C)))
+
+
 def collab_params(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, geom: MotionGeom, step=4,
-                  profile=None, sigmas=None):
+                  profile=None, sigmas=None, pooled=False):
     p = CollabParams()
+    p.pool_ratio = pool_ratio(C) if pooled else 0.0
     p.W, p.H, p.C = W, H, C
     p.step = step
     p.refs_x, p.refs_y = refs_along(W, step), refs_along(H, step)
@@ -376,6 +392,22 @@ def collab_filter(values_ring, params: CollabParams, centre_slot, nslots, member
     fn(_p(values_ring, F32P), ctypes.byref(params), centre_slot, _p(ns, U32P), _p(members, U32P), _p(accum, I32P),
        _p(wsum, I32P), _p(gw, F32P))
     return gw
+
+
+def pooled_threshold(group, variances, profile, k_use, threshold, dc_lambda, precision="f32"):
+    """pooled_threshold (fused/pooled.rs:24-147) over one group given as [v][j][u] (the
+    reference's [sub][j][i]); returns the thresholded group in that layout and the per-lane
+    retained variances."""
+    dt = np.float32 if precision == "f32" else np.float64
+    ptr = F32P if precision == "f32" else ctypes.POINTER(ctypes.c_double)
+    stk = np.ascontiguousarray(np.transpose(np.asarray(group, dt), (1, 0, 2)))  # [j][v][u]
+    var = np.ascontiguousarray(variances, dt)
+    prof = np.ascontiguousarray(profile, np.float32)
+    ret = np.zeros(8, dt)
+    fn = lib().avdo_pooled_threshold_f32 if precision == "f32" else lib().avdo_pooled_threshold_f64
+    cast = ctypes.c_float if precision == "f32" else ctypes.c_double
+    fn(_p(stk, ptr), _p(var, ptr), _p(prof, F32P), int(k_use), cast(threshold), cast(dc_lambda), _p(ret, ptr))
+    return np.transpose(stk, (1, 0, 2)).copy(), ret
 
 
 def collab_normalise(accum_region, wsum_region, pixels, C):
@@ -556,6 +588,7 @@ class Nl4dSpec:
     thsad_scale: float = 1.0
     motion: MotionGeom = field(default_factory=lambda: MotionGeom(0, 0))
     step: int = 4
+    pooled: bool = False
 
 
 class Nl4dPass:
@@ -572,7 +605,7 @@ class Nl4dPass:
                                spec.motion.pyramid_levels)
         self.maxv = max_value(depth_in)
         self.params = collab_params(W, H, C, self.r, spec.spatial_radius, spec.refine, spec.c_min, spec.lambda_ht,
-                                    spec.sigma, spec.kaiser_beta, self.geom, spec.step)
+                                    spec.sigma, spec.kaiser_beta, self.geom, spec.step, pooled=spec.pooled)
         px = W * H
         self.accum = np.zeros(self.N * px * C, np.int32)
         self.wsum = np.zeros(self.N * px, np.int32)

@@ -43,9 +43,10 @@ def words_host(t: torch.Tensor) -> np.ndarray:
 
 
 def collab_pass(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, centre, slots, geom, profile=None,
-                code_bits=0):
+                code_bits=0, pooled=False):
     p = L.CollabPass()
     p.code_bits = code_bits
+    p.pool_ratio = O.pool_ratio(C) if pooled else 0.0
     p.width, p.height, p.channels = W, H, C
     p.step, p.spatial_radius, p.temporal_radius, p.refine, p.group_size = 4, S, radius, refine, 8
     p.blk_step, p.blksize, p.blocks_x, p.blocks_y = geom.step, geom.blksize, geom.blocks_x, geom.blocks_y

@@ -218,8 +218,9 @@ def test_codes_search_edge_cases_are_bit_exact(case):
     assert_members_equal(got, want, name)
 
 
-@pytest.mark.parametrize("C,radius,S", [(1, 2, 9), (2, 1, 6), (1, 3, 12), (1, 0, 5)])
-def test_collab_codes_search_and_filter_are_bit_exact(C, radius, S):
+@pytest.mark.parametrize("C,radius,S,pooled", [(1, 2, 9, False), (2, 1, 6, False), (1, 3, 12, False), (1, 0, 5, False),
+                                                (1, 2, 9, True), (2, 2, 6, True), (1, 0, 5, True)])
+def test_collab_codes_search_and_filter_are_bit_exact(C, radius, S, pooled):
     W, H, depth = 80, 56, 10
     frames = 2 * radius + 1
     clip = avd.synth.clip(frames, W, H, depth, 5 + C, chroma="444" if C == 2 else "gray", sigma_luma=12.0)
@@ -241,8 +242,9 @@ def test_collab_codes_search_and_filter_are_bit_exact(C, radius, S):
     mv = np.concatenate(mvs) if mvs else np.zeros(2, np.int32)
     conf = np.concatenate(confs) if confs else np.ones(1, np.float32)
     lam = 2.7
-    p = collab_pass(W, H, C, radius, S, 2, 0.05, lam, sigma, 2.0, centre, frames, geom, code_bits=depth)
-    op = O.collab_params(W, H, C, radius, S, 2, 0.05, lam, sigma, 2.0, geom)
+    p = collab_pass(W, H, C, radius, S, 2, 0.05, lam, sigma, 2.0, centre, frames, geom, code_bits=depth,
+                    pooled=pooled)
+    op = O.collab_params(W, H, C, radius, S, 2, 0.05, lam, sigma, 2.0, geom, pooled=pooled)
     want = O.collab_search(codes, op, centre, slots, mv, conf)
     d_codes = dev(codes.view(np.int16))
     got = site_collab_search(p, codes=d_codes, nslots=slots, mv=mv, conf=conf)
@@ -260,9 +262,9 @@ def test_collab_codes_search_and_filter_are_bit_exact(C, radius, S):
 
 # ---- engines ------------------------------------------------------------------------------
 
-def _nl4d_spec(r, S, sigma, lam=2.7):
+def _nl4d_spec(r, S, sigma, lam=2.7, pooled=False):
     return O.Nl4dSpec(temporal_radius=r, refine=2, spatial_radius=S, lambda_ht=lam, c_min=0.05, kaiser_beta=2.0,
-                      sigma=sigma, motion=O.MotionGeom(0, 0, 16, 8, 4, 2))
+                      sigma=sigma, motion=O.MotionGeom(0, 0, 16, 8, 4, 2), pooled=pooled)
 
 
 @pytest.mark.parametrize("frames,r", [(7, 1), (9, 2), (2, 2), (5, 3)])
@@ -303,6 +305,32 @@ def test_nl4d_engine_420_luma_and_chroma_are_bit_exact():
         assert np.array_equal(got[i][0], want_l[i][0]), f"luma {i}"
         assert np.array_equal(got[i][1], want_c[i][0]), f"U {i}"
         assert np.array_equal(got[i][2], want_c[i][1]), f"V {i}"
+
+
+@pytest.mark.parametrize("chroma", ["gray", "420"])
+def test_nl4d_engine_with_pooled_threshold_is_bit_exact(chroma):
+    """The reference's default threshold (pooled) through the engine, luma and 4:2:0 chroma."""
+    W, H, depth, r, S = 64, 48, 10, 2, 6
+    sigma = 16.0 / 1023
+    frames = 6
+    clip = avd.synth.clip(frames, W, H, depth, 91, chroma=chroma, sigma_luma=16.0, sigma_chroma=8.0)
+    geom = L.Geometry(W, H, 0 if chroma == "gray" else 1, L.SampleFormat.u16(10), L.SampleFormat.u16(10), 0)
+    opt = avd.nl4d_options(temporal_radius=r, spatial_radius=S, sigma=sigma, pooled_threshold=1, noise_map=0,
+                           motion={"estimation": 1})
+    eng = avd.build_engine("nl4d", geom, luma=opt, chroma=opt if chroma != "gray" else None)
+    dims = [(W, H)] if chroma == "gray" else [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
+    got = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f[:len(dims)]] for f in clip], depth, dims)
+    pl = O.Nl4dPass(_nl4d_spec(r, S, sigma, lam=4.158, pooled=True), W, H, 1, depth, depth)
+    want_l, _ = O.run_stream(pl, [[O.pack_plane(f[0].numpy(), depth)] for f in clip])
+    for i in range(frames):
+        assert np.array_equal(got[i][0], want_l[i][0]), f"luma {i}"
+    if chroma != "gray":
+        pc = O.Nl4dPass(_nl4d_spec(r, S, sigma, lam=3.234, pooled=True), W // 2, H // 2, 2, depth, depth)
+        want_c, _ = O.run_stream(pc, [[O.pack_plane(f[1].numpy(), depth), O.pack_plane(f[2].numpy(), depth)]
+                                      for f in clip])
+        for i in range(frames):
+            assert np.array_equal(got[i][1], want_c[i][0]), f"U {i}"
+            assert np.array_equal(got[i][2], want_c[i][1]), f"V {i}"
 
 
 def test_nl4d_chroma_passes_through_when_only_luma_is_cleaned():
