@@ -1053,13 +1053,16 @@ __device__ __forceinline__ void unpack_aggregate(unsigned long long v, int32_t &
     weight = (int32_t)(uint32_t)(v >> 32) + (accum < 0 ? 1 : 0);
 }
 
-template <int C>
+// STAGED: instead of atomics into the accumulators, each member's fixed-point values go to
+// `stage` ([ref][c][row][member][column] i32, one coalesced row per store) and every 32x32 tile a
+// member patch overlaps counts it in `tiles.count`; k_tile_gather then sums them per tile.
+template <int C, bool STAGED>
 __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p, const float *__restrict__ values,
                                                                const uint32_t *__restrict__ nslots,
                                                                const uint32_t *__restrict__ members,
                                                                unsigned long long *__restrict__ aggregate,
                                                                int32_t *__restrict__ accum_second,
-                                                               float *__restrict__ group_weight)
+                                                               float *__restrict__ group_weight, TileBins tiles)
 {
     __shared__ float s_t[kWaves][8][8][9];  // [wave][member][row][col], padded
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1194,7 +1197,36 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
         wave_sync();
         dct8_inv(col);  // rows of column q
 
-        if (live && in_group) {
+        if (STAGED) {
+            if (live) {
+                int32_t *st = tiles.stage + (((size_t)ref * C + c) * 8) * 64 + lane;
+                const float kq = p.kaiser[q];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const float w = scaled_weight * (p.kaiser[r] * kq);
+                    st[r * 64] = in_group ? to_fixed(col[r] * w, scale) : 0;
+                }
+                if (c == 0 && in_group && q == 0) {
+                    // Claim an entry in the list of every tile the patch overlaps; a full list sends
+                    // the (member, tile) pair to the overflow list.
+                    const uint32_t id = ((uint32_t)ref << 3) | (uint32_t)m;
+                    const int tx0 = mx >> kTileShift, tx1 = (mx + 7) >> kTileShift;
+                    const int ty0 = my >> kTileShift, ty1 = (my + 7) >> kTileShift;
+                    for (int ty = ty0; ty <= ty1; ++ty)
+                        for (int tx = tx0; tx <= tx1; ++tx) {
+                            const uint32_t tile = (uint32_t)slot * tiles.per_slot + ty * tiles.tiles_x + tx;
+                            const uint32_t k = atomicAdd(&tiles.count[tile], 1u);
+                            if (k < kTileCap) {
+                                tiles.list[(size_t)tile * kTileCap + k] = id;
+                            } else {
+                                const uint32_t o = atomicAdd(tiles.overflow_count, 1u);
+                                tiles.overflow[2 * (size_t)o] = id;
+                                tiles.overflow[2 * (size_t)o + 1] = tile;
+                            }
+                        }
+                }
+            }
+        } else if (live && in_group) {
             const size_t base = (size_t)slot * px;
             const float kq = p.kaiser[q];
 #pragma unroll
@@ -1206,6 +1238,121 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
                 else atomicAdd(&accum_second[pix], a);
             }
         }
+    }
+}
+
+// Adds member `id`'s staged patch (sums, and weights recomputed from the group weight in the
+// filter's own arithmetic) for the pixels of row r that fall in [x0, x0 + T) x [y0, y0 + T).
+template <int C, typename Add>
+__device__ __forceinline__ void member_row(const avd_collab_pass &p, const uint32_t *__restrict__ members,
+                                           const float *__restrict__ group_weight, const TileBins &tiles,
+                                           uint32_t id, int r, int x0, int y0, int T, Add &&add)
+{
+    const uint32_t ref = id >> 3, m = id & 7;
+    const uint32_t pos = members[(size_t)ref * 9 + m];
+    const int y = (int)((pos >> 13) & 0x1FFF) + r - y0;
+    if (y < 0 || y >= T) return;
+    const int xb = (int)(pos & 0x1FFF) - x0;
+    const float sw = group_weight[ref] * p.weight_scale;
+    const float kr = p.kaiser[r];
+    const int32_t *st = tiles.stage + ((size_t)ref * C * 8 + r) * 64 + m * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int x = xb + q;
+        if (x < 0 || x >= T) continue;
+        const float w = sw * (kr * p.kaiser[q]);
+        add(y * T + x, pack_aggregate(st[q], to_fixed_weight(w, p.accum_scale)), C == 2 ? st[8 * 64 + q] : 0);
+    }
+}
+
+// One workgroup per (slot, 32x32 tile): the listed members' positions and group weights are
+// fetched 256 at a time into LDS, then each wave walks members with lane (r, q) on pixel (r, q)
+// of the patch, adding its staged sum and its weight (recomputed from the group weight in the
+// filter's own arithmetic) to the tile in LDS.  The tile then adds to the accumulators once, as
+// the only writer of those pixels this pass (overflowing pairs are added by k_tile_overflow).
+template <int C>
+__global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const uint32_t *__restrict__ members,
+                                                     const float *__restrict__ group_weight, TileBins tiles,
+                                                     unsigned long long *__restrict__ aggregate,
+                                                     int32_t *__restrict__ accum_second)
+{
+    constexpr int T = 1 << kTileShift;
+    __shared__ unsigned long long s_agg[T * T];
+    __shared__ int32_t s_second[C == 2 ? T * T : 1];
+    __shared__ uint32_t s_id[256], s_pos[256];
+    __shared__ float s_sw[256];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t slot = tile / tiles.per_slot, in_slot = tile - slot * tiles.per_slot;
+    const int ty = (int)(in_slot / tiles.tiles_x), tx = (int)(in_slot - ty * tiles.tiles_x);
+    const int x0 = tx * T, y0 = ty * T;
+    for (int i = threadIdx.x; i < T * T; i += 256) {
+        s_agg[i] = 0;
+        if (C == 2) s_second[i] = 0;
+    }
+    const uint32_t n = min(tiles.count[tile], (uint32_t)kTileCap);
+    const uint32_t *list = tiles.list + (size_t)tile * kTileCap;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane >> 3, q = lane & 7;
+    const float kr_kq = p.kaiser[r] * p.kaiser[q];
+    const float scale = p.accum_scale;
+    for (uint32_t base = 0; base < n; base += 256) {
+        __syncthreads();
+        const uint32_t k = base + threadIdx.x;
+        if (k < n) {
+            const uint32_t id = list[k];
+            s_id[threadIdx.x] = id;
+            s_pos[threadIdx.x] = members[(size_t)(id >> 3) * 9 + (id & 7)];
+            s_sw[threadIdx.x] = group_weight[id >> 3] * p.weight_scale;
+        }
+        __syncthreads();
+        const uint32_t batch = min(256u, n - base);
+        for (uint32_t e = wave; e < batch; e += 4) {
+            const uint32_t id = s_id[e], pos = s_pos[e];
+            const int lx = (int)(pos & 0x1FFF) + q - x0, ly = (int)((pos >> 13) & 0x1FFF) + r - y0;
+            if (lx < 0 || lx >= T || ly < 0 || ly >= T) continue;
+            const int32_t *st = tiles.stage + ((size_t)(id >> 3) * C * 8) * 64 + r * 64 + (id & 7) * 8 + q;
+            const float w = s_sw[e] * kr_kq;
+            atomicAdd(&s_agg[ly * T + lx], pack_aggregate(st[0], to_fixed_weight(w, scale)));
+            if (C == 2) atomicAdd(&s_second[ly * T + lx], st[8 * 64]);
+        }
+    }
+    __syncthreads();
+    const size_t px = (size_t)p.width * p.height;
+    for (int i = threadIdx.x; i < T * T; i += 256) {
+        const int lx = i & (T - 1), ly = i >> kTileShift;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x >= (int)p.width || y >= (int)p.height) continue;
+        const size_t pix = (size_t)slot * px + (size_t)y * p.width + x;
+        if (s_agg[i]) aggregate[pix] += s_agg[i];
+        if (C == 2 && s_second[i]) accum_second[pix] += s_second[i];
+    }
+}
+
+// The (member, tile) pairs that did not fit a tile's list: thread (pair, row) adds its row's pixels
+// inside the pair's tile with global atomics.
+template <int C>
+__global__ __launch_bounds__(256) void k_tile_overflow(avd_collab_pass p, const uint32_t *__restrict__ members,
+                                                       const float *__restrict__ group_weight, TileBins tiles,
+                                                       unsigned long long *__restrict__ aggregate,
+                                                       int32_t *__restrict__ accum_second)
+{
+    constexpr int T = 1 << kTileShift;
+    const uint32_t pairs = *tiles.overflow_count;
+    const size_t px = (size_t)p.width * p.height;
+    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < pairs * 8; k += gridDim.x * 256) {
+        const uint32_t pair = k >> 3;
+        const int r = (int)(k & 7);
+        const uint32_t id = tiles.overflow[2 * (size_t)pair], tile = tiles.overflow[2 * (size_t)pair + 1];
+        const uint32_t slot = tile / tiles.per_slot, in_slot = tile - slot * tiles.per_slot;
+        const int ty = (int)(in_slot / tiles.tiles_x), tx = (int)(in_slot - ty * tiles.tiles_x);
+        const int x0 = tx * T, y0 = ty * T;
+        member_row<C>(p, members, group_weight, tiles, id, r, x0, y0, T,
+                      [&](int i, unsigned long long v, int32_t v2) {
+                          const size_t pix = (size_t)slot * px + (size_t)(y0 + (i >> kTileShift)) * p.width +
+                                             x0 + (i & (T - 1));
+                          atomicAdd(&aggregate[pix], v);
+                          if (C == 2) atomicAdd(&accum_second[pix], v2);
+                      });
     }
 }
 
@@ -1315,10 +1462,45 @@ hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, c
     hipError_t e = upload_taps(s);
     if (e != hipSuccess) return e;
     const unsigned g = div_up(ref_count(p), kWaves);
+    const TileBins none{};
     if (p.channels == 1)
-        k_collab_filter<1><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second, group_weight);
+        k_collab_filter<1, false><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
+                                                             group_weight, none);
     else
-        k_collab_filter<2><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second, group_weight);
+        k_collab_filter<2, false><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
+                                                             group_weight, none);
+    return hipGetLastError();
+}
+
+hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
+                                       const uint32_t *members, const TileBins &tiles,
+                                       unsigned long long *aggregate, int32_t *accum_second, float *group_weight,
+                                       hipStream_t s)
+{
+    if (!pass_ok(p) || (p.channels == 2 && !accum_second) || !tiles.stage || !tiles.count || !tiles.list ||
+        !tiles.overflow || !tiles.overflow_count ||
+        tiles.tiles_x != (p.width + (1u << kTileShift) - 1) >> kTileShift ||
+        tiles.per_slot != tiles.tiles_x * ((p.height + (1u << kTileShift) - 1) >> kTileShift))
+        return hipErrorInvalidValue;
+    hipError_t e = upload_taps(s);
+    if (e != hipSuccess) return e;
+    const uint32_t n_tiles = tiles.per_slot * p.slots;
+    e = launch_fill_u32(tiles.count, 0, n_tiles, s);
+    if (e == hipSuccess) e = launch_fill_u32(tiles.overflow_count, 0, 1, s);
+    if (e != hipSuccess) return e;
+    const uint32_t refs = (uint32_t)ref_count(p);
+    const unsigned g = div_up(refs, kWaves);
+    if (p.channels == 1) {
+        k_collab_filter<1, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
+                                                            group_weight, tiles);
+        k_tile_gather<1><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+        k_tile_overflow<1><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+    } else {
+        k_collab_filter<2, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
+                                                            group_weight, tiles);
+        k_tile_gather<2><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+        k_tile_overflow<2><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+    }
     return hipGetLastError();
 }
 

@@ -637,6 +637,18 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     if (C == 2) accum_second_.allocate(N_ * px * sizeof(int32_t));
     members_.allocate(refs_ * 9 * sizeof(uint32_t));
     group_weight_.allocate(refs_ * sizeof(float));
+    {
+        const uint32_t side = 1u << kTileShift;
+        const uint32_t tiles_x = (W + side - 1) / side, per_slot = tiles_x * ((H + side - 1) / side);
+        const size_t n_tiles = (size_t)per_slot * N_;
+        stage_.allocate(refs_ * C * 512 * sizeof(int32_t));
+        tile_count_.allocate((n_tiles + 1) * sizeof(uint32_t));  // the last word counts overflow pairs
+        tile_list_.allocate(n_tiles * kTileCap * sizeof(uint32_t));
+        tile_overflow_.allocate(refs_ * 8 * 4 * 2 * sizeof(uint32_t));
+        uint32_t *counts = tile_count_.as<uint32_t>();
+        bins_ = TileBins{stage_.as<int32_t>(), counts, tile_list_.as<uint32_t>(), tile_overflow_.as<uint32_t>(),
+                         counts + n_tiles, tiles_x, per_slot};
+    }
     output_.allocate(px * C * sizeof(float));
     const size_t blocks = (size_t)mg_.blocks_x * mg_.blocks_y;
     mv_.allocate(2 * r_ * blocks * 2 * sizeof(int32_t));
@@ -689,9 +701,9 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
     });
     const uint64_t filter_ops = (uint64_t)refs_ * 8 * ((uint64_t)C_ * (4096 + 576 + 256) + 64 * (C_ + 1));
     timed(AVD_STAGE_COLLAB_FILTER, filter_ops, stream_, "collab filter", [&] {
-        return launch_collab_filter(p, ring_.value_ptr(), nslots, members_.as<uint32_t>(),
-                                    aggregate_.as<unsigned long long>(), accum_second_.as<int32_t>(),
-                                    group_weight_.as<float>(), stream_);
+        return launch_collab_filter_binned(p, ring_.value_ptr(), nslots, members_.as<uint32_t>(), bins_,
+                                           aggregate_.as<unsigned long long>(), accum_second_.as<int32_t>(),
+                                           group_weight_.as<float>(), stream_);
     });
     ++passes_;
 }

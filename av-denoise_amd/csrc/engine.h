@@ -202,6 +202,9 @@ class Nl4dPass : public Pass {
     // Per ring slot: aggregate_ holds (weight sum << 32) + first-channel sum per pixel, accum_second_
     // the second channel's sum (chroma passes only).
     DeviceBuffer aggregate_, accum_second_, members_, group_weight_, output_, mv_, conf_;
+    // Binned aggregation: staged member values, per-tile counts / offsets / cursors, member lists.
+    DeviceBuffer stage_, tile_count_, tile_list_, tile_overflow_;
+    TileBins bins_{};
     std::vector<uint32_t> pending_, tail_;
     std::vector<bool> zeroed_;
     uint32_t passes_ = 0;
