@@ -1206,24 +1206,45 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
                     const float w = scaled_weight * (p.kaiser[r] * kq);
                     st[r * 64] = in_group ? to_fixed(col[r] * w, scale) : 0;
                 }
-                if (c == 0 && in_group && q == 0) {
-                    // Claim an entry in the list of every tile the patch overlaps; a full list sends
-                    // the (member, tile) pair to the overflow list.
+            }
+            if (c == 0) {
+                // Enter each member in the list of every tile its patch overlaps.  Lane (m, q < 4)
+                // holds claim q of member m.  The workgroup counts its claims per tile in an LDS
+                // hash table (rank = the bucket's count before the claim), one atomic per
+                // distinct tile reserves the entries, and a full list sends the (member, tile)
+                // pair to the overflow list.
+                constexpr int kBuckets = kWaves * 64;
+                __shared__ uint32_t s_key[kBuckets], s_cnt[kBuckets], s_base[kBuckets];
+                s_key[threadIdx.x] = ~0u;
+                s_cnt[threadIdx.x] = 0;
+                __syncthreads();
+                const int tx0 = mx >> kTileShift, ty0 = my >> kTileShift;
+                const int nx = ((mx + 7) >> kTileShift) - tx0 + 1, ny = ((my + 7) >> kTileShift) - ty0 + 1;
+                uint32_t tile = ~0u, bucket = 0, rank = 0;
+                if (live && in_group && q < nx * ny) {
+                    tile = (uint32_t)slot * tiles.per_slot + (ty0 + q / nx) * tiles.tiles_x + tx0 + q % nx;
+                    bucket = (tile * 2654435761u) >> 24;
+                    for (;;) {  // at most 128 keys in 256 buckets: a free or equal bucket exists
+                        const uint32_t old = atomicCAS(&s_key[bucket], ~0u, tile);
+                        if (old == ~0u || old == tile) break;
+                        bucket = (bucket + 1) & (kBuckets - 1);
+                    }
+                    rank = atomicAdd(&s_cnt[bucket], 1u);
+                }
+                __syncthreads();
+                if (s_key[threadIdx.x] != ~0u)
+                    s_base[threadIdx.x] = atomicAdd(&tiles.count[s_key[threadIdx.x]], s_cnt[threadIdx.x]);
+                __syncthreads();
+                if (tile != ~0u) {
+                    const uint32_t k = s_base[bucket] + rank;
                     const uint32_t id = ((uint32_t)ref << 3) | (uint32_t)m;
-                    const int tx0 = mx >> kTileShift, tx1 = (mx + 7) >> kTileShift;
-                    const int ty0 = my >> kTileShift, ty1 = (my + 7) >> kTileShift;
-                    for (int ty = ty0; ty <= ty1; ++ty)
-                        for (int tx = tx0; tx <= tx1; ++tx) {
-                            const uint32_t tile = (uint32_t)slot * tiles.per_slot + ty * tiles.tiles_x + tx;
-                            const uint32_t k = atomicAdd(&tiles.count[tile], 1u);
-                            if (k < kTileCap) {
-                                tiles.list[(size_t)tile * kTileCap + k] = id;
-                            } else {
-                                const uint32_t o = atomicAdd(tiles.overflow_count, 1u);
-                                tiles.overflow[2 * (size_t)o] = id;
-                                tiles.overflow[2 * (size_t)o + 1] = tile;
-                            }
-                        }
+                    if (k < kTileCap) {
+                        tiles.list[(size_t)tile * kTileCap + k] = id;
+                    } else {
+                        const uint32_t o = atomicAdd(tiles.overflow_count, 1u);
+                        tiles.overflow[2 * (size_t)o] = id;
+                        tiles.overflow[2 * (size_t)o + 1] = tile;
+                    }
                 }
             }
         } else if (live && in_group) {
