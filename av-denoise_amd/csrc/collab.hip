@@ -1054,8 +1054,9 @@ __device__ __forceinline__ void unpack_aggregate(unsigned long long v, int32_t &
 }
 
 // STAGED: instead of atomics into the accumulators, each member's fixed-point values go to
-// `stage` ([ref][c][row][member][column] i32, one coalesced row per store) and every 32x32 tile a
-// member patch overlaps counts it in `tiles.count`; k_tile_gather then sums them per tile.
+// `stage` ([ref][member][c][row][column] i32: a member's patch is 256 contiguous bytes per channel)
+// and the member enters the list of every 32x32 tile its patch overlaps; k_tile_gather then sums
+// them per tile.
 template <int C, bool STAGED>
 __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p, const float *__restrict__ values,
                                                                const uint32_t *__restrict__ nslots,
@@ -1199,12 +1200,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
 
         if (STAGED) {
             if (live) {
-                int32_t *st = tiles.stage + (((size_t)ref * C + c) * 8) * 64 + lane;
+                int32_t *st = tiles.stage + (((size_t)ref * 8 + m) * C + c) * 64 + q;
                 const float kq = p.kaiser[q];
 #pragma unroll
                 for (int r = 0; r < 8; ++r) {
                     const float w = scaled_weight * (p.kaiser[r] * kq);
-                    st[r * 64] = in_group ? to_fixed(col[r] * w, scale) : 0;
+                    st[r * 8] = in_group ? to_fixed(col[r] * w, scale) : 0;
                 }
             }
             if (c == 0) {
@@ -1276,13 +1277,13 @@ __device__ __forceinline__ void member_row(const avd_collab_pass &p, const uint3
     const int xb = (int)(pos & 0x1FFF) - x0;
     const float sw = group_weight[ref] * p.weight_scale;
     const float kr = p.kaiser[r];
-    const int32_t *st = tiles.stage + ((size_t)ref * C * 8 + r) * 64 + m * 8;
+    const int32_t *st = tiles.stage + (size_t)id * C * 64 + r * 8;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int x = xb + q;
         if (x < 0 || x >= T) continue;
         const float w = sw * (kr * p.kaiser[q]);
-        add(y * T + x, pack_aggregate(st[q], to_fixed_weight(w, p.accum_scale)), C == 2 ? st[8 * 64 + q] : 0);
+        add(y * T + x, pack_aggregate(st[q], to_fixed_weight(w, p.accum_scale)), C == 2 ? st[64 + q] : 0);
     }
 }
 
@@ -1327,14 +1328,33 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
         }
         __syncthreads();
         const uint32_t batch = min(256u, n - base);
-        for (uint32_t e = wave; e < batch; e += 4) {
-            const uint32_t id = s_id[e], pos = s_pos[e];
-            const int lx = (int)(pos & 0x1FFF) + q - x0, ly = (int)((pos >> 13) & 0x1FFF) + r - y0;
-            if (lx < 0 || lx >= T || ly < 0 || ly >= T) continue;
-            const int32_t *st = tiles.stage + ((size_t)(id >> 3) * C * 8) * 64 + r * 64 + (id & 7) * 8 + q;
-            const float w = s_sw[e] * kr_kq;
-            atomicAdd(&s_agg[ly * T + lx], pack_aggregate(st[0], to_fixed_weight(w, scale)));
-            if (C == 2) atomicAdd(&s_second[ly * T + lx], st[8 * 64]);
+        // Four members per wave per step: every staged load is issued before the first add.
+        for (uint32_t e0 = wave; e0 < batch; e0 += 16) {
+            int32_t a[4], a2[4];
+            int cell[4];
+            float w[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t e = e0 + 4 * u;
+                cell[u] = -1;
+                if (e < batch) {
+                    const uint32_t id = s_id[e], pos = s_pos[e];
+                    const int lx = (int)(pos & 0x1FFF) + q - x0, ly = (int)((pos >> 13) & 0x1FFF) + r - y0;
+                    if (lx >= 0 && lx < T && ly >= 0 && ly < T) {
+                        const int32_t *st = tiles.stage + (size_t)id * C * 64 + lane;
+                        cell[u] = ly * T + lx;
+                        a[u] = st[0];
+                        if (C == 2) a2[u] = st[64];
+                        w[u] = s_sw[e] * kr_kq;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (cell[u] < 0) continue;
+                atomicAdd(&s_agg[cell[u]], pack_aggregate(a[u], to_fixed_weight(w[u], scale)));
+                if (C == 2) atomicAdd(&s_second[cell[u]], a2[u]);
+            }
         }
     }
     __syncthreads();

@@ -68,8 +68,8 @@ hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, c
 hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,
                                    int C, hipStream_t s);
 
-// Binned aggregation: the filter stages each member's fixed-point values (`stage`, refs * C * 512
-// i32) and enters the member in the list of every 32x32 tile of its ring slot that its patch
+// Binned aggregation: the filter stages each member's fixed-point values (`stage`, [ref][member][c]
+// [64] i32) and enters the member in the list of every 32x32 tile of its ring slot that its patch
 // overlaps (`count` / `list`: kTileCap entries per tile; pairs past that go to `overflow`, up to
 // refs * 8 * 4 pairs); each tile then sums its members in LDS and adds to the accumulators once.
 constexpr int kTileShift = 5;
