@@ -722,7 +722,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
 
     const int radius = p.temporal_radius;
     const int gf = radius == 1 ? 2 : 4;
-    const int volumes = 8 / gf, tail = gf - 1;
+    const int volumes = 8 / gf;
     const int bstep = p.blk_step, bsz = p.blksize, bx_n = p.blocks_x, by_n = p.blocks_y;
     const int covers = (bsz + bstep - 1) / bstep;
     const int cshift = covers <= 1 ? 0 : (covers <= 2 ? 1 : 2);
@@ -772,14 +772,14 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     }
     wave_sync();
 
-    // The volume lists: positions and distances (SSD + 1 fits 32 bits; ~0u marks an empty slot).
+    // final[] collects the grid's members.  A volume's list holds its gf - 1 frame picks and their
+    // distances (SSD + 1 fits 32 bits; ~0u marks an empty slot).  Only earlier volumes' picks can
+    // equal a candidate (anchors sit in the centre frame), at most three of them.
     uint32_t scored = 0;
-    uint32_t gpos[8], gd[8];
+    uint32_t final_pos[8], held[3] = {0u, 0u, 0u};
+    bool filled = true;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        gpos[i] = 0;
-        gd[i] = ~0u;
-    }
+    for (int i = 0; i < 8; ++i) final_pos[i] = 0;
     uint32_t P[8][4 * C];
     for (int v = 0; v < volumes; ++v) {
         const int first = v * gf;
@@ -792,17 +792,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
 #pragma unroll
             for (int i = 0; i < 4 * C; ++i) P[r][i] = s_patch[wave][v][r][i];
         const uint32_t A2 = patch_energy<C>(P);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if (i == first) {
-                gpos[i] = anchor;
-                gd[i] = 0;
-            } else if (i > first && i < first + gf) {
-                gpos[i] = 0;
-                gd[i] = ~0u;
-            }
-        }
-        const int held_n = first + 1;
+        uint32_t vd[3] = {~0u, ~0u, ~0u}, vp[3] = {0u, 0u, 0u};
         const uint32_t(*rects)[2] = s_rect[wave] + v * per_vol;
         // Compact the non-empty rectangles, in entry order.
         int n_valid = 0;
@@ -840,13 +830,11 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                 }
             }
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                if (i < held_n) {
-                    const uint32_t g = gpos[i];
-                    const int gy = (int)((g >> 13) & 0x1FFF);
-                    if ((int)(g >> 26) == t + 1 && (int)(g & 0x1FFF) == cx && gy >= tp && gy <= bt)
-                        skip |= 1u << (gy - tp);
-                }
+            for (int i = 0; i < 3; ++i) {
+                const uint32_t g = held[i];
+                const int gy = (int)((g >> 13) & 0x1FFF);
+                if ((int)(g >> 26) == t + 1 && (int)(g & 0x1FFF) == cx && gy >= tp && gy <= bt)
+                    skip |= 1u << (gy - tp);
             }
             scored += (uint32_t)__popc(((1u << h) - 1u) & ~skip);
             uint64_t best = kNoKey;
@@ -888,27 +876,39 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
             uint32_t d = (uint32_t)__shfl((int)cd, u);
             uint32_t q = (uint32_t)__shfl((int)cp, u);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                if (i > first && i < first + gf && d < gd[i]) {
-                    const uint32_t hd = gd[i], hp = gpos[i];
-                    gd[i] = d;
-                    gpos[i] = q;
+            for (int j = 0; j < 3; ++j) {
+                if (j < gf - 1 && d < vd[j]) {
+                    const uint32_t hd = vd[j], hp = vp[j];
+                    vd[j] = d;
+                    vp[j] = q;
                     d = hd;
                     q = hp;
                 }
             }
         }
+        filled = filled && (gf == 2 ? vd[0] : vd[2]) != ~0u;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int j = i - first - 1;
+            if (i == first) final_pos[i] = anchor;
+            else if (j >= 0 && j < gf - 1) final_pos[i] = j == 0 ? vp[0] : (j == 1 ? vp[1] : vp[2]);
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (j < gf - 1) {  // this volume's picks are held from the next volume on
+                held[2] = held[1];
+                held[1] = held[0];
+                held[0] = vp[j];
+            }
+        }
         wave_sync();
     }
-    bool use_grid = k_use == 8;
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-        if (i % gf == tail) use_grid = use_grid && (gd[i] != ~0u);
+    const bool use_grid = k_use == 8 && filled;
     if (use_grid && rp.live && lane < 9) {
         uint32_t word = 8u | 0x100u;
 #pragma unroll
         for (int m = 0; m < 8; ++m)
-            if (lane == m) word = gpos[m];
+            if (lane == m) word = final_pos[m];
         members[(size_t)rp.ref * 9 + lane] = word;
     }
     add_scored(scored_count, scored, rp.live, C);
