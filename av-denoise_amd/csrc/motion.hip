@@ -169,6 +169,34 @@ size_t match_lds_bytes(int blk, int R, int neighbours)
     return (size_t)(blk * blk + 16 + 16 + 32 + 16) * 4 + (size_t)batch * win * 4;
 }
 
+// Copies the `cols` x `rows` rectangle at (x0, y0) of a level into dst (row stride `stride`
+// words), clamping to the level: lanes take whole rows; a row inside the level is read with plain
+// offset loads, a row crossing its left or right edge sample by sample.
+template <int COLS>
+__device__ __forceinline__ void stage_rect(const Level &l, int x0, int y0, int rows, uint32_t *dst, int stride,
+                                           int lane, int lanes)
+{
+    const bool inside = x0 >= 0 && x0 + COLS <= l.w;
+    for (int row = lane; row < rows; row += lanes) {
+        const int y = clampi(y0 + row, l.h - 1);
+        uint32_t *d = dst + row * stride;
+        if (inside) {
+            if (l.codes) {
+                const uint16_t *src = l.codes + ((size_t)y * l.w + x0) * l.C;
+#pragma unroll
+                for (int x = 0; x < COLS; ++x) d[x] = src[x * l.C];
+            } else {
+                const int32_t *src = l.plane + (size_t)y * l.w + x0;
+#pragma unroll
+                for (int x = 0; x < COLS; ++x) d[x] = (uint32_t)src[x];
+            }
+        } else {
+#pragma unroll
+            for (int x = 0; x < COLS; ++x) d[x] = l.at(x0 + x, y);
+        }
+    }
+}
+
 // The same search specialised for a compile-time block size and radius: one wave per block, lane
 // (neighbour, dy) scores the whole row of offsets (dx = 0..2R) for its dy.  Per block row it holds
 // the block row and the window row (BLK + 2R samples) in registers, so each window row is read from
@@ -187,10 +215,7 @@ __device__ __forceinline__ void match_rows(const Level &cen, const Level &nb, si
     int *s_seed = (int *)(s_csad + 16);           // 32
     uint32_t *s_slot = (uint32_t *)(s_seed + 32); // 16
     uint32_t *s_win = s_slot + 16;                // batch * WIN, rows 16-byte aligned
-    for (int i = lane; i < BLK * BLK; i += 64) {
-        const int py = i / BLK, px = i - py * BLK;
-        s_blk[i] = cen.at(ox + px, oy + py);
-    }
+    stage_rect<BLK>(cen, ox, oy, BLK, s_blk, BLK, lane, 64);
     for (int n0 = 0; n0 < neighbours; n0 += batch) {
         const int nb_n = min(batch, neighbours - n0);
         if (lane < nb_n) {
@@ -202,13 +227,17 @@ __device__ __forceinline__ void match_rows(const Level &cen, const Level &nb, si
             s_key[lane] = ~0u;
         }
         __syncthreads();
-        for (int i = lane; i < nb_n * WIN; i += 64) {
-            const int nl = i / WIN, j = i - nl * WIN;
-            const int wy = j / WW, wx = j - wy * WW;
-            Level l = nb;
-            if (l.codes) l.codes += (size_t)s_slot[nl] * level_len;
-            else l.plane += (size_t)s_slot[nl] * level_len;
-            s_win[i] = l.at(ox + s_seed[2 * nl] - R + wx, oy + s_seed[2 * nl + 1] - R + wy);
+        // Lanes split the batch's neighbours into groups of 64 / nb_n lanes, each staging one window.
+        {
+            const int per = 64 / nb_n;  // nb_n <= 16
+            const int nl = lane / per, sub = lane - nl * per;
+            if (nl < nb_n) {
+                Level l = nb;
+                if (l.codes) l.codes += (size_t)s_slot[nl] * level_len;
+                else l.plane += (size_t)s_slot[nl] * level_len;
+                stage_rect<WW>(l, ox + s_seed[2 * nl] - R, oy + s_seed[2 * nl + 1] - R, WW, s_win + nl * WIN, WW,
+                               sub, per);
+            }
         }
         __syncthreads();
         for (int task = lane; task < nb_n * SIDE; task += 64) {
