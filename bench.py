@@ -43,7 +43,25 @@ def parse():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sample-steps", type=int, default=4, help="extra steps timed per stage with HIP events")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the run, one plain device copy of this many MiB (a known-size kernel that "
+                         "hardware-counter runs calibrate FETCH_SIZE / WRITE_SIZE against)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "nl4d_4k_radius3_traffic.json")
+
+
+def stage_traffic():
+    """HBM bytes per stage launch (fetch + write) of this workload, from the committed counter
+    summary (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled by the same counter on a
+    256 MiB device copy); empty when the summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    return {k: v["bytes_per_launch"] for k, v in data.get("stages", {}).items()}
 
 
 def make_engine(avd):
@@ -139,6 +157,13 @@ def main():
     run_steps(eng, frames, outs, warm + args.steps, args.sample_steps)
     stages = eng.timing()
     eng.sample_device_time(0)
+    if args.copy_mib > 0:
+        src = torch.full((args.copy_mib << 20,), 7, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     rooflines = {}
     for name in OP_STAGES:
         st = stages[name]
@@ -150,8 +175,10 @@ def main():
         rooflines[name] = {"kernels": STAGE_KERNELS[name], "avg_launch_ms": round(avg_ms, 4),
                            "ops_per_launch": int(ops), "achieved": round(tops, 3),
                            "frac": round(tops / VALU_PEAK_TOPS, 4)}
-    dominant = max(rooflines, key=lambda k: stages[k]["ms"])
-    dom = rooflines[dominant]
+    dominant = max(rooflines, key=lambda k: stages[k]["ms"]) if rooflines else None
+    dom = rooflines.get(dominant, {"kernels": "", "avg_launch_ms": None, "ops_per_launch": None, "achieved": None,
+                                   "frac": None})
+    traffic = stage_traffic().get(dominant) if dominant else None
 
     result = {
         "metric": METRIC,
@@ -181,7 +208,7 @@ def main():
             "peak": VALU_PEAK_TOPS,
             "unit": "TFLOP/s",
             "frac": dom["frac"],
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": dom["avg_launch_ms"],
             "ops_per_launch": dom["ops_per_launch"],
         },
