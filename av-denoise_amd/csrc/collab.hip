@@ -794,11 +794,23 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
         const uint32_t A2 = patch_energy<C>(P);
         uint32_t vd[3] = {~0u, ~0u, ~0u}, vp[3] = {0u, 0u, 0u};
         const uint32_t(*rects)[2] = s_rect[wave] + v * per_vol;
-        // Compact the non-empty rectangles, in entry order.
+        // Compact the rectangles that hold a position no earlier rectangle of their frame covers
+        // (an empty or fully covered one adds nothing but its share of the walk index), in entry
+        // order.
         int n_valid = 0;
         for (int base = 0; base < per_vol; base += 64) {
             const int e = base + lane;
-            const bool valid = e < per_vol && (rects[e][0] & 0xFFFFu) <= (rects[e][0] >> 16);
+            bool valid = e < per_vol && (rects[e][0] & 0xFFFFu) <= (rects[e][0] >> 16);
+            if (valid) {
+                const int l = (int)(rects[e][0] & 0xFFFFu), r = (int)(rects[e][0] >> 16);
+                const int tp = (int)(rects[e][1] & 0xFFFFu), bt = (int)(rects[e][1] >> 16);
+                const int f0 = e & ~((1 << nshift) - 1);
+                for (int e2 = f0; e2 < e; ++e2) {
+                    const int l2 = (int)(rects[e2][0] & 0xFFFFu), r2 = (int)(rects[e2][0] >> 16);
+                    const int tp2 = (int)(rects[e2][1] & 0xFFFFu), bt2 = (int)(rects[e2][1] >> 16);
+                    if (l2 <= l && r <= r2 && tp2 <= tp && bt <= bt2) valid = false;
+                }
+            }
             const uint64_t bal = __ballot(valid);
             if (valid) s_list[wave][n_valid + lane_mask_lt(bal)] = (uint8_t)e;
             n_valid += __popcll(bal);
@@ -836,7 +848,9 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                 if ((int)(g >> 26) == t + 1 && (int)(g & 0x1FFF) == cx && gy >= tp && gy <= bt)
                     skip |= 1u << (gy - tp);
             }
-            scored += (uint32_t)__popc(((1u << h) - 1u) & ~skip);
+            const uint32_t live_rows = ((1u << h) - 1u) & ~skip;
+            if (!live_rows) continue;  // every position of this column was searched or is held
+            scored += (uint32_t)__popc(live_rows);
             uint64_t best = kNoKey;
             // Every lane runs K rows of candidates; a rectangle clipped shorter drops the rest.
             slide_fixed<C, K>(ring, (size_t)s_slot[wave][t] * slot_len, W, H, cx, tp, h, P, A2,
