@@ -557,7 +557,6 @@ __device__ __forceinline__ int xcd_block()
     return xcd * q + min(xcd, rem) + i;
 }
 
-constexpr int kMaxEntries = 16 * 16;  // neighbour frames x covering blocks per frame
 
 // Reference `ref`'s top-left corner on the STEP grid, the last position clamped to dim - 8.
 struct RefPos {
@@ -683,19 +682,39 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_codes(avd_collab_pass p,
 // group already holds; the best per frame joins the volume.  When every volume fills and the
 // stack holds eight, the members become the volume grid.  Needs temporal_radius >= 1.
 //
-// K = 2 * refine + 1.  The rectangles of every volume are built up front (entry (v, t, b), b the
-// covering block in row-major order, padded to a power-of-two square); per volume the non-empty
-// ones are compacted and lane task k * K + col slides column col of rectangle k.  Each frame's
-// best (distance, walk index) is an LDS atomic min; lane t decodes frame t, and the frames join the
-// volume through the reference's ordered insertion.
-constexpr int kMaxRects = 512;  // volumes * neighbour frames * padded covering blocks
+// K = 2 * refine + 1.  A wave serves NR references (lanes r * 64 / NR onwards keep reference r's
+// lists).  The rectangles of every volume are built up front (entry (v, t, b), b the covering
+// block in row-major order, padded to a power-of-two square); per volume the non-empty ones of all
+// NR references are compacted into one list and lane task k * K + col slides column col of
+// rectangle k, so the references share the lanes their own few rectangles would leave idle.  Each
+// frame's best (distance, walk index) is an LDS atomic min; lane t of a reference's lanes decodes
+// frame t, and the frames join the volume through the reference's ordered insertion.
+// LDS of one trajectory workgroup with NR references per wave: the rectangle tables (dynamic) and
+// the lists, frame slots, per-frame bests, anchor patches, energies and held picks, and the scored
+// counters' per-wave sums (static).
+constexpr size_t kLdsBudget = 64 * 1024;
+
+__host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR)
+{
+    return (size_t)kWaves * (NR * 256 * 2 + 16 * 4 + NR * 16 * 8 + NR * 4 * 8 * 4 * C * 4 + NR * 4 * 4 + NR * 3 * 4 + 4);
+}
 
 __device__ __forceinline__ uint32_t lane_mask_lt(uint64_t bal)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
 }
 
-template <int C, int K>
+// Entries of one reference's rectangle table, and the dynamic LDS the kernel needs for NR
+// references per wave.
+__host__ __device__ __forceinline__ int trajectory_entries(const avd_collab_pass &p)
+{
+    const int gf = p.temporal_radius == 1 ? 2 : 4;
+    const int covers = ((int)p.blksize + (int)p.blk_step - 1) / (int)p.blk_step;
+    const int cshift = covers <= 1 ? 0 : (covers <= 2 ? 1 : 2);
+    return (8 / gf) * ((2 * (int)p.temporal_radius) << (2 * cshift));
+}
+
+template <int C, int K, int NR>
 __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                                                                   const uint32_t *__restrict__ nslots,
                                                                   const int32_t *__restrict__ mv,
@@ -703,22 +722,30 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                                                                   uint32_t *__restrict__ members,
                                                                   unsigned long long *__restrict__ scored_count)
 {
-    __shared__ uint32_t s_rect[kWaves][kMaxRects][2];  // (left | right << 16), (top | bottom << 16)
-    __shared__ uint8_t s_list[kWaves][256];             // non-empty entries of the current volume
+    constexpr int LPR = 64 / NR;  // lanes per reference
+    extern __shared__ uint32_t s_dyn[];  // [kWaves][NR * entries][2]: (left | right << 16), (top | bottom << 16)
+    __shared__ uint16_t s_list[kWaves][NR * 256];  // non-empty entries of the current volume: e | r << 8
     __shared__ uint32_t s_slot[kWaves][16];
-    __shared__ unsigned long long s_fbest[kWaves][16];
-    __shared__ uint32_t s_patch[kWaves][4][8][4 * C];  // every volume's anchor patch
+    __shared__ unsigned long long s_fbest[kWaves][NR][16];
+    __shared__ uint32_t s_patch[kWaves][NR][4][8][4 * C];  // every volume's anchor patch
+    __shared__ uint32_t s_energy[kWaves][NR][4];
+    __shared__ uint32_t s_held[kWaves][NR][3];
     constexpr int refine = (K - 1) / 2;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const RefPos rp = ref_of_wave(p, wave);
+    const int hr = lane / LPR, hl = lane % LPR, hbase = hr * LPR;
     const int W = p.width, H = p.height, maxx = W - 8, maxy = H - 8;
+    const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
+    const int refs = refs_x * refs_y;
+    const int ref0 = (xcd_block() * kWaves + wave) * NR;  // the wave's first reference
+    const int my_ref = min(ref0 + hr, refs - 1);
+    const bool live = ref0 + hr < refs;
     const size_t slot_len = (size_t)W * H * C;
     const size_t cframe = (size_t)p.centre_slot * slot_len;
-    const uint32_t mine = members[(size_t)rp.ref * 9 + min(lane, 8)];
-    const uint32_t k_use = (uint32_t)__shfl((int)mine, 8);
+    const uint32_t mine = members[(size_t)my_ref * 9 + min(hl, 8)];
+    const uint32_t k_use = (uint32_t)__shfl((int)mine, hbase + 8);
     uint32_t anchors[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) anchors[v] = (uint32_t)__shfl((int)mine, v);
+    for (int v = 0; v < 4; ++v) anchors[v] = (uint32_t)__shfl((int)mine, hbase + v);
 
     const int radius = p.temporal_radius;
     const int gf = radius == 1 ? 2 : 4;
@@ -729,22 +756,24 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     const int nshift = 2 * cshift, cmask = (1 << cshift) - 1;
     const int frames_n = 2 * radius;
     const int per_vol = frames_n << nshift;
+    const int ents = volumes * per_vol;
     const size_t nblk = (size_t)bx_n * by_n;
+    uint32_t(*const rect_tab)[2] = reinterpret_cast<uint32_t(*)[2]>(s_dyn) + (size_t)wave * NR * ents;
     if (lane < frames_n) s_slot[wave][lane] = nslots[lane];
-    if (lane < volumes * 8) {  // lane (v, r) stages row r of volume v's anchor patch
-        const int v = lane >> 3, r = lane & 7;
+    for (int i = hl; i < volumes * 8; i += LPR) {  // lane (v, r) stages row r of volume v's anchor patch
+        const int v = i >> 3, r = i & 7;
         uint32_t anchor = anchors[0];
 #pragma unroll
-        for (int i = 1; i < 4; ++i)
-            if (v == i) anchor = anchors[i];
+        for (int j = 1; j < 4; ++j)
+            if (v == j) anchor = anchors[j];
         uint32_t row[4 * C];
         load_row<C>(ring, cframe + ((size_t)(((anchor >> 13) & 0x1FFF) + r) * W + (anchor & 0x1FFF)) * C, row);
 #pragma unroll
-        for (int i = 0; i < 4 * C; ++i) s_patch[wave][v][r][i] = row[i];
+        for (int j = 0; j < 4 * C; ++j) s_patch[wave][hr][v][r][j] = row[j];
     }
 
-    // ---- every volume's rectangles.
-    for (int idx = lane; idx < volumes * per_vol; idx += 64) {
+    // ---- every volume's rectangles, per reference.
+    for (int idx = hl; idx < ents; idx += LPR) {
         const int v = idx / per_vol, e = idx - v * per_vol;
         const int t = e >> nshift, b = e & ((1 << nshift) - 1);
         uint32_t anchor = anchors[0];
@@ -767,40 +796,47 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                 yr = (uint32_t)clampi(py - refine, 0, maxy) | ((uint32_t)clampi(py + refine, 0, maxy) << 16);
             }
         }
-        s_rect[wave][idx][0] = xr;
-        s_rect[wave][idx][1] = yr;
+        rect_tab[hr * ents + idx][0] = xr;
+        rect_tab[hr * ents + idx][1] = yr;
     }
     wave_sync();
+    if (hl < volumes) {
+        uint32_t e = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int i = 0; i < 4 * C; ++i)
+                e = __builtin_amdgcn_udot2(as_u2(s_patch[wave][hr][hl][r][i]), as_u2(s_patch[wave][hr][hl][r][i]), e,
+                                           false);
+        s_energy[wave][hr][hl] = e;
+    }
 
     // final[] collects the grid's members.  A volume's list holds its gf - 1 frame picks and their
     // distances (SSD + 1 fits 32 bits; ~0u marks an empty slot).  Only earlier volumes' picks can
-    // equal a candidate (anchors sit in the centre frame), at most three of them.
+    // equal a candidate (anchors sit in the centre frame), at most three of them.  All of this is
+    // the same on every lane of one reference.
     uint32_t scored = 0;
     uint32_t final_pos[8], held[3] = {0u, 0u, 0u};
     bool filled = true;
 #pragma unroll
     for (int i = 0; i < 8; ++i) final_pos[i] = 0;
-    uint32_t P[8][4 * C];
     for (int v = 0; v < volumes; ++v) {
         const int first = v * gf;
         uint32_t anchor = anchors[0];
 #pragma unroll
         for (int i = 1; i < 4; ++i)
             if (v == i) anchor = anchors[i];
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-#pragma unroll
-            for (int i = 0; i < 4 * C; ++i) P[r][i] = s_patch[wave][v][r][i];
-        const uint32_t A2 = patch_energy<C>(P);
         uint32_t vd[3] = {~0u, ~0u, ~0u}, vp[3] = {0u, 0u, 0u};
-        const uint32_t(*rects)[2] = s_rect[wave] + v * per_vol;
+        if (hl < 3) s_held[wave][hr][hl] = hl == 0 ? held[0] : (hl == 1 ? held[1] : held[2]);
         // Compact the rectangles that hold a position no earlier rectangle of their frame covers
-        // (an empty or fully covered one adds nothing but its share of the walk index), in entry
-        // order.
+        // (an empty or fully covered one adds nothing but its share of the walk index), in
+        // (reference, entry) order.
         int n_valid = 0;
-        for (int base = 0; base < per_vol; base += 64) {
-            const int e = base + lane;
-            bool valid = e < per_vol && (rects[e][0] & 0xFFFFu) <= (rects[e][0] >> 16);
+        for (int base = 0; base < NR * per_vol; base += 64) {
+            const int idx = base + lane;
+            const int r_of = min(idx / per_vol, NR - 1), e = idx - r_of * per_vol;
+            const uint32_t(*rects)[2] = rect_tab + r_of * ents + v * per_vol;
+            bool valid = idx < NR * per_vol && (rects[e][0] & 0xFFFFu) <= (rects[e][0] >> 16);
             if (valid) {
                 const int l = (int)(rects[e][0] & 0xFFFFu), r = (int)(rects[e][0] >> 16);
                 const int tp = (int)(rects[e][1] & 0xFFFFu), bt = (int)(rects[e][1] >> 16);
@@ -812,14 +848,16 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                 }
             }
             const uint64_t bal = __ballot(valid);
-            if (valid) s_list[wave][n_valid + lane_mask_lt(bal)] = (uint8_t)e;
+            if (valid) s_list[wave][n_valid + lane_mask_lt(bal)] = (uint16_t)(e | (r_of << 8));
             n_valid += __popcll(bal);
         }
-        if (lane < frames_n) s_fbest[wave][lane] = kNoKey;
+        if (hl < frames_n) s_fbest[wave][hr][hl] = kNoKey;
         wave_sync();
         for (int task = lane; task < n_valid * K; task += 64) {
             const int k = task / K, col = task - k * K;
-            const int e = s_list[wave][k];
+            const uint32_t ent = s_list[wave][k];
+            const int tr = (int)(ent >> 8), e = (int)(ent & 0xFFu);
+            const uint32_t(*rects)[2] = rect_tab + tr * ents + v * per_vol;
             const int t = e >> nshift, b = e & ((1 << nshift) - 1);
             const int l = (int)(rects[e][0] & 0xFFFFu), r = (int)(rects[e][0] >> 16);
             const int tp = (int)(rects[e][1] & 0xFFFFu), bt = (int)(rects[e][1] >> 16);
@@ -843,39 +881,45 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const uint32_t g = held[i];
+                const uint32_t g = s_held[wave][tr][i];
                 const int gy = (int)((g >> 13) & 0x1FFF);
                 if ((int)(g >> 26) == t + 1 && (int)(g & 0x1FFF) == cx && gy >= tp && gy <= bt)
                     skip |= 1u << (gy - tp);
             }
             const uint32_t live_rows = ((1u << h) - 1u) & ~skip;
             if (!live_rows) continue;  // every position of this column was searched or is held
-            scored += (uint32_t)__popc(live_rows);
+            if (ref0 + tr < refs) scored += (uint32_t)__popc(live_rows);
+            uint32_t P[8][4 * C];
+#pragma unroll
+            for (int rr = 0; rr < 8; ++rr)
+#pragma unroll
+                for (int i = 0; i < 4 * C; ++i) P[rr][i] = s_patch[wave][tr][v][rr][i];
             uint64_t best = kNoKey;
             // Every lane runs K rows of candidates; a rectangle clipped shorter drops the rest.
-            slide_fixed<C, K>(ring, (size_t)s_slot[wave][t] * slot_len, W, H, cx, tp, h, P, A2,
+            slide_fixed<C, K>(ring, (size_t)s_slot[wave][t] * slot_len, W, H, cx, tp, h, P, s_energy[wave][tr][v],
                               [&](int j, uint32_t ssd) {
                                   const uint64_t key = ((uint64_t)(ssd + 1u) << 32) | (uint32_t)(walk + j * w + col);
                                   if (!((skip >> j) & 1u) && key < best) best = key;
                               });
-            if (best != kNoKey) atomicMin(&s_fbest[wave][t], (unsigned long long)best);
+            if (best != kNoKey) atomicMin(&s_fbest[wave][tr][t], (unsigned long long)best);
         }
         wave_sync();
-        // Lane t decodes frame t's match.
+        // Lane t of each reference's lanes decodes that reference's frame t.
         uint32_t cd = ~0u;
         uint32_t cp = 0;
-        if (lane < frames_n) {
-            const uint64_t best = s_fbest[wave][lane];
+        if (hl < frames_n) {
+            const uint32_t(*rects)[2] = rect_tab + hr * ents + v * per_vol;
+            const uint64_t best = s_fbest[wave][hr][hl];
             if (best != kNoKey) {
                 int i = (int)(best & 0xFFFFFFFFu);
                 for (int b = 0; b < (1 << nshift); ++b) {
-                    const uint32_t x2 = rects[(lane << nshift) + b][0], y2 = rects[(lane << nshift) + b][1];
+                    const uint32_t x2 = rects[(hl << nshift) + b][0], y2 = rects[(hl << nshift) + b][1];
                     const int l2 = (int)(x2 & 0xFFFFu), r2 = (int)(x2 >> 16);
                     if (l2 > r2) continue;
                     const int tp2 = (int)(y2 & 0xFFFFu), bt2 = (int)(y2 >> 16);
                     const int w2 = r2 - l2 + 1, n2 = w2 * (bt2 - tp2 + 1);
                     if (i < n2) {
-                        cp = pack_pos_t((uint32_t)(l2 + i % w2), (uint32_t)(tp2 + i / w2), (uint32_t)lane + 1);
+                        cp = pack_pos_t((uint32_t)(l2 + i % w2), (uint32_t)(tp2 + i / w2), (uint32_t)hl + 1);
                         break;
                     }
                     i -= n2;
@@ -887,8 +931,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
         // equal one, and a displaced entry moves on past equal ones (the reference's ordered
         // insertion, which is not a stable sort when distances tie).
         for (int u = 0; u < frames_n; ++u) {
-            uint32_t d = (uint32_t)__shfl((int)cd, u);
-            uint32_t q = (uint32_t)__shfl((int)cp, u);
+            uint32_t d = (uint32_t)__shfl((int)cd, hbase + u);
+            uint32_t q = (uint32_t)__shfl((int)cp, hbase + u);
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 if (j < gf - 1 && d < vd[j]) {
@@ -918,14 +962,14 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
         wave_sync();
     }
     const bool use_grid = k_use == 8 && filled;
-    if (use_grid && rp.live && lane < 9) {
+    if (use_grid && live && hl < 9) {
         uint32_t word = 8u | 0x100u;
 #pragma unroll
         for (int m = 0; m < 8; ++m)
-            if (lane == m) word = final_pos[m];
-        members[(size_t)rp.ref * 9 + lane] = word;
+            if (hl == m) word = final_pos[m];
+        members[(size_t)my_ref * 9 + hl] = word;
     }
-    add_scored(scored_count, scored, rp.live, C);
+    add_scored(scored_count, scored, true, C);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1489,7 +1533,21 @@ hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes,
             else k_centre_codes<2, false><<<g, blk, 0, s>>>(p, codes, members, scored_count);
         }
         if (p.temporal_radius >= 1 && p.group_size == 8) {
-#define AVD_TRAJ(Cn, Kn) k_trajectory_codes<Cn, Kn><<<g, blk, 0, s>>>(p, codes, nslots, mv, conf, members, scored_count)
+            // Four references per wave where their tables fit the LDS budget, else two (the largest
+            // table, 512 entries, fits at two).
+            const size_t ents = (size_t)trajectory_entries(p) * 2 * sizeof(uint32_t) * kWaves;
+            const int C = (int)p.channels;
+            const bool four = trajectory_static_lds(C, 4) + 4 * ents <= kLdsBudget;
+            const int nr = four ? 4 : 2;
+            const size_t lds = nr * ents;
+            const unsigned gt = div_up(ref_count(p), kWaves * nr);
+#define AVD_TRAJ(Cn, Kn)                                                                                       \
+    do {                                                                                                       \
+        if (four)                                                                                              \
+            k_trajectory_codes<Cn, Kn, 4><<<gt, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count); \
+        else                                                                                                   \
+            k_trajectory_codes<Cn, Kn, 2><<<gt, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count); \
+    } while (0)
             switch (p.refine * 2 + 1 + (p.channels == 2 ? 100 : 0)) {
             case 3: AVD_TRAJ(1, 3); break;
             case 5: AVD_TRAJ(1, 5); break;

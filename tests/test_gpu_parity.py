@@ -162,7 +162,7 @@ def _codes_ring(planes, C):
 def _oracle_fields(codes, W, H, C, centre, slots, sigma, max_value, geom):
     pyr = {s: [codes[s].reshape(-1, C)[:, 0].astype(np.int32),
                O.downscale_sum(codes[s].reshape(-1, C)[:, 0].astype(np.int32), W, H)] for s in [centre] + slots}
-    floor, th = float(O.sad_noise_floor(16, sigma)), float(O.thsad(16, 1.0))
+    floor, th = float(O.sad_noise_floor(geom.blksize, sigma)), float(O.thsad(geom.blksize, 1.0))
     mvs, confs = [], []
     for s in slots:
         m, c = O.motion_search(geom, pyr[centre], pyr[s], max_value, floor, th)
@@ -186,12 +186,19 @@ def _oracle_fields(codes, W, H, C, centre, slots, sigma, max_value, geom):
     ("tiny frame", 9, 8, 1, 10, 1, 3, 10, "scene"),
     ("narrow frame", 8, 40, 2, 10, 2, 5, 10, "scene"),
     ("8-bit full range noise", 56, 48, 1, 8, 3, 12, 8, "noise"),
+    # 32-pixel blocks at step 8 cover an anchor with up to 16 blocks; at these radii the rectangle
+    # tables only fit two references per wave.
+    ("deep window, 16 covering blocks", 64, 48, 1, 10, 6, 6, 10, "scene", 32, 24),
+    ("deep window chroma, 16 covering blocks", 48, 40, 2, 10, 5, 5, 10, "scene", 32, 24),
+    ("radius 4, 9 covering blocks", 64, 48, 1, 10, 4, 7, 10, "scene", 24, 16),
 ])
 def test_codes_search_edge_cases_are_bit_exact(case):
     """Member lists of the integer search against the oracle over the shapes and contents that
     exercise its paths: odd rows (unaligned loads), 32- and 64-bit keys, run lengths past the
-    unrolled forms, clipped rectangles, and equal distances resolved by visit order."""
-    name, W, H, C, depth, radius, S, code_bits, content = case
+    unrolled forms, clipped rectangles, equal distances resolved by visit order, and rectangle
+    tables of every size (four or two references per wave)."""
+    name, W, H, C, depth, radius, S, code_bits, content = case[:9]
+    blksize, overlap = case[9:] if len(case) > 9 else (16, 8)
     print(f"search case: {name}", flush=True)
     frames = 2 * radius + 1
     rng = np.random.default_rng(W * 1000 + H + depth)
@@ -205,7 +212,7 @@ def test_codes_search_edge_cases_are_bit_exact(case):
     else:
         planes = [[rng.integers(0, top + 1, (H, W)) for _ in range(C)] for _ in range(frames)]
     codes = _codes_ring(planes, C)
-    geom = O.MotionGeom(W, H, 16, 8, 4, 2)
+    geom = O.MotionGeom(W, H, blksize, overlap, 4, 2)
     centre = radius
     slots = [s for s in range(frames) if s != centre]
     sigma = 12.0 / 1023
