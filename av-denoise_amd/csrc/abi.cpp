@@ -252,9 +252,20 @@ avd_status avd_site_collab_search(const avd_collab_pass *pass, const uint16_t *r
     if (!pass || !members || (!ring_codes && !ring_values)) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
     if (pass->temporal_radius > 0 && (!neighbour_slots || !mv_field || !confidence))
         return fail(AVD_ERR_NULL_ARGUMENT, "a temporal pass needs neighbour slots, motion and confidence");
-    return hip_status(avd::launch_collab_search(*pass, ring_codes, ring_values, neighbour_slots, mv_field, confidence,
-                                                members, static_cast<hipStream_t>(stream)),
-                      "collab_search");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!ring_codes || !avd::centre_on_matrix_cores(*pass))
+        return hip_status(avd::launch_collab_search(*pass, ring_codes, ring_values, neighbour_slots, mv_field,
+                                                    confidence, members, s),
+                          "collab_search");
+    // The matrix-core centre search's scratch lives for this call, in stream order.
+    void *scratch = nullptr;
+    hipError_t e = hipMallocAsync(&scratch, avd::centre_scratch_bytes(pass->width, pass->height, pass->channels), s);
+    if (e != hipSuccess) return hip_status(e, "collab_search scratch");
+    const avd::CentreScratch cs = avd::centre_scratch_at(scratch, pass->width, pass->height, pass->channels);
+    e = avd::launch_collab_search(*pass, ring_codes, ring_values, neighbour_slots, mv_field, confidence, members, s,
+                                  nullptr, &cs);
+    const hipError_t f = hipFreeAsync(scratch, s);
+    return hip_status(e != hipSuccess ? e : f, "collab_search");
 }
 
 avd_status avd_site_collab_filter(const avd_collab_pass *pass, const float *ring_values,

@@ -60,6 +60,35 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
     return v;
 }
 
+// One DPP step of a 64-bit minimum: every lane takes the smaller of its own key and the key of
+// the lane CTRL selects within its row of 16 (quad permutes and row rotations read no other row).
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_min_step(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    return o < v ? o : v;
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Minimum of every row of 16 lanes, left in all of the row's lanes: quad swaps (xor 1, xor 2),
+// then row rotations by 4 and 8.  Needs every lane active.
+__device__ __forceinline__ uint64_t row16_min_u64(uint64_t v)
+{
+    v = dpp_min_step<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_min_step<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_min_step<0x124>(v);  // row_ror 4
+    v = dpp_min_step<0x128>(v);  // row_ror 8
+    return v;
+}
+
 // Orders a non-negative f32 distance as an unsigned integer; 0 stays for the pinned self-match.
 __device__ __forceinline__ uint32_t float_key(float d) { return __float_as_uint(d) + 1u; }
 
@@ -447,9 +476,12 @@ __device__ __forceinline__ void slide_fixed(const uint16_t *__restrict__ ring, s
     }
 }
 
-// The same for a run length k known only at run time (k up to 33 with spatial_radius 16).
+// The same for a run length k known only at run time (k up to 33 with spatial_radius 16), in
+// blocks of eight rows so that every sum slot stays a compile-time register.  After the first block
+// every row adds its cross products into all eight slots: a slot of a candidate past k collects
+// sums nobody reads (candidate c and c + 8 never overlap), which spares a condition per product.
 template <int C, typename Emit>
-__device__ __forceinline__ void slide_column(const uint16_t *__restrict__ ring, size_t frame, int W, int H, int x,
+__device__ __forceinline__ void slide_blocks(const uint16_t *__restrict__ ring, size_t frame, int W, int H, int x,
                                              int y0, int k, int kk, const uint32_t (&P)[8][4 * C], uint32_t A2,
                                              Emit &&emit)
 {
@@ -458,39 +490,46 @@ __device__ __forceinline__ void slide_column(const uint16_t *__restrict__ ring, 
     for (int i = 0; i < 8; ++i) acc[i] = 0;
     uint32_t window = 0;
     const size_t col = frame + (size_t)x * C;
+    const int rows = k + 7;
 #pragma unroll
-    for (int t = 0; t < 7; ++t) {
+    for (int t = 0; t < 8; ++t) {  // rows < 8 start candidates 0..t only
         uint32_t row[4 * C];
         load_row<C>(ring, col + (size_t)min(y0 + t, H - 1) * W * C, row);
-        energy[t] = row_dot<C>(row, row, 0u);
-        window += energy[t];
+        uint32_t e = 0;
 #pragma unroll
-        for (int r = 0; r <= t; ++r)
-            if (t - r < k) acc[(t - r) & 7] = row_dot<C>(P[r], row, acc[(t - r) & 7]);
+        for (int i = 0; i < 4 * C; ++i) {
+            e = __builtin_amdgcn_udot2(as_u2(row[i]), as_u2(row[i]), e, false);
+#pragma unroll
+            for (int r = 0; r <= t; ++r)
+                acc[(t - r) & 7] = __builtin_amdgcn_udot2(as_u2(P[r][i]), as_u2(row[i]), acc[(t - r) & 7], false);
+        }
+        energy[t] = e;
+        window += e;
+        if (t == 7) {
+            if (kk > 0) emit(0, A2 + window - 2u * acc[0]);
+            acc[0] = 0;
+            window -= energy[0];
+        }
     }
-    for (int t0 = 7; t0 < k + 7; t0 += 8) {
+    for (int t0 = 8; t0 < rows; t0 += 8) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int t = t0 + u;  // t % 8 == (7 + u) % 8
-            if (t < k + 7) {
-                uint32_t row[4 * C];
-                load_row<C>(ring, col + (size_t)min(y0 + t, H - 1) * W * C, row);
-                uint32_t e = 0;
+            const int t = t0 + u;  // t % 8 == u; candidate t - 7 sits in slot (u + 1) % 8
+            uint32_t row[4 * C];
+            load_row<C>(ring, col + (size_t)min(y0 + t, H - 1) * W * C, row);
+            uint32_t e = 0;
 #pragma unroll
-                for (int i = 0; i < 4 * C; ++i) {
-                    e = __builtin_amdgcn_udot2(as_u2(row[i]), as_u2(row[i]), e, false);
+            for (int i = 0; i < 4 * C; ++i) {
+                e = __builtin_amdgcn_udot2(as_u2(row[i]), as_u2(row[i]), e, false);
 #pragma unroll
-                    for (int r = 0; r < 8; ++r)
-                        if (t - r < k)
-                            acc[(7 + u - r) & 7] =
-                                __builtin_amdgcn_udot2(as_u2(P[r][i]), as_u2(row[i]), acc[(7 + u - r) & 7], false);
-                }
-                energy[(7 + u) & 7] = e;
-                window += e;
-                if (t - 7 < kk) emit(t - 7, A2 + window - 2u * acc[u]);
-                acc[u] = 0;
-                window -= energy[u];
+                for (int r = 0; r < 8; ++r)
+                    acc[(u - r) & 7] = __builtin_amdgcn_udot2(as_u2(P[r][i]), as_u2(row[i]), acc[(u - r) & 7], false);
             }
+            energy[u] = e;
+            window += e;
+            if (t - 7 < kk) emit(t - 7, A2 + window - 2u * acc[(u + 1) & 7]);
+            acc[(u + 1) & 7] = 0;
+            window -= energy[(u + 1) & 7];
         }
     }
 }
@@ -508,7 +547,7 @@ __device__ __forceinline__ void slide(const uint16_t *__restrict__ ring, size_t 
         AVD_SLIDE_CASE(12) AVD_SLIDE_CASE(13) AVD_SLIDE_CASE(14) AVD_SLIDE_CASE(15) AVD_SLIDE_CASE(16)
         AVD_SLIDE_CASE(17)
 #undef AVD_SLIDE_CASE
-    default: slide_column<C>(ring, frame, W, H, x, y0, k, kk, P, A2, emit); break;
+    default: slide_blocks<C>(ring, frame, W, H, x, y0, k, kk, P, A2, emit); break;
     }
 }
 
@@ -558,27 +597,6 @@ __device__ __forceinline__ int xcd_block()
 }
 
 
-// Reference `ref`'s top-left corner on the STEP grid, the last position clamped to dim - 8.
-struct RefPos {
-    int ref, rx, ry;
-    bool live;
-};
-
-__device__ __forceinline__ RefPos ref_of_wave(const avd_collab_pass &p, int wave)
-{
-    const int W = p.width, H = p.height;
-    const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
-    const int refs = refs_x * refs_y;
-    RefPos r;
-    r.ref = xcd_block() * kWaves + wave;
-    r.live = r.ref < refs;
-    if (!r.live) r.ref = refs - 1;
-    const int iy = r.ref / refs_x, ix = r.ref - iy * refs_x;
-    r.rx = min(ix * (int)p.step, W - 8);
-    r.ry = min(iy * (int)p.step, H - 8);
-    return r;
-}
-
 // Adds this workgroup's scored candidates (times the channels) to one of kScoreSlots counters, so
 // the atomics of many workgroups do not queue on one address.  Every thread of the block calls it.
 __device__ __forceinline__ void add_scored(unsigned long long *scored_count, uint32_t scored, bool live, int C)
@@ -604,14 +622,17 @@ __device__ __forceinline__ void add_scored(unsigned long long *scored_count, uin
 // NARROW: codes of at most 10 bits and spatial_radius <= 15, so a lane's keys are
 // (distance << 5 | candidate) in 32 bits; otherwise (distance << 32 | visit index) in 64.
 template <int C, bool NARROW>
-__global__ __launch_bounds__(kWaves * 64) void k_centre_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
-                                                              uint32_t *__restrict__ members,
-                                                              unsigned long long *__restrict__ scored_count)
+__device__ __forceinline__ uint32_t centre_search_wave(const avd_collab_pass &p, const uint16_t *__restrict__ ring,
+                                                       uint32_t *__restrict__ members, int ref_raw)
 {
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const RefPos rp = ref_of_wave(p, wave);
+    const int lane = threadIdx.x & 63;
     const int W = p.width, H = p.height, maxx = W - 8, maxy = H - 8;
-    const int rx = rp.rx, ry = rp.ry;
+    const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
+    const int refs = refs_x * refs_y;
+    const bool live = ref_raw < refs;
+    const int ref = min(ref_raw, refs - 1);
+    const int iy = ref / refs_x, ix = ref - iy * refs_x;
+    const int rx = min(ix * (int)p.step, W - 8), ry = min(iy * (int)p.step, H - 8);
     const size_t cframe = (size_t)p.centre_slot * ((size_t)W * H * C);
     const int S = p.spatial_radius;
 
@@ -653,7 +674,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_codes(avd_collab_pass p,
                                : (((uint64_t)(L[0] >> 5) << 32) | (uint32_t)((y0 + (int)(L[0] & 31) - top) * rw + col));
         else
             head = LW[0];
-        const uint64_t best = wave_min_u64(head);
+        // The wave minimum: rows of 16 by DPP, then the four rows read out.
+        const uint64_t rows = row16_min_u64(head);
+        const uint64_t a = readlane_u64(rows, 0), b = readlane_u64(rows, 16);
+        const uint64_t c = readlane_u64(rows, 32), d = readlane_u64(rows, 48);
+        const uint64_t ab = a < b ? a : b, cd = c < d ? c : d;
+        const uint64_t best = ab < cd ? ab : cd;
         if (best != kNoKey) {  // an unfilled slot keeps position word 0
             const int idx = (int)(best & 0xFFFFFFFFu);
             if (lane == m) word = pack_pos_t((uint32_t)(left + idx % rw), (uint32_t)(top + idx / rw), 0);
@@ -671,8 +697,326 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_codes(avd_collab_pass p,
     uint32_t k_use = 1;
     while (k_use * 2 <= (uint32_t)n_live && k_use * 2 <= p.group_size) k_use *= 2;
     if (lane == 8) word = k_use;
-    if (rp.live && lane < 9) members[(size_t)rp.ref * 9 + lane] = word;
-    add_scored(scored_count, lane == 0 ? (uint32_t)(n_live - 1) : 0u, rp.live, C);  // the self-match is not scored
+    if (live && lane < 9) members[(size_t)ref * 9 + lane] = word;
+    return live && lane == 0 ? (uint32_t)(n_live - 1) : 0u;  // the self-match is not scored
+}
+
+template <int C, bool NARROW>
+__global__ __launch_bounds__(kWaves * 64) void k_centre_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
+                                                              uint32_t *__restrict__ members,
+                                                              unsigned long long *__restrict__ scored_count)
+{
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t scored = centre_search_wave<C, NARROW>(p, ring, members, xcd_block() * kWaves + wave);
+    add_scored(scored_count, scored, true, C);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Centre search on the matrix cores.
+//
+// The cross term <P, B> of every (reference, candidate) pair is a matrix product: the references'
+// patches are rows of A, the candidates' patches columns of B, patch elements the shared K.  With
+// codes below 1024 it is exact on v_mfma_f32_16x16x32_f16: B holds 1024 + code (the f16 bit
+// pattern 0x6400 | code, prepared once per pass), A holds the reference codes split into their
+// high and low five bits, so every product and partial sum is an integer below 2^24 and the fp32
+// accumulation is exact in any order; <P, B> = 32 * C_hi + C_lo - 1024 * sum(P).
+//
+// A wave takes a block of 2 x 4 references (2 grid columns, 4 grid rows).  A's 16 rows are the
+// (reference, half) pairs; B's columns are 16 consecutive candidate columns of one candidate row
+// cy, two tiles covering every column any of the 8 references searches, and the wave walks cy down
+// the union of their row ranges, so the 16 lanes of a k-block read consecutive codes of one frame
+// row.  Lane (n, g) receives rows 4g..4g+3 of each product, both halves of references 2g and 2g+1
+// (grid row g) for candidate column n of the tile, forms their keys and keeps a sorted top 8 per
+// reference; the 16 lanes of a row then merge each reference's lists by DPP.
+//
+// Keys are (min(d, kKeyCap) << 10 | row-major index) in 32 bits.  A reference whose eighth key
+// reaches the cap (a distance that large could tie-break differently) is listed and searched again
+// with the exact 64-bit keys of the vector search (k_centre_redo).
+// ---------------------------------------------------------------------------------------------
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr uint32_t kKeyCap = (1u << 22) - 1u;
+constexpr int kPrepRows = 32;
+
+// The centre frame as f16 patterns of 1024 + code (`plane`, and for one channel `plane_odd`, the
+// same shifted by one code so that an 8-code row starting on an odd code is a 4-byte-aligned read)
+// and the energy sum(code^2) of the 8x8 patch at every position x <= W - 8, y <= H - 8.  Thread x
+// of the grid walks kPrepRows rows, keeping the last 8 rows' energies.
+template <int C>
+__global__ __launch_bounds__(256) void k_centre_prep(const uint16_t *__restrict__ frame, int W, int H,
+                                                     uint16_t *__restrict__ plane, uint16_t *__restrict__ plane_odd,
+                                                     uint32_t *__restrict__ energy)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y0 = blockIdx.y * kPrepRows;
+    if (x >= W) return;
+    const size_t n_codes = (size_t)W * H * C;
+    const int y_end = min(y0 + kPrepRows, H);
+    for (int y = y0; y < y_end; ++y) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const size_t i = ((size_t)y * W + x) * C + c;
+            plane[i] = (uint16_t)(0x6400u | frame[i]);
+            if (C == 1) plane_odd[i] = (uint16_t)(0x6400u | (i + 1 < n_codes ? frame[i + 1] : 0u));
+        }
+    }
+    if (x > W - 8) return;
+    uint32_t rows[8];
+    uint32_t window = 0;
+    const int last = min(y0 + kPrepRows, H - 7) + 7;  // energy rows y0 .. min(y0 + kPrepRows, H - 7) - 1
+    for (int y = y0; y < last; ++y) {
+        uint32_t e = 0;
+#pragma unroll
+        for (int u = 0; u < 8 * C; ++u) {
+            const uint32_t v = frame[((size_t)y * W + x) * C + u];
+            e += v * v;
+        }
+        const int slot = (y - y0) & 7;
+        if (y - y0 >= 8) window -= rows[slot];
+        rows[slot] = e;
+        window += e;
+        if (y - y0 >= 7) energy[(size_t)(y - 7) * W + x] = window;
+    }
+}
+
+// Row minimum of 32-bit keys over the 16 lanes of each DPP row (the value lands in every lane).
+__device__ __forceinline__ uint32_t row16_min_u32(uint32_t v)
+{
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false));  // row_ror 4
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false));  // row_ror 8
+    return v;
+}
+
+// Eight consecutive codes of the f16 plane starting at code `a`, as 4-byte-aligned reads: for one
+// channel an odd start reads the shifted copy, `odd_at` elements past the plane.
+template <int C>
+__device__ __forceinline__ half8_t load_b8(const uint16_t *__restrict__ plane, uint32_t odd_at, uint32_t a)
+{
+    const uint32_t i = (C == 1 && (a & 1u)) ? odd_at + a - 1u : a;
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(plane) + (i >> 1);
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = q[k];
+    return __builtin_bit_cast(half8_t, w);
+}
+
+template <int C>
+__global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, const uint16_t *__restrict__ ring,
+                                                             const uint16_t *__restrict__ plane, uint32_t odd_at,
+                                                             const uint32_t *__restrict__ energy,
+                                                             uint32_t *__restrict__ members,
+                                                             uint32_t *__restrict__ redo_list,
+                                                             unsigned long long *__restrict__ scored_count)
+{
+    constexpr int KS = 2 * C;  // K steps of 32 patch elements
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int n = lane & 15, g = lane >> 4;
+    const int W = p.width, H = p.height, maxx = W - 8, maxy = H - 8;
+    const int step = p.step, S = p.spatial_radius;
+    const int refs_x = (W - 8 + step - 1) / step + 1, refs_y = (H - 8 + step - 1) / step + 1;
+    const int blocks_x = (refs_x + 1) >> 1, blocks_y = (refs_y + 3) >> 2;
+    const int blk_raw = xcd_block() * kWaves + wave;
+    const bool blk_live = blk_raw < blocks_x * blocks_y;
+    const int blk = min(blk_raw, blocks_x * blocks_y - 1);
+    const int by = blk / blocks_x, bx = blk - by * blocks_x;
+    const size_t cframe = (size_t)p.centre_slot * ((size_t)W * H * C);
+
+    // Reference j = 2q + i of the block sits in grid column 2bx + i and grid row 4by + q (both
+    // clamped to the grid; a clamped one is searched but not written).
+    auto ref_col = [&](int i) { return min(2 * bx + i, refs_x - 1); };
+    auto ref_row = [&](int q) { return min(4 * by + q, refs_y - 1); };
+    auto pos_x = [&](int ix) { return min(ix * step, W - 8); };
+    auto pos_y = [&](int iy) { return min(iy * step, H - 8); };
+
+    // ---- A: lane (n, g) holds row n (reference n / 2; its high five bits when n is even) at the
+    // k-block g of every K step: patch row 4s + g (one channel) or half g & 1 of row 2s + g / 2.
+    half8_t A[KS];
+    {
+        const int j = n >> 1;
+        const int ax = pos_x(ref_col(j & 1)), ay = pos_y(ref_row(j >> 1));
+        const bool hi = (n & 1) == 0;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            uint32_t w[4];
+            if (C == 1) {
+                load_row<1>(ring, cframe + (size_t)(ay + 4 * s + g) * W + ax, w);
+            } else {
+                const uint32_t *q = reinterpret_cast<const uint32_t *>(
+                    ring + cframe + ((size_t)(ay + 2 * s + (g >> 1)) * W + ax) * 2 + 8 * (g & 1));
+#pragma unroll
+                for (int i = 0; i < 4; ++i) w[i] = q[i];
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t code = (w[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+                A[s][e] = (_Float16)(float)(hi ? (code >> 5) : (code & 31u));
+            }
+        }
+    }
+
+    // ---- the two references whose keys this lane forms: j = 2g + r, grid row g, column r.
+    int rx_[2], ry_[2], left_[2], top_[2], bottom_[2], rw_[2], nlive_[2];
+    uint32_t kc_[2];
+    bool live_[2];
+    int ref_[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int ix = ref_col(r), iy = ref_row(g);
+        live_[r] = blk_live && 2 * bx + r < refs_x && 4 * by + g < refs_y;
+        ref_[r] = iy * refs_x + ix;
+        rx_[r] = pos_x(ix);
+        ry_[r] = pos_y(iy);
+        left_[r] = clampi(rx_[r] - S, 0, maxx);
+        const int right = clampi(rx_[r] + S, 0, maxx);
+        top_[r] = clampi(ry_[r] - S, 0, maxy);
+        bottom_[r] = clampi(ry_[r] + S, 0, maxy);
+        rw_[r] = right - left_[r] + 1;
+        nlive_[r] = rw_[r] * (bottom_[r] - top_[r] + 1);
+        // kc = |P|^2 + 2 * 1024 * sum(P): SSD = kc + |B|^2 - 2 * (32 * C_hi + C_lo).
+        uint32_t e2 = 0, sum = 0;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            uint32_t w[4 * C];
+            load_row<C>(ring, cframe + ((size_t)(ry_[r] + v) * W + rx_[r]) * C, w);
+            e2 = row_dot<C>(w, w, e2);
+#pragma unroll
+            for (int t = 0; t < 4 * C; ++t) sum = __builtin_amdgcn_udot2(as_u2(w[t]), as_u2(0x00010001u), sum, false);
+        }
+        kc_[r] = e2 + 2048u * sum;
+    }
+    // Per tile t and reference r: candidate column cx = cx_base + 16 t + n, whether it is searched,
+    // its offset from the rectangle's left edge and whether it is the reference's own column.  A
+    // column past the frame reads the last one (its keys are dropped).
+    const int cx_base = clampi(pos_x(ref_col(0)) - S, 0, maxx);
+    bool colok[2][2], selfcol[2][2];
+    int colidx[2][2];
+    uint32_t ecol[2], bcol[KS][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int cx = cx_base + 16 * t + n;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            colok[t][r] = cx >= left_[r] && cx < left_[r] + rw_[r];
+            selfcol[t][r] = cx == rx_[r];
+            colidx[t][r] = cx - left_[r];
+        }
+        const int lx = min(cx, maxx);
+        ecol[t] = (uint32_t)lx;
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+            bcol[s][t] = C == 1 ? (uint32_t)((4 * s + g) * W + lx) : (uint32_t)(((2 * s + (g >> 1)) * W + lx) * 2 + 8 * (g & 1));
+    }
+    const int cy_lo = clampi(pos_y(ref_row(0)) - S, 0, maxy), cy_hi = clampi(pos_y(ref_row(3)) + S, 0, maxy);
+
+    uint32_t L[2][8];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) L[r][i] = ~0u;
+    // The next row's operands load while this row's products and keys are formed (two operand
+    // sets in turn, so no copy waits on a load in flight).  cy <= maxy, so every patch row exists.
+    auto load_row_ops = [&](int cy, half8_t (&B)[KS][2], uint32_t (&eb)[2]) {
+        const uint32_t base = (uint32_t)(cy * W * C);
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) B[s][t] = load_b8<C>(plane, odd_at, base + bcol[s][t]);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) eb[t] = energy[(uint32_t)(cy * W) + ecol[t]];
+    };
+    auto score_row = [&](int cy, const half8_t (&B)[KS][2], const uint32_t (&eb)[2]) {
+        float4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[s], B[s][t], acc[t], 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const bool rowok = cy >= top_[r] && cy <= bottom_[r];
+            const int rowbase = (cy - top_[r]) * rw_[r];
+            const bool selfrow = cy == ry_[r];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const bool ok = rowok && colok[t][r];
+                const uint32_t cross = ((uint32_t)acc[t][2 * r] << 5) + (uint32_t)acc[t][2 * r + 1];
+                const uint32_t ssd = kc_[r] + eb[t] - 2u * cross;
+                const uint32_t d = (selfrow && selfcol[t][r]) ? 0u : ssd + 1u;
+                const uint32_t key = (min(d, kKeyCap) << 10) | (uint32_t)(rowbase + colidx[t][r]);
+                insert8_u32(L[r], key | (ok ? 0u : ~0u));
+            }
+        }
+    };
+    half8_t Bx[KS][2], By[KS][2];
+    uint32_t ex[2], ey[2];
+    load_row_ops(cy_lo, Bx, ex);
+    for (int cy = cy_lo; cy <= cy_hi; cy += 2) {
+        load_row_ops(min(cy + 1, cy_hi), By, ey);
+        score_row(cy, Bx, ex);
+        if (cy + 1 > cy_hi) break;
+        load_row_ops(min(cy + 2, cy_hi), Bx, ex);
+        score_row(cy + 1, By, ey);
+    }
+
+    // ---- merge each reference's 16 lists (one DPP row) into its best eight.
+    uint32_t word[2] = {0u, 0u};
+    bool redo[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t inv = (65536u + (uint32_t)rw_[r] - 1u) / (uint32_t)rw_[r];  // idx / rw exact below 1024
+        uint32_t last = ~0u;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const uint32_t head = L[r][0];
+            const uint32_t best = row16_min_u32(head);
+            if (best != ~0u) {
+                last = best;
+                const uint32_t idx = best & 1023u;
+                const uint32_t qy = (idx * inv) >> 16, qx = idx - qy * (uint32_t)rw_[r];
+                if (n == m) word[r] = pack_pos_t((uint32_t)left_[r] + qx, (uint32_t)top_[r] + qy, 0);
+                if (head == best) {
+#pragma unroll
+                    for (int i = 0; i < 7; ++i) L[r][i] = L[r][i + 1];
+                    L[r][7] = ~0u;
+                }
+            }
+        }
+        // Keys sort by the capped distance; once a chosen one is capped, their order may not be
+        // the distances' order.
+        redo[r] = last != ~0u && (last >> 10) >= kKeyCap;
+        uint32_t k_use = 1;
+        while (k_use * 2 <= (uint32_t)nlive_[r] && k_use * 2 <= p.group_size) k_use *= 2;
+        if (n == 8) word[r] = k_use;
+        if (live_[r] && !redo[r] && n < 9) members[(size_t)ref_[r] * 9 + n] = word[r];
+    }
+
+    // ---- references whose keys reached the cap go to the exact vector search (k_centre_redo).
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+        if (n == 0 && redo[r] && live_[r]) redo_list[1 + atomicAdd(redo_list, 1u)] = (uint32_t)ref_[r];
+    uint32_t scored = 0;
+    if (n == 0) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+            if (live_[r]) scored += (uint32_t)(nlive_[r] - 1);  // the self-match is not scored
+    }
+    add_scored(scored_count, scored, true, C);
+}
+
+// The exact vector centre search (64-bit keys) of the references in `redo_list` (count, then
+// references); every wave walks the list with the grid's stride.
+template <int C>
+__global__ __launch_bounds__(kWaves * 64) void k_centre_redo(avd_collab_pass p, const uint16_t *__restrict__ ring,
+                                                             const uint32_t *__restrict__ redo_list,
+                                                             uint32_t *__restrict__ members)
+{
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t count = __builtin_amdgcn_readfirstlane(redo_list[0]);
+    for (uint32_t i = blockIdx.x * kWaves + wave; i < count; i += gridDim.x * kWaves)
+        centre_search_wave<C, false>(p, ring, members, (int)__builtin_amdgcn_readfirstlane(redo_list[1 + i]));
 }
 
 // Trajectory search (trajectory.rs:142-450) and the grid decision (fused/mod.rs:407-553) of every
@@ -1514,23 +1858,70 @@ bool pass_ok(const avd_collab_pass &p)
 
 }  // namespace
 
+size_t centre_scratch_bytes(uint32_t W, uint32_t H, uint32_t C)
+{
+    const size_t px = (size_t)W * H;
+    const size_t plane = (px * C * 2 + 255) & ~(size_t)255, odd = C == 1 ? (px * 2 + 255) & ~(size_t)255 : 0;
+    // the redo list holds a count and at most one entry per reference (refs < px)
+    return plane + odd + px * 4 + (px + 1) * 4;
+}
+
+CentreScratch centre_scratch_at(void *base, uint32_t W, uint32_t H, uint32_t C)
+{
+    const size_t px = (size_t)W * H;
+    const size_t plane = (px * C * 2 + 255) & ~(size_t)255, odd = C == 1 ? (px * 2 + 255) & ~(size_t)255 : 0;
+    char *b = static_cast<char *>(base);
+    return CentreScratch{reinterpret_cast<uint16_t *>(b), C == 1 ? reinterpret_cast<uint16_t *>(b + plane) : nullptr,
+                         reinterpret_cast<uint32_t *>(b + plane + odd),
+                         reinterpret_cast<uint32_t *>(b + plane + odd + px * 4)};
+}
+
+bool centre_on_matrix_cores(const avd_collab_pass &p)
+{
+    return p.code_bits >= 1 && p.code_bits <= 10 && p.step + 2 * p.spatial_radius <= 31;
+}
+
 hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
                                 const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
-                                hipStream_t s, unsigned long long *scored_count)
+                                hipStream_t s, unsigned long long *scored_count, const CentreScratch *centre)
 {
     if (!pass_ok(p)) return hipErrorInvalidValue;
     const unsigned g = div_up(ref_count(p), kWaves);
     if (codes) {
-        // 32-bit keys hold (SSD + 1) << 5: SSD < 64 * C * 1023^2 + 1 < 2^27 for codes of at most
-        // 10 bits, and a lane's run of candidates stays below 32 while rw <= 32.
-        const bool narrow = p.code_bits >= 1 && p.code_bits <= 10 && p.spatial_radius <= 15;
         const dim3 blk(kWaves * 64);
-        if (p.channels == 1) {
-            if (narrow) k_centre_codes<1, true><<<g, blk, 0, s>>>(p, codes, members, scored_count);
-            else k_centre_codes<1, false><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+        if (centre && centre_on_matrix_cores(p)) {
+            if (!centre->plane || !centre->energy || !centre->redo || (p.channels == 1 && !centre->plane_odd))
+                return hipErrorInvalidValue;
+            const int W = (int)p.width, H = (int)p.height, st = (int)p.step;
+            const uint16_t *frame = codes + (size_t)p.centre_slot * W * H * p.channels;
+            const dim3 pg(div_up(W, 256), div_up(H, kPrepRows));
+            const int refs_x = (W - 8 + st - 1) / st + 1, refs_y = (H - 8 + st - 1) / st + 1;
+            const unsigned gm = div_up(((refs_x + 1) / 2) * ((refs_y + 3) / 4), kWaves);
+            const hipError_t e = launch_fill_u32(centre->redo, 0, 1, s);
+            if (e != hipSuccess) return e;
+            if (p.channels == 1) {
+                k_centre_prep<1><<<pg, 256, 0, s>>>(frame, W, H, centre->plane, centre->plane_odd, centre->energy);
+                k_centre_mfma<1><<<gm, blk, 0, s>>>(p, codes, centre->plane,
+                                                    (uint32_t)(centre->plane_odd - centre->plane), centre->energy,
+                                                    members, centre->redo, scored_count);
+                k_centre_redo<1><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
+            } else {
+                k_centre_prep<2><<<pg, 256, 0, s>>>(frame, W, H, centre->plane, nullptr, centre->energy);
+                k_centre_mfma<2><<<gm, blk, 0, s>>>(p, codes, centre->plane, 0u, centre->energy, members,
+                                                    centre->redo, scored_count);
+                k_centre_redo<2><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
+            }
         } else {
-            if (narrow) k_centre_codes<2, true><<<g, blk, 0, s>>>(p, codes, members, scored_count);
-            else k_centre_codes<2, false><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+            // 32-bit keys hold (SSD + 1) << 5: SSD < 64 * C * 1023^2 + 1 < 2^27 for codes of at
+            // most 10 bits, and a lane's run of candidates stays below 32 while rw <= 32.
+            const bool narrow = p.code_bits >= 1 && p.code_bits <= 10 && p.spatial_radius <= 15;
+            if (p.channels == 1) {
+                if (narrow) k_centre_codes<1, true><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+                else k_centre_codes<1, false><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+            } else {
+                if (narrow) k_centre_codes<2, true><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+                else k_centre_codes<2, false><<<g, blk, 0, s>>>(p, codes, members, scored_count);
+            }
         }
         if (p.temporal_radius >= 1 && p.group_size == 8) {
             // Four references per wave where their tables fit the LDS budget, else two (the largest

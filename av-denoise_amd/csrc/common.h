@@ -57,9 +57,23 @@ hipError_t launch_warp(const float *src, float *dst, const int32_t *mv, int W, i
 // `scored_count`, when non-null, points at kScoreSlots counters kScoreStride apart; their sum gains
 // the candidate patches the launch scores times the channels.
 constexpr int kScoreSlots = 64, kScoreStride = 16;
+// Scratch of the matrix-core centre search: the centre frame as f16 patterns of 1024 + code
+// (`plane`, W*H*C u16; `plane_odd`, the same shifted by one code, W*H u16, one channel only) and
+// the 8x8 patch energy of every position (`energy`, W*H u32), and the references searched again
+// with exact keys (`redo`: a count, then the references).
+struct CentreScratch {
+    uint16_t *plane, *plane_odd;
+    uint32_t *energy, *redo;
+};
+size_t centre_scratch_bytes(uint32_t W, uint32_t H, uint32_t C);
+CentreScratch centre_scratch_at(void *base, uint32_t W, uint32_t H, uint32_t C);
+// Whether a pass takes the matrix-core centre search (integer codes below 1024 and a spatial
+// radius whose two grid rows of candidates fit 32 rows); it then needs a CentreScratch.
+bool centre_on_matrix_cores(const avd_collab_pass &p);
 hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
                                 const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
-                                hipStream_t s, unsigned long long *scored_count = nullptr);
+                                hipStream_t s, unsigned long long *scored_count = nullptr,
+                                const CentreScratch *centre = nullptr);
 // Accumulators of the fixed-point scatter: `aggregate` holds per pixel (weight sum << 32) plus the
 // first channel's sum, one 64-bit atomic add per pixel; `accum_second` the second channel's sum.
 hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, const uint32_t *nslots,

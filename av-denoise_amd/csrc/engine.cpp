@@ -649,6 +649,10 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
         bins_ = TileBins{stage_.as<int32_t>(), counts, tile_list_.as<uint32_t>(), tile_overflow_.as<uint32_t>(),
                          counts + n_tiles, tiles_x, per_slot};
     }
+    if (centre_on_matrix_cores(pass_)) {
+        centre_buf_.allocate(centre_scratch_bytes(W, H, (uint32_t)C));
+        centre_ = centre_scratch_at(centre_buf_.ptr, W, H, (uint32_t)C);
+    }
     output_.allocate(px * C * sizeof(float));
     const size_t blocks = (size_t)mg_.blocks_x * mg_.blocks_y;
     mv_.allocate(2 * r_ * blocks * 2 * sizeof(int32_t));
@@ -697,7 +701,8 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
     unsigned long long *counter = timer_ ? timer_->scored_counter() : nullptr;
     timed(AVD_STAGE_COLLAB_SEARCH, 0, stream_, "collab search", [&] {
         return launch_collab_search(p, ring_.code_ptr(), ring_.value_ptr(), nslots, mv_.as<int32_t>(),
-                                    conf_.as<float>(), members_.as<uint32_t>(), stream_, counter);
+                                    conf_.as<float>(), members_.as<uint32_t>(), stream_, counter,
+                                    centre_.plane ? &centre_ : nullptr);
     });
     const uint64_t filter_ops = (uint64_t)refs_ * 8 * ((uint64_t)C_ * (4096 + 576 + 256) + 64 * (C_ + 1));
     timed(AVD_STAGE_COLLAB_FILTER, filter_ops, stream_, "collab filter", [&] {

@@ -205,6 +205,9 @@ class Nl4dPass : public Pass {
     // Binned aggregation: staged member values, per-tile counts / offsets / cursors, member lists.
     DeviceBuffer stage_, tile_count_, tile_list_, tile_overflow_;
     TileBins bins_{};
+    // The matrix-core centre search's f16 centre frame and patch energies (when the pass takes it).
+    DeviceBuffer centre_buf_;
+    CentreScratch centre_{};
     std::vector<uint32_t> pending_, tail_;
     std::vector<bool> zeroed_;
     uint32_t passes_ = 0;

@@ -30,7 +30,7 @@ CLIP_FRAMES = 16
 VALU_PEAK_TOPS = 157.3
 OP_STAGES = ["collab_search", "motion", "collab_filter"]
 STAGE_KERNELS = {
-    "collab_search": "k_centre_codes + k_trajectory_codes",
+    "collab_search": "k_centre_prep + k_centre_mfma + k_trajectory_codes",
     "motion": "k_bm_coarse + k_bm_fine",
     "collab_filter": "k_collab_filter",
 }

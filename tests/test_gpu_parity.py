@@ -186,6 +186,12 @@ def _oracle_fields(codes, W, H, C, centre, slots, sigma, max_value, geom):
     ("tiny frame", 9, 8, 1, 10, 1, 3, 10, "scene"),
     ("narrow frame", 8, 40, 2, 10, 2, 5, 10, "scene"),
     ("8-bit full range noise", 56, 48, 1, 8, 3, 12, 8, "noise"),
+    # Radius 14 leaves the matrix-core centre search (two grid rows of candidates exceed 32 rows).
+    ("radius 14, vector centre search", 64, 48, 1, 10, 2, 14, 10, "scene"),
+    # Full-range 10-bit noise puts the best distances past the 32-bit key range of the matrix-core
+    # centre search, whose references are then searched again with exact keys.
+    ("10-bit full range noise, capped keys", 40, 32, 1, 10, 2, 2, 10, "noise"),
+    ("10-bit chroma noise, capped keys", 36, 28, 2, 10, 1, 2, 10, "noise"),
     # 32-pixel blocks at step 8 cover an anchor with up to 16 blocks; at these radii the rectangle
     # tables only fit two references per wave.
     ("deep window, 16 covering blocks", 64, 48, 1, 10, 6, 6, 10, "scene", 32, 24),
