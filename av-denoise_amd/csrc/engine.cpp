@@ -368,9 +368,17 @@ NlmPass::NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, av
     if (hq_ && !has_sigma_)
         throw Failure(AVD_ERR_INVALID_OPTIONS,
                       "HQ noise estimation is not built yet; pass a fixed sigma (HqParams::sigma_override)");
-    if (confidence_ && !motion_)
-        throw Failure(AVD_ERR_INVALID_OPTIONS,
-                      "HQ temporal confidence without motion compensation is not supported by this build");
+    // Confidence with motion compensation off scores each block where it stands
+    // (MotionCtx::confidence_only, motion/context.rs:182-200: 16x16 blocks at step 8, one level,
+    // radius 0) and leaves the neighbours unwarped.
+    conf_only_ = confidence_ && !motion_;
+    if (conf_only_) {
+        if (in.kind == AVD_FORMAT_U16 && in.depth > 12)
+            throw Failure(AVD_ERR_INVALID_GEOMETRY, "motion search on more than 12-bit samples is not supported");
+        mg_ = MotionGeom{(int)W, (int)H, C, 16, 8, 0, 1, 0, 0};
+        mg_.blocks_x = std::max((int)div_up(W, mg_.step), 1);
+        mg_.blocks_y = std::max((int)div_up(H, mg_.step), 1);
+    }
     if (motion_) {
         const avd_motion_search &m = o.motion;
         if (m.overlap >= m.blksize || m.blksize > 32 || m.blksize < 2 || m.search_radius > 8 || m.pyramid_levels < 1 ||
@@ -405,9 +413,9 @@ NlmPass::NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, av
     lut[S_ * side + S_] = 0.0f;  // build_spatial_offset_lut at rho = 0 (noise/correlation.rs:75-88)
     lut_.allocate(lut.size() * sizeof(float));
     check_hip(hipMemcpy(lut_.ptr, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice), "upload lut");
-    if (motion_) {
+    if (motion_ || conf_only_) {
         const size_t blocks = (size_t)mg_.blocks_x * mg_.blocks_y;
-        comp_.allocate(N * px * C * sizeof(float));
+        if (motion_) comp_.allocate(N * px * C * sizeof(float));
         mv_.allocate(2 * r_ * blocks * 2 * sizeof(int32_t));
         conf_.allocate(2 * r_ * blocks * sizeof(float));
     }
@@ -463,8 +471,9 @@ void NlmPass::denoise()
     auto frame_of = [&](int k) -> const float * { return ring_.value_slot(ring_.phys((int)r_ + k)); };
     std::vector<const float *> nb(2 * r_ + 1, nullptr);
     for (int k = -(int)r_; k <= (int)r_; ++k) nb[k + r_] = frame_of(k);
-    if (motion_) {
-        // run_motion_compensation (nlmeans/dispatch/motion.rs:427-520): neighbours oldest first.
+    if (motion_ || conf_only_) {
+        // run_motion_compensation (nlmeans/dispatch/motion.rs:427-520), or run_confidence_pass
+        // (:554-612) when only confidence is on: neighbours oldest first.
         std::vector<uint32_t> slots;
         for (int k = -(int)r_; k <= (int)r_; ++k)
             if (k != 0) slots.push_back(ring_.phys((int)r_ + k));
@@ -475,7 +484,7 @@ void NlmPass::denoise()
                                  (int)slots.size(), mv_.as<int32_t>(), conf_.as<float>(), mx,
                                  sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), stream_);
         });
-        for (size_t i = 0; i < slots.size(); ++i) {
+        for (size_t i = 0; i < slots.size() && motion_; ++i) {
             const int k = (int)i < (int)r_ ? (int)i - (int)r_ : (int)i - (int)r_ + 1;
             float *dst = comp_.as<float>() + (size_t)slots[i] * px * C;
             timed(AVD_STAGE_WARP, px, stream_, "warp", [&] {
@@ -489,12 +498,12 @@ void NlmPass::denoise()
     check_hip(launch_fill_u32(wsum_.ptr, 0, px, stream_), "zero");
     check_hip(launch_fill_u32(maxw_.ptr, 0, px, stream_), "zero");
     NlmWindow w{(int)W_, (int)H_, C, (int)P_, (int)S_, h2_ * 1.4426950408889634f, offset_, 1, 1, 1};
-    if (motion_) {
+    if (motion_ || conf_only_) {
         w.step = mg_.step;
         w.blocks_x = mg_.blocks_x;
         w.blocks_y = mg_.blocks_y;
     }
-    const size_t blocks = motion_ ? (size_t)mg_.blocks_x * mg_.blocks_y : 0;
+    const size_t blocks = (motion_ || conf_only_) ? (size_t)mg_.blocks_x * mg_.blocks_y : 0;
     auto conf_of = [&](int k) -> const float * {
         if (!confidence_) return nullptr;
         const int idx = k < 0 ? k + (int)r_ : k + (int)r_ - 1;

@@ -160,7 +160,7 @@ class NlmPass : public Pass {
     avd_sample_format in_, out_;
     hipStream_t stream_;
     uint32_t r_ = 0, S_ = 2, P_ = 4;
-    bool hq_ = false, has_sigma_ = false, motion_ = false, confidence_ = false, pushed_ = false;
+    bool hq_ = false, has_sigma_ = false, motion_ = false, confidence_ = false, conf_only_ = false, pushed_ = false;
     float sigma_ = 0.0f, self_weight_ = 1.0f, h2_ = 0.0f, offset_ = 0.0f, thsad_scale_ = 1.0f;
     MotionGeom mg_{};
     Ring ring_;

@@ -548,7 +548,21 @@ class NlmPass:
         frames = {}
         confs = {}
         geom = s.motion
-        if r > 0 and geom is not None:
+        conf_only = r > 0 and geom is None and s.hq and s.temporal_confidence
+        if conf_only:
+            # run_confidence_pass (nlmeans/dispatch/motion.rs:554-612) with the confidence_only block
+            # geometry (motion/context.rs:182-200): 16x16 blocks at step 8 scored where they stand
+            # (radius 0, one level); the neighbours stay unwarped.
+            geom = MotionGeom(W, H, 16, 8, 0, 1)
+            sigma = s.sigma if s.sigma is not None else 0.0
+            floor = float(sad_noise_floor(geom.blksize, sigma))
+            th = float(thsad(geom.blksize, s.thsad_scale))
+            for k in list(range(-r, 0)) + list(range(1, r + 1)):
+                slot = self.ring.phys(r + k)
+                _, conf = motion_search(geom, self.ring.pyr[centre], self.ring.pyr[slot], self.maxv, floor, th)
+                frames[k] = self.ring.values[slot]
+                confs[k] = conf
+        elif r > 0 and geom is not None:
             sigma = s.sigma if s.sigma is not None else 0.0
             floor = float(sad_noise_floor(geom.blksize, sigma))
             th = float(thsad(geom.blksize, s.thsad_scale))

@@ -366,7 +366,7 @@ def _max_lsb(a_words, b_words, depth, pixels):
     return int(np.max(np.abs(a - b)))
 
 
-@pytest.mark.parametrize("case", ["spatial8", "temporal10", "hq_motion10"])
+@pytest.mark.parametrize("case", ["spatial8", "temporal10", "hq_motion10", "hq_confidence_only10"])
 def test_nlm_engine_stream_within_one_lsb(case):
     if case == "spatial8":
         W, H, depth, frames = 80, 48, 8, 3
@@ -377,6 +377,14 @@ def test_nlm_engine_stream_within_one_lsb(case):
         W, H, depth, frames = 72, 40, 10, 5
         opt = avd.nlmeans_options(temporal=1, temporal_radius=2, search_radius=4, patch_radius=3)
         spec = O.NlmSpec(2, 4, 3, 1.2, 1.0)
+        sig = 12.0
+    elif case == "hq_confidence_only10":
+        # HQ confidence with motion compensation off: blocks scored where they stand, frames unwarped.
+        W, H, depth, frames = 72, 56, 10, 5
+        sigma = 12.0 / 1023
+        opt = avd.nlmeans_options(hq=1, temporal=1, temporal_radius=2, search_radius=3, patch_radius=2, sigma=sigma,
+                                  motion_compensation=0)
+        spec = O.NlmSpec(2, 3, 2, O.HQ_STRENGTH_LUMA[2], 1.0, hq=True, sigma=sigma)
         sig = 12.0
     else:
         W, H, depth, frames = 80, 64, 10, 6
