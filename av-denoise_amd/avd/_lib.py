@@ -80,7 +80,7 @@ class WindowSpan(ctypes.Structure):
 
 
 STAGES = ["ingest", "motion", "collab_search", "collab_filter", "normalise", "egress", "nlm_pair", "nlm_single",
-          "nlm_finish", "warp"]
+          "nlm_finish", "warp", "collab_centre"]
 
 
 class Timing(ctypes.Structure):

@@ -1881,57 +1881,63 @@ bool centre_on_matrix_cores(const avd_collab_pass &p)
     return p.code_bits >= 1 && p.code_bits <= 10 && p.step + 2 * p.spatial_radius <= 31;
 }
 
-hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
-                                const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
-                                hipStream_t s, unsigned long long *scored_count, const CentreScratch *centre)
+hipError_t launch_collab_centre(const avd_collab_pass &p, const uint16_t *codes, uint32_t *members, hipStream_t s,
+                                const CentreScratch *centre)
 {
-    if (!pass_ok(p)) return hipErrorInvalidValue;
-    const unsigned g = div_up(ref_count(p), kWaves);
-    if (codes) {
-        const dim3 blk(kWaves * 64);
-        if (centre && centre_on_matrix_cores(p)) {
-            if (!centre->plane || !centre->energy || !centre->redo || (p.channels == 1 && !centre->plane_odd))
-                return hipErrorInvalidValue;
-            const int W = (int)p.width, H = (int)p.height, st = (int)p.step;
-            const uint16_t *frame = codes + (size_t)p.centre_slot * W * H * p.channels;
-            const dim3 pg(div_up(W, 256), div_up(H, kPrepRows));
-            const int refs_x = (W - 8 + st - 1) / st + 1, refs_y = (H - 8 + st - 1) / st + 1;
-            const unsigned gm = div_up(((refs_x + 1) / 2) * ((refs_y + 3) / 4), kWaves);
-            const hipError_t e = launch_fill_u32(centre->redo, 0, 1, s);
-            if (e != hipSuccess) return e;
-            if (p.channels == 1) {
-                k_centre_prep<1><<<pg, 256, 0, s>>>(frame, W, H, centre->plane, centre->plane_odd, centre->energy);
-                k_centre_mfma<1><<<gm, blk, 0, s>>>(p, codes, centre->plane,
-                                                    (uint32_t)(centre->plane_odd - centre->plane), centre->energy,
-                                                    members, centre->redo, scored_count);
-                k_centre_redo<1><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
-            } else {
-                k_centre_prep<2><<<pg, 256, 0, s>>>(frame, W, H, centre->plane, nullptr, centre->energy);
-                k_centre_mfma<2><<<gm, blk, 0, s>>>(p, codes, centre->plane, 0u, centre->energy, members,
-                                                    centre->redo, scored_count);
-                k_centre_redo<2><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
-            }
+    if (!pass_ok(p) || !codes) return hipErrorInvalidValue;
+    const dim3 blk(kWaves * 64);
+    if (centre && centre_on_matrix_cores(p)) {
+        if (!centre->plane || !centre->energy || !centre->redo || (p.channels == 1 && !centre->plane_odd))
+            return hipErrorInvalidValue;
+        const int W = (int)p.width, H = (int)p.height, st = (int)p.step;
+        const uint16_t *frame = codes + (size_t)p.centre_slot * W * H * p.channels;
+        const dim3 pg(div_up(W, 256), div_up(H, kPrepRows));
+        const int refs_x = (W - 8 + st - 1) / st + 1, refs_y = (H - 8 + st - 1) / st + 1;
+        const unsigned gm = div_up(((refs_x + 1) / 2) * ((refs_y + 3) / 4), kWaves);
+        const hipError_t e = launch_fill_u32(centre->redo, 0, 1, s);
+        if (e != hipSuccess) return e;
+        if (p.channels == 1) {
+            k_centre_prep<1><<<pg, 256, 0, s>>>(frame, W, H, centre->plane, centre->plane_odd, centre->energy);
+            k_centre_mfma<1><<<gm, blk, 0, s>>>(p, codes, centre->plane, (uint32_t)(centre->plane_odd - centre->plane),
+                                                centre->energy, members, centre->redo, nullptr);
+            k_centre_redo<1><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
         } else {
-            // 32-bit keys hold (SSD + 1) << 5: SSD < 64 * C * 1023^2 + 1 < 2^27 for codes of at
-            // most 10 bits, and a lane's run of candidates stays below 32 while rw <= 32.
-            const bool narrow = p.code_bits >= 1 && p.code_bits <= 10 && p.spatial_radius <= 15;
-            if (p.channels == 1) {
-                if (narrow) k_centre_codes<1, true><<<g, blk, 0, s>>>(p, codes, members, scored_count);
-                else k_centre_codes<1, false><<<g, blk, 0, s>>>(p, codes, members, scored_count);
-            } else {
-                if (narrow) k_centre_codes<2, true><<<g, blk, 0, s>>>(p, codes, members, scored_count);
-                else k_centre_codes<2, false><<<g, blk, 0, s>>>(p, codes, members, scored_count);
-            }
+            k_centre_prep<2><<<pg, 256, 0, s>>>(frame, W, H, centre->plane, nullptr, centre->energy);
+            k_centre_mfma<2><<<gm, blk, 0, s>>>(p, codes, centre->plane, 0u, centre->energy, members, centre->redo,
+                                                nullptr);
+            k_centre_redo<2><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
         }
-        if (p.temporal_radius >= 1 && p.group_size == 8) {
-            // Four references per wave where their tables fit the LDS budget, else two (the largest
-            // table, 512 entries, fits at two).
-            const size_t ents = (size_t)trajectory_entries(p) * 2 * sizeof(uint32_t) * kWaves;
-            const int C = (int)p.channels;
-            const bool four = trajectory_static_lds(C, 4) + 4 * ents <= kLdsBudget;
-            const int nr = four ? 4 : 2;
-            const size_t lds = nr * ents;
-            const unsigned gt = div_up(ref_count(p), kWaves * nr);
+        return hipGetLastError();
+    }
+    // 32-bit keys hold (SSD + 1) << 5: SSD < 64 * C * 1023^2 + 1 < 2^27 for codes of at most 10
+    // bits, and a lane's run of candidates stays below 32 while rw <= 32.
+    const bool narrow = p.code_bits >= 1 && p.code_bits <= 10 && p.spatial_radius <= 15;
+    const unsigned g = div_up(ref_count(p), kWaves);
+    if (p.channels == 1) {
+        if (narrow) k_centre_codes<1, true><<<g, blk, 0, s>>>(p, codes, members, nullptr);
+        else k_centre_codes<1, false><<<g, blk, 0, s>>>(p, codes, members, nullptr);
+    } else {
+        if (narrow) k_centre_codes<2, true><<<g, blk, 0, s>>>(p, codes, members, nullptr);
+        else k_centre_codes<2, false><<<g, blk, 0, s>>>(p, codes, members, nullptr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_collab_trajectory(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
+                                    const int32_t *mv, const float *conf, uint32_t *members, hipStream_t s,
+                                    unsigned long long *scored_count)
+{
+    if (!pass_ok(p) || !codes) return hipErrorInvalidValue;
+    if (p.temporal_radius < 1 || p.group_size != 8) return hipSuccess;
+    const dim3 blk(kWaves * 64);
+    // Four references per wave where their tables fit the LDS budget, else two (the largest
+    // table, 512 entries, fits at two).
+    const size_t ents = (size_t)trajectory_entries(p) * 2 * sizeof(uint32_t) * kWaves;
+    const int C = (int)p.channels;
+    const bool four = trajectory_static_lds(C, 4) + 4 * ents <= kLdsBudget;
+    const int nr = four ? 4 : 2;
+    const size_t lds = nr * ents;
+    const unsigned gt = div_up(ref_count(p), kWaves * nr);
 #define AVD_TRAJ(Cn, Kn)                                                                                       \
     do {                                                                                                       \
         if (four)                                                                                              \
@@ -1939,22 +1945,34 @@ hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes,
         else                                                                                                   \
             k_trajectory_codes<Cn, Kn, 2><<<gt, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count); \
     } while (0)
-            switch (p.refine * 2 + 1 + (p.channels == 2 ? 100 : 0)) {
-            case 3: AVD_TRAJ(1, 3); break;
-            case 5: AVD_TRAJ(1, 5); break;
-            case 7: AVD_TRAJ(1, 7); break;
-            case 9: AVD_TRAJ(1, 9); break;
-            case 103: AVD_TRAJ(2, 3); break;
-            case 105: AVD_TRAJ(2, 5); break;
-            case 107: AVD_TRAJ(2, 7); break;
-            default: AVD_TRAJ(2, 9); break;
-            }
-#undef AVD_TRAJ
-        }
-    } else {
-        if (p.channels == 1) k_collab_search<1, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
-        else k_collab_search<2, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
+    switch (p.refine * 2 + 1 + (p.channels == 2 ? 100 : 0)) {
+    case 3: AVD_TRAJ(1, 3); break;
+    case 5: AVD_TRAJ(1, 5); break;
+    case 7: AVD_TRAJ(1, 7); break;
+    case 9: AVD_TRAJ(1, 9); break;
+    case 103: AVD_TRAJ(2, 3); break;
+    case 105: AVD_TRAJ(2, 5); break;
+    case 107: AVD_TRAJ(2, 7); break;
+    default: AVD_TRAJ(2, 9); break;
     }
+#undef AVD_TRAJ
+    return hipGetLastError();
+}
+
+hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
+                                const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
+                                hipStream_t s, unsigned long long *scored_count, const CentreScratch *centre)
+{
+    if (!pass_ok(p)) return hipErrorInvalidValue;
+    if (codes) {
+        const hipError_t e = launch_collab_centre(p, codes, members, s, centre);
+        return e != hipSuccess ? e : launch_collab_trajectory(p, codes, nslots, mv, conf, members, s, scored_count);
+    }
+    const unsigned g = div_up(ref_count(p), kWaves);
+    if (p.channels == 1)
+        k_collab_search<1, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
+    else
+        k_collab_search<2, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
     return hipGetLastError();
 }
 

@@ -70,6 +70,13 @@ CentreScratch centre_scratch_at(void *base, uint32_t W, uint32_t H, uint32_t C);
 // Whether a pass takes the matrix-core centre search (integer codes below 1024 and a spatial
 // radius whose two grid rows of candidates fit 32 rows); it then needs a CentreScratch.
 bool centre_on_matrix_cores(const avd_collab_pass &p);
+// The search as two launches (the engine times them apart): the centre search of every
+// reference, then the trajectory search and grid decision over integer codes.
+hipError_t launch_collab_centre(const avd_collab_pass &p, const uint16_t *codes, uint32_t *members, hipStream_t s,
+                                const CentreScratch *centre = nullptr);
+hipError_t launch_collab_trajectory(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
+                                    const int32_t *mv, const float *conf, uint32_t *members, hipStream_t s,
+                                    unsigned long long *scored_count = nullptr);
 hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
                                 const uint32_t *nslots, const int32_t *mv, const float *conf, uint32_t *members,
                                 hipStream_t s, unsigned long long *scored_count = nullptr,

@@ -658,6 +658,22 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
         bins_ = TileBins{stage_.as<int32_t>(), counts, tile_list_.as<uint32_t>(), tile_overflow_.as<uint32_t>(),
                          counts + n_tiles, tiles_x, per_slot};
     }
+    // The centre search scores every position of each reference's clipped rectangle but its own.
+    centre_ops_ = 0;
+    {
+        const int st = (int)o.step, S = (int)o.spatial_radius;
+        const int rx_n = ((int)W - 8 + st - 1) / st + 1, ry_n = ((int)H - 8 + st - 1) / st + 1;
+        uint64_t rows = 0, cols = 0;
+        for (int i = 0; i < rx_n; ++i) {
+            const int x = std::min(i * st, (int)W - 8);
+            cols += (uint64_t)(std::min(x + S, (int)W - 8) - std::max(x - S, 0) + 1);
+        }
+        for (int i = 0; i < ry_n; ++i) {
+            const int y = std::min(i * st, (int)H - 8);
+            rows += (uint64_t)(std::min(y + S, (int)H - 8) - std::max(y - S, 0) + 1);
+        }
+        centre_ops_ = (rows * cols - (uint64_t)rx_n * ry_n) * 192u * (uint64_t)C;
+    }
     if (centre_on_matrix_cores(pass_)) {
         centre_buf_.allocate(centre_scratch_bytes(W, H, (uint32_t)C));
         centre_ = centre_scratch_at(centre_buf_.ptr, W, H, (uint32_t)C);
@@ -708,10 +724,13 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
     avd_collab_pass p = pass_;
     p.centre_slot = centre;
     unsigned long long *counter = timer_ ? timer_->scored_counter() : nullptr;
-    timed(AVD_STAGE_COLLAB_SEARCH, 0, stream_, "collab search", [&] {
-        return launch_collab_search(p, ring_.code_ptr(), ring_.value_ptr(), nslots, mv_.as<int32_t>(),
-                                    conf_.as<float>(), members_.as<uint32_t>(), stream_, counter,
+    timed(AVD_STAGE_COLLAB_CENTRE, centre_ops_, stream_, "collab centre", [&] {
+        return launch_collab_centre(p, ring_.code_ptr(), members_.as<uint32_t>(), stream_,
                                     centre_.plane ? &centre_ : nullptr);
+    });
+    timed(AVD_STAGE_COLLAB_SEARCH, 0, stream_, "collab search", [&] {
+        return launch_collab_trajectory(p, ring_.code_ptr(), nslots, mv_.as<int32_t>(), conf_.as<float>(),
+                                        members_.as<uint32_t>(), stream_, counter);
     });
     const uint64_t filter_ops = (uint64_t)refs_ * 8 * ((uint64_t)C_ * (4096 + 576 + 256) + 64 * (C_ + 1));
     timed(AVD_STAGE_COLLAB_FILTER, filter_ops, stream_, "collab filter", [&] {

@@ -208,6 +208,7 @@ class Nl4dPass : public Pass {
     // The matrix-core centre search's f16 centre frame and patch energies (when the pass takes it).
     DeviceBuffer centre_buf_;
     CentreScratch centre_{};
+    uint64_t centre_ops_ = 0;  // algorithmic operations of one centre search launch
     std::vector<uint32_t> pending_, tail_;
     std::vector<bool> zeroed_;
     uint32_t passes_ = 0;

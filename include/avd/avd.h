@@ -207,8 +207,10 @@ avd_status avd_engine_synchronize(avd_engine *engine);
 avd_status avd_engine_fail_for_test(avd_engine *engine);
 
 /* Device time per stage (EngineTiming / DeviceTimer, engine/timing.rs:13-200): with sampling on,
- * every launch of a stage is bracketed by HIP events on the engine's stream.  `work` counts the
- * stage's algorithmic operations: AVD_STAGE_COLLAB_SEARCH 192 per candidate patch scored per
+ * every launch of a stage is bracketed by HIP events on the engine's stream.  NL4D's grouping
+ * search is two stages: AVD_STAGE_COLLAB_CENTRE (the centre search of every reference) and
+ * AVD_STAGE_COLLAB_SEARCH (the trajectory search and the grid decision).  `work` counts the
+ * stage's algorithmic operations: both search stages 192 per candidate patch scored per
  * channel (64 samples x subtract, multiply, add); AVD_STAGE_MOTION 3 per pixel of every scored
  * block offset (subtract, absolute value, add) on both pyramid levels; AVD_STAGE_COLLAB_FILTER per
  * reference 8 members x (C x (4096 + 576 + 256) + 64 x (C + 1)) (forward and inverse 2-D DCT as
@@ -227,7 +229,8 @@ enum {
     AVD_STAGE_NLM_SINGLE = 7,
     AVD_STAGE_NLM_FINISH = 8,
     AVD_STAGE_WARP = 9,
-    AVD_STAGE_COUNT = 10
+    AVD_STAGE_COLLAB_CENTRE = 10,
+    AVD_STAGE_COUNT = 11
 };
 
 typedef struct avd_timing {
