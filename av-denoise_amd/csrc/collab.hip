@@ -38,6 +38,7 @@ constexpr int kMaxS = 16;
 constexpr uint64_t kNoKey = ~0ull;
 
 typedef short short2_t __attribute__((ext_vector_type(2)));
+typedef int int4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -470,6 +471,73 @@ __device__ __forceinline__ void slide_fixed(const uint16_t *__restrict__ ring, s
         window += e;
         if (t >= 7) {
             if (t - 7 < kk) emit(t - 7, A2 + window - 2u * acc[(t - 7) & 7]);
+            acc[(t - 7) & 7] = 0;
+            window -= energy[(t - 7) & 7];
+        }
+    }
+}
+
+// slide_fixed for the trajectory search, where every lane starts a new column: candidates
+// (x, y0 + j), j < kk <= K, of a frame with H >= K + 7.  The K + 7 rows are read from y0s =
+// min(y0, H - K - 7), so none passes the frame (a rectangle clipped at the bottom slides from
+// higher up and skips its first y0 - y0s candidates).  Rows are read as raw buffer loads from the
+// ring (`ring_bytes` long): a scalar resource and one 32-bit byte offset per row; one-channel rows
+// start on the even code at or below the row's first and are shifted into place by the lane's
+// fixed byte shift of that row parity.
+template <int C, int K, typename Emit>
+__device__ __forceinline__ void slide_traj(const uint16_t *__restrict__ ring, uint32_t ring_bytes, size_t frame, int W,
+                                           int H, int x, int y0, int kk, const uint32_t (&P)[8][4 * C], uint32_t A2,
+                                           Emit &&emit)
+{
+    const int y0s = min(y0, H - K - 7);
+    const int delta = y0 - y0s;
+    const size_t a0 = frame + ((size_t)y0s * W + x) * C;  // first code of row 0
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(ring), (short)0, (int)ring_bytes, 0x00020000);
+    uint32_t acc[8], energy[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = 0;
+    uint32_t window = 0;
+    const uint32_t sh_e = C == 1 ? (uint32_t)(a0 & 1) : 0u, sh_o = C == 1 ? (uint32_t)((a0 + W) & 1) : 0u;
+    const uint32_t be = (uint32_t)((a0 - sh_e) * 2), bo = (uint32_t)((a0 + (size_t)W * C - sh_o) * 2);
+    const uint32_t pitch2 = (uint32_t)(4 * W * C);  // bytes per two rows
+#pragma unroll
+    for (int t = 0; t < K + 7; ++t) {
+        uint32_t row[4 * C];
+        const bool odd = (t & 1) != 0;
+        const uint32_t off = (odd ? bo : be) + (uint32_t)(t >> 1) * pitch2;
+        const int4_t v0 = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)off, 0, 0);
+        if (C == 1) {
+            // The row's eighth code alone (a dword holding it can pass the ring's end, and a buffer
+            // load drops a dword that is not wholly inside).
+            const uint32_t d4 = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc, (int)off + 16, 0, 0);
+            const uint32_t sh = (odd ? sh_o : sh_e) * 2u;
+            row[0] = __builtin_amdgcn_alignbyte((uint32_t)v0[1], (uint32_t)v0[0], sh);
+            row[1] = __builtin_amdgcn_alignbyte((uint32_t)v0[2], (uint32_t)v0[1], sh);
+            row[2] = __builtin_amdgcn_alignbyte((uint32_t)v0[3], (uint32_t)v0[2], sh);
+            row[3] = __builtin_amdgcn_alignbyte(d4, (uint32_t)v0[3], sh);
+        } else {
+            const int4_t v1 = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)off + 16, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                row[i] = (uint32_t)v0[i];
+                row[4 + i] = (uint32_t)v1[i];
+            }
+        }
+        uint32_t e2 = 0;
+#pragma unroll
+        for (int i = 0; i < 4 * C; ++i) {
+            e2 = __builtin_amdgcn_udot2(as_u2(row[i]), as_u2(row[i]), e2, false);
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (t - r >= 0 && t - r < K)
+                    acc[(t - r) & 7] = __builtin_amdgcn_udot2(as_u2(P[r][i]), as_u2(row[i]), acc[(t - r) & 7], false);
+        }
+        energy[t & 7] = e2;
+        window += e2;
+        if (t >= 7) {
+            const int j = t - 7 - delta;
+            if (j >= 0 && j < kk) emit(j, A2 + window - 2u * acc[(t - 7) & 7]);
             acc[(t - 7) & 7] = 0;
             window -= energy[(t - 7) & 7];
         }
@@ -1058,7 +1126,7 @@ __host__ __device__ __forceinline__ int trajectory_entries(const avd_collab_pass
     return (8 / gf) * ((2 * (int)p.temporal_radius) << (2 * cshift));
 }
 
-template <int C, int K, int NR>
+template <int C, int K, int NR, bool SHORT>
 __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                                                                   const uint32_t *__restrict__ nslots,
                                                                   const int32_t *__restrict__ mv,
@@ -1085,6 +1153,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     const bool live = ref0 + hr < refs;
     const size_t slot_len = (size_t)W * H * C;
     const size_t cframe = (size_t)p.centre_slot * slot_len;
+    const uint32_t ring_bytes = (uint32_t)(slot_len * p.slots * 2);
     const uint32_t mine = members[(size_t)my_ref * 9 + min(hl, 8)];
     const uint32_t k_use = (uint32_t)__shfl((int)mine, hbase + 8);
     uint32_t anchors[4];
@@ -1240,11 +1309,16 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                 for (int i = 0; i < 4 * C; ++i) P[rr][i] = s_patch[wave][tr][v][rr][i];
             uint64_t best = kNoKey;
             // Every lane runs K rows of candidates; a rectangle clipped shorter drops the rest.
-            slide_fixed<C, K>(ring, (size_t)s_slot[wave][t] * slot_len, W, H, cx, tp, h, P, s_energy[wave][tr][v],
-                              [&](int j, uint32_t ssd) {
-                                  const uint64_t key = ((uint64_t)(ssd + 1u) << 32) | (uint32_t)(walk + j * w + col);
-                                  if (!((skip >> j) & 1u) && key < best) best = key;
-                              });
+            auto keep = [&](int j, uint32_t ssd) {
+                const uint64_t key = ((uint64_t)(ssd + 1u) << 32) | (uint32_t)(walk + j * w + col);
+                if (!((skip >> j) & 1u) && key < best) best = key;
+            };
+            if (SHORT)  // rows past the frame are clamped
+                slide_fixed<C, K>(ring, (size_t)s_slot[wave][t] * slot_len, W, H, cx, tp, h, P, s_energy[wave][tr][v],
+                                  keep);
+            else
+                slide_traj<C, K>(ring, ring_bytes, (size_t)s_slot[wave][t] * slot_len, W, H, cx, tp, h, P,
+                                 s_energy[wave][tr][v], keep);
             if (best != kNoKey) atomicMin(&s_fbest[wave][tr][t], (unsigned long long)best);
         }
         wave_sync();
@@ -1935,15 +2009,24 @@ hipError_t launch_collab_trajectory(const avd_collab_pass &p, const uint16_t *co
     const size_t ents = (size_t)trajectory_entries(p) * 2 * sizeof(uint32_t) * kWaves;
     const int C = (int)p.channels;
     const bool four = trajectory_static_lds(C, 4) + 4 * ents <= kLdsBudget;
-    const int nr = four ? 4 : 2;
-    const size_t lds = nr * ents;
-    const unsigned gt = div_up(ref_count(p), kWaves * nr);
+    // Frames shorter than K + 7 rows, and rings past 2^31 bytes (beyond a buffer load's offset),
+    // take the clamping slide (at two references per wave).
+    const bool short_frame = (int)p.height < 2 * (int)p.refine + 8 ||
+                             (uint64_t)p.width * p.height * p.channels * p.slots * 2 >= (1ull << 31);
+    const bool four_rows = four && !short_frame;
+    const size_t lds_at = (four_rows ? 4 : 2) * ents;
+    const unsigned g_at = div_up(ref_count(p), kWaves * (four_rows ? 4 : 2));
 #define AVD_TRAJ(Cn, Kn)                                                                                       \
     do {                                                                                                       \
-        if (four)                                                                                              \
-            k_trajectory_codes<Cn, Kn, 4><<<gt, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count); \
+        if (short_frame)                                                                                       \
+            k_trajectory_codes<Cn, Kn, 2, true><<<g_at, blk, lds_at, s>>>(p, codes, nslots, mv, conf, members,  \
+                                                                          scored_count);                       \
+        else if (four_rows)                                                                                    \
+            k_trajectory_codes<Cn, Kn, 4, false><<<g_at, blk, lds_at, s>>>(p, codes, nslots, mv, conf, members, \
+                                                                           scored_count);                      \
         else                                                                                                   \
-            k_trajectory_codes<Cn, Kn, 2><<<gt, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count); \
+            k_trajectory_codes<Cn, Kn, 2, false><<<g_at, blk, lds_at, s>>>(p, codes, nslots, mv, conf, members, \
+                                                                           scored_count);                      \
     } while (0)
     switch (p.refine * 2 + 1 + (p.channels == 2 ? 100 : 0)) {
     case 3: AVD_TRAJ(1, 3); break;
