@@ -339,7 +339,7 @@ void Ring::duplicate_last(hipStream_t s)
 
 NlmPass::NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, avd_sample_format in,
                  avd_sample_format out, hipStream_t s)
-    : W_(W), H_(H), C_(C), in_(in), out_(out), stream_(s)
+    : Pass(s), W_(W), H_(H), C_(C), in_(in), out_(out)
 {
     // resolve_params (nlmeans/options.rs:106-137)
     r_ = o.temporal ? o.temporal_radius : 0;
@@ -571,7 +571,7 @@ bool NlmPass::has_pending() const { return ready_; }
 
 Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd_sample_format in,
                    avd_sample_format out, hipStream_t s)
-    : W_(W), H_(H), C_(C), in_(in), out_(out), stream_(s)
+    : Pass(s), W_(W), H_(H), C_(C), in_(in), out_(out)
 {
     r_ = o.temporal_radius;
     if (r_ == 0) throw Failure(AVD_ERR_INVALID_OPTIONS, "nl4d needs a temporal radius of at least 1");
@@ -873,14 +873,21 @@ Engine::Engine(const avd_engine_spec &spec) : spec_(spec)
     } else if (g.chroma == AVD_CHROMA_422) {
         chroma_w_ = (g.width + 1) / 2;
     }
+    hipStream_t chroma_stream = stream_;
+    if (spec.clean_luma && spec.clean_chroma && !gray) {
+        check_hip(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "create chroma stream");
+        check_hip(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming), "create event");
+        check_hip(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming), "create event");
+        chroma_stream = side_;
+    }
     if (spec.kind == AVD_ENGINE_NLMEANS) {
         if (spec.clean_luma) luma_.reset(new NlmPass(spec.nlm_luma, g.width, g.height, 1, g.input, g.output, stream_));
         if (spec.clean_chroma)
-            chroma_.reset(new NlmPass(spec.nlm_chroma, chroma_w_, chroma_h_, 2, g.input, g.output, stream_));
+            chroma_.reset(new NlmPass(spec.nlm_chroma, chroma_w_, chroma_h_, 2, g.input, g.output, chroma_stream));
     } else {
         if (spec.clean_luma) luma_.reset(new Nl4dPass(spec.nl4d_luma, g.width, g.height, 1, g.input, g.output, stream_));
         if (spec.clean_chroma)
-            chroma_.reset(new Nl4dPass(spec.nl4d_chroma, chroma_w_, chroma_h_, 2, g.input, g.output, stream_));
+            chroma_.reset(new Nl4dPass(spec.nl4d_chroma, chroma_w_, chroma_h_, 2, g.input, g.output, chroma_stream));
     }
     if (luma_) luma_->set_timer(&timer_);
     if (chroma_) chroma_->set_timer(&timer_);
@@ -894,6 +901,33 @@ Engine::Engine(const avd_engine_spec &spec) : spec_(spec)
         pt_words_ = plane_bytes(g.input, (uint64_t)w * h) / 4;
         passthrough_.allocate(pt_slots_ * pt_planes_ * pt_words_ * 4);
     }
+}
+
+Engine::~Engine()
+{
+    if (side_) {
+        // Work already enqueued on the chroma stream finishes before the passes free their buffers.
+        hipStreamSynchronize(side_);
+        luma_.reset();
+        chroma_.reset();
+        hipStreamDestroy(side_);
+    }
+    if (fork_ev_) hipEventDestroy(fork_ev_);
+    if (join_ev_) hipEventDestroy(join_ev_);
+}
+
+void Engine::fork()
+{
+    if (!side_) return;
+    check_hip(hipEventRecord(fork_ev_, stream_), "fork");
+    check_hip(hipStreamWaitEvent(side_, fork_ev_, 0), "fork");
+}
+
+void Engine::join()
+{
+    if (!side_) return;
+    check_hip(hipEventRecord(join_ev_, side_), "join");
+    check_hip(hipStreamWaitEvent(stream_, join_ev_, 0), "join");
 }
 
 Pass &Engine::lead() { return luma_ ? *luma_ : *chroma_; }
@@ -956,6 +990,7 @@ uint32_t Engine::push(const avd_device_plane *planes, uint32_t count)
         const uint32_t frame = next_frame_++;
         pushed_ = true;
         uint32_t ready = 0;
+        fork();
         if (luma_) ready = luma_->push(planes);
         if (chroma_) {
             const uint32_t rc = chroma_->push(planes + 1);
@@ -965,6 +1000,7 @@ uint32_t Engine::push(const avd_device_plane *planes, uint32_t count)
             }
             ready = rc;
         }
+        join();
         store_uncleaned(planes, frame);
         return ready;
     });
@@ -977,8 +1013,10 @@ void Engine::push_context(const avd_device_plane *planes, uint32_t count)
         if (lead().has_pending()) throw Failure(AVD_ERR_OUTPUTS_PENDING, "every ready frame must be emitted before the next push");
         if (pushed_) throw Failure(AVD_ERR_CONTEXT_AFTER_PUSH, "context frames must come before the first push of a stream");
         const uint32_t frame = next_frame_++;
+        fork();
         if (luma_) luma_->push_context(planes);
         if (chroma_) chroma_->push_context(planes + 1);
+        join();
         store_uncleaned(planes, frame);
         return 0;
     });
@@ -988,8 +1026,10 @@ void Engine::emit(const avd_device_plane *planes, uint32_t count)
 {
     guarded([&] {
         check_planes_for(planes, count, spec_.geometry.output);
+        fork();
         if (luma_) luma_->emit(planes);
         if (chroma_) chroma_->emit(planes + 1);
+        join();
         load_uncleaned(planes, lead().emitted_frame());
         if (tail_left_ > 0 && --tail_left_ == 0) restart();
         return 0;
@@ -1053,6 +1093,9 @@ void Engine::sample_device_time(uint32_t every)
 {
     guarded([&] {
         timer_.set_sampling(every != 0, stream_);
+        // While device time is sampled the chroma pass runs on the caller's stream, so each stage's
+        // launches are timed alone rather than beside the other plane group's.
+        if (side_ && chroma_) chroma_->set_stream(every != 0 ? stream_ : side_);
         return 0;
     });
 }

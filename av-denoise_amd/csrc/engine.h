@@ -124,8 +124,13 @@ class Pass {
     virtual bool has_pending() const = 0;
     uint32_t emitted_frame() const { return emitted_frame_; }
     void set_timer(StageTimer *t) { timer_ = t; }
+    // The stream the pass enqueues on (the engine moves the chroma pass between its own stream and
+    // the caller's; the engine's fork and join order the switch).
+    void set_stream(hipStream_t s) { stream_ = s; }
 
   protected:
+    explicit Pass(hipStream_t s) : stream_(s) {}
+    hipStream_t stream_;
     // Runs `launch` (returning hipError_t) as one timed launch of `stage`.
     template <typename F>
     void timed(int stage, uint64_t work, hipStream_t s, const char *what, F &&launch)
@@ -158,7 +163,6 @@ class NlmPass : public Pass {
     uint32_t W_, H_;
     int C_;
     avd_sample_format in_, out_;
-    hipStream_t stream_;
     uint32_t r_ = 0, S_ = 2, P_ = 4;
     bool hq_ = false, has_sigma_ = false, motion_ = false, confidence_ = false, conf_only_ = false, pushed_ = false;
     float sigma_ = 0.0f, self_weight_ = 1.0f, h2_ = 0.0f, offset_ = 0.0f, thsad_scale_ = 1.0f;
@@ -192,7 +196,6 @@ class Nl4dPass : public Pass {
     uint32_t W_, H_;
     int C_;
     avd_sample_format in_, out_;
-    hipStream_t stream_;
     uint32_t r_ = 1, N_ = 3;
     float sigma_ = 0.0f, thsad_scale_ = 1.0f;
     MotionGeom mg_{};
@@ -218,6 +221,7 @@ class Nl4dPass : public Pass {
 class Engine {
   public:
     explicit Engine(const avd_engine_spec &spec);
+    ~Engine();
     uint32_t push(const avd_device_plane *planes, uint32_t count);
     void push_context(const avd_device_plane *planes, uint32_t count);
     void emit(const avd_device_plane *planes, uint32_t count);
@@ -240,9 +244,15 @@ class Engine {
     void restart();
     template <typename F>
     auto guarded(F &&f) -> decltype(f());
+    // With both plane groups cleaned, the chroma pass runs on its own stream (side_): each call
+    // forks it from the caller's stream at its start and joins it back at its end, so the caller
+    // sees the engine's work in order on one stream while the two passes overlap.
+    void fork();
+    void join();
 
     avd_engine_spec spec_;
-    hipStream_t stream_ = nullptr;
+    hipStream_t stream_ = nullptr, side_ = nullptr;
+    hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
     StageTimer timer_;
     std::unique_ptr<Pass> luma_, chroma_;
     uint32_t planes_ = 1, chroma_w_ = 0, chroma_h_ = 0;

@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sample-steps", type=int, default=4, help="extra steps timed per stage with HIP events")
+    ap.add_argument("--sample-all", action="store_true",
+                    help="sample device time from the first step (every launch timed alone, the plane groups "
+                         "one after the other; for kernel traces whose durations match the stage figures)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, one plain device copy of this many MiB (a known-size kernel that "
                          "hardware-counter runs calibrate FETCH_SIZE / WRITE_SIZE against)")
@@ -144,6 +147,8 @@ def main():
             for (w, h) in [(W, H), ((W + 1) // 2, (H + 1) // 2), ((W + 1) // 2, (H + 1) // 2)]]
 
     warm = max(args.warmup, 2 * RADIUS + 1)
+    if args.sample_all:
+        eng.sample_device_time(1)
     run_steps(eng, frames, outs, 0, warm)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -157,7 +162,8 @@ def main():
     # engine counts each stage's algorithmic operations (both search stages: 192 per candidate
     # patch and channel; motion: 3 per pixel of every scored block offset; filter: 8 members x
     # (C x 4928 + 64 x (C + 1)) per reference).
-    eng.sample_device_time(1)
+    if not args.sample_all:
+        eng.sample_device_time(1)
     run_steps(eng, frames, outs, warm + args.steps, args.sample_steps)
     stages = eng.timing()
     eng.sample_device_time(0)
