@@ -145,10 +145,14 @@ __device__ __forceinline__ void match_block(const Level &cen, const Level &nb, s
 
 constexpr int kRowsLdsBudget = 48 * 1024;
 
+// Window rows of the rows form are u16 samples (codes of at most 12 bits, or level-1 sums of four
+// of them), each row padded to a multiple of 8 samples so rows start on 16-byte boundaries.
+__host__ __device__ constexpr int rows_stride(int ww) { return (ww + 7) & ~7; }
+
 __host__ __device__ inline int match_rows_batch(int blk, int R, int neighbours)
 {
     const int ww = blk + 2 * R;
-    int batch = kRowsLdsBudget / (ww * ww * 4);
+    int batch = kRowsLdsBudget / (ww * rows_stride(ww) * 2);
     batch = batch < 1 ? 1 : batch;
     batch = batch < neighbours ? batch : neighbours;
     return batch < 16 ? batch : 16;
@@ -157,7 +161,8 @@ __host__ __device__ inline int match_rows_batch(int blk, int R, int neighbours)
 size_t match_rows_lds_bytes(int blk, int R, int neighbours)
 {
     const int ww = blk + 2 * R;
-    return (size_t)(blk * blk + 16 + 16 + 32 + 16) * 4 + (size_t)match_rows_batch(blk, R, neighbours) * ww * ww * 4;
+    return (size_t)(blk * blk / 2 + 16 + 16 + 32 + 16) * 4 +
+           (size_t)match_rows_batch(blk, R, neighbours) * ww * rows_stride(ww) * 2;
 }
 
 size_t match_lds_bytes(int blk, int R, int neighbours)
@@ -197,6 +202,34 @@ __device__ __forceinline__ void stage_rect(const Level &l, int x0, int y0, int r
     }
 }
 
+// The COLS x ROWS rectangle at (x0, y0) of a level into u16 rows `stride` samples apart, the
+// wave's lanes sweeping consecutive samples (so a load instruction covers a few cache lines, not
+// one per lane); a rectangle inside the level reads without clamps.
+template <int COLS, int ROWS>
+__device__ __forceinline__ void stage_tile16(const Level &l, int x0, int y0, uint16_t *dst, int stride, int lane)
+{
+    const bool inside = x0 >= 0 && y0 >= 0 && x0 + COLS <= l.w && y0 + ROWS <= l.h;
+    for (int i = lane; i < COLS * ROWS; i += 64) {
+        const int r = i / COLS, c = i - r * COLS;
+        uint32_t v;
+        if (inside)
+            v = l.codes ? (uint32_t)l.codes[((size_t)(y0 + r) * l.w + x0 + c) * l.C]
+                        : (uint32_t)l.plane[(size_t)(y0 + r) * l.w + x0 + c];
+        else
+            v = l.at(x0 + c, y0 + r);
+        dst[r * stride + c] = (uint16_t)v;
+    }
+}
+
+// |a.lo16 - b.lo16| + |a.hi16 - b.hi16| + acc: two packed samples per instruction (gfx950's
+// v_sad_u16 sums both halves into the 32-bit accumulator).
+__device__ __forceinline__ uint32_t sad2_u16(uint32_t a, uint32_t b, uint32_t acc)
+{
+    uint32_t r;
+    asm("v_sad_u16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(acc));
+    return r;
+}
+
 // The same search specialised for a compile-time block size and radius: one wave per block, lane
 // (neighbour, dy) scores the whole row of offsets (dx = 0..2R) for its dy.  Per block row it holds
 // the block row and the window row (BLK + 2R samples) in registers, so each window row is read from
@@ -206,16 +239,17 @@ __device__ __forceinline__ void match_rows(const Level &cen, const Level &nb, si
                                            const uint32_t *__restrict__ nslots, int neighbours, int ox, int oy,
                                            Seed &&seed, Finish &&finish, uint32_t *lds)
 {
-    constexpr int SIDE = 2 * R + 1, WW = BLK + 2 * R, WIN = WW * WW;
+    constexpr int SIDE = 2 * R + 1, WW = BLK + 2 * R, WS = rows_stride(WW), WIN = WW * WS;
     const int lane = threadIdx.x;
     const int batch = match_rows_batch(BLK, R, neighbours);
-    uint32_t *s_blk = lds;                        // BLK * BLK
-    uint32_t *s_key = s_blk + BLK * BLK;          // 16
-    uint32_t *s_csad = s_key + 16;                // 16
-    int *s_seed = (int *)(s_csad + 16);           // 32
-    uint32_t *s_slot = (uint32_t *)(s_seed + 32); // 16
-    uint32_t *s_win = s_slot + 16;                // batch * WIN, rows 16-byte aligned
-    stage_rect<BLK>(cen, ox, oy, BLK, s_blk, BLK, lane, 64);
+    uint16_t *s_blk = reinterpret_cast<uint16_t *>(lds);  // BLK * BLK u16
+    uint32_t *s_key = lds + BLK * BLK / 2;                   // 16
+    uint32_t *s_csad = s_key + 16;                           // 16
+    int *s_seed = (int *)(s_csad + 16);                      // 32
+    uint32_t *s_slot = (uint32_t *)(s_seed + 32);            // 16
+    // batch windows of WW rows of WS u16 samples, rows 16-byte aligned
+    uint16_t *s_win = reinterpret_cast<uint16_t *>(s_slot + 16);
+    stage_tile16<BLK, BLK>(cen, ox, oy, s_blk, BLK, lane);
     for (int n0 = 0; n0 < neighbours; n0 += batch) {
         const int nb_n = min(batch, neighbours - n0);
         if (lane < nb_n) {
@@ -227,17 +261,12 @@ __device__ __forceinline__ void match_rows(const Level &cen, const Level &nb, si
             s_key[lane] = ~0u;
         }
         __syncthreads();
-        // Lanes split the batch's neighbours into groups of 64 / nb_n lanes, each staging one window.
-        {
-            const int per = 64 / nb_n;  // nb_n <= 16
-            const int nl = lane / per, sub = lane - nl * per;
-            if (nl < nb_n) {
-                Level l = nb;
-                if (l.codes) l.codes += (size_t)s_slot[nl] * level_len;
-                else l.plane += (size_t)s_slot[nl] * level_len;
-                stage_rect<WW>(l, ox + s_seed[2 * nl] - R, oy + s_seed[2 * nl + 1] - R, WW, s_win + nl * WIN, WW,
-                               sub, per);
-            }
+        // The whole wave stages each neighbour's window in turn.
+        for (int nl = 0; nl < nb_n; ++nl) {
+            Level l = nb;
+            if (l.codes) l.codes += (size_t)s_slot[nl] * level_len;
+            else l.plane += (size_t)s_slot[nl] * level_len;
+            stage_tile16<WW, WW>(l, ox + s_seed[2 * nl] - R, oy + s_seed[2 * nl + 1] - R, s_win + nl * WIN, WS, lane);
         }
         __syncthreads();
         for (int task = lane; task < nb_n * SIDE; task += 64) {
@@ -245,22 +274,27 @@ __device__ __forceinline__ void match_rows(const Level &cen, const Level &nb, si
             uint32_t sad[SIDE];
 #pragma unroll
             for (int j = 0; j < SIDE; ++j) sad[j] = 0;
-            const uint32_t *wbase = s_win + nl * WIN + dy * WW;
+            const uint16_t *wbase = s_win + nl * WIN + dy * WS;
 #pragma unroll 2
             for (int py = 0; py < BLK; ++py) {
-                // Rows start on 16-byte boundaries: BLK and WW are multiples of 4 words, and the
-                // window area starts (BLK^2 + 80) words into the 16-byte aligned LDS block.
+                // Rows start on 16-byte boundaries: BLK is a multiple of 8 samples, WS of 8, and the
+                // window area starts (BLK^2 / 2 + 80) words into the 16-byte aligned LDS block.
                 const uint32_t *brow = (const uint32_t *)__builtin_assume_aligned(s_blk + py * BLK, 16);
-                const uint32_t *wrow = (const uint32_t *)__builtin_assume_aligned(wbase + py * WW, 16);
-                uint32_t b[BLK], w[WW];
+                const uint32_t *wrow = (const uint32_t *)__builtin_assume_aligned(wbase + py * WS, 16);
+                // Packed pairs: the block's (2k, 2k + 1), the window's from an even sample (wp) and
+                // from an odd one (wo[k] = samples 2k + 1, 2k + 2).
+                uint32_t bp[BLK / 2], wp[(WW + 1) / 2], wo[WW / 2];
 #pragma unroll
-                for (int x = 0; x < BLK; ++x) b[x] = brow[x];
+                for (int k = 0; k < BLK / 2; ++k) bp[k] = brow[k];
 #pragma unroll
-                for (int x = 0; x < WW; ++x) w[x] = wrow[x];
+                for (int k = 0; k < (WW + 1) / 2; ++k) wp[k] = wrow[k];
+#pragma unroll
+                for (int k = 0; k + 1 < (WW + 1) / 2; ++k) wo[k] = __builtin_amdgcn_alignbyte(wp[k + 1], wp[k], 2);
 #pragma unroll
                 for (int j = 0; j < SIDE; ++j)
 #pragma unroll
-                    for (int x = 0; x < BLK; ++x) sad[j] = sad_u32(b[x], w[x + j], sad[j]);
+                    for (int k = 0; k < BLK / 2; ++k)
+                        sad[j] = sad2_u16(bp[k], (j & 1) ? wo[(j >> 1) + k] : wp[(j >> 1) + k], sad[j]);
             }
             uint32_t best = ~0u;
 #pragma unroll
