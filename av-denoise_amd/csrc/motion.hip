@@ -221,6 +221,18 @@ __device__ __forceinline__ void stage_tile16(const Level &l, int x0, int y0, uin
     }
 }
 
+// This workgroup's block (column, row) of the 2-D grid, renumbered so that each XCD (workgroups are
+// dealt round-robin over eight) takes a contiguous run of rows: neighbouring blocks' overlapping
+// windows then meet in one L2.
+__device__ __forceinline__ void xcd_block_2d(int &bc, int &br)
+{
+    const int n = (int)(gridDim.x * gridDim.y), b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+    const int xcd = b & 7, i = b >> 3, q = n >> 3, rem = n & 7;
+    const int v = xcd * q + min(xcd, rem) + i;
+    br = v / (int)gridDim.x;
+    bc = v - br * (int)gridDim.x;
+}
+
 // |a.lo16 - b.lo16| + |a.hi16 - b.hi16| + acc: two packed samples per instruction (gfx950's
 // v_sad_u16 sums both halves into the 32-bit accumulator).
 __device__ __forceinline__ uint32_t sad2_u16(uint32_t a, uint32_t b, uint32_t acc)
@@ -324,7 +336,8 @@ __global__ __launch_bounds__(kBmThreads) void k_bm_coarse(const int32_t *__restr
                                                           int fine_step)
 {
     extern __shared__ uint32_t lds[];
-    const int bc = blockIdx.x, br = blockIdx.y;
+    int bc, br;
+    xcd_block_2d(bc, br);
     const size_t lp = (size_t)lw * lh;
     const Level cen{nullptr, pyr1 + (size_t)centre_slot * lp, lw, lh, 1};
     const Level nb{nullptr, pyr1, lw, lh, 1};
@@ -357,7 +370,8 @@ __global__ __launch_bounds__(kBmThreads) void k_bm_fine(const uint16_t *__restri
                                                         float max_value, float sad_floor, float thsad)
 {
     extern __shared__ uint32_t lds[];
-    const int bc = blockIdx.x, br = blockIdx.y;
+    int bc, br;
+    xcd_block_2d(bc, br);
     const int bx = gridDim.x, by = gridDim.y;
     const size_t sp = (size_t)W * H * C;
     const Level cen{codes + (size_t)centre_slot * sp, nullptr, W, H, C};
@@ -393,7 +407,8 @@ __global__ __launch_bounds__(64) void k_bm_coarse_rows(const int32_t *__restrict
                                                        int fine_by, int fine_step)
 {
     extern __shared__ __align__(16) uint32_t lds_rows[];
-    const int bc = blockIdx.x, br = blockIdx.y;
+    int bc, br;
+    xcd_block_2d(bc, br);
     const size_t lp = (size_t)lw * lh;
     const Level cen{nullptr, pyr1 + (size_t)centre_slot * lp, lw, lh, 1};
     const Level nb{nullptr, pyr1, lw, lh, 1};
@@ -425,7 +440,8 @@ __global__ __launch_bounds__(64) void k_bm_fine_rows(const uint16_t *__restrict_
                                                      float max_value, float sad_floor, float thsad)
 {
     extern __shared__ __align__(16) uint32_t lds_rows[];
-    const int bc = blockIdx.x, br = blockIdx.y;
+    int bc, br;
+    xcd_block_2d(bc, br);
     const int bx = gridDim.x, by = gridDim.y;
     const size_t sp = (size_t)W * H * C;
     const Level cen{codes + (size_t)centre_slot * sp, nullptr, W, H, C};
