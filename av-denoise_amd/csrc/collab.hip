@@ -1542,6 +1542,17 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
                                                                float *__restrict__ group_weight, TileBins tiles)
 {
     __shared__ float s_t[kWaves][8][8][9];  // [wave][member][row][col], padded
+    // Per channel and coefficient (v, u): the variance sigma^2 prof[u] prof[v] and the hard
+    // threshold lambda sqrt(variance), the same f32 expressions for every reference, once per
+    // workgroup.
+    __shared__ float s_cvs[C][64], s_thr[C][64];
+    if (threadIdx.x < 64 * C) {
+        const int c = threadIdx.x >> 6, v = (threadIdx.x >> 3) & 7, u = threadIdx.x & 7;
+        const float cvs = (p.sigma[c] * p.sigma[c]) * (p.profile[u] * p.profile[v]);
+        s_cvs[c][v * 8 + u] = cvs;
+        s_thr[c][v * 8 + u] = p.lambda[c] * sqrtf(cvs);
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int m = lane >> 3, q = lane & 7;
     const int W = p.width, H = p.height;
@@ -1635,9 +1646,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
             float cvs[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                cvs[u] = sig2 * (p.profile[u] * pv);
-                const float thr = lam * sqrtf(cvs[u]);
-                bool keep = fabsf(row[u]) >= thr;
+                cvs[u] = s_cvs[c][v * 8 + u];
+                bool keep = fabsf(row[u]) >= s_thr[c][v * 8 + u];
                 if (u == 0 && m == 0 && v == 0) keep = true;
                 keep = keep && in_group;
                 if (keep) counts += 1u << (4 * u);
