@@ -231,6 +231,27 @@ def test_codes_search_edge_cases_are_bit_exact(case):
     assert_members_equal(got, want, name)
 
 
+@pytest.mark.parametrize("C,step,S", [(1, 3, 12), (1, 8, 9), (2, 2, 6), (1, 1, 13)])
+def test_codes_search_other_grid_steps_are_bit_exact(C, step, S):
+    """Reference grids at other strides through the integer search (the matrix-core centre search
+    takes every stride whose two candidate rows of references fit 32 rows)."""
+    W, H, depth, radius = 72, 52, 10, 2
+    print(f"search step {step} C {C} S {S}", flush=True)
+    frames = 2 * radius + 1
+    clip = avd.synth.clip(frames, W, H, depth, 17 + step, chroma="444" if C == 2 else "gray", sigma_luma=12.0)
+    planes = [[f[0].numpy()] if C == 1 else [f[1].numpy(), f[2].numpy()] for f in clip]
+    codes = _codes_ring(planes, C)
+    geom = O.MotionGeom(W, H, 16, 8, 4, 2)
+    slots = [s for s in range(frames) if s != radius]
+    sigma = 12.0 / 1023
+    mv, conf = _oracle_fields(codes, W, H, C, radius, slots, sigma, 1023.0, geom)
+    p = collab_pass(W, H, C, radius, S, 2, 0.05, 2.7, sigma, 2.0, radius, frames, geom, code_bits=depth, step=step)
+    op = O.collab_params(W, H, C, radius, S, 2, 0.05, 2.7, sigma, 2.0, geom, step=step)
+    want = O.collab_search(codes, op, radius, slots, mv, conf)
+    got = site_collab_search(p, codes=dev(codes.view(np.int16)), nslots=slots, mv=mv, conf=conf)
+    assert_members_equal(got, want, f"step {step}")
+
+
 @pytest.mark.parametrize("C,radius,S,pooled", [(1, 2, 9, False), (2, 1, 6, False), (1, 3, 12, False), (1, 0, 5, False),
                                                 (1, 2, 9, True), (2, 2, 6, True), (1, 0, 5, True)])
 def test_collab_codes_search_and_filter_are_bit_exact(C, radius, S, pooled):
