@@ -65,7 +65,7 @@ class Nl4dOptions(ctypes.Structure):
                 ("thsad_scale", f32), ("refine", u32), ("spatial_radius", u32), ("has_lambda_ht", u32),
                 ("lambda_ht", f32), ("lambda_ht_scale", f32), ("c_min", f32), ("kaiser_beta", f32),
                 ("field_lambda", f32), ("noise_map", u32), ("pooled_threshold", u32), ("grain_data", u32),
-                ("temporal_radius", u32), ("step", u32), ("group_size", u32)]
+                ("temporal_radius", u32), ("step", u32), ("group_size", u32), ("transform_form", u32)]
 
 
 class EngineSpec(ctypes.Structure):

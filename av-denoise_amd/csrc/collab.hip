@@ -1778,6 +1778,243 @@ __device__ __forceinline__ void member_row(const avd_collab_pass &p, const uint3
 // of the patch, adding its staged sum and its weight (recomputed from the group weight in the
 // filter's own arithmetic) to the tile in LDS.  The tile then adds to the accumulators once, as
 // the only writer of those pixels this pass (overflowing pairs are added by k_tile_overflow).
+// ---------------------------------------------------------------------------------------------
+// The mixed-precision filter form: the 2-D DCT and its inverse on the matrix cores.
+//
+// A member's 8x8 patch is a 64-vector x (pixel k = 8r + c); its coefficients are (D (x) D) x
+// (coefficient k' = 8v + u), the Kronecker square of the 8-point DCT.  A wave filters four
+// references: B's 32 columns are their 32 members, A is D (x) D in f16 (from LDS), and
+// v_mfma_f32_32x32x8_f16 accumulates in f32 (2 row tiles x 8 K steps).  Lane (n, h) receives the
+// coefficients (v, u = 4h..4h+3) of member n, so the group Haar pairs lanes n ^ 1, ^ 2, ^ 4 of one
+// DPP row; the same layout is the B operand of the inverse, (D (x) D)^T.  The samples and the
+// transform matrix round to f16 (about 2^-11 relative), so the output differs from the FP32 form
+// by well under 0.05 dB; the hard threshold and the group weight's variance sum follow the FP32
+// form's rules in another summation order.  Hard threshold only.
+// ---------------------------------------------------------------------------------------------
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float dct_coef(int v, int r)
+{
+    const int m = v >> 1, k = r < 4 ? r : 7 - r;
+    if ((v & 1) == 0) return c_taps.e[m * 4 + k];
+    return r < 4 ? c_taps.o[m * 4 + k] : -c_taps.o[m * 4 + k];
+}
+
+// Value of lane (member) `src_lane` of this lane's DPP row, for partners XOR 1, 2 or 4.
+template <int XOR>
+__device__ __forceinline__ float member_partner(float v, int n)
+{
+    const int b = __builtin_bit_cast(int, v);
+    int o;
+    if (XOR == 1) o = __builtin_amdgcn_update_dpp(0, b, 0xB1, 0xF, 0xF, false);       // quad_perm [1,0,3,2]
+    else if (XOR == 2) o = __builtin_amdgcn_update_dpp(0, b, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    else {
+        // row_ror:k hands lane i the value of lane i - k of its row of 16.
+        const int dn = __builtin_amdgcn_update_dpp(0, b, 0x124, 0xF, 0xF, false);    // row_ror 4: lane - 4
+        const int up = __builtin_amdgcn_update_dpp(0, b, 0x12C, 0xF, 0xF, false);    // row_ror 12: lane + 4
+        o = (n & 4) ? dn : up;
+    }
+    return __builtin_bit_cast(float, o);
+}
+
+template <int XOR>
+__device__ __forceinline__ void haar_level_dpp(float (&y)[32], bool active, bool low, int n)
+{
+    const float f = 0.70710678118654752440f;
+    const float sgn = low ? 1.0f : -1.0f;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+        const float other = member_partner<XOR>(y[i], n);
+        const float t = __builtin_fmaf(sgn, y[i], other);  // low: v + other; high: other - v
+        if (active) y[i] = t * f;
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_pass p, const float *__restrict__ values,
+                                                                     const uint32_t *__restrict__ nslots,
+                                                                     const uint32_t *__restrict__ members,
+                                                                     float *__restrict__ group_weight, TileBins tiles)
+{
+    __shared__ _Float16 s_afwd[64][64], s_ainv[64][64];  // [row][k] of D (x) D and of its transpose
+    __shared__ float s_cvs[C][64], s_thr[C][64];
+    for (int i = threadIdx.x; i < 64 * 64; i += kWaves * 64) {
+        const int a = i >> 6, b = i & 63;  // a = 8v + u, b = 8r + c
+        const float d = dct_coef(a >> 3, b >> 3) * dct_coef(a & 7, b & 7);
+        s_afwd[a][b] = (_Float16)d;
+        s_ainv[b][a] = (_Float16)d;
+    }
+    if (threadIdx.x < 64 * C) {
+        const int c = threadIdx.x >> 6, v = (threadIdx.x >> 3) & 7, u = threadIdx.x & 7;
+        const float cvs = (p.sigma[c] * p.sigma[c]) * (p.profile[u] * p.profile[v]);
+        s_cvs[c][v * 8 + u] = cvs;
+        s_thr[c][v * 8 + u] = p.lambda[c] * sqrtf(cvs);
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int n = lane & 31, h = lane >> 5, m = n & 7;
+    const int W = p.width, H = p.height;
+    const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
+    const int refs = refs_x * refs_y;
+    int ref = (xcd_block() * kWaves + wave) * 4 + (n >> 3);
+    const bool live = ref < refs;
+    if (!live) ref = refs - 1;
+    const uint32_t *mw = members + (size_t)ref * 9;
+    const uint32_t gword = mw[8];
+    const int k_use = (int)(gword & 0xFF);
+    const bool grid = (gword & 0x100u) != 0;
+    const int gf = p.temporal_radius == 0 ? 1 : (p.temporal_radius == 1 ? 2 : 4);
+    const HaarPlan plan = haar_plan(m, k_use, grid, gf);
+    const uint32_t pos = mw[m];
+    const uint32_t t = pos >> 26;
+    const int slot = t > 0 ? (int)nslots[t - 1] : (int)p.centre_slot;
+    const int mx = (int)(pos & 0x1FFF), my = (int)((pos >> 13) & 0x1FFF);
+    const size_t px = (size_t)W * H;
+    const float *src = values + (size_t)slot * px * C;
+    const bool in_group = m < k_use;
+    const float scale = p.accum_scale;
+    float scaled_weight = 0.0f;
+
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        // ---- forward: coefficient rows 32 mt + (lane's rows) of member n.
+        float16_t acc[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[mt][i] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            half4_t b;  // pixels (r = s, c = 4h .. 4h + 3) of member n
+#pragma unroll
+            for (int i = 0; i < 4; ++i) b[i] = (_Float16)src[((size_t)(my + s) * W + mx + 4 * h + i) * C + c];
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                half4_t a;  // row 32 mt + n of D (x) D, columns 8 s + 4 h .. + 3
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = s_afwd[32 * mt + n][8 * s + 4 * h + i];
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc[mt], 0, 0, 0);
+            }
+        }
+        // Lane (n, h), register 4 j + i of tile mt: coefficient (v = 4 mt + j, u = 4 h + i).
+        float y[32];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) y[mt * 16 + i] = acc[mt][i];
+        if (plan.levels > 0) haar_level_dpp<1>(y, plan.act[0], plan.low[0], n);
+        if (plan.levels > 1) haar_level_dpp<2>(y, plan.act[1], plan.low[1], n);
+        if (plan.levels > 2) haar_level_dpp<4>(y, plan.act[2], plan.low[2], n);
+        // ---- hard threshold (the group DC always kept) and the retained variance.
+        float part = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            const int v = (i >> 4) * 4 + ((i >> 2) & 3), u = 4 * h + (i & 3);
+            bool keep = fabsf(y[i]) >= s_thr[c][v * 8 + u];
+            if (m == 0 && v == 0 && u == 0) keep = true;
+            keep = keep && in_group;
+            if (keep) part += s_cvs[c][v * 8 + u];
+            else if (in_group) y[i] = 0.0f;
+        }
+        // Sum over the reference's 8 member lanes and both halves.
+        part += member_partner<1>(part, n);
+        part += member_partner<2>(part, n);
+        part += member_partner<4>(part, n);
+        part += __shfl_xor(part, 32);
+        const float weight = (isnan(part) || isinf(part)) ? 0.0f : 1.0f / fmaxf(part, 1e-12f);
+        if (c == 0) {
+            if (live && m == 0 && h == 0) group_weight[ref] = weight;
+            scaled_weight = weight * p.weight_scale;
+        }
+        if (plan.levels > 2) haar_level_dpp<4>(y, plan.act[2], plan.low[2], n);
+        if (plan.levels > 1) haar_level_dpp<2>(y, plan.act[1], plan.low[1], n);
+        if (plan.levels > 0) haar_level_dpp<1>(y, plan.act[0], plan.low[0], n);
+        // ---- inverse: the coefficients in this layout are B's K steps.
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[mt][i] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            half4_t b;  // coefficients 8 s + 4 h + i = registers 4 (s & 3) + i of tile s >> 2
+#pragma unroll
+            for (int i = 0; i < 4; ++i) b[i] = (_Float16)y[(s >> 2) * 16 + (s & 3) * 4 + i];
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                half4_t a;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = s_ainv[32 * mt + n][8 * s + 4 * h + i];
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc[mt], 0, 0, 0);
+            }
+        }
+        // Register 4 j + i of tile mt: pixel (r = 4 mt + j, c = 4 h + i).
+        if (live) {
+            int32_t *st = tiles.stage + (((size_t)ref * 8 + m) * C + c) * 64;
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int r = 4 * mt + j;
+                    int4_t o;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float w = scaled_weight * (p.kaiser[r] * p.kaiser[4 * h + i]);
+                        o[i] = in_group ? to_fixed(acc[mt][4 * j + i] * w, scale) : 0;
+                    }
+                    *reinterpret_cast<int4_t *>(st + r * 8 + 4 * h) = o;
+                }
+        }
+    }
+
+    // Enter each member in the list of every tile its patch overlaps (claims 2h, 2h + 1 of member
+    // n), counted per tile in an LDS hash table as in k_collab_filter.
+    constexpr int kBuckets = kWaves * 256;
+    __shared__ uint32_t s_key[kBuckets], s_cnt[kBuckets], s_base[kBuckets];
+    for (int i = threadIdx.x; i < kBuckets; i += kWaves * 64) {
+        s_key[i] = ~0u;
+        s_cnt[i] = 0;
+    }
+    __syncthreads();
+    const int tx0 = mx >> kTileShift, ty0 = my >> kTileShift;
+    const int nx = ((mx + 7) >> kTileShift) - tx0 + 1, ny = ((my + 7) >> kTileShift) - ty0 + 1;
+    uint32_t tile[2] = {~0u, ~0u}, bucket[2] = {0, 0}, rank[2] = {0, 0};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int q = 2 * h + k;
+        if (live && in_group && q < nx * ny) {
+            tile[k] = (uint32_t)slot * tiles.per_slot + (ty0 + q / nx) * tiles.tiles_x + tx0 + q % nx;
+            uint32_t bk = (tile[k] * 2654435761u) >> 22;
+            for (;;) {  // at most 512 keys in 1024 buckets: a free or equal bucket exists
+                const uint32_t old = atomicCAS(&s_key[bk], ~0u, tile[k]);
+                if (old == ~0u || old == tile[k]) break;
+                bk = (bk + 1) & (kBuckets - 1);
+            }
+            bucket[k] = bk;
+            rank[k] = atomicAdd(&s_cnt[bk], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kBuckets; i += kWaves * 64)
+        if (s_key[i] != ~0u) s_base[i] = atomicAdd(&tiles.count[s_key[i]], s_cnt[i]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (tile[k] == ~0u) continue;
+        const uint32_t e = s_base[bucket[k]] + rank[k];
+        const uint32_t id = ((uint32_t)ref << 3) | (uint32_t)m;
+        if (e < kTileCap) {
+            tiles.list[(size_t)tile[k] * kTileCap + e] = id;
+        } else {
+            const uint32_t o = atomicAdd(tiles.overflow_count, 1u);
+            tiles.overflow[2 * (size_t)o] = id;
+            tiles.overflow[2 * (size_t)o + 1] = tile[k];
+        }
+    }
+}
+
 template <int C>
 __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const uint32_t *__restrict__ members,
                                                      const float *__restrict__ group_weight, TileBins tiles,
@@ -2090,8 +2327,9 @@ hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, c
 hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
                                        const uint32_t *members, const TileBins &tiles,
                                        unsigned long long *aggregate, int32_t *accum_second, float *group_weight,
-                                       hipStream_t s)
+                                       hipStream_t s, bool mixed)
 {
+    if (mixed && p.pool_ratio > 0.0f) return hipErrorInvalidValue;  // the mixed form thresholds hard
     if (!pass_ok(p) || (p.channels == 2 && !accum_second) || !tiles.stage || !tiles.count || !tiles.list ||
         !tiles.overflow || !tiles.overflow_count ||
         tiles.tiles_x != (p.width + (1u << kTileShift) - 1) >> kTileShift ||
@@ -2105,14 +2343,21 @@ hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *va
     if (e != hipSuccess) return e;
     const uint32_t refs = (uint32_t)ref_count(p);
     const unsigned g = div_up(refs, kWaves);
+    const unsigned gm = div_up(refs, kWaves * 4);
     if (p.channels == 1) {
-        k_collab_filter<1, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
-                                                            group_weight, tiles);
+        if (mixed)
+            k_collab_filter_mixed<1><<<gm, kWaves * 64, 0, s>>>(p, values, nslots, members, group_weight, tiles);
+        else
+            k_collab_filter<1, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
+                                                                group_weight, tiles);
         k_tile_gather<1><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
         k_tile_overflow<1><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
     } else {
-        k_collab_filter<2, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
-                                                            group_weight, tiles);
+        if (mixed)
+            k_collab_filter_mixed<2><<<gm, kWaves * 64, 0, s>>>(p, values, nslots, members, group_weight, tiles);
+        else
+            k_collab_filter<2, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
+                                                                group_weight, tiles);
         k_tile_gather<2><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
         k_tile_overflow<2><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
     }

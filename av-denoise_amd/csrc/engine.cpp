@@ -600,6 +600,10 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     if (!o.has_sigma)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "noise estimation is not built yet; pass a fixed sigma");
     if (o.grain_data) throw Failure(AVD_ERR_INVALID_OPTIONS, "grain measurement is not built yet");
+    if (o.transform_form > 1) throw Failure(AVD_ERR_INVALID_OPTIONS, "transform_form must be 0 (FP32) or 1 (mixed)");
+    if (o.transform_form == 1 && o.pooled_threshold)
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "the mixed transform form thresholds hard: set pooled_threshold = 0");
+    mixed_ = o.transform_form == 1;
     if (o.step != 4 || o.group_size != 8)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "this build runs the reference grid: step 4, group size 8");
     if (W < 8 || H < 8) throw Failure(AVD_ERR_INVALID_GEOMETRY, "frames must be at least 8x8 for the patch grid");
@@ -736,7 +740,7 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
     timed(AVD_STAGE_COLLAB_FILTER, filter_ops, stream_, "collab filter", [&] {
         return launch_collab_filter_binned(p, ring_.value_ptr(), nslots, members_.as<uint32_t>(), bins_,
                                            aggregate_.as<unsigned long long>(), accum_second_.as<int32_t>(),
-                                           group_weight_.as<float>(), stream_);
+                                           group_weight_.as<float>(), stream_, mixed_);
     });
     ++passes_;
 }

@@ -212,6 +212,7 @@ class Nl4dPass : public Pass {
     DeviceBuffer centre_buf_;
     CentreScratch centre_{};
     uint64_t centre_ops_ = 0;  // algorithmic operations of one centre search launch
+    bool mixed_ = false;       // the mixed-precision (f16 matrix) transform form
     std::vector<uint32_t> pending_, tail_;
     std::vector<bool> zeroed_;
     uint32_t passes_ = 0;

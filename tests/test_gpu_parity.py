@@ -367,6 +367,44 @@ def test_nl4d_engine_with_pooled_threshold_is_bit_exact(chroma):
             assert np.array_equal(got[i][2], want_c[i][1]), f"V {i}"
 
 
+def _psnr(a, b, peak):
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return 99.0 if mse == 0 else 10.0 * np.log10(peak * peak / mse)
+
+
+def test_nl4d_mixed_transform_form_stays_within_0_05_db():
+    """The mixed form (f16 matrix-core DCTs, f32 accumulation) against the FP32 form on the same
+    4:2:0 clip: PSNR against the clean scene within 0.05 dB on every plane, outputs within a few
+    codes of each other, and the FP32 form still bit-exact with the oracle."""
+    W, H, depth, r, S = 96, 64, 10, 2, 8
+    frames = 7
+    sig_y, sig_c = 16.0, 8.0
+    noisy = avd.synth.clip(frames, W, H, depth, 123, chroma="420", sigma_luma=sig_y, sigma_chroma=sig_c)
+    clean = avd.synth.clip(frames, W, H, depth, 123, chroma="420", sigma_luma=0.0, sigma_chroma=0.0)
+    geom = L.Geometry(W, H, 1, L.SampleFormat.u16(10), L.SampleFormat.u16(10), 0)
+    dims = [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
+    outs = {}
+    for form in (0, 1):
+        common = dict(temporal_radius=r, spatial_radius=S, lambda_ht=2.7, pooled_threshold=0, noise_map=0,
+                      motion={"estimation": 1}, transform_form=form)
+        eng = avd.build_engine("nl4d", geom, luma=avd.nl4d_options(sigma=sig_y / 1023, **common),
+                               chroma=avd.nl4d_options(sigma=sig_c / 1023, **common))
+        outs[form] = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f] for f in noisy], depth, dims)
+    pl = O.Nl4dPass(_nl4d_spec(r, S, sig_y / 1023), W, H, 1, depth, depth)
+    want_l, _ = O.run_stream(pl, [[O.pack_plane(f[0].numpy(), depth)] for f in noisy])
+    for i in range(frames):
+        assert np.array_equal(outs[0][i][0], want_l[i][0]), f"FP32 form, luma {i}"
+    for k, (w, h) in enumerate(dims):
+        ref = np.concatenate([clean[i][k].numpy().reshape(-1) for i in range(frames)])
+        dec = {f: np.concatenate([O.unpack_plane(outs[f][i][k], depth, w * h) for i in range(frames)])
+               for f in (0, 1)}
+        p32, pmx = _psnr(dec[0], ref, 1023.0), _psnr(dec[1], ref, 1023.0)
+        print(f"plane {k}: FP32 {p32:.3f} dB, mixed {pmx:.3f} dB, max |diff| {int(np.max(np.abs(dec[0] - dec[1])))}")
+        assert abs(pmx - p32) <= 0.05, (k, p32, pmx)
+        assert int(np.max(np.abs(dec[0] - dec[1]))) <= 8, k
+        assert not np.array_equal(dec[0], dec[1]) or k > 0, "the mixed form ran"
+
+
 def test_nl4d_chroma_passes_through_when_only_luma_is_cleaned():
     W, H, depth = 48, 40, 8
     clip = avd.synth.clip(5, W, H, depth, 3, chroma="420", sigma_luma=6.0)
