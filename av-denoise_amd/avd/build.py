@@ -24,10 +24,13 @@ COMMON = [
 ]
 # Exact-arithmetic units keep the operation order they spell out (explicit fmaf only).
 EXACT = ["-ffp-contract=off"]
+# Matrix-core accumulators in VGPRs rather than AGPRs: the grouping and filter kernels read every
+# product's result with vector instructions, which AGPRs would first copy out.
+MFMA_VGPR = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 SOURCES = {
     "io.hip": EXACT,
     "motion.hip": EXACT,
-    "collab.hip": EXACT,
+    "collab.hip": EXACT + MFMA_VGPR,
     "nlm.hip": [],
     "engine.cpp": EXACT,
     "abi.cpp": EXACT,

@@ -28,6 +28,7 @@
 // permutes member slots, every member carries the same variance, and slot 0 holds the group
 // approximation in both layouts, so every kept coefficient and every output value is identical.
 #include "common.h"
+#include <cstring>
 
 namespace avd {
 
@@ -1751,7 +1752,9 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
 
 // Adds member `id`'s staged patch (sums, and weights recomputed from the group weight in the
 // filter's own arithmetic) for the pixels of row r that fall in [x0, x0 + T) x [y0, y0 + T).
-template <int C, typename Add>
+// MIXED: the stage holds the filtered samples as f16; the window and the group weight are applied
+// here, as the FP32 form's filter applies them.
+template <int C, bool MIXED, typename Add>
 __device__ __forceinline__ void member_row(const avd_collab_pass &p, const uint32_t *__restrict__ members,
                                            const float *__restrict__ group_weight, const TileBins &tiles,
                                            uint32_t id, int r, int x0, int y0, int T, Add &&add)
@@ -1764,71 +1767,82 @@ __device__ __forceinline__ void member_row(const avd_collab_pass &p, const uint3
     const float sw = group_weight[ref] * p.weight_scale;
     const float kr = p.kaiser[r];
     const int32_t *st = tiles.stage + (size_t)id * C * 64 + r * 8;
+    const _Float16 *st16 = reinterpret_cast<const _Float16 *>(tiles.stage) + (size_t)id * C * 64 + r * 8;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int x = xb + q;
         if (x < 0 || x >= T) continue;
         const float w = sw * (kr * p.kaiser[q]);
-        add(y * T + x, pack_aggregate(st[q], to_fixed_weight(w, p.accum_scale)), C == 2 ? st[64 + q] : 0);
+        if (MIXED) {
+            const int32_t a = to_fixed((float)st16[q] * w, p.accum_scale);
+            const int32_t a2 = C == 2 ? to_fixed((float)st16[64 + q] * w, p.accum_scale) : 0;
+            add(y * T + x, pack_aggregate(a, to_fixed_weight(w, p.accum_scale)), a2);
+        } else {
+            add(y * T + x, pack_aggregate(st[q], to_fixed_weight(w, p.accum_scale)), C == 2 ? st[64 + q] : 0);
+        }
     }
 }
 
-// One workgroup per (slot, 32x32 tile): the listed members' positions and group weights are
-// fetched 256 at a time into LDS, then each wave walks members with lane (r, q) on pixel (r, q)
-// of the patch, adding its staged sum and its weight (recomputed from the group weight in the
-// filter's own arithmetic) to the tile in LDS.  The tile then adds to the accumulators once, as
-// the only writer of those pixels this pass (overflowing pairs are added by k_tile_overflow).
 // ---------------------------------------------------------------------------------------------
-// The mixed-precision filter form: the 2-D DCT and its inverse on the matrix cores.
+// The mixed-precision filter form: every transform of a group on the matrix cores.
 //
-// A member's 8x8 patch is a 64-vector x (pixel k = 8r + c); its coefficients are (D (x) D) x
-// (coefficient k' = 8v + u), the Kronecker square of the 8-point DCT.  A wave filters four
-// references: B's 32 columns are their 32 members, A is D (x) D in f16 (from LDS), and
-// v_mfma_f32_32x32x8_f16 accumulates in f32 (2 row tiles x 8 K steps).  Lane (n, h) receives the
-// coefficients (v, u = 4h..4h+3) of member n, so the group Haar pairs lanes n ^ 1, ^ 2, ^ 4 of one
-// DPP row; the same layout is the B operand of the inverse, (D (x) D)^T.  The samples and the
-// transform matrix round to f16 (about 2^-11 relative), so the output differs from the FP32 form
-// by well under 0.05 dB; the hard threshold and the group weight's variance sum follow the FP32
-// form's rules in another summation order.  Hard threshold only.
+// A wave filters four references (32 members).  The FP32 form's 3-D transform (the 2-D DCT of each
+// patch, then the group Haar across members) and its inverse become five steps, four of them
+// products on v_mfma_f32_32x32x16_f16 (f16 operands, f32 accumulation):
+//   1. X1[member][coef]  = P[member][pixel] T[pixel][coef]           2-D DCT, T = (D (x) D)^T
+//   2. X2[member][coef]  = H[member][member'] X1[member'][coef]      group Haar: block diagonal,
+//                                                                     one 8x8 block per reference
+//   3. the hard threshold of X2 and the group weight (the retained variance)
+//   4. X3[coef][member]  = X2^T[coef][member'] Hi^T[member'][member]  inverse Haar
+//   5. X4[pixel][member] = Ti[pixel][coef] X3[coef][member]           inverse DCT, Ti = T^T
+// An accumulator tile keeps its rows in registers (register 4 j + i holds row 8 j + 4 (lane / 32)
+// + i) and its columns on lanes, while each operand takes its rows (or columns) on lanes and its K
+// values in registers.  So a tile is, as it lies, the operand of the next product contracting over
+// its rows: K slot i' of lane half h in K step s is row 16 s + 8 (i' / 4) + 4 h + i' % 4, and the
+// tables (T, Ti, H, Hi) are laid out in that order.  No lane exchanges the transforms.  The Haar
+// blocks are the linear maps of each reference's Haar plan (the FP32 form's levels, applied to
+// unit vectors).  The samples, the tables and each product's inputs round to f16 (2^-11
+// relative), which moves PSNR by a few thousandths of a dB against the FP32 form; the threshold
+// and the retained variance follow the FP32 form's rules, summed in another order.  Hard
+// threshold only.  The filtered samples are staged as f16 (half the FP32 form's bytes); the
+// gather applies the window and the group weight in fixed point.
 // ---------------------------------------------------------------------------------------------
 
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float16_t __attribute__((ext_vector_type(16)));
+typedef float float4u_t __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float dct_coef(int v, int r)
-{
-    const int m = v >> 1, k = r < 4 ? r : 7 - r;
-    if ((v & 1) == 0) return c_taps.e[m * 4 + k];
-    return r < 4 ? c_taps.o[m * 4 + k] : -c_taps.o[m * 4 + k];
-}
+// DCT tables in operand order, uploaded once per device.  Fragment 2 s + mt (s < 4, mt < 2), the
+// B operand of step 1: lane (n, h) holds T[pixel 16 s + 8 h + i'][coef 32 mt + n] for i' < 8.
+// Fragment 8 + 2 s + pt, the A operand of step 5: lane (n, h) holds Ti[pixel 32 pt + n][coef 16 s +
+// 8 (i' / 4) + 4 h + i' % 4].
+__device__ uint4 g_mixed_frags[16][64];
 
-// Value of lane (member) `src_lane` of this lane's DPP row, for partners XOR 1, 2 or 4.
-template <int XOR>
-__device__ __forceinline__ float member_partner(float v, int n)
+__device__ __forceinline__ half8_t frag_from(const float16_t &acc, int base)
 {
-    const int b = __builtin_bit_cast(int, v);
-    int o;
-    if (XOR == 1) o = __builtin_amdgcn_update_dpp(0, b, 0xB1, 0xF, 0xF, false);       // quad_perm [1,0,3,2]
-    else if (XOR == 2) o = __builtin_amdgcn_update_dpp(0, b, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-    else {
-        // row_ror:k hands lane i the value of lane i - k of its row of 16.
-        const int dn = __builtin_amdgcn_update_dpp(0, b, 0x124, 0xF, 0xF, false);    // row_ror 4: lane - 4
-        const int up = __builtin_amdgcn_update_dpp(0, b, 0x12C, 0xF, 0xF, false);    // row_ror 12: lane + 4
-        o = (n & 4) ? dn : up;
-    }
-    return __builtin_bit_cast(float, o);
-}
-
-template <int XOR>
-__device__ __forceinline__ void haar_level_dpp(float (&y)[32], bool active, bool low, int n)
-{
-    const float f = 0.70710678118654752440f;
-    const float sgn = low ? 1.0f : -1.0f;
+    half8_t f;
 #pragma unroll
-    for (int i = 0; i < 32; ++i) {
-        const float other = member_partner<XOR>(y[i], n);
-        const float t = __builtin_fmaf(sgn, y[i], other);  // low: v + other; high: other - v
-        if (active) y[i] = t * f;
+    for (int i = 0; i < 8; ++i) f[i] = (_Float16)acc[base + i];
+    return f;
+}
+
+__device__ __forceinline__ half8_t frag_from(const float (&acc)[16], int base)
+{
+    half8_t f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) f[i] = (_Float16)acc[base + i];
+    return f;
+}
+
+// One Haar butterfly level, scaled by f, applied to four unit-vector columns held across the
+// member lanes (a reference's members are 8 consecutive lanes; partners are XOR apart).
+template <int XOR>
+__device__ __forceinline__ void haar_columns(float (&v)[4], bool active, bool low, float f)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float other = __shfl_xor(v[k], XOR);
+        if (active) v[k] = low ? (v[k] + other) * f : (other - v[k]) * f;
     }
 }
 
@@ -1838,14 +1852,9 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
                                                                      const uint32_t *__restrict__ members,
                                                                      float *__restrict__ group_weight, TileBins tiles)
 {
-    __shared__ _Float16 s_afwd[64][64], s_ainv[64][64];  // [row][k] of D (x) D and of its transpose
+    __shared__ uint4 s_frag[16][64];
     __shared__ float s_cvs[C][64], s_thr[C][64];
-    for (int i = threadIdx.x; i < 64 * 64; i += kWaves * 64) {
-        const int a = i >> 6, b = i & 63;  // a = 8v + u, b = 8r + c
-        const float d = dct_coef(a >> 3, b >> 3) * dct_coef(a & 7, b & 7);
-        s_afwd[a][b] = (_Float16)d;
-        s_ainv[b][a] = (_Float16)d;
-    }
+    for (int i = threadIdx.x; i < 16 * 64; i += kWaves * 64) s_frag[i >> 6][i & 63] = g_mixed_frags[i >> 6][i & 63];
     if (threadIdx.x < 64 * C) {
         const int c = threadIdx.x >> 6, v = (threadIdx.x >> 3) & 7, u = threadIdx.x & 7;
         const float cvs = (p.sigma[c] * p.sigma[c]) * (p.profile[u] * p.profile[v]);
@@ -1855,116 +1864,169 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int n = lane & 31, h = lane >> 5, m = n & 7;
+    const int n = lane & 31, h = lane >> 5, m = n & 7, jn = n >> 3;
     const int W = p.width, H = p.height;
     const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
     const int refs = refs_x * refs_y;
-    int ref = (xcd_block() * kWaves + wave) * 4 + (n >> 3);
-    const bool live = ref < refs;
-    if (!live) ref = refs - 1;
+    const int first = (xcd_block() * kWaves + wave) * 4;  // the wave's first reference
+    int k_use[4];  // members in the group of each of the wave's references
+#pragma unroll
+    for (int j = 0; j < 4; ++j) k_use[j] = (int)(members[(size_t)min(first + j, refs - 1) * 9 + 8] & 0xFF);
+
+    // This lane's member: member m of reference `ref`.
+    const int ref = min(first + jn, refs - 1);
+    const bool live = first + jn < refs;
     const uint32_t *mw = members + (size_t)ref * 9;
     const uint32_t gword = mw[8];
-    const int k_use = (int)(gword & 0xFF);
+    const int k_use_n = (int)(gword & 0xFF);
     const bool grid = (gword & 0x100u) != 0;
     const int gf = p.temporal_radius == 0 ? 1 : (p.temporal_radius == 1 ? 2 : 4);
-    const HaarPlan plan = haar_plan(m, k_use, grid, gf);
+    const bool in_group = m < k_use_n;
     const uint32_t pos = mw[m];
     const uint32_t t = pos >> 26;
     const int slot = t > 0 ? (int)nslots[t - 1] : (int)p.centre_slot;
     const int mx = (int)(pos & 0x1FFF), my = (int)((pos >> 13) & 0x1FFF);
+
+    // Rows 2 s + h of the member's patch: 8 C samples each.
     const size_t px = (size_t)W * H;
-    const float *src = values + (size_t)slot * px * C;
-    const bool in_group = m < k_use;
-    const float scale = p.accum_scale;
-    float scaled_weight = 0.0f;
+    const float *src = values + (size_t)slot * px * C + ((size_t)(my + h) * W + mx) * C;
+    float4u_t raw[4][2 * C];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int k = 0; k < 2 * C; ++k) raw[s][k] = *reinterpret_cast<const float4u_t *>(src + (size_t)2 * s * W * C + 4 * k);
+
+    // The Haar blocks.  A member that passes L active levels of its reference's plan leaves the
+    // forward transform as f^L (f = 1/sqrt 2) times a sum of its group's coefficients with signs
+    // +-1, so H = D S with S of +-1 and 0 (exact in f16) and D = diag(f^L); the inverse is
+    // H^T = S^T D.  Step 2 multiplies by S only, step 3 compares |S X1| f^L with the threshold, and
+    // step 4 multiplies by S^T D^2, whose entries +-2^-L are exact too.  Lane (n, h) holds
+    // S[m][4 h + k] and (S^T D^2)[m][4 h + k] of its reference (the butterflies run on unit vectors
+    // with factors 1 and 1/2), which sit in K step jn / 2 at slots 4 (jn % 2) + k of the step-2 A
+    // and step-4 B operands.
+    half8_t hfrag[2], gfrag[2];
+    {
+        const HaarPlan plan = haar_plan(m, k_use_n, grid, gf);
+        float hs[4], hg[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) hs[k] = hg[k] = m == 4 * h + k ? 1.0f : 0.0f;
+        haar_columns<1>(hs, plan.levels > 0 && plan.act[0], plan.low[0], 1.0f);
+        haar_columns<2>(hs, plan.levels > 1 && plan.act[1], plan.low[1], 1.0f);
+        haar_columns<4>(hs, plan.levels > 2 && plan.act[2], plan.low[2], 1.0f);
+        haar_columns<4>(hg, plan.levels > 2 && plan.act[2], plan.low[2], 0.5f);
+        haar_columns<2>(hg, plan.levels > 1 && plan.act[1], plan.low[1], 0.5f);
+        haar_columns<1>(hg, plan.levels > 0 && plan.act[0], plan.low[0], 0.5f);
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const bool mine = s == (jn >> 1) && (i >> 2) == (jn & 1);
+                hfrag[s][i] = (_Float16)(mine ? hs[i & 3] : 0.0f);
+                gfrag[s][i] = (_Float16)(mine ? hg[i & 3] : 0.0f);
+            }
+    }
+    // f^L of member 4 h + i of reference j (the rows this lane holds after step 2), from the lane
+    // of that member.
+    float row_scale[4][4];
+    {
+        const HaarPlan plan = haar_plan(m, k_use_n, grid, gf);
+        const int L = (plan.levels > 0 && plan.act[0]) + (plan.levels > 1 && plan.act[1]) +
+                      (plan.levels > 2 && plan.act[2]);
+        const float own = ((L & 1) ? 0.70710678118654752440f : 1.0f) * (L >= 2 ? 0.5f : 1.0f);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) row_scale[j][i] = __shfl(own, 8 * j + 4 * h + i);
+    }
 
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        // ---- forward: coefficient rows 32 mt + (lane's rows) of member n.
-        float16_t acc[2];
+        // ---- 1. 2-D DCT: X1 tile mt holds coefficients 32 mt + n of members 8 j + 4 h + i.
+        half8_t pa[4];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+        for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int i = 0; i < 16; ++i) acc[mt][i] = 0.0f;
+            for (int i = 0; i < 8; ++i) pa[s][i] = (_Float16)raw[s][(i * C + c) >> 2][(i * C + c) & 3];
+        float16_t x1[2], x2[2];
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            half4_t b;  // pixels (r = s, c = 4h .. 4h + 3) of member n
+        for (int mt = 0; mt < 2; ++mt) {
+            x1[mt] = float16_t{};
 #pragma unroll
-            for (int i = 0; i < 4; ++i) b[i] = (_Float16)src[((size_t)(my + s) * W + mx + 4 * h + i) * C + c];
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                half4_t a;  // row 32 mt + n of D (x) D, columns 8 s + 4 h .. + 3
-#pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = s_afwd[32 * mt + n][8 * s + 4 * h + i];
-                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc[mt], 0, 0, 0);
-            }
+            for (int s = 0; s < 4; ++s)
+                x1[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                    pa[s], __builtin_bit_cast(half8_t, s_frag[2 * s + mt][lane]), x1[mt], 0, 0, 0);
         }
-        // Lane (n, h), register 4 j + i of tile mt: coefficient (v = 4 mt + j, u = 4 h + i).
-        float y[32];
+        // ---- 2. group Haar (S).
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+        for (int mt = 0; mt < 2; ++mt) {
+            x2[mt] = float16_t{};
 #pragma unroll
-            for (int i = 0; i < 16; ++i) y[mt * 16 + i] = acc[mt][i];
-        if (plan.levels > 0) haar_level_dpp<1>(y, plan.act[0], plan.low[0], n);
-        if (plan.levels > 1) haar_level_dpp<2>(y, plan.act[1], plan.low[1], n);
-        if (plan.levels > 2) haar_level_dpp<4>(y, plan.act[2], plan.low[2], n);
-        // ---- hard threshold (the group DC always kept) and the retained variance.
-        float part = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 32; ++i) {
-            const int v = (i >> 4) * 4 + ((i >> 2) & 3), u = 4 * h + (i & 3);
-            bool keep = fabsf(y[i]) >= s_thr[c][v * 8 + u];
-            if (m == 0 && v == 0 && u == 0) keep = true;
-            keep = keep && in_group;
-            if (keep) part += s_cvs[c][v * 8 + u];
-            else if (in_group) y[i] = 0.0f;
+            for (int s = 0; s < 2; ++s)
+                x2[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hfrag[s], frag_from(x1[mt], 8 * s), x2[mt], 0, 0, 0);
         }
-        // Sum over the reference's 8 member lanes and both halves.
-        part += member_partner<1>(part, n);
-        part += member_partner<2>(part, n);
-        part += member_partner<4>(part, n);
-        part += __shfl_xor(part, 32);
-        const float weight = (isnan(part) || isinf(part)) ? 0.0f : 1.0f / fmaxf(part, 1e-12f);
+        // ---- 3. hard threshold (the group DC always kept) and the retained variance of each
+        // reference, from the first channel.  Groups hold 1, 2, 4 or 8 members and the Haar pairs
+        // members inside the group, so the members past k_use (zeroed here like any dropped
+        // coefficient) never reach the group's; they only leave the variance sum.
+        float y[2][16];
+        float kept[4][4];  // [reference j][register i]: retained variance over both tiles
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            const float thr = s_thr[c][32 * mt + n], cvs = s_cvs[c][32 * mt + n];
+            const float thr_dc = mt == 0 && lane == 0 ? -INFINITY : thr;  // coefficient 0 of member 0
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float v = x2[mt][4 * j + i];
+                    const bool keep = fabsf(v) * row_scale[j][i] >= (i == 0 ? thr_dc : thr);
+                    y[mt][4 * j + i] = keep ? v : 0.0f;
+                    const float add = keep ? cvs : 0.0f;
+                    kept[j][i] = mt == 0 ? add : kept[j][i] + add;
+                }
+        }
         if (c == 0) {
-            if (live && m == 0 && h == 0) group_weight[ref] = weight;
-            scaled_weight = weight * p.weight_scale;
-        }
-        if (plan.levels > 2) haar_level_dpp<4>(y, plan.act[2], plan.low[2], n);
-        if (plan.levels > 1) haar_level_dpp<2>(y, plan.act[1], plan.low[1], n);
-        if (plan.levels > 0) haar_level_dpp<1>(y, plan.act[0], plan.low[0], n);
-        // ---- inverse: the coefficients in this layout are B's K steps.
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+            for (int j = 0; j < 4; ++j) {
+                float part = 0.0f;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) acc[mt][i] = 0.0f;
+                for (int i = 0; i < 4; ++i) part += 4 * h + i < k_use[j] ? kept[j][i] : 0.0f;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            half4_t b;  // coefficients 8 s + 4 h + i = registers 4 (s & 3) + i of tile s >> 2
-#pragma unroll
-            for (int i = 0; i < 4; ++i) b[i] = (_Float16)y[(s >> 2) * 16 + (s & 3) * 4 + i];
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                half4_t a;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = s_ainv[32 * mt + n][8 * s + 4 * h + i];
-                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc[mt], 0, 0, 0);
+                for (int o = 1; o < 64; o <<= 1) part += __shfl_xor(part, o);
+                const float weight = (isnan(part) || isinf(part)) ? 0.0f : 1.0f / fmaxf(part, 1e-12f);
+                if (lane == 0 && first + j < refs) group_weight[first + j] = weight;
             }
         }
-        // Register 4 j + i of tile mt: pixel (r = 4 mt + j, c = 4 h + i).
-        if (live) {
-            int32_t *st = tiles.stage + (((size_t)ref * 8 + m) * C + c) * 64;
+        // ---- 4. inverse Haar (S^T D^2): X3 tile mt holds coefficients 32 mt + 8 j + 4 h + i of member n.
+        float16_t x3[2];
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
+        for (int mt = 0; mt < 2; ++mt) {
+            x3[mt] = float16_t{};
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                x3[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(frag_from(y[mt], 8 * s), gfrag[s], x3[mt], 0, 0, 0);
+        }
+        // ---- 5. inverse DCT: X4 tile pt holds pixel (4 pt + j, 4 h + i) of member n.
+        float16_t x4[2];
+#pragma unroll
+        for (int pt = 0; pt < 2; ++pt) {
+            x4[pt] = float16_t{};
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                x4[pt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                    __builtin_bit_cast(half8_t, s_frag[8 + 2 * s + pt][lane]), frag_from(x3[s >> 1], 8 * (s & 1)), x4[pt],
+                    0, 0, 0);
+        }
+        if (live && in_group) {
+            _Float16 *st = reinterpret_cast<_Float16 *>(tiles.stage) + (((size_t)ref * 8 + m) * C + c) * 64 + 4 * h;
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int r = 4 * mt + j;
-                    int4_t o;
+                    half4_t o;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const float w = scaled_weight * (p.kaiser[r] * p.kaiser[4 * h + i]);
-                        o[i] = in_group ? to_fixed(acc[mt][4 * j + i] * w, scale) : 0;
-                    }
-                    *reinterpret_cast<int4_t *>(st + r * 8 + 4 * h) = o;
+                    for (int i = 0; i < 4; ++i) o[i] = (_Float16)x4[pt][4 * j + i];
+                    *reinterpret_cast<half4_t *>(st + (4 * pt + j) * 8) = o;
                 }
         }
     }
@@ -2015,7 +2077,13 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
     }
 }
 
-template <int C>
+// One workgroup per (slot, 32x32 tile): the listed members' positions and group weights are
+// fetched 256 at a time into LDS, then each wave walks members with lane (r, q) on pixel (r, q)
+// of the patch, adding its staged sum and its weight (recomputed from the group weight in the
+// filter's own arithmetic) to the tile in LDS.  The tile then adds to the accumulators once, as
+// the only writer of those pixels this pass (overflowing pairs are added by k_tile_overflow).
+// MIXED: the stage holds f16 samples, weighted and converted to fixed point here.
+template <int C, bool MIXED>
 __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const uint32_t *__restrict__ members,
                                                      const float *__restrict__ group_weight, TileBins tiles,
                                                      unsigned long long *__restrict__ aggregate,
@@ -2064,11 +2132,18 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
                     const uint32_t id = s_id[e], pos = s_pos[e];
                     const int lx = (int)(pos & 0x1FFF) + q - x0, ly = (int)((pos >> 13) & 0x1FFF) + r - y0;
                     if (lx >= 0 && lx < T && ly >= 0 && ly < T) {
-                        const int32_t *st = tiles.stage + (size_t)id * C * 64 + lane;
                         cell[u] = ly * T + lx;
-                        a[u] = st[0];
-                        if (C == 2) a2[u] = st[64];
                         w[u] = s_sw[e] * kr_kq;
+                        if (MIXED) {
+                            const _Float16 *st =
+                                reinterpret_cast<const _Float16 *>(tiles.stage) + (size_t)id * C * 64 + lane;
+                            a[u] = to_fixed((float)st[0] * w[u], scale);
+                            if (C == 2) a2[u] = to_fixed((float)st[64] * w[u], scale);
+                        } else {
+                            const int32_t *st = tiles.stage + (size_t)id * C * 64 + lane;
+                            a[u] = st[0];
+                            if (C == 2) a2[u] = st[64];
+                        }
                     }
                 }
             }
@@ -2094,7 +2169,7 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
 
 // The (member, tile) pairs that did not fit a tile's list: thread (pair, row) adds its row's pixels
 // inside the pair's tile with global atomics.
-template <int C>
+template <int C, bool MIXED>
 __global__ __launch_bounds__(256) void k_tile_overflow(avd_collab_pass p, const uint32_t *__restrict__ members,
                                                        const float *__restrict__ group_weight, TileBins tiles,
                                                        unsigned long long *__restrict__ aggregate,
@@ -2110,7 +2185,7 @@ __global__ __launch_bounds__(256) void k_tile_overflow(avd_collab_pass p, const 
         const uint32_t slot = tile / tiles.per_slot, in_slot = tile - slot * tiles.per_slot;
         const int ty = (int)(in_slot / tiles.tiles_x), tx = (int)(in_slot - ty * tiles.tiles_x);
         const int x0 = tx * T, y0 = ty * T;
-        member_row<C>(p, members, group_weight, tiles, id, r, x0, y0, T,
+        member_row<C, MIXED>(p, members, group_weight, tiles, id, r, x0, y0, T,
                       [&](int i, unsigned long long v, int32_t v2) {
                           const size_t pix = (size_t)slot * px + (size_t)(y0 + (i >> kTileShift)) * p.width +
                                              x0 + (i & (T - 1));
@@ -2146,19 +2221,54 @@ double dct8_basis(int row, int col)
     return rs * cos(M_PI * (2.0 * col + 1.0) * row / 16.0);
 }
 
+// The mixed form's DCT tables in operand order (see g_mixed_frags): products of two DCT basis
+// values, rounded once to f16.
+void mixed_frags(uint4 (&out)[16][64])
+{
+    for (int lane = 0; lane < 64; ++lane) {
+        const int h = lane >> 5, n = lane & 31;
+        for (int f = 0; f < 16; ++f) {
+            _Float16 v[8];
+            for (int i = 0; i < 8; ++i) {
+                int coef, pixel;
+                if (f < 8) {
+                    coef = 32 * (f & 1) + n;
+                    pixel = 16 * (f >> 1) + 8 * h + i;
+                } else {
+                    pixel = 32 * ((f - 8) & 1) + n;
+                    coef = 16 * ((f - 8) >> 1) + 8 * (i >> 2) + 4 * h + (i & 3);
+                }
+                v[i] = (_Float16)(dct8_basis(coef >> 3, pixel >> 3) * dct8_basis(coef & 7, pixel & 7));
+            }
+            std::memcpy(&out[f][lane], v, sizeof(v));
+        }
+    }
+}
+
 hipError_t upload_taps(hipStream_t s)
 {
-    static bool done = false;  // per process; the taps are constants
-    if (done) return hipSuccess;
+    // Per device (the constants live in each device's copy of the code object); the engine and
+    // the site calls of one device run on one host thread at a time.
+    constexpr int kMaxDevices = 64;
+    static bool done[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (done[dev]) return hipSuccess;
     DctTaps t;
     for (int m = 0; m < 4; ++m)
         for (int k = 0; k < 4; ++k) {
             t.e[m * 4 + k] = (float)dct8_basis(2 * m, k);
             t.o[m * 4 + k] = (float)dct8_basis(2 * m + 1, k);
         }
-    hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_taps), &t, sizeof(t), 0, hipMemcpyHostToDevice, s);
+    static uint4 frags[16][64];
+    mixed_frags(frags);
+    e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_taps), &t, sizeof(t), 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyToSymbolAsync(HIP_SYMBOL(g_mixed_frags), frags, sizeof(frags), 0, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e == hipSuccess) done = true;
+    if (e == hipSuccess) done[dev] = true;
     return e;
 }
 
@@ -2344,22 +2454,26 @@ hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *va
     const uint32_t refs = (uint32_t)ref_count(p);
     const unsigned g = div_up(refs, kWaves);
     const unsigned gm = div_up(refs, kWaves * 4);
-    if (p.channels == 1) {
-        if (mixed)
+    if (mixed) {
+        if (p.channels == 1) {
             k_collab_filter_mixed<1><<<gm, kWaves * 64, 0, s>>>(p, values, nslots, members, group_weight, tiles);
-        else
-            k_collab_filter<1, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
-                                                                group_weight, tiles);
-        k_tile_gather<1><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
-        k_tile_overflow<1><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
-    } else {
-        if (mixed)
+            k_tile_gather<1, true><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+            k_tile_overflow<1, true><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+        } else {
             k_collab_filter_mixed<2><<<gm, kWaves * 64, 0, s>>>(p, values, nslots, members, group_weight, tiles);
-        else
-            k_collab_filter<2, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
-                                                                group_weight, tiles);
-        k_tile_gather<2><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
-        k_tile_overflow<2><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+            k_tile_gather<2, true><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+            k_tile_overflow<2, true><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+        }
+    } else if (p.channels == 1) {
+        k_collab_filter<1, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
+                                                            group_weight, tiles);
+        k_tile_gather<1, false><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+        k_tile_overflow<1, false><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+    } else {
+        k_collab_filter<2, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
+                                                            group_weight, tiles);
+        k_tile_gather<2, false><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+        k_tile_overflow<2, false><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
     }
     return hipGetLastError();
 }

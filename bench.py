@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -35,7 +36,7 @@ STAGE_KERNELS = {
     "collab_search": "k_trajectory_codes",
     "collab_centre": "k_centre_prep + k_centre_mfma + k_centre_redo",
     "motion": "k_bm_coarse + k_bm_fine",
-    "collab_filter": "k_collab_filter + k_tile_gather + k_tile_overflow",
+    "collab_filter": "k_collab_filter_mixed + k_tile_gather + k_tile_overflow",
 }
 STAGE_PEAK = {"collab_centre": MFMA_F16_PEAK_TOPS}
 
@@ -50,6 +51,9 @@ def parse():
     ap.add_argument("--sample-all", action="store_true",
                     help="sample device time from the first step (every launch timed alone, the plane groups "
                          "one after the other; for kernel traces whose durations match the stage figures)")
+    ap.add_argument("--form", choices=["both", "mixed", "fp32"], default="both",
+                    help="transform form(s) to run; the headline value is the mixed form's when it runs")
+    ap.add_argument("--no-quality", action="store_true", help="skip the PSNR comparison against the clean scene")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, one plain device copy of this many MiB (a known-size kernel that "
                          "hardware-counter runs calibrate FETCH_SIZE / WRITE_SIZE against)")
@@ -71,21 +75,53 @@ def stage_traffic():
     return {k: v["bytes_per_launch"] for k, v in data.get("stages", {}).items()}
 
 
-def make_engine(avd):
+FORMS = {"fp32": 0, "mixed": 1}
+
+
+def make_engine(avd, form="mixed"):
     geom = avd.Geometry(W, H, 1, avd.SampleFormat.u16(DEPTH), avd.SampleFormat.u16(DEPTH), 0)
     common = dict(temporal_radius=RADIUS, spatial_radius=SPATIAL, refine=REFINE, lambda_ht=LAMBDA,
-                  pooled_threshold=0, noise_map=0, motion={"estimation": 1})
+                  pooled_threshold=0, noise_map=0, motion={"estimation": 1}, transform_form=FORMS[form])
     luma = avd.nl4d_options(sigma=SIGMA_Y / 1023.0, **common)
     chroma = avd.nl4d_options(sigma=SIGMA_C / 1023.0, **common)
     return avd.build_engine("nl4d", geom, luma=luma, chroma=chroma), geom
 
 
-def gpu_clip(avd, torch):
-    clip = avd.synth.clip(CLIP_FRAMES, W, H, DEPTH, 0xA5D00004, chroma="420", sigma_luma=SIGMA_Y,
-                          sigma_chroma=SIGMA_C, device="cuda", lo_hi=(64, 940))
+def gpu_clip(avd, torch, sigma=True):
+    sy, sc = (SIGMA_Y, SIGMA_C) if sigma else (0.0, 0.0)
+    clip = avd.synth.clip(CLIP_FRAMES, W, H, DEPTH, 0xA5D00004, chroma="420", sigma_luma=sy, sigma_chroma=sc,
+                          device="cuda", lo_hi=(64, 940))
     frames = [[avd.synth.pack_words(p, DEPTH) for p in f] for f in clip]
     torch.cuda.synchronize()
     return clip, frames
+
+
+def plane_dims():
+    return [(W, H), ((W + 1) // 2, (H + 1) // 2), ((W + 1) // 2, (H + 1) // 2)]
+
+
+def quality(avd, torch, frames, clean, form, first=4, count=4):
+    """PSNR (dB, per plane) against the noise-free scene of output frames first..first+count-1."""
+    eng, geom = make_engine(avd, form)
+    sq = [0.0, 0.0, 0.0]
+    n = [0, 0, 0]
+    emitted = 0
+    for t in range(len(frames)):
+        for _ in range(eng.push(frames[t])):
+            outs = [torch.zeros(avd.plane_bytes(geom.output, w, h), dtype=torch.uint8, device="cuda")
+                    for (w, h) in plane_dims()]
+            eng.emit_into(outs)
+            if first <= emitted < first + count:
+                for k, (w, h) in enumerate(plane_dims()):
+                    got = outs[k].view(torch.int16)[: w * h].to(torch.int64) & 0xFFFF
+                    ref = clean[emitted][k].reshape(-1).to(torch.int64)
+                    sq[k] += float(((got - ref).to(torch.float64) ** 2).sum())
+                    n[k] += w * h
+            emitted += 1
+        if emitted >= first + count:
+            break
+    peak = float((1 << DEPTH) - 1)
+    return [10.0 * math.log10(peak * peak / (sq[k] / n[k])) for k in range(3)]
 
 
 def run_steps(eng, frames, outs, start, count):
@@ -141,22 +177,39 @@ def main():
     import avd
 
     torch.cuda.set_device(0)
-    eng, geom = make_engine(avd)
     clip, frames = gpu_clip(avd, torch)
-    outs = [torch.zeros(avd.plane_bytes(geom.output, w, h), dtype=torch.uint8, device="cuda")
-            for (w, h) in [(W, H), ((W + 1) // 2, (H + 1) // 2), ((W + 1) // 2, (H + 1) // 2)]]
-
     warm = max(args.warmup, 2 * RADIUS + 1)
-    if args.sample_all:
-        eng.sample_device_time(1)
-    run_steps(eng, frames, outs, 0, warm)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    emitted = run_steps(eng, frames, outs, warm, args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    assert emitted == args.steps, (emitted, args.steps)
+
+    def timed_run(form):
+        eng, geom = make_engine(avd, form)
+        outs = [torch.zeros(avd.plane_bytes(geom.output, w, h), dtype=torch.uint8, device="cuda")
+                for (w, h) in plane_dims()]
+        if args.sample_all:
+            eng.sample_device_time(1)
+        run_steps(eng, frames, outs, 0, warm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        emitted = run_steps(eng, frames, outs, warm, args.steps)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        assert emitted == args.steps, (emitted, args.steps)
+        return eng, outs, elapsed
+
+    # The FP32 form (bit-exact with the reference's f32 arithmetic) and the mixed form (f16 matrix
+    # transforms): frames/s of each, and PSNR against the noise-free scene on the same frames.
+    forms = {}
+    for form in (["fp32", "mixed"] if args.form == "both" else [args.form]):
+        eng, outs, elapsed = timed_run(form)
+        forms[form] = {"value": round(args.steps / elapsed, 3), "elapsed": elapsed, "eng": eng, "outs": outs}
+    headline = "mixed" if "mixed" in forms else "fp32"
+    eng, outs, elapsed = forms[headline]["eng"], forms[headline]["outs"], forms[headline]["elapsed"]
     fps = args.steps / elapsed
+    psnr = {}
+    if not args.no_quality:
+        clean, _ = gpu_clip(avd, torch, sigma=False)
+        for form in forms:
+            psnr[form] = [round(x, 4) for x in quality(avd, torch, frames, clean, form)]
+        del clean
 
     # Per-stage device time over a few more steps, with HIP events on the engine's stream; the
     # engine counts each stage's algorithmic operations (both search stages: 192 per candidate
@@ -202,12 +255,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32 search (f16 matrix-core products, exact) + f32 filter (FP32 form)",
+        "dtype": ("int32 search (f16 matrix-core products, exact) + " +
+                  ("f16 matrix-core DCTs with f32 accumulation (mixed form)" if headline == "mixed"
+                   else "f32 filter (FP32 form)")),
         "data": "synthetic (seeded moving sinusoid/gradient scene, Gaussian noise 16/8 codes, generated on GPU)",
         "config": {
             "workload": "NL4D 3840x2160 10-bit 4:2:0, temporal radius 3, spatial radius 12, refine 2, 8x8 patches "
                         "at stride 4, groups of 8, hard threshold 2.7 sigma (sigma fixed: 16/1023 luma, 8/1023 "
-                        "chroma), motion 16x16/8 blocks radius 4 two-level Direct; FP32 form",
+                        "chroma), motion 16x16/8 blocks radius 4 two-level Direct; " + headline + " form",
             "global_batch": 1,
             "frames_in_clip": CLIP_FRAMES,
             "mpix_per_s": round(fps * W * H / 1e6, 3),
@@ -227,7 +282,12 @@ def main():
         "stages_ms_per_frame": {k: round(v["ms"] / max(args.sample_steps, 1), 4) for k, v in stages.items()
                                 if v["launches"]},
         "dominant_stage": dominant,
+        "forms": {f: {"frames_per_s": forms[f]["value"], "psnr_db_vs_clean": psnr.get(f)} for f in forms},
     }
+    if "mixed" in forms and "fp32" in forms:
+        result["mixed_vs_fp32"] = round(forms["mixed"]["value"] / forms["fp32"]["value"], 4)
+        if psnr:
+            result["delta_psnr_db"] = [round(m - f, 4) for m, f in zip(psnr["mixed"], psnr["fp32"])]
 
     if not args.no_cpu_baseline:
         import numpy as np
