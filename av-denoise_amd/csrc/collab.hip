@@ -1773,10 +1773,12 @@ __device__ __forceinline__ void member_row(const avd_collab_pass &p, const uint3
         const int x = xb + q;
         if (x < 0 || x >= T) continue;
         const float w = sw * (kr * p.kaiser[q]);
-        if (MIXED) {
-            const int32_t a = to_fixed((float)st16[q] * w, p.accum_scale);
-            const int32_t a2 = C == 2 ? to_fixed((float)st16[64 + q] * w, p.accum_scale) : 0;
-            add(y * T + x, pack_aggregate(a, to_fixed_weight(w, p.accum_scale)), a2);
+        if (MIXED) {  // the mixed gather's arithmetic
+            const float ws = sw * (kr * p.kaiser[q] * p.accum_scale);
+            const float lim = 5.0f * p.accum_scale;
+            const int32_t a = (int32_t)rintf(fminf(fmaxf((float)st16[q] * ws, -lim), lim));
+            const int32_t a2 = C == 2 ? (int32_t)rintf(fminf(fmaxf((float)st16[64 + q] * ws, -lim), lim)) : 0;
+            add(y * T + x, pack_aggregate(a, (int32_t)rintf(fminf(ws, p.accum_scale) * 5.0f)), a2);
         } else {
             add(y * T + x, pack_aggregate(st[q], to_fixed_weight(w, p.accum_scale)), C == 2 ? st[64 + q] : 0);
         }
@@ -1846,55 +1848,23 @@ __device__ __forceinline__ void haar_columns(float (&v)[4], bool active, bool lo
     }
 }
 
+// The filter of one group of four references (the wave's), from its samples converted to f16
+// (`pa[c][s]`: rows 2 s + h of member n, channel c), writing the group weights and the staged
+// samples.
 template <int C>
-__global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_pass p, const float *__restrict__ values,
-                                                                     const uint32_t *__restrict__ nslots,
-                                                                     const uint32_t *__restrict__ members,
-                                                                     float *__restrict__ group_weight, TileBins tiles)
+__device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint4 (*s_frag)[64],
+                                            const float (*s_cvs)[64], const float (*s_thr)[64],
+                                            const half8_t (&pa)[C][4], int first, int refs, const int (&k_use)[4],
+                                            uint32_t gword, int lane, float *__restrict__ group_weight,
+                                            const TileBins &tiles)
 {
-    __shared__ uint4 s_frag[16][64];
-    __shared__ float s_cvs[C][64], s_thr[C][64];
-    for (int i = threadIdx.x; i < 16 * 64; i += kWaves * 64) s_frag[i >> 6][i & 63] = g_mixed_frags[i >> 6][i & 63];
-    if (threadIdx.x < 64 * C) {
-        const int c = threadIdx.x >> 6, v = (threadIdx.x >> 3) & 7, u = threadIdx.x & 7;
-        const float cvs = (p.sigma[c] * p.sigma[c]) * (p.profile[u] * p.profile[v]);
-        s_cvs[c][v * 8 + u] = cvs;
-        s_thr[c][v * 8 + u] = p.lambda[c] * sqrtf(cvs);
-    }
-    __syncthreads();
-
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int n = lane & 31, h = lane >> 5, m = n & 7, jn = n >> 3;
-    const int W = p.width, H = p.height;
-    const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
-    const int refs = refs_x * refs_y;
-    const int first = (xcd_block() * kWaves + wave) * 4;  // the wave's first reference
-    int k_use[4];  // members in the group of each of the wave's references
-#pragma unroll
-    for (int j = 0; j < 4; ++j) k_use[j] = (int)(members[(size_t)min(first + j, refs - 1) * 9 + 8] & 0xFF);
-
-    // This lane's member: member m of reference `ref`.
     const int ref = min(first + jn, refs - 1);
     const bool live = first + jn < refs;
-    const uint32_t *mw = members + (size_t)ref * 9;
-    const uint32_t gword = mw[8];
     const int k_use_n = (int)(gword & 0xFF);
     const bool grid = (gword & 0x100u) != 0;
     const int gf = p.temporal_radius == 0 ? 1 : (p.temporal_radius == 1 ? 2 : 4);
     const bool in_group = m < k_use_n;
-    const uint32_t pos = mw[m];
-    const uint32_t t = pos >> 26;
-    const int slot = t > 0 ? (int)nslots[t - 1] : (int)p.centre_slot;
-    const int mx = (int)(pos & 0x1FFF), my = (int)((pos >> 13) & 0x1FFF);
-
-    // Rows 2 s + h of the member's patch: 8 C samples each.
-    const size_t px = (size_t)W * H;
-    const float *src = values + (size_t)slot * px * C + ((size_t)(my + h) * W + mx) * C;
-    float4u_t raw[4][2 * C];
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int k = 0; k < 2 * C; ++k) raw[s][k] = *reinterpret_cast<const float4u_t *>(src + (size_t)2 * s * W * C + 4 * k);
 
     // The Haar blocks.  A member that passes L active levels of its reference's plan leaves the
     // forward transform as f^L (f = 1/sqrt 2) times a sum of its group's coefficients with signs
@@ -1905,6 +1875,9 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
     // with factors 1 and 1/2), which sit in K step jn / 2 at slots 4 (jn % 2) + k of the step-2 A
     // and step-4 B operands.
     half8_t hfrag[2], gfrag[2];
+    // f^L of member 4 h + i of reference j (the rows this lane holds after step 2), from the lane
+    // of that member.
+    float row_scale[4][4];
     {
         const HaarPlan plan = haar_plan(m, k_use_n, grid, gf);
         float hs[4], hg[4];
@@ -1924,12 +1897,6 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
                 hfrag[s][i] = (_Float16)(mine ? hs[i & 3] : 0.0f);
                 gfrag[s][i] = (_Float16)(mine ? hg[i & 3] : 0.0f);
             }
-    }
-    // f^L of member 4 h + i of reference j (the rows this lane holds after step 2), from the lane
-    // of that member.
-    float row_scale[4][4];
-    {
-        const HaarPlan plan = haar_plan(m, k_use_n, grid, gf);
         const int L = (plan.levels > 0 && plan.act[0]) + (plan.levels > 1 && plan.act[1]) +
                       (plan.levels > 2 && plan.act[2]);
         const float own = ((L & 1) ? 0.70710678118654752440f : 1.0f) * (L >= 2 ? 0.5f : 1.0f);
@@ -1942,11 +1909,6 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         // ---- 1. 2-D DCT: X1 tile mt holds coefficients 32 mt + n of members 8 j + 4 h + i.
-        half8_t pa[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int i = 0; i < 8; ++i) pa[s][i] = (_Float16)raw[s][(i * C + c) >> 2][(i * C + c) & 3];
         float16_t x1[2], x2[2];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
@@ -1954,7 +1916,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
 #pragma unroll
             for (int s = 0; s < 4; ++s)
                 x1[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
-                    pa[s], __builtin_bit_cast(half8_t, s_frag[2 * s + mt][lane]), x1[mt], 0, 0, 0);
+                    pa[c][s], __builtin_bit_cast(half8_t, s_frag[2 * s + mt][lane]), x1[mt], 0, 0, 0);
         }
         // ---- 2. group Haar (S).
 #pragma unroll
@@ -2030,16 +1992,51 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
                 }
         }
     }
+}
 
-    // Enter each member in the list of every tile its patch overlaps (claims 2h, 2h + 1 of member
-    // n), counted per tile in an LDS hash table as in k_collab_filter.
+// One group of four references per wave.  The workgroup first enters each member in the list of
+// every tile its patch overlaps (lane (n, h) holds claims 2 h, 2 h + 1 of member n): its claims are
+// counted per tile in an LDS hash table (rank = the bucket's count before the claim) and one
+// atomic per distinct tile reserves the entries.  Those atomics, the sample loads and the table
+// copy are then in flight together, and the reserved positions are only read after the filter,
+// when the list entries are written (a full list sends the (member, tile) pair to the overflow
+// list).
+template <int C>
+__global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_pass p, const float *__restrict__ values,
+                                                                     const uint32_t *__restrict__ nslots,
+                                                                     const uint32_t *__restrict__ members,
+                                                                     float *__restrict__ group_weight, TileBins tiles)
+{
     constexpr int kBuckets = kWaves * 256;
-    __shared__ uint32_t s_key[kBuckets], s_cnt[kBuckets], s_base[kBuckets];
+    __shared__ uint4 s_frag[16][64];
+    __shared__ float s_cvs[C][64], s_thr[C][64];
+    __shared__ uint32_t s_key[kBuckets], s_cnt[kBuckets];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int n = lane & 31, h = lane >> 5, m = n & 7, jn = n >> 3;
+    const int W = p.width, H = p.height;
+    const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
+    const int refs = refs_x * refs_y;
+    const int first = (xcd_block() * kWaves + wave) * 4;  // the wave's first reference
     for (int i = threadIdx.x; i < kBuckets; i += kWaves * 64) {
         s_key[i] = ~0u;
         s_cnt[i] = 0;
     }
+    int k_use[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) k_use[j] = (int)(members[(size_t)min(first + j, refs - 1) * 9 + 8] & 0xFF);
+    const uint32_t ref = (uint32_t)min(first + jn, refs - 1);
+    const bool live = first + jn < refs;
+    const uint32_t *mw = members + (size_t)ref * 9;
+    const uint32_t gword = mw[8], pos = mw[m];
+    const bool in_group = m < (int)(gword & 0xFF);
+    // The ring slot of trajectory index t (0: the centre frame) from the 2 r neighbour slots.
+    const uint32_t t = pos >> 26;
+    uint32_t slot = p.centre_slot;
+    for (int k = 0; k < 2 * (int)p.temporal_radius; ++k) slot = t == (uint32_t)(k + 1) ? nslots[k] : slot;
+    const int mx = (int)(pos & 0x1FFF), my = (int)((pos >> 13) & 0x1FFF);
     __syncthreads();
+
+    // ---- claims: hash, then one reserving atomic per distinct tile.
     const int tx0 = mx >> kTileShift, ty0 = my >> kTileShift;
     const int nx = ((mx + 7) >> kTileShift) - tx0 + 1, ny = ((my + 7) >> kTileShift) - ty0 + 1;
     uint32_t tile[2] = {~0u, ~0u}, bucket[2] = {0, 0}, rank[2] = {0, 0};
@@ -2047,7 +2044,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
     for (int k = 0; k < 2; ++k) {
         const int q = 2 * h + k;
         if (live && in_group && q < nx * ny) {
-            tile[k] = (uint32_t)slot * tiles.per_slot + (ty0 + q / nx) * tiles.tiles_x + tx0 + q % nx;
+            tile[k] = slot * tiles.per_slot + (ty0 + q / nx) * tiles.tiles_x + tx0 + q % nx;
             uint32_t bk = (tile[k] * 2654435761u) >> 22;
             for (;;) {  // at most 512 keys in 1024 buckets: a free or equal bucket exists
                 const uint32_t old = atomicCAS(&s_key[bk], ~0u, tile[k]);
@@ -2059,14 +2056,47 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kBuckets; i += kWaves * 64)
-        if (s_key[i] != ~0u) s_base[i] = atomicAdd(&tiles.count[s_key[i]], s_cnt[i]);
+    uint32_t base[kBuckets / (kWaves * 64)];  // reserved positions of this thread's buckets
+#pragma unroll
+    for (int k = 0; k < kBuckets / (kWaves * 64); ++k) {
+        const int i = (int)threadIdx.x + k * kWaves * 64;
+        base[k] = s_key[i] != ~0u ? atomicAdd(&tiles.count[s_key[i]], s_cnt[i]) : 0u;
+    }
+
+    // ---- samples: rows 2 s + h of the member's patch, 8 C values each; the tables.
+    const float *src = values + (size_t)slot * W * H * C + ((size_t)(my + h) * W + mx) * C;
+    float4u_t raw[4][2 * C];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int k = 0; k < 2 * C; ++k) raw[s][k] = *reinterpret_cast<const float4u_t *>(src + (size_t)2 * s * W * C + 4 * k);
+    for (int i = threadIdx.x; i < 16 * 64; i += kWaves * 64) s_frag[i >> 6][i & 63] = g_mixed_frags[i >> 6][i & 63];
+    if (threadIdx.x < 64 * C) {
+        const int c = threadIdx.x >> 6, v = (threadIdx.x >> 3) & 7, u = threadIdx.x & 7;
+        const float cvs = (p.sigma[c] * p.sigma[c]) * (p.profile[u] * p.profile[v]);
+        s_cvs[c][v * 8 + u] = cvs;
+        s_thr[c][v * 8 + u] = p.lambda[c] * sqrtf(cvs);
+    }
+    __syncthreads();
+
+    half8_t pa[C][4];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) pa[c][s][i] = (_Float16)raw[s][(i * C + c) >> 2][(i * C + c) & 3];
+    mixed_group<C>(p, s_frag, s_cvs, s_thr, pa, first, refs, k_use, gword, lane, group_weight, tiles);
+
+    // ---- list entries, at the reserved positions (s_cnt now holds the bases).
+#pragma unroll
+    for (int k = 0; k < kBuckets / (kWaves * 64); ++k) s_cnt[threadIdx.x + k * kWaves * 64] = base[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         if (tile[k] == ~0u) continue;
-        const uint32_t e = s_base[bucket[k]] + rank[k];
-        const uint32_t id = ((uint32_t)ref << 3) | (uint32_t)m;
+        const uint32_t e = s_cnt[bucket[k]] + rank[k];
+        const uint32_t id = (ref << 3) | (uint32_t)m;
         if (e < kTileCap) {
             tiles.list[(size_t)tile[k] * kTileCap + e] = id;
         } else {
@@ -2090,67 +2120,80 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
                                                      int32_t *__restrict__ accum_second)
 {
     constexpr int T = 1 << kTileShift;
-    __shared__ unsigned long long s_agg[T * T];
-    __shared__ int32_t s_second[C == 2 ? T * T : 1];
+    // Rows padded to 40 cells: the eight rows of a patch then fall on distinct banks (two passes
+    // per 64-bit add, the minimum, instead of eight).
+    constexpr int kRow = T + 8;
+    __shared__ unsigned long long s_agg[T * kRow];
+    __shared__ int32_t s_second[C == 2 ? T * kRow : 1];
     __shared__ uint32_t s_id[256], s_pos[256];
     __shared__ float s_sw[256];
     const uint32_t tile = blockIdx.x;
     const uint32_t slot = tile / tiles.per_slot, in_slot = tile - slot * tiles.per_slot;
     const int ty = (int)(in_slot / tiles.tiles_x), tx = (int)(in_slot - ty * tiles.tiles_x);
     const int x0 = tx * T, y0 = ty * T;
-    for (int i = threadIdx.x; i < T * T; i += 256) {
+    for (int i = threadIdx.x; i < T * kRow; i += 256) {
         s_agg[i] = 0;
         if (C == 2) s_second[i] = 0;
     }
-    const uint32_t n = min(tiles.count[tile], (uint32_t)kTileCap);
     const uint32_t *list = tiles.list + (size_t)tile * kTileCap;
+    // The first entries are read with the count (a list always holds kTileCap entries, so the read
+    // stays in bounds; entries past the count are ignored).
+    const uint32_t id0 = list[threadIdx.x];
+    const uint32_t n = min(tiles.count[tile], (uint32_t)kTileCap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane >> 3, q = lane & 7;
     const float kr_kq = p.kaiser[r] * p.kaiser[q];
     const float scale = p.accum_scale;
+    const float kk_scale = kr_kq * scale;  // MIXED: the window and the fixed-point scale in one factor
     for (uint32_t base = 0; base < n; base += 256) {
         __syncthreads();
         const uint32_t k = base + threadIdx.x;
         if (k < n) {
-            const uint32_t id = list[k];
+            const uint32_t id = base == 0 ? id0 : list[k];
             s_id[threadIdx.x] = id;
             s_pos[threadIdx.x] = members[(size_t)(id >> 3) * 9 + (id & 7)];
             s_sw[threadIdx.x] = group_weight[id >> 3] * p.weight_scale;
         }
         __syncthreads();
         const uint32_t batch = min(256u, n - base);
-        // Four members per wave per step: every staged load is issued before the first add.
-        for (uint32_t e0 = wave; e0 < batch; e0 += 16) {
-            int32_t a[4], a2[4];
-            int cell[4];
-            float w[4];
+        // kPer members per wave per step: every staged load is issued before the first add.
+        constexpr int kPer = 8;
+        for (uint32_t e0 = wave; e0 < batch; e0 += 4 * kPer) {
+            int32_t a[kPer], a2[kPer], wq[kPer];
+            int cell[kPer];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kPer; ++u) {
                 const uint32_t e = e0 + 4 * u;
                 cell[u] = -1;
                 if (e < batch) {
                     const uint32_t id = s_id[e], pos = s_pos[e];
                     const int lx = (int)(pos & 0x1FFF) + q - x0, ly = (int)((pos >> 13) & 0x1FFF) + r - y0;
                     if (lx >= 0 && lx < T && ly >= 0 && ly < T) {
-                        cell[u] = ly * T + lx;
-                        w[u] = s_sw[e] * kr_kq;
+                        cell[u] = ly * kRow + lx;
                         if (MIXED) {
+                            // The FP32 form's weight and clamp with round-to-nearest-even: to_fixed
+                            // (v w) = rint(clamp(v w scale, +-5 scale)), and the weight term
+                            // rint(min(w scale, scale) 5), with w = sw kaiser[r] kaiser[q].
                             const _Float16 *st =
                                 reinterpret_cast<const _Float16 *>(tiles.stage) + (size_t)id * C * 64 + lane;
-                            a[u] = to_fixed((float)st[0] * w[u], scale);
-                            if (C == 2) a2[u] = to_fixed((float)st[64] * w[u], scale);
+                            const float ws = s_sw[e] * kk_scale;
+                            const float lim = 5.0f * scale;
+                            a[u] = (int32_t)rintf(fminf(fmaxf((float)st[0] * ws, -lim), lim));
+                            if (C == 2) a2[u] = (int32_t)rintf(fminf(fmaxf((float)st[64] * ws, -lim), lim));
+                            wq[u] = (int32_t)rintf(fminf(ws, scale) * 5.0f);
                         } else {
                             const int32_t *st = tiles.stage + (size_t)id * C * 64 + lane;
                             a[u] = st[0];
                             if (C == 2) a2[u] = st[64];
+                            wq[u] = to_fixed_weight(s_sw[e] * kr_kq, scale);
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kPer; ++u) {
                 if (cell[u] < 0) continue;
-                atomicAdd(&s_agg[cell[u]], pack_aggregate(a[u], to_fixed_weight(w[u], scale)));
+                atomicAdd(&s_agg[cell[u]], pack_aggregate(a[u], wq[u]));
                 if (C == 2) atomicAdd(&s_second[cell[u]], a2[u]);
             }
         }
@@ -2161,9 +2204,12 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
         const int lx = i & (T - 1), ly = i >> kTileShift;
         const int x = x0 + lx, y = y0 + ly;
         if (x >= (int)p.width || y >= (int)p.height) continue;
+        // The only writer of these pixels this pass: the adds need no result, so they are atomics
+        // (the L2 does the read-modify-write and the workgroup does not wait for it).
         const size_t pix = (size_t)slot * px + (size_t)y * p.width + x;
-        if (s_agg[i]) aggregate[pix] += s_agg[i];
-        if (C == 2 && s_second[i]) accum_second[pix] += s_second[i];
+        const unsigned long long v = s_agg[ly * kRow + lx];
+        if (v) atomicAdd(&aggregate[pix], v);
+        if (C == 2 && s_second[ly * kRow + lx]) atomicAdd(&accum_second[pix], s_second[ly * kRow + lx]);
     }
 }
 
