@@ -142,9 +142,9 @@ typedef struct avd_nl4d_options {
     uint32_t step;
     uint32_t group_size;
     /* Filter transform form (no counterpart in the reference): 0, the FP32 form, bit-exact with
-     * the reference's f32 arithmetic (default); 1, the mixed form, the 2-D DCT and its inverse as
-     * f16 matrix products with f32 accumulation (hard threshold only; output within 0.05 dB PSNR
-     * of the FP32 form). */
+     * the reference's f32 arithmetic (default); 1, the mixed form, the 2-D DCT, the group Haar and
+     * their inverses as f16 matrix products with f32 accumulation (hard threshold only, so
+     * pooled_threshold must be 0; output within 0.05 dB PSNR of the FP32 form). */
     uint32_t transform_form;
 } avd_nl4d_options;
 
