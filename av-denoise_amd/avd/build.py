@@ -32,6 +32,7 @@ SOURCES = {
     "motion.hip": EXACT,
     "collab.hip": EXACT + MFMA_VGPR,
     "nlm.hip": [],
+    "noise.hip": EXACT,
     "engine.cpp": EXACT,
     "abi.cpp": EXACT,
 }

@@ -1,5 +1,6 @@
 // The extern "C" boundary: status codes, thread-local error messages, defaults and presets, the
 // engine entry points and the per-stage launch sites.  No exception crosses this file.
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -290,6 +291,25 @@ avd_status avd_site_collab_normalise(uint64_t *aggregate_region, int32_t *accum_
                                                    accum_second_region, out, pixels, (int)channels,
                                                    static_cast<hipStream_t>(stream)),
                       "collab_normalise");
+}
+
+avd_status avd_site_noise_spatial(const float *ring_values, uint32_t width, uint32_t height, uint32_t channels,
+                                  uint32_t frame, float *partials, float *results, uint32_t slot, void *stream)
+{
+    if (!ring_values || !partials || !results) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (width == 0 || height == 0 || (channels != 1 && channels != 2))
+        return fail(AVD_ERR_INVALID_GEOMETRY, "noise measurement needs a non-empty frame of 1 or 2 channels");
+    return hip_status(avd::launch_noise_spatial(ring_values, width, height, channels, frame, partials, results, slot,
+                                                static_cast<hipStream_t>(stream)),
+                      "noise_spatial");
+}
+
+uint32_t avd_noise_partials_len(uint32_t width, uint32_t height) { return avd::noise_partials_len(width, height); }
+
+float avd_noise_sigma_from_abs_sum(float abs_sum, uint32_t width, uint32_t height)
+{
+    const float interior = (float)(width - 2) * (float)(height - 2);
+    return std::sqrt(1.57079632679489661923f) * abs_sum / (6.0f * interior);
 }
 
 avd_status avd_site_motion(const avd_motion_pass *p, const uint16_t *ring_codes, const uint32_t *neighbour_slots,

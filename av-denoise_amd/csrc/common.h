@@ -86,6 +86,11 @@ hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes,
 hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
                                 const uint32_t *members, unsigned long long *aggregate, int32_t *accum_second,
                                 float *group_weight, hipStream_t s);
+// Spatial noise measurement (noise.hip): the Immerkaer partials of frame `frame` of a ring of f32
+// [frame][H][W][C] samples, then their per-channel totals into row `slot` of `results`.
+uint32_t noise_partials_len(uint32_t width, uint32_t height);
+hipError_t launch_noise_spatial(const float *ring, uint32_t width, uint32_t height, uint32_t channels,
+                                uint32_t frame, float *partials, float *results, uint32_t slot, hipStream_t s);
 hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,
                                    int C, hipStream_t s);
 

@@ -124,6 +124,42 @@ def quality(avd, torch, frames, clean, form, first=4, count=4):
     return [10.0 * math.log10(peak * peak / (sq[k] / n[k])) for k in range(3)]
 
 
+HBM_PEAK_GBPS = 8000.0
+
+
+def noise_spatial_measure(avd, torch, clip, calls=20):
+    """The spatial noise measurement (both Immerkaer stages) on the clip's first 4K luma frame as
+    f32 samples in HBM: average time per call from HIP events on torch's stream (the one the
+    calls run on), against the HBM roofline of one read of the frame."""
+    import ctypes
+
+    lib = avd._lib.load()
+    frame = (clip[0][0].to(torch.float32) / float((1 << DEPTH) - 1)).contiguous()
+    partials = torch.empty(lib.avd_noise_partials_len(W, H), dtype=torch.float32, device="cuda")
+    results = torch.empty(2, dtype=torch.float32, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def call():
+        avd._lib.check(lib.avd_site_noise_spatial(ctypes.c_void_p(frame.data_ptr()), W, H, 1, 0,
+                                                  ctypes.c_void_p(partials.data_ptr()),
+                                                  ctypes.c_void_p(results.data_ptr()), 0, stream))
+
+    for _ in range(3):
+        call()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(calls):
+        call()
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / calls
+    nbytes = W * H * 4
+    gbps = nbytes / (ms * 1e-3) / 1e9
+    return {"kernels": "k_noise_partial + k_noise_reduce", "workload": "one 3840x2160 luma frame, f32 samples",
+            "avg_launch_ms": round(ms, 4), "bytes_per_launch": nbytes, "achieved": round(gbps, 1),
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+
+
 def run_steps(eng, frames, outs, start, count):
     done = 0
     for i in range(count):
@@ -284,6 +320,7 @@ def main():
         "dominant_stage": dominant,
         "forms": {f: {"frames_per_s": forms[f]["value"], "psnr_db_vs_clean": psnr.get(f)} for f in forms},
     }
+    result["next_rows"] = {"noise_spatial": noise_spatial_measure(avd, torch, clip)}
     if "mixed" in forms and "fp32" in forms:
         result["mixed_vs_fp32"] = round(forms["mixed"]["value"] / forms["fp32"]["value"], 4)
         if psnr:

@@ -338,6 +338,20 @@ avd_status avd_site_nlm_pair(const avd_nlm_window *w, const float *centre, const
 avd_status avd_site_nlm_finish(const float *frame, const float *accum, const float *wsum, const float *maxw,
                                float self_weight, uint32_t pixels, uint32_t channels, float *out, void *stream);
 
+/* Spatial noise measurement: both stages of the Immerkaer estimate of frame `frame` of a ring of
+ * f32 [frame][height][width][channels] samples (run_noise_estimate, noise/spatial.rs:93-125, with
+ * NoiseCtx's fields; kernels nlm_noise_partial / nlm_noise_reduce, noise/kernels/stats.rs:24-140).
+ * `partials` holds avd_noise_partials_len(width, height) floats (two per 32x8 tile); the two
+ * per-channel sums of absolute mask responses land in results[2 slot], results[2 slot + 1] (lane 1
+ * is 0 for one channel), bit-identical to the reference's f32 sums. */
+avd_status avd_site_noise_spatial(const float *ring_values, uint32_t width, uint32_t height, uint32_t channels,
+                                  uint32_t frame, float *partials, float *results, uint32_t slot, void *stream);
+/* partials_len (noise/spatial.rs:31-34): floats of one frame's partials. */
+uint32_t avd_noise_partials_len(uint32_t width, uint32_t height);
+/* sigma_from_abs_sum (noise/spatial.rs:134-138): sqrt(pi / 2) sum / (6 (width - 2)(height - 2)),
+ * the Immerkaer sigma of a frame-wide sum (in the samples' normalised units). */
+float avd_noise_sigma_from_abs_sum(float abs_sum, uint32_t width, uint32_t height);
+
 #ifdef __cplusplus
 }
 #endif

@@ -914,3 +914,66 @@ void avdo_dct8_inv_f32(float *line)
 
 void avdo_group_fwd_f32(float *stk, int k_use, int grid, int gf) { group_fwd_f32(stk, k_use, grid, gf, 0.70710678118654752440f); }
 void avdo_group_inv_f32(float *stk, int k_use, int grid, int gf) { group_inv_f32(stk, k_use, grid, gf, 0.70710678118654752440f); }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Spatial noise measurement (the Immerkaer estimate), noise/kernels/stats.rs:24-140.         */
+/* ------------------------------------------------------------------------------------------ */
+
+/* nlm_noise_partial (stats.rs:31-99): per 32x8 tile, the sum over the tile's threads in thread
+ * order (row * 32 + column) of |4 c - 2 (l + r + u + d) + (ul + ur + dl + dr)| per channel;
+ * border pixels, pixels past the frame and lanes past the channel count add zero. */
+void avdo_noise_partial(const float *ring, uint32_t W, uint32_t H, uint32_t C, uint32_t frame, float *partials)
+{
+    const uint32_t bx = 32, by = 8;
+    const uint32_t cx = (W + bx - 1) / bx, cy = (H + by - 1) / by;
+    const float *f = ring + (size_t)frame * W * H * C;
+    for (uint32_t ty = 0; ty < cy; ++ty)
+        for (uint32_t tx = 0; tx < cx; ++tx) {
+            float sum[2] = {0.0f, 0.0f};
+            for (uint32_t t = 0; t < bx * by; ++t) {
+                const uint32_t x = tx * bx + t % bx, y = ty * by + t / bx;
+                const int interior = x >= 1 && x + 1 < W && y >= 1 && y + 1 < H;
+                for (uint32_t lane = 0; lane < 2; ++lane) {
+                    float v = 0.0f;
+                    if (interior && lane < C) {
+#define AT(xx, yy) f[((size_t)(yy) * W + (xx)) * C + lane]
+                        const float centre = AT(x, y), left = AT(x - 1, y), right = AT(x + 1, y);
+                        const float up = AT(x, y - 1), down = AT(x, y + 1);
+                        const float ul = AT(x - 1, y - 1), ur = AT(x + 1, y - 1), dl = AT(x - 1, y + 1),
+                                    dr = AT(x + 1, y + 1);
+#undef AT
+                        const float a = centre * 4.0f;
+                        const float b = (((left + right) + up) + down) * 2.0f;
+                        const float d = ((ul + ur) + dl) + dr;
+                        v = fabsf((a - b) + d);
+                    }
+                    sum[lane] += v;
+                }
+            }
+            partials[((size_t)ty * cx + tx) * 2] = sum[0];
+            partials[((size_t)ty * cx + tx) * 2 + 1] = sum[1];
+        }
+}
+
+/* nlm_noise_reduce (stats.rs:101-140): thread t (of 256) adds partials t, t + 256, ... of each
+ * lane in turn; each lane's total then adds the 256 thread sums in thread order. */
+void avdo_noise_reduce(const float *partials, uint32_t count, float *results, uint32_t slot)
+{
+    float s0[256], s1[256];
+    for (uint32_t t = 0; t < 256; ++t) {
+        float a = 0.0f, b = 0.0f;
+        for (uint32_t i = t; i < count; i += 256) {
+            a += partials[(size_t)i * 2];
+            b += partials[(size_t)i * 2 + 1];
+        }
+        s0[t] = a;
+        s1[t] = b;
+    }
+    float r0 = 0.0f, r1 = 0.0f;
+    for (uint32_t t = 0; t < 256; ++t) {
+        r0 += s0[t];
+        r1 += s1[t];
+    }
+    results[(size_t)slot * 2] = r0;
+    results[(size_t)slot * 2 + 1] = r1;
+}

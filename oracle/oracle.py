@@ -13,7 +13,8 @@ It wraps ``liboracle.so`` (``avd_oracle.c``) and adds, in plain Python/numpy:
   ``nlmeans/dispatch/mod.rs:653-679``) and NL4D with shifted edges and head/middle/tail passes
   (``nl4d/denoiser.rs:370-600``, ``nl4d/engine.rs:264-360``, ``nl4d/collab_pass.rs:50-200``).
 
-Parity status of each function is stated in DESIGN.md's oracle table.
+Each function cites the reference file:line it restates; the tests under ``tests/`` pin them
+against the reference's golden vectors and known answers.
 """
 from __future__ import annotations
 
@@ -720,3 +721,86 @@ def run_stream(pass_obj, frames_words):
         outputs.append(pass_obj.out.pop(0))
     px = pass_obj.W * pass_obj.H
     return [egress(f, px, pass_obj.C, pass_obj.depth_out) for f in outputs], outputs
+
+
+# ---------------------------------------------------------------------------------------------
+# Spatial noise measurement (the Immerkaer estimate)
+# ---------------------------------------------------------------------------------------------
+
+IMMERKAER_BLOCK_X, IMMERKAER_BLOCK_Y, IMMERKAER_LANES = 32, 8, 2  # noise/mod.rs:114-122
+
+
+def noise_partials_len(width: int, height: int) -> int:
+    """partials_len (noise/spatial.rs:31-34): two floats per 32x8 tile."""
+    return -(-width // IMMERKAER_BLOCK_X) * -(-height // IMMERKAER_BLOCK_Y) * IMMERKAER_LANES
+
+
+def noise_partial(ring: np.ndarray, width: int, height: int, channels: int, frame: int = 0) -> np.ndarray:
+    """nlm_noise_partial (noise/kernels/stats.rs:31-99) over frame ``frame`` of an f32 ring
+    [frame][H][W][C], in the reference's summation order (avdo_noise_partial)."""
+    ring = np.ascontiguousarray(ring, np.float32).reshape(-1)
+    out = np.zeros(noise_partials_len(width, height), np.float32)
+    lib().avdo_noise_partial(_p(ring, F32P), ctypes.c_uint32(width), ctypes.c_uint32(height),
+                             ctypes.c_uint32(channels), ctypes.c_uint32(frame), _p(out, F32P))
+    return out
+
+
+def noise_reduce(partials: np.ndarray, results: np.ndarray | None = None, slot: int = 0) -> np.ndarray:
+    """nlm_noise_reduce (noise/kernels/stats.rs:101-140): the two per-channel totals into row
+    ``slot`` of ``results`` (avdo_noise_reduce)."""
+    partials = np.ascontiguousarray(partials, np.float32)
+    if results is None:
+        results = np.zeros(2 * (slot + 1), np.float32)
+    lib().avdo_noise_reduce(_p(partials, F32P), ctypes.c_uint32(partials.size // 2), _p(results, F32P),
+                            ctypes.c_uint32(slot))
+    return results
+
+
+def immerkaer_sigma_scale() -> np.float32:
+    """sqrt(pi / 2) in f32 (noise/spatial.rs:127-130)."""
+    return np.sqrt(np.float32(math.pi / 2))
+
+
+def sigma_from_abs_sum(abs_sum: float, width: int, height: int) -> np.float32:
+    """sigma_from_abs_sum (noise/spatial.rs:132-138), f32 arithmetic."""
+    interior = np.float32(width - 2) * np.float32(height - 2)
+    return immerkaer_sigma_scale() * np.float32(abs_sum) / (np.float32(6.0) * interior)
+
+
+def lower_quartile(sorted_values) -> np.float32:
+    """lower_quartile (noise/stats.rs:20-38) of a sorted, non-empty sequence: the quarter point,
+    interpolated between its neighbours, in f32."""
+    v = [np.float32(x) for x in sorted_values]
+    if len(v) == 1:
+        return v[0]
+    position = np.float32(0.25) * np.float32(len(v) - 1)
+    lo, hi = int(math.floor(position)), int(math.ceil(position))
+    if lo == hi:
+        return v[lo]
+    fraction = position - np.float32(lo)
+    return v[lo] + fraction * (v[hi] - v[lo])
+
+
+def sigma_block_p25_from_partials(partials, channels: int, width: int, height: int) -> np.ndarray:
+    """sigma_block_p25_from_partials (noise/spatial.rs:140-200): each tile's sigma over its overlap
+    with the frame interior (tiles with none skipped), then the lower quartile per channel."""
+    cx, cy = -(-width // IMMERKAER_BLOCK_X), -(-height // IMMERKAER_BLOCK_Y)
+    scale = immerkaer_sigma_scale()
+    sigmas = [[] for _ in range(channels)]
+    for ty in range(cy):
+        y0, y1 = ty * IMMERKAER_BLOCK_Y, min((ty + 1) * IMMERKAER_BLOCK_Y, height)
+        oy0, oy1 = max(y0, 1), min(y1, height - 1)
+        for tx in range(cx):
+            x0, x1 = tx * IMMERKAER_BLOCK_X, min((tx + 1) * IMMERKAER_BLOCK_X, width)
+            ox0, ox1 = max(x0, 1), min(x1, width - 1)
+            if ox1 <= ox0 or oy1 <= oy0:
+                continue
+            area = np.float32((ox1 - ox0) * (oy1 - oy0))
+            base = (ty * cx + tx) * IMMERKAER_LANES
+            for c in range(channels):
+                sigmas[c].append(scale * np.float32(partials[base + c]) / (np.float32(6.0) * area))
+    out = np.zeros(IMMERKAER_LANES, np.float32)
+    for c in range(channels):
+        if sigmas[c]:
+            out[c] = lower_quartile(sorted(sigmas[c]))
+    return out
