@@ -77,6 +77,8 @@ def geom(chroma=0, fmt_in=None, fmt_out=None, w=64, h=48):
     ("nl4d", dict(sigma=0.02, pooled_threshold=0), dict(sigma=0.02), geom(0), avd.InvalidGeometry),  # gray chroma
     ("nl4d", dict(sigma=0.02, pooled_threshold=0), None, geom(0, avd.SampleFormat.u16(16)), avd.InvalidGeometry),
     ("nl4d", dict(sigma=0.02, pooled_threshold=0), None, geom(0, w=7, h=7), avd.InvalidGeometry),
+    ("nl4d", dict(sigma=0.02, pooled_threshold=0, transform_form=2), None, geom(), avd.InvalidOptions),
+    ("nl4d", dict(sigma=0.02, pooled_threshold=1, transform_form=1), None, geom(), avd.InvalidOptions),  # mixed: hard only
     ("nlmeans", dict(patch_radius=17), None, geom(), avd.InvalidOptions),
     ("nlmeans", dict(search_radius=9), None, geom(), avd.InvalidOptions),
     ("nlmeans", dict(hq=1), None, geom(), avd.InvalidOptions),  # HQ needs a fixed sigma here
@@ -92,3 +94,15 @@ def test_engine_validation_errors(kind, luma, chroma, g, err):
 def test_a_spec_must_clean_something():
     with pytest.raises(avd.InvalidOptions):
         avd.build_engine("nlmeans", geom(1), luma=None, chroma=None)
+
+
+def test_noise_site_rejects_bad_arguments_before_any_launch():
+    import ctypes
+
+    from avd import _lib as L
+
+    lib = L.load()
+    buf = ctypes.c_void_p(1)  # never dereferenced: validation fails first
+    assert lib.avd_site_noise_spatial(None, 64, 48, 1, 0, buf, buf, 0, None) == avd.NullArgument.status
+    assert lib.avd_site_noise_spatial(buf, 64, 48, 3, 0, buf, buf, 0, None) == avd.InvalidGeometry.status
+    assert lib.avd_site_noise_spatial(buf, 0, 48, 1, 0, buf, buf, 0, None) == avd.InvalidGeometry.status
