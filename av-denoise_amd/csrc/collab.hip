@@ -29,6 +29,7 @@
 // approximation in both layouts, so every kept coefficient and every output value is identical.
 #include "common.h"
 #include <cstring>
+#include <mutex>
 
 namespace avd {
 
@@ -1803,10 +1804,11 @@ __device__ __forceinline__ void member_row(const avd_collab_pass &p, const uint3
 // its rows: K slot i' of lane half h in K step s is row 16 s + 8 (i' / 4) + 4 h + i' % 4, and the
 // tables (T, Ti, H, Hi) are laid out in that order.  No lane exchanges the transforms.  The Haar
 // blocks are the linear maps of each reference's Haar plan (the FP32 form's levels, applied to
-// unit vectors).  The samples, the tables and each product's inputs round to f16 (2^-11
-// relative), which moves PSNR by a few thousandths of a dB against the FP32 form; the threshold
-// and the retained variance follow the FP32 form's rules, summed in another order.  Hard
-// threshold only.  The filtered samples are staged as f16 (half the FP32 form's bytes); the
+// unit vectors), split into exact signs and an f32 scale (below).  Samples of up to 11-bit codes
+// are exact in f16; the DCT tables and the inverse DCT's input round to f16 (2^-11 relative) and
+// the Haar's inputs enter as hi + lo pairs, which keeps PSNR within a thousandth of a dB of the
+// FP32 form; the threshold and the retained variance follow the FP32 form's rules, summed in
+// another order.  Hard threshold only.  The filtered samples are staged as f16 (half the FP32 form's bytes); the
 // gather applies the window and the group weight in fixed point.
 // ---------------------------------------------------------------------------------------------
 
@@ -1836,6 +1838,20 @@ __device__ __forceinline__ half8_t frag_from(const float (&acc)[16], int base)
     return f;
 }
 
+// f32 values as two f16 fragments, hi = f16(v) and lo = f16(v - hi): hi + lo holds v to about
+// 2^-22 relative, so a product with an exact f16 operand (the Haar signs, +-2^-L) keeps f32-like
+// precision in two matrix-core steps.
+template <typename V>
+__device__ __forceinline__ void split_frag(const V &acc, int base, half8_t &hi, half8_t &lo)
+{
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const _Float16 h = (_Float16)acc[base + i];
+        hi[i] = h;
+        lo[i] = (_Float16)(acc[base + i] - (float)h);
+    }
+}
+
 // One Haar butterfly level, scaled by f, applied to four unit-vector columns held across the
 // member lanes (a reference's members are 8 consecutive lanes; partners are XOR apart).
 template <int XOR>
@@ -1854,9 +1870,9 @@ __device__ __forceinline__ void haar_columns(float (&v)[4], bool active, bool lo
 template <int C>
 __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint4 (*s_frag)[64],
                                             const float (*s_cvs)[64], const float (*s_thr)[64],
-                                            const half8_t (&pa)[C][4], int first, int refs, const int (&k_use)[4],
-                                            uint32_t gword, int lane, float *__restrict__ group_weight,
-                                            const TileBins &tiles)
+                                            const half8_t (&pa)[C][4], float inv_s, int first, int refs,
+                                            const int (&k_use)[4], uint32_t gword, int lane,
+                                            float *__restrict__ group_weight, const TileBins &tiles)
 {
     const int n = lane & 31, h = lane >> 5, m = n & 7, jn = n >> 3;
     const int ref = min(first + jn, refs - 1);
@@ -1903,7 +1919,7 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) row_scale[j][i] = __shfl(own, 8 * j + 4 * h + i);
+            for (int i = 0; i < 4; ++i) row_scale[j][i] = __shfl(own, 8 * j + 4 * h + i) * inv_s;
     }
 
 #pragma unroll
@@ -1918,13 +1934,18 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
                 x1[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
                     pa[c][s], __builtin_bit_cast(half8_t, s_frag[2 * s + mt][lane]), x1[mt], 0, 0, 0);
         }
-        // ---- 2. group Haar (S).
+        // ---- 2. group Haar (S), on X1 split into hi + lo: the members' large DC terms would
+        // otherwise leave their f16 rounding in the small differences the Haar forms.
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             x2[mt] = float16_t{};
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
-                x2[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hfrag[s], frag_from(x1[mt], 8 * s), x2[mt], 0, 0, 0);
+            for (int s = 0; s < 2; ++s) {
+                half8_t hi, lo;
+                split_frag(x1[mt], 8 * s, hi, lo);
+                x2[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hfrag[s], hi, x2[mt], 0, 0, 0);
+                x2[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hfrag[s], lo, x2[mt], 0, 0, 0);
+            }
         }
         // ---- 3. hard threshold (the group DC always kept) and the retained variance of each
         // reference, from the first channel.  Groups hold 1, 2, 4 or 8 members and the Haar pairs
@@ -1959,14 +1980,18 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
                 if (lane == 0 && first + j < refs) group_weight[first + j] = weight;
             }
         }
-        // ---- 4. inverse Haar (S^T D^2): X3 tile mt holds coefficients 32 mt + 8 j + 4 h + i of member n.
+        // ---- 4. inverse Haar (S^T D^2, on hi + lo): X3 tile mt holds coefficients 32 mt + 8 j + 4 h + i of member n.
         float16_t x3[2];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             x3[mt] = float16_t{};
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
-                x3[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(frag_from(y[mt], 8 * s), gfrag[s], x3[mt], 0, 0, 0);
+            for (int s = 0; s < 2; ++s) {
+                half8_t hi, lo;
+                split_frag(y[mt], 8 * s, hi, lo);
+                x3[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi, gfrag[s], x3[mt], 0, 0, 0);
+                x3[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo, gfrag[s], x3[mt], 0, 0, 0);
+            }
         }
         // ---- 5. inverse DCT: X4 tile pt holds pixel (4 pt + j, 4 h + i) of member n.
         float16_t x4[2];
@@ -1987,7 +2012,7 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
                 for (int j = 0; j < 4; ++j) {
                     half4_t o;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) o[i] = (_Float16)x4[pt][4 * j + i];
+                    for (int i = 0; i < 4; ++i) o[i] = (_Float16)(x4[pt][4 * j + i] * inv_s);
                     *reinterpret_cast<half4_t *>(st + (4 * pt + j) * 8) = o;
                 }
         }
@@ -2079,14 +2104,23 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
     }
     __syncthreads();
 
+    // Samples of up to 11-bit codes enter exactly, as code 2^-bits = v s (s = (2^bits - 1) / 2^bits);
+    // the threshold and the output take 1 / s back in f32.  Deeper samples round to f16.
+    const bool exact = p.code_bits >= 1 && p.code_bits <= 11;
+    const float maxv = exact ? (float)((1u << p.code_bits) - 1u) : 1.0f;
+    const float pow2 = exact ? __builtin_ldexpf(1.0f, -(int)p.code_bits) : 1.0f;
+    const float inv_s = exact ? (float)(1u << p.code_bits) / maxv : 1.0f;
     half8_t pa[C][4];
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int i = 0; i < 8; ++i) pa[c][s][i] = (_Float16)raw[s][(i * C + c) >> 2][(i * C + c) & 3];
-    mixed_group<C>(p, s_frag, s_cvs, s_thr, pa, first, refs, k_use, gword, lane, group_weight, tiles);
+            for (int i = 0; i < 8; ++i) {
+                const float v = raw[s][(i * C + c) >> 2][(i * C + c) & 3];
+                pa[c][s][i] = (_Float16)(exact ? rintf(v * maxv) * pow2 : v);
+            }
+    mixed_group<C>(p, s_frag, s_cvs, s_thr, pa, inv_s, first, refs, k_use, gword, lane, group_weight, tiles);
 
     // ---- list entries, at the reserved positions (s_cnt now holds the bases).
 #pragma unroll
@@ -2293,14 +2327,16 @@ void mixed_frags(uint4 (&out)[16][64])
 
 hipError_t upload_taps(hipStream_t s)
 {
-    // Per device (the constants live in each device's copy of the code object); the engine and
-    // the site calls of one device run on one host thread at a time.
+    // Once per device (the constants live in each device's copy of the code object), under a lock:
+    // engines on several host threads may reach here together.
     constexpr int kMaxDevices = 64;
+    static std::mutex lock;
     static bool done[kMaxDevices] = {};
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(lock);
     if (done[dev]) return hipSuccess;
     DctTaps t;
     for (int m = 0; m < 4; ++m)
