@@ -373,11 +373,15 @@ def _psnr(a, b, peak):
 
 
 MIXED_CASES = [
-    # name, W, H, depth, temporal radius, S, frames, seed, luma / chroma sigma in codes
-    ("r2_96x64_10bit", 96, 64, 10, 2, 8, 7, 123, 16.0, 8.0),  # grid of 4 frames per volume
-    ("r1_70x46_8bit", 70, 46, 8, 1, 6, 6, 77, 5.0, 3.0),  # grid of 2, ragged odd chroma, 8-bit codes
-    ("r3_80x56_10bit", 80, 56, 10, 3, 8, 8, 31, 12.0, 6.0),
+    # name, W, H, depth, temporal radius, S, frames, seed, luma / chroma sigma in codes, chroma
+    ("r2_96x64_10bit", 96, 64, 10, 2, 8, 7, 123, 16.0, 8.0, "420"),  # grid of 4 frames per volume
+    ("r1_70x46_8bit", 70, 46, 8, 1, 6, 6, 77, 5.0, 3.0, "420"),  # grid of 2, ragged odd chroma, 8-bit
+    ("r3_80x56_10bit", 80, 56, 10, 3, 8, 8, 31, 12.0, 6.0, "420"),
+    ("r2_64x48_422", 64, 48, 10, 2, 7, 6, 5, 14.0, 7.0, "422"),
+    ("r1_56x40_444", 56, 40, 10, 1, 7, 5, 6, 14.0, 7.0, "444"),
+    ("r2_72x56_gray", 72, 56, 10, 2, 8, 6, 9, 16.0, 0.0, "gray"),
 ]
+LAYOUT = {"gray": 0, "420": 1, "422": 2, "444": 3}
 
 
 @pytest.mark.parametrize("case", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
@@ -389,20 +393,21 @@ def test_nl4d_mixed_transform_form_stays_within_0_05_db(case):
     also moves a sparse group's weight: at frame borders, where one to four groups cover a pixel,
     that shows as a few codes; without the hi/lo split of the Haar inputs 1.2 % of the r = 3 chroma
     samples moved more than 2 codes, with it 0.3 %.)"""
-    name, W, H, depth, r, S, frames, seed, sig_y, sig_c = case
+    name, W, H, depth, r, S, frames, seed, sig_y, sig_c, chroma = case
     print(f"mixed-form case {name}", flush=True)
     peak = float((1 << depth) - 1)
-    noisy = avd.synth.clip(frames, W, H, depth, seed, chroma="420", sigma_luma=sig_y, sigma_chroma=sig_c)
-    clean = avd.synth.clip(frames, W, H, depth, seed, chroma="420", sigma_luma=0.0, sigma_chroma=0.0)
+    noisy = avd.synth.clip(frames, W, H, depth, seed, chroma=chroma, sigma_luma=sig_y, sigma_chroma=sig_c)
+    clean = avd.synth.clip(frames, W, H, depth, seed, chroma=chroma, sigma_luma=0.0, sigma_chroma=0.0)
     fmt = L.SampleFormat.u8() if depth == 8 else L.SampleFormat.u16(depth)
-    geom = L.Geometry(W, H, 1, fmt, fmt, 0)
-    dims = [(W, H), ((W + 1) // 2, (H + 1) // 2), ((W + 1) // 2, (H + 1) // 2)]
+    geom = L.Geometry(W, H, LAYOUT[chroma], fmt, fmt, 0)
+    cd = avd.chroma_dims(LAYOUT[chroma], W, H)
+    dims = [(W, H)] + ([cd, cd] if cd else [])
     outs = {}
     for form in (0, 1):
         common = dict(temporal_radius=r, spatial_radius=S, lambda_ht=2.7, pooled_threshold=0, noise_map=0,
                       motion={"estimation": 1}, transform_form=form)
         eng = avd.build_engine("nl4d", geom, luma=avd.nl4d_options(sigma=sig_y / peak, **common),
-                               chroma=avd.nl4d_options(sigma=sig_c / peak, **common))
+                               chroma=avd.nl4d_options(sigma=sig_c / peak, **common) if cd else None)
         outs[form] = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f] for f in noisy], depth, dims)
     pl = O.Nl4dPass(_nl4d_spec(r, S, sig_y / peak), W, H, 1, depth, depth)
     want_l, _ = O.run_stream(pl, [[O.pack_plane(f[0].numpy(), depth)] for f in noisy])
