@@ -1924,6 +1924,8 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
 
 #pragma unroll
     for (int c = 0; c < C; ++c) {
+        // One channel at a time: overlapping the two would hold both channels' tiles at once.
+        if (c > 0) __builtin_amdgcn_sched_barrier(0);
         // ---- 1. 2-D DCT: X1 tile mt holds coefficients 32 mt + n of members 8 j + 4 h + i.
         float16_t x1[2], x2[2];
 #pragma unroll
