@@ -8,6 +8,8 @@ It wraps ``liboracle.so`` (``avd_oracle.c``) and adds, in plain Python/numpy:
 * the host-side closed forms the reference computes once per engine
   (``nlmeans/params.rs:229-268``, ``noise/correlation.rs:43-88``, ``collab/kernels/aggregate.rs:36-125``,
   ``collab/geometry.rs:6-29``, ``motion/confidence.rs:17-29``);
+* the spatial noise measurement's host closed forms (``noise/spatial.rs:31-200``,
+  ``noise/stats.rs:20-38``) around the C restatement of its two kernels;
 * stream drivers that replay the reference's ring and pass scheduling:
   NLMeans with repeated edges (``nlmeans/denoiser/ring.rs``, ``nlmeans/engine.rs:201-262``,
   ``nlmeans/dispatch/mod.rs:653-679``) and NL4D with shifted edges and head/middle/tail passes
