@@ -2,6 +2,7 @@
 // engine entry points and the per-stage launch sites.  No exception crosses this file.
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 
@@ -308,6 +309,9 @@ uint32_t avd_noise_partials_len(uint32_t width, uint32_t height) { return avd::n
 
 float avd_noise_sigma_from_abs_sum(float abs_sum, uint32_t width, uint32_t height)
 {
+    // The 3x3 mask has no interior response below 3 samples on an axis (the reference's version
+    // underflows there); NaN says that no estimate exists.
+    if (width < 3 || height < 3) return std::numeric_limits<float>::quiet_NaN();
     const float interior = (float)(width - 2) * (float)(height - 2);
     return std::sqrt(1.57079632679489661923f) * abs_sum / (6.0f * interior);
 }

@@ -2207,16 +2207,16 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
                     if (lx >= 0 && lx < T && ly >= 0 && ly < T) {
                         cell[u] = ly * kRow + lx;
                         if (MIXED) {
-                            // The FP32 form's weight and clamp with round-to-nearest-even: to_fixed
-                            // (v w) = rint(clamp(v w scale, +-5 scale)), and the weight term
-                            // rint(min(w scale, scale) 5), with w = sw kaiser[r] kaiser[q].
+                            // The FP32 form's weight, clamp and rounding (half away from zero, as
+                            // f32::round): to_fixed(v w) = round(clamp(v w scale, +-5 scale)), and
+                            // the weight term round(min(w scale, scale) 5), w = sw kaiser[r] kaiser[q].
                             const _Float16 *st =
                                 reinterpret_cast<const _Float16 *>(tiles.stage) + (size_t)id * C * 64 + lane;
                             const float ws = s_sw[e] * kk_scale;
                             const float lim = 5.0f * scale;
-                            a[u] = (int32_t)rintf(fminf(fmaxf((float)st[0] * ws, -lim), lim));
-                            if (C == 2) a2[u] = (int32_t)rintf(fminf(fmaxf((float)st[64] * ws, -lim), lim));
-                            wq[u] = (int32_t)rintf(fminf(ws, scale) * 5.0f);
+                            a[u] = (int32_t)roundf(fminf(fmaxf((float)st[0] * ws, -lim), lim));
+                            if (C == 2) a2[u] = (int32_t)roundf(fminf(fmaxf((float)st[64] * ws, -lim), lim));
+                            wq[u] = (int32_t)roundf(fminf(ws, scale) * 5.0f);
                         } else {
                             const int32_t *st = tiles.stage + (size_t)id * C * 64 + lane;
                             a[u] = st[0];

@@ -333,6 +333,20 @@ void Ring::duplicate_last(hipStream_t s)
     ++head;
 }
 
+// MotionEstimation::resolve (motion/search.rs:118-140): Auto is Direct below temporal radius 3
+// and Chained from it on.  Only Direct is built, so a request that resolves to Chained is refused
+// rather than run as something else.
+static void check_estimation(const avd_motion_search &m, uint32_t temporal_radius)
+{
+    if (m.estimation > 2) throw Failure(AVD_ERR_INVALID_OPTIONS, "estimation must be 0 (Auto), 1 (Direct) or 2 (Chained)");
+    if (m.estimation == 2)
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "Chained motion estimation is not built yet; use Direct");
+    if (m.estimation == 0 && temporal_radius >= 3)
+        throw Failure(AVD_ERR_INVALID_OPTIONS,
+                      "Auto motion estimation resolves to Chained at temporal radius 3 and above, which is not built "
+                      "yet; use Direct");
+}
+
 // ---------------------------------------------------------------------------------------------
 // NLMeans pass
 // ---------------------------------------------------------------------------------------------
@@ -384,8 +398,7 @@ NlmPass::NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, av
         if (m.overlap >= m.blksize || m.blksize > 32 || m.blksize < 2 || m.search_radius > 8 || m.pyramid_levels < 1 ||
             m.pyramid_levels > 2)
             throw Failure(AVD_ERR_INVALID_OPTIONS, "unsupported motion search geometry");
-        if (m.estimation == 2)
-            throw Failure(AVD_ERR_INVALID_OPTIONS, "Chained motion estimation is not built yet; use Direct");
+        check_estimation(m, r_);
         if (m.max_search_escalation > m.search_radius)
             throw Failure(AVD_ERR_INVALID_OPTIONS, "search escalation is not built yet");
         if (in.kind == AVD_FORMAT_U16 && in.depth > 12)
@@ -594,9 +607,13 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
         throw Failure(AVD_ERR_INVALID_OPTIONS, "more than 4 motion blocks would cover a patch on each axis");
     if (m.blksize > 32 || m.blksize < 2 || m.search_radius > 8 || m.pyramid_levels < 1 || m.pyramid_levels > 2)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "unsupported motion search geometry");
-    if (m.estimation == 2) throw Failure(AVD_ERR_INVALID_OPTIONS, "Chained motion estimation is not built yet; use Direct");
+    check_estimation(m, r_);
     if (m.max_search_escalation > m.search_radius)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "search escalation is not built yet");
+    // The truemotion regulariser (nl4d/denoiser.rs:235, truemotion/stage.rs:61) runs whenever
+    // field_lambda is above 0; it is not built, so only field_lambda = 0 keeps the reference's output.
+    if (o.field_lambda != 0.0f)
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "the truemotion regulariser is not built yet; set field_lambda = 0");
     if (!o.has_sigma)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "noise estimation is not built yet; pass a fixed sigma");
     if (o.grain_data) throw Failure(AVD_ERR_INVALID_OPTIONS, "grain measurement is not built yet");

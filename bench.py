@@ -81,7 +81,7 @@ FORMS = {"fp32": 0, "mixed": 1}
 def make_engine(avd, form="mixed"):
     geom = avd.Geometry(W, H, 1, avd.SampleFormat.u16(DEPTH), avd.SampleFormat.u16(DEPTH), 0)
     common = dict(temporal_radius=RADIUS, spatial_radius=SPATIAL, refine=REFINE, lambda_ht=LAMBDA,
-                  pooled_threshold=0, noise_map=0, motion={"estimation": 1}, transform_form=FORMS[form])
+                  pooled_threshold=0, noise_map=0, motion={"estimation": 1}, field_lambda=0.0, transform_form=FORMS[form])
     luma = avd.nl4d_options(sigma=SIGMA_Y / 1023.0, **common)
     chroma = avd.nl4d_options(sigma=SIGMA_C / 1023.0, **common)
     return avd.build_engine("nl4d", geom, luma=luma, chroma=chroma), geom

@@ -349,7 +349,8 @@ avd_status avd_site_noise_spatial(const float *ring_values, uint32_t width, uint
 /* partials_len (noise/spatial.rs:31-34): floats of one frame's partials. */
 uint32_t avd_noise_partials_len(uint32_t width, uint32_t height);
 /* sigma_from_abs_sum (noise/spatial.rs:134-138): sqrt(pi / 2) sum / (6 (width - 2)(height - 2)),
- * the Immerkaer sigma of a frame-wide sum (in the samples' normalised units). */
+ * the Immerkaer sigma of a frame-wide sum (in the samples' normalised units).  NaN when width or
+ * height is below 3, where the mask has no interior sample. */
 float avd_noise_sigma_from_abs_sum(float abs_sum, uint32_t width, uint32_t height);
 
 #ifdef __cplusplus

@@ -79,6 +79,17 @@ def geom(chroma=0, fmt_in=None, fmt_out=None, w=64, h=48):
     ("nl4d", dict(sigma=0.02, pooled_threshold=0), None, geom(0, w=7, h=7), avd.InvalidGeometry),
     ("nl4d", dict(sigma=0.02, pooled_threshold=0, transform_form=2), None, geom(), avd.InvalidOptions),
     ("nl4d", dict(sigma=0.02, pooled_threshold=1, transform_form=1), None, geom(), avd.InvalidOptions),  # mixed: hard only
+    # Unbuilt reference behaviour is refused, not replaced: the truemotion regulariser
+    # (field_lambda > 0, nl4d/denoiser.rs:235) and Auto estimation at temporal radius >= 3, which
+    # the reference resolves to Chained (motion/search.rs:133-138).
+    ("nl4d", dict(sigma=0.02, pooled_threshold=0, field_lambda=1.0, motion={"estimation": 1}), None, geom(),
+     avd.InvalidOptions),
+    ("nl4d", dict(sigma=0.02, pooled_threshold=0, field_lambda=0.0, temporal_radius=3, motion={"estimation": 0}),
+     None, geom(), avd.InvalidOptions),
+    ("nl4d", dict(sigma=0.02, pooled_threshold=0, field_lambda=0.0, motion={"estimation": 2}), None, geom(),
+     avd.InvalidOptions),
+    ("nlmeans", dict(hq=1, sigma=0.02, temporal=1, temporal_radius=3, motion_compensation=1,
+                     motion={"estimation": 0}), None, geom(), avd.InvalidOptions),
     ("nlmeans", dict(patch_radius=17), None, geom(), avd.InvalidOptions),
     ("nlmeans", dict(search_radius=9), None, geom(), avd.InvalidOptions),
     ("nlmeans", dict(hq=1), None, geom(), avd.InvalidOptions),  # HQ needs a fixed sigma here
@@ -86,9 +97,25 @@ def geom(chroma=0, fmt_in=None, fmt_out=None, w=64, h=48):
     ("nlmeans", dict(), None, geom(1, avd.SampleFormat.u8(), avd.SampleFormat.u16(10)), avd.InvalidGeometry),
 ])
 def test_engine_validation_errors(kind, luma, chroma, g, err):
+    if kind == "nl4d":
+        # Options the build runs, unless the case sets them: no truemotion, Direct estimation.
+        luma = {"field_lambda": 0.0, "motion": {"estimation": 1}, **luma}
+        chroma = None if chroma is None else {"field_lambda": 0.0, "motion": {"estimation": 1}, **chroma}
     mk = avd.nl4d_options if kind == "nl4d" else avd.nlmeans_options
     with pytest.raises(err):
         avd.build_engine(kind, g, luma=mk(**luma), chroma=mk(**chroma) if chroma is not None else None)
+
+
+def test_the_defaults_that_resolve_to_built_behaviour_are_accepted():
+    # Auto below temporal radius 3 is Direct (built); field_lambda 0 runs no truemotion.  Creating
+    # an engine allocates device memory, so only the option check is reached here: without a GPU
+    # the failure, if any, is a GPU error, never InvalidOptions.
+    for opts in (dict(sigma=0.02, pooled_threshold=0, field_lambda=0.0, temporal_radius=2, motion={"estimation": 0}),
+                 dict(sigma=0.02, pooled_threshold=0, field_lambda=0.0, temporal_radius=3, motion={"estimation": 1})):
+        try:
+            avd.build_engine("nl4d", geom(), luma=avd.nl4d_options(**opts))
+        except avd.Gpu:
+            pass
 
 
 def test_a_spec_must_clean_something():

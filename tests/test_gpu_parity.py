@@ -309,7 +309,7 @@ def test_nl4d_engine_stream_is_bit_exact(frames, r):
     clip = avd.synth.clip(frames, W, H, depth, 100 + r, chroma="gray", sigma_luma=16.0)
     codes = [f[0].numpy() for f in clip]
     opt = avd.nl4d_options(temporal_radius=r, spatial_radius=S, sigma=sigma, lambda_ht=2.7, pooled_threshold=0,
-                           noise_map=0, motion={"estimation": 1})
+                           noise_map=0, motion={"estimation": 1}, field_lambda=0.0)
     eng = avd.build_engine("nl4d", gray_geometry(W, H, depth), luma=opt)
     got = run_engine(eng, [[words_dev(c, depth)] for c in codes], depth, [(W, H)])
     p = O.Nl4dPass(_nl4d_spec(r, S, sigma), W, H, 1, depth, depth)
@@ -326,7 +326,7 @@ def test_nl4d_engine_420_luma_and_chroma_are_bit_exact():
     clip = avd.synth.clip(frames, W, H, depth, 77, chroma="420", sigma_luma=16.0, sigma_chroma=8.0)
     geom = L.Geometry(W, H, 1, L.SampleFormat.u16(10), L.SampleFormat.u16(10), 0)
     opt = avd.nl4d_options(temporal_radius=r, spatial_radius=S, sigma=sigma, lambda_ht=2.7, pooled_threshold=0,
-                           noise_map=0, motion={"estimation": 1})
+                           noise_map=0, motion={"estimation": 1}, field_lambda=0.0)
     eng = avd.build_engine("nl4d", geom, luma=opt, chroma=opt)
     dims = [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
     got = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f] for f in clip], depth, dims)
@@ -350,7 +350,7 @@ def test_nl4d_engine_with_pooled_threshold_is_bit_exact(chroma):
     clip = avd.synth.clip(frames, W, H, depth, 91, chroma=chroma, sigma_luma=16.0, sigma_chroma=8.0)
     geom = L.Geometry(W, H, 0 if chroma == "gray" else 1, L.SampleFormat.u16(10), L.SampleFormat.u16(10), 0)
     opt = avd.nl4d_options(temporal_radius=r, spatial_radius=S, sigma=sigma, pooled_threshold=1, noise_map=0,
-                           motion={"estimation": 1})
+                           motion={"estimation": 1}, field_lambda=0.0)
     eng = avd.build_engine("nl4d", geom, luma=opt, chroma=opt if chroma != "gray" else None)
     dims = [(W, H)] if chroma == "gray" else [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
     got = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f[:len(dims)]] for f in clip], depth, dims)
@@ -380,6 +380,7 @@ MIXED_CASES = [
     ("r2_64x48_422", 64, 48, 10, 2, 7, 6, 5, 14.0, 7.0, "422"),
     ("r1_56x40_444", 56, 40, 10, 1, 7, 5, 6, 14.0, 7.0, "444"),
     ("r2_72x56_gray", 72, 56, 10, 2, 8, 6, 9, 16.0, 0.0, "gray"),
+    ("r2_64x48_12bit", 64, 48, 12, 2, 8, 6, 41, 64.0, 32.0, "420"),  # codes above 11 bits round to f16
 ]
 LAYOUT = {"gray": 0, "420": 1, "422": 2, "444": 3}
 
@@ -405,7 +406,7 @@ def test_nl4d_mixed_transform_form_stays_within_0_05_db(case):
     outs = {}
     for form in (0, 1):
         common = dict(temporal_radius=r, spatial_radius=S, lambda_ht=2.7, pooled_threshold=0, noise_map=0,
-                      motion={"estimation": 1}, transform_form=form)
+                      motion={"estimation": 1}, field_lambda=0.0, transform_form=form)
         eng = avd.build_engine("nl4d", geom, luma=avd.nl4d_options(sigma=sig_y / peak, **common),
                                chroma=avd.nl4d_options(sigma=sig_c / peak, **common) if cd else None)
         outs[form] = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f] for f in noisy], depth, dims)
@@ -433,7 +434,7 @@ def test_nl4d_chroma_passes_through_when_only_luma_is_cleaned():
     clip = avd.synth.clip(5, W, H, depth, 3, chroma="420", sigma_luma=6.0)
     geom = L.Geometry(W, H, 1, L.SampleFormat.u8(), L.SampleFormat.u8(), 0)
     opt = avd.nl4d_options(temporal_radius=1, spatial_radius=5, sigma=6 / 255, pooled_threshold=0, noise_map=0,
-                           motion={"estimation": 1})
+                           motion={"estimation": 1}, field_lambda=0.0)
     eng = avd.build_engine("nl4d", geom, luma=opt)
     dims = [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
     got = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f] for f in clip], depth, dims)

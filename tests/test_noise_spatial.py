@@ -29,6 +29,13 @@ def test_partials_len_matches_tile_grid():
         assert L.load().avd_noise_partials_len(w, h) == want
 
 
+def test_sigma_from_abs_sum_below_three_samples_is_nan():
+    # No interior sample of the 3x3 mask exists there (the reference's version underflows).
+    for w, h in [(2, 2), (1, 64), (64, 2), (0, 0)]:
+        assert np.isnan(L.load().avd_noise_sigma_from_abs_sum(1.0, w, h)), (w, h)
+    assert np.isfinite(L.load().avd_noise_sigma_from_abs_sum(1.0, 3, 3))
+
+
 def test_sigma_from_abs_sum_zero_and_formula():
     assert O.sigma_from_abs_sum(0.0, 64, 64) == 0.0
     assert L.load().avd_noise_sigma_from_abs_sum(0.0, 64, 64) == 0.0
