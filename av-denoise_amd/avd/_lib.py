@@ -192,6 +192,7 @@ SIGNATURES = {
     "avd_site_nlm_pair": (ctypes.c_int, [ctypes.POINTER(NlmWindow), P, P, P, P, P, P, P, P, P]),
     "avd_site_nlm_finish": (ctypes.c_int, [P, P, P, P, f32, u32, u32, P, P]),
     "avd_site_noise_spatial": (ctypes.c_int, [P, u32, u32, u32, u32, P, P, u32, P]),
+    "avd_site_noise_spatial_noalloc": (ctypes.c_int, [P, u32, u32, u32, u32, P, u32, P]),
     "avd_noise_partials_len": (u32, [u32, u32]),
     "avd_noise_sigma_from_abs_sum": (f32, [f32, u32, u32]),
 }

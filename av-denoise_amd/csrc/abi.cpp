@@ -305,6 +305,25 @@ avd_status avd_site_noise_spatial(const float *ring_values, uint32_t width, uint
                       "noise_spatial");
 }
 
+avd_status avd_site_noise_spatial_noalloc(const float *ring_values, uint32_t width, uint32_t height,
+                                          uint32_t channels, uint32_t frame, float *results, uint32_t slot,
+                                          void *stream)
+{
+    if (!ring_values || !results) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (width == 0 || height == 0 || (channels != 1 && channels != 2))
+        return fail(AVD_ERR_INVALID_GEOMETRY, "noise measurement needs a non-empty frame of 1 or 2 channels");
+    if (avd::noise_partials_len(width, height) > avd::kNoiseScratchFloats)
+        return fail(AVD_ERR_INVALID_GEOMETRY, "frame of " + std::to_string(width) + "x" + std::to_string(height) +
+                                                  " needs more partials than the static scratch holds (" +
+                                                  std::to_string(avd::kNoiseScratchFloats) + " floats)");
+    float *partials = nullptr;
+    const hipError_t e = avd::noise_scratch_address(&partials);
+    if (e != hipSuccess) return hip_status(e, "noise_spatial_noalloc scratch");
+    return hip_status(avd::launch_noise_spatial(ring_values, width, height, channels, frame, partials, results, slot,
+                                                static_cast<hipStream_t>(stream)),
+                      "noise_spatial_noalloc");
+}
+
 uint32_t avd_noise_partials_len(uint32_t width, uint32_t height) { return avd::noise_partials_len(width, height); }
 
 float avd_noise_sigma_from_abs_sum(float abs_sum, uint32_t width, uint32_t height)

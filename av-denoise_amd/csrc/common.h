@@ -91,6 +91,10 @@ hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, c
 uint32_t noise_partials_len(uint32_t width, uint32_t height);
 hipError_t launch_noise_spatial(const float *ring, uint32_t width, uint32_t height, uint32_t channels,
                                 uint32_t frame, float *partials, float *results, uint32_t slot, hipStream_t s);
+// The current device's copy of the static partials array of the allocation-free entry point
+// (kNoiseScratchFloats floats), resolved on the first call per device.
+constexpr uint32_t kNoiseScratchFloats = AVD_NOISE_SCRATCH_FLOATS;
+hipError_t noise_scratch_address(float **out);
 hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,
                                    int C, hipStream_t s);
 

@@ -12,9 +12,18 @@
 // turn; each total then adds its 256 thread sums in thread order, as does each tile in stage 1),
 // so the totals are bit-identical to the reference's.  The work is one read of the frame (HBM).
 
+#include <mutex>
+
 #include "common.h"
 
 namespace avd {
+
+// The partials of the entry point that allocates nothing: a file-scope device array, one copy per
+// GPU at its own address, resolved once per device by hipGetSymbolAddress.  Host code only ever
+// uses that resolved address.  Every partial the reduce reads is written by the partial kernel of
+// the same call first, so the array needs no clearing; calls that overlap on one GPU race on it.
+__device__ __align__(256) float scratch[kNoiseScratchFloats];
+
 namespace {
 
 constexpr int kImmBlockX = 32, kImmBlockY = 8;  // the tile (noise/mod.rs:117-119)
@@ -131,6 +140,26 @@ hipError_t launch_noise_spatial(const float *ring, uint32_t width, uint32_t heig
         k_noise_partial<2><<<grid, kImmBlockX * kImmBlockY, 0, s>>>(ring, partials, frame, (int)width, (int)height);
     k_noise_reduce<<<1, kReduceBlock, 0, s>>>(partials, results, slot, tiles_x * tiles_y);
     return hipGetLastError();
+}
+
+hipError_t noise_scratch_address(float **out)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static float *resolved[kMaxDevices] = {};
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!resolved[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        resolved[dev] = static_cast<float *>(ptr);
+    }
+    *out = resolved[dev];
+    return hipSuccess;
 }
 
 }  // namespace avd

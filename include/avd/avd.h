@@ -346,6 +346,18 @@ avd_status avd_site_nlm_finish(const float *frame, const float *accum, const flo
  * is 0 for one channel), bit-identical to the reference's f32 sums. */
 avd_status avd_site_noise_spatial(const float *ring_values, uint32_t width, uint32_t height, uint32_t channels,
                                   uint32_t frame, float *partials, float *results, uint32_t slot, void *stream);
+/* No counterpart in the reference: avd_site_noise_spatial for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * takes no partials workspace and allocates nothing: the partials live in a static device array of
+ * AVD_NOISE_SCRATCH_FLOATS floats (2 MiB), one per GPU the library runs on, held until the process
+ * ends; frames whose avd_noise_partials_len exceeds it (above about 8192x8192) return
+ * AVD_ERR_INVALID_GEOMETRY.  Runs on the current device; `stream` must belong to it.  Calls on one
+ * GPU share the array, so calls that overlap on one GPU race.  Results match avd_site_noise_spatial
+ * bit for bit. */
+#define AVD_NOISE_SCRATCH_FLOATS (1u << 19)
+avd_status avd_site_noise_spatial_noalloc(const float *ring_values, uint32_t width, uint32_t height,
+                                          uint32_t channels, uint32_t frame, float *results, uint32_t slot,
+                                          void *stream);
 /* partials_len (noise/spatial.rs:31-34): floats of one frame's partials. */
 uint32_t avd_noise_partials_len(uint32_t width, uint32_t height);
 /* sigma_from_abs_sum (noise/spatial.rs:134-138): sqrt(pi / 2) sum / (6 (width - 2)(height - 2)),

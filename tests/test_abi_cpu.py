@@ -133,3 +133,18 @@ def test_noise_site_rejects_bad_arguments_before_any_launch():
     assert lib.avd_site_noise_spatial(None, 64, 48, 1, 0, buf, buf, 0, None) == avd.NullArgument.status
     assert lib.avd_site_noise_spatial(buf, 64, 48, 3, 0, buf, buf, 0, None) == avd.InvalidGeometry.status
     assert lib.avd_site_noise_spatial(buf, 0, 48, 1, 0, buf, buf, 0, None) == avd.InvalidGeometry.status
+
+
+def test_noalloc_noise_entry_refuses_frames_past_its_static_scratch():
+    import ctypes
+
+    from avd import _lib as L
+
+    lib = L.load()
+    w, h = 8192, 8200  # 256 x 1025 tiles: 524800 partials, above the 2^19 floats of the array
+    assert lib.avd_noise_partials_len(w, h) > (1 << 19)
+    assert lib.avd_noise_partials_len(8192, 8192) == (1 << 19)
+    buf = ctypes.c_void_p(1)  # never dereferenced: the checks fail before any launch
+    assert lib.avd_site_noise_spatial_noalloc(buf, w, h, 1, 0, buf, 0, None) == avd.InvalidGeometry.status
+    assert lib.avd_site_noise_spatial_noalloc(None, 64, 48, 1, 0, buf, 0, None) == avd.NullArgument.status
+    assert lib.avd_site_noise_spatial_noalloc(buf, 64, 48, 3, 0, buf, 0, None) == avd.InvalidGeometry.status
