@@ -1777,9 +1777,9 @@ __device__ __forceinline__ void member_row(const avd_collab_pass &p, const uint3
         if (MIXED) {  // the mixed gather's arithmetic
             const float ws = sw * (kr * p.kaiser[q] * p.accum_scale);
             const float lim = 5.0f * p.accum_scale;
-            const int32_t a = (int32_t)rintf(fminf(fmaxf((float)st16[q] * ws, -lim), lim));
-            const int32_t a2 = C == 2 ? (int32_t)rintf(fminf(fmaxf((float)st16[64 + q] * ws, -lim), lim)) : 0;
-            add(y * T + x, pack_aggregate(a, (int32_t)rintf(fminf(ws, p.accum_scale) * 5.0f)), a2);
+            const int32_t a = (int32_t)roundf(fminf(fmaxf((float)st16[q] * ws, -lim), lim));
+            const int32_t a2 = C == 2 ? (int32_t)roundf(fminf(fmaxf((float)st16[64 + q] * ws, -lim), lim)) : 0;
+            add(y * T + x, pack_aggregate(a, (int32_t)roundf(fminf(ws, p.accum_scale) * 5.0f)), a2);
         } else {
             add(y * T + x, pack_aggregate(st[q], to_fixed_weight(w, p.accum_scale)), C == 2 ? st[64 + q] : 0);
         }
