@@ -31,6 +31,9 @@
 
 #define PATCH 8
 #define MAX_K 8
+/* Group lists hold up to 16 members: the stride-3 / 16-member grid extends the reference's
+ * MAX_K = 8 (collab/mod.rs:16-20); every size below is pp->k_max (8 or 16). */
+#define KMAX 16
 #define UNFILLED_D 0xFFFFFFFFFFFFFFFFull
 
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -409,9 +412,10 @@ static inline int covering_lo(int p, int blksize, int step)
         const int ry = (iy * pp->step < maxy) ? iy * pp->step : maxy;                                    \
         const T *cf = ring + (size_t)centre_slot * slot_len;                                             \
         /* centre_search, clipped walk, shift_insert8: a tie never displaces an earlier visit. */        \
-        D bd[8];                                                                                         \
-        uint32_t bp[8];                                                                                  \
-        for (int i = 0; i < 8; ++i) {                                                                    \
+        const int K = pp->k_max;                                                                         \
+        D bd[KMAX];                                                                                      \
+        uint32_t bp[KMAX];                                                                               \
+        for (int i = 0; i < K; ++i) {                                                                    \
             bd[i] = UNF;                                                                                 \
             bp[i] = 0;                                                                                   \
         }                                                                                                \
@@ -420,8 +424,8 @@ static inline int covering_lo(int p, int blksize, int step)
         for (int y = top; y <= bottom; ++y)                                                              \
             for (int x = left; x <= right; ++x) {                                                        \
                 D d = (x == rx && y == ry) ? SELF_D : DIST(cf, cf, W, C, rx, ry, x, y);                  \
-                if (!(d < bd[7])) continue;                                                              \
-                int k = 7;                                                                               \
+                if (!(d < bd[K - 1])) continue;                                                          \
+                int k = K - 1;                                                                           \
                 while (k > 0 && d < bd[k - 1]) {                                                         \
                     bd[k] = bd[k - 1];                                                                   \
                     bp[k] = bp[k - 1];                                                                   \
@@ -433,16 +437,18 @@ static inline int covering_lo(int p, int blksize, int step)
         uint32_t n_live = (uint32_t)(right - left + 1) * (uint32_t)(bottom - top + 1);                   \
         uint32_t k_use = 1;                                                                              \
         while (k_use * 2 <= n_live && k_use * 2 <= (uint32_t)pp->k_max) k_use *= 2;                      \
-        uint32_t members[8];                                                                             \
-        for (int m = 0; m < 8; ++m) members[m] = bp[m];                                                  \
+        uint32_t members[KMAX];                                                                          \
+        for (int m = 0; m < K; ++m) members[m] = bp[m];                                                  \
         int use_grid = 0;                                                                                \
-        if (pp->grid_frames > 1 && pp->k_max == MAX_K) {                                                 \
-            const int gf = pp->grid_frames, volumes = MAX_K / gf, tail = gf - 1;                         \
+        /* The volume grid of the reference's 8 members (grid_frames x 8 / grid_frames), and of 16  */  \
+        /* in the extension (grid_frames x 16 / grid_frames volumes).                                */  \
+        if (pp->grid_frames > 1 && (K == MAX_K || K == KMAX)) {                                          \
+            const int gf = pp->grid_frames, volumes = K / gf, tail = gf - 1;                             \
             const int covers = (pp->blksize + pp->blk_step - 1) / pp->blk_step;                          \
             const int nnb = 2 * pp->radius;                                                              \
             const size_t nblk = (size_t)pp->blocks_x * pp->blocks_y;                                     \
-            uint32_t gpos[8];                                                                            \
-            D gd[8];                                                                                     \
+            uint32_t gpos[KMAX];                                                                         \
+            D gd[KMAX];                                                                                  \
             for (int v = 0; v < volumes; ++v) {                                                          \
                 int first = v * gf;                                                                      \
                 uint32_t anchor = members[v];                                                            \
@@ -522,15 +528,15 @@ static inline int covering_lo(int p, int blksize, int step)
                     }                                                                                    \
                 }                                                                                        \
             }                                                                                            \
-            use_grid = (k_use == MAX_K);                                                                 \
+            use_grid = (k_use == (uint32_t)K);                                                           \
             for (int v = 0; v < volumes; ++v) use_grid = use_grid && (gd[v * gf + tail] < UNF);          \
             if (use_grid) {                                                                              \
-                for (int m = 0; m < 8; ++m) members[m] = gpos[m];                                        \
-                k_use = MAX_K;                                                                           \
+                for (int m = 0; m < K; ++m) members[m] = gpos[m];                                        \
+                k_use = (uint32_t)K;                                                                     \
             }                                                                                            \
         }                                                                                                \
-        for (int m = 0; m < 8; ++m) out9[m] = members[m];                                                \
-        out9[8] = k_use | (use_grid ? 0x100u : 0u);                                                      \
+        for (int m = 0; m < K; ++m) out9[m] = members[m];                                                \
+        out9[K] = k_use | (use_grid ? 0x100u : 0u);                                                      \
     }                                                                                                    \
     void avdo_collab_search##SFX(const T *ring, const avdo_collab_params *pp, int centre_slot,            \
                                  const uint32_t *nslots, const int32_t *mv, const float *conf,            \
@@ -541,7 +547,7 @@ static inline int covering_lo(int p, int blksize, int step)
         {                                                                                                \
             for (int ix = 0; ix < pp->refs_x; ++ix)                                                      \
                 collab_search_one##SFX(ring, slot_len, pp, centre_slot, nslots, mv, conf, ix, iy,          \
-                                       members + ((size_t)iy * pp->refs_x + ix) * 9);                    \
+                                       members + ((size_t)iy * pp->refs_x + ix) * (pp->k_max + 1));      \
         }                                                                                                \
     }
 
@@ -618,7 +624,7 @@ static double dct8_basis(int row, int col)
     static void group_fwd##SFX(R *stk, int k_use, int grid, int gf, R f)                            \
     {                                                                                               \
         if (grid) {                                                                                 \
-            int vols = MAX_K / gf;                                                                  \
+            int vols = k_use / gf;                                                                  \
             for (int v = 0; v < vols; ++v)                                                          \
                 for (int len = gf, st = 1; len >= 2; len /= 2, st *= 2)                             \
                     for (int p = 0; p < gf; p += 2 * st) haar_pair##SFX(stk, v * gf + p, v * gf + p + st, f); \
@@ -633,7 +639,7 @@ static double dct8_basis(int row, int col)
     static void group_inv##SFX(R *stk, int k_use, int grid, int gf, R f)                            \
     {                                                                                               \
         if (grid) {                                                                                 \
-            int vols = MAX_K / gf;                                                                  \
+            int vols = k_use / gf;                                                                  \
             int top = 1;                                                                            \
             while (top * 2 < vols) top *= 2;                                                        \
             for (int fr = 0; fr < gf; ++fr)                                                         \
@@ -673,15 +679,16 @@ static double dct8_basis(int row, int col)
                             R *retained)                                                            \
     {                                                                                               \
         const R bar = threshold * threshold;                                                        \
-        R orig[8 * 64]; /* every lane reads the coefficients before any is dropped */               \
-        for (int i = 0; i < 8 * 64; ++i) orig[i] = stk[i];                                          \
+        R orig[KMAX * 64]; /* every lane reads the coefficients before any is dropped */            \
+        const int rows = k_use > MAX_K ? k_use : MAX_K;                                             \
+        for (int i = 0; i < rows * 64; ++i) orig[i] = stk[i];                                       \
         for (int v = 0; v < 8; ++v) {                                                               \
             const int dc_lane = v == 0, has_up = v > 0, has_down = v < 7;                           \
             const R vertical = (R)1 + (has_up ? (R)1 : (R)0) + (has_down ? (R)1 : (R)0);           \
             const R bars[4] = {bar * (vertical + (v == 1 ? (R)0 : (R)1)), bar * (vertical + (dc_lane ? (R)1 : (R)2)), \
                                bar * (vertical + (R)2), bar * (vertical + (R)1)};                  \
             R acc = 0;                                                                              \
-            for (int j = 0; j < 8; ++j) {                                                           \
+            for (int j = 0; j < rows; ++j) {                                                        \
                 R e[3][8]; /* lanes v-1, v, v+1 */                                                  \
                 for (int d = -1; d <= 1; ++d) {                                                     \
                     const int vv = v + d;                                                           \
@@ -745,14 +752,15 @@ static double dct8_basis(int row, int col)
         const R scale = (R)pp->accum_scale;                                                         \
         _Pragma("omp parallel for schedule(dynamic, 2)") for (int iy = 0; iy < pp->refs_y; ++iy)     \
         {                                                                                           \
-            R stk[8 * 64];                                                                          \
+            R stk[KMAX * 64];                                                                       \
             for (int ix = 0; ix < pp->refs_x; ++ix) {                                               \
                 size_t ref = (size_t)iy * pp->refs_x + ix;                                          \
-                const uint32_t *mem = members + ref * 9;                                            \
-                uint32_t gw = mem[8];                                                               \
+                const int K = pp->k_max;                                                            \
+                const uint32_t *mem = members + ref * (size_t)(K + 1);                              \
+                uint32_t gw = mem[K];                                                               \
                 int k_use = (int)(gw & 0xFF), grid = (gw & 0x100) != 0;                             \
-                int slot_of[8];                                                                     \
-                for (int m = 0; m < 8; ++m) {                                                       \
+                int slot_of[KMAX];                                                                  \
+                for (int m = 0; m < K; ++m) {                                                       \
                     uint32_t t = mem[m] >> 26;                                                      \
                     slot_of[m] = t > 0 ? (int)nslots[t - 1] : centre_slot;                          \
                 }                                                                                   \
@@ -778,8 +786,8 @@ static double dct8_basis(int row, int col)
                     group_fwd##SFX(stk, k_use, grid, pp->grid_frames, f);                           \
                     R part[8];                                                                      \
                     if (pp->pool_ratio > 0.0f) {                                                    \
-                        R var[8];                                                                   \
-                        for (int j = 0; j < 8; ++j) var[j] = sig2;                                  \
+                        R var[KMAX];                                                                \
+                        for (int j = 0; j < KMAX; ++j) var[j] = sig2;                               \
                         const R lam_c = (R)pp->lambda[c];                                           \
                         pooled##SFX(stk, var, pp->profile, k_use, lam_c * (R)pp->pool_ratio, lam_c, part); \
                     } else {                                                                        \

@@ -195,10 +195,13 @@ def kaiser_window(beta: float) -> np.ndarray:
     return out
 
 
-def cross_frame_accum_scale(spatial_radius: int, temporal_radius: int, step: int = 4) -> np.float32:
-    """aggregate.rs:36-55: the largest power of two keeping the worst case under i32::MAX/2."""
+def cross_frame_accum_scale(spatial_radius: int, temporal_radius: int, step: int = 4,
+                            group_size: int = 8) -> np.float32:
+    """aggregate.rs:36-55: the largest power of two keeping the worst case under i32::MAX/2.  The
+    reference's STEP (4) and MAX_K (8) enter as the grid stride and the group size, so a stride-3,
+    16-member grid gets the bound of its own worst case."""
     refs_per_axis = ((8 - 1) + 2 * spatial_radius) // step + 1
-    contribs = float(refs_per_axis) * float(refs_per_axis) * 8.0
+    contribs = float(refs_per_axis) * float(refs_per_axis) * float(group_size)
     passes = float(4 * temporal_radius + 1)
     max_raw = contribs * passes * 5.0
     budget = float(2**31 - 1) / 2.0
@@ -348,13 +351,13 @@ def pool_ratio(C: int) -> float:
 
 
 def collab_params(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, geom: MotionGeom, step=4,
-                  profile=None, sigmas=None, pooled=False):
+                  profile=None, sigmas=None, pooled=False, group_size=8):
     p = CollabParams()
     p.pool_ratio = pool_ratio(C) if pooled else 0.0
     p.W, p.H, p.C = W, H, C
     p.step = step
     p.refs_x, p.refs_y = refs_along(W, step), refs_along(H, step)
-    p.S, p.radius, p.grid_frames, p.refine, p.k_max = S, radius, grid_frames(radius), refine, 8
+    p.S, p.radius, p.grid_frames, p.refine, p.k_max = S, radius, grid_frames(radius), refine, group_size
     p.blk_step, p.blksize, p.blocks_x, p.blocks_y = geom.step, geom.blksize, geom.blocks_x, geom.blocks_y
     p.c_min = c_min
     sig = list(sigmas) if sigmas is not None else [sigma, sigma]
@@ -365,7 +368,7 @@ def collab_params(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, ge
     p.weight_scale = float(weight_scale(sig[0], float(prof.max())))
     for i in range(8):
         p.profile[i] = float(prof[i])
-    p.accum_scale = float(cross_frame_accum_scale(S, radius, step))
+    p.accum_scale = float(cross_frame_accum_scale(S, radius, step, group_size))
     k = kaiser_window(kaiser_beta)
     for i in range(8):
         p.kaiser[i] = float(k[i])
@@ -375,7 +378,7 @@ def collab_params(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, ge
 def collab_search(ring, params: CollabParams, centre_slot, nslots, mv, conf):
     """search_group over every reference; ``ring`` is u16 codes (exact SSD) or f32 values."""
     refs = params.refs_x * params.refs_y
-    members = np.zeros(refs * 9, np.uint32)
+    members = np.zeros(refs * (params.k_max + 1), np.uint32)  # k_max member words, then the group word
     ns = np.ascontiguousarray(nslots, np.uint32) if len(nslots) else np.zeros(1, np.uint32)
     mvv = np.ascontiguousarray(mv, np.int32) if mv is not None and mv.size else np.zeros(2, np.int32)
     cf = np.ascontiguousarray(conf, np.float32) if conf is not None and conf.size else np.zeros(1, np.float32)
@@ -606,6 +609,7 @@ class Nl4dSpec:
     motion: MotionGeom = field(default_factory=lambda: MotionGeom(0, 0))
     step: int = 4
     pooled: bool = False
+    group_size: int = 8
 
 
 class Nl4dPass:
@@ -622,7 +626,8 @@ class Nl4dPass:
                                spec.motion.pyramid_levels)
         self.maxv = max_value(depth_in)
         self.params = collab_params(W, H, C, self.r, spec.spatial_radius, spec.refine, spec.c_min, spec.lambda_ht,
-                                    spec.sigma, spec.kaiser_beta, self.geom, spec.step, pooled=spec.pooled)
+                                    spec.sigma, spec.kaiser_beta, self.geom, spec.step, pooled=spec.pooled,
+                                    group_size=spec.group_size)
         px = W * H
         self.accum = np.zeros(self.N * px * C, np.int32)
         self.wsum = np.zeros(self.N * px, np.int32)
