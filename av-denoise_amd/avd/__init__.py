@@ -177,6 +177,14 @@ class Engine:
     def fail_for_test(self) -> None:
         check(self._lib.avd_engine_fail_for_test(self._h))
 
+    def set_tile_cap_for_test(self, cap: int) -> None:
+        check(self._lib.avd_engine_set_tile_cap_for_test(self._h, cap))
+
+    def tile_overflows_for_test(self) -> int:
+        out = ctypes.c_uint64()
+        check(self._lib.avd_engine_tile_overflows_for_test(self._h, ctypes.byref(out)))
+        return int(out.value)
+
     def close(self) -> None:
         if self._h:
             self._lib.avd_engine_destroy(self._h)

@@ -180,6 +180,8 @@ SIGNATURES = {
     "avd_engine_max_held_frames": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64)]),
     "avd_engine_synchronize": (ctypes.c_int, [P]),
     "avd_engine_fail_for_test": (ctypes.c_int, [P]),
+    "avd_engine_set_tile_cap_for_test": (ctypes.c_int, [P, u32]),
+    "avd_engine_tile_overflows_for_test": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64)]),
     "avd_engine_sample_device_time": (ctypes.c_int, [P, u32]),
     "avd_engine_timing": (ctypes.c_int, [P, ctypes.POINTER(Timing)]),
     "avd_site_ingest": (ctypes.c_int, [P, P, P, P, u32, u32, SampleFormat, P]),

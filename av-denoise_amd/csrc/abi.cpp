@@ -205,6 +205,18 @@ avd_status avd_engine_fail_for_test(avd_engine *e)
     return guard([&] { e->impl.fail_for_test(); });
 }
 
+avd_status avd_engine_set_tile_cap_for_test(avd_engine *e, uint32_t cap)
+{
+    if (!e) return fail(AVD_ERR_NULL_ARGUMENT, "engine is null");
+    return guard([&] { e->impl.set_tile_cap_for_test(cap); });
+}
+
+avd_status avd_engine_tile_overflows_for_test(const avd_engine *e, uint64_t *pairs)
+{
+    if (!e || !pairs) return fail(AVD_ERR_NULL_ARGUMENT, "engine or pairs is null");
+    return guard([&] { *pairs = e->impl.tile_overflows_for_test(); });
+}
+
 avd_status avd_engine_sample_device_time(avd_engine *e, uint32_t every)
 {
     if (!e) return fail(AVD_ERR_NULL_ARGUMENT, "engine is null");

@@ -155,11 +155,12 @@ __device__ __forceinline__ uint32_t dist_key(const uint16_t *ca, const uint16_t 
 // collab_search
 // ---------------------------------------------------------------------------------------------
 
-// Inserts `k` into the ascending 8-entry list `a`, dropping the largest (keys are unique).
-__device__ __forceinline__ void insert8(uint64_t *a, uint64_t k)
+// Inserts `k` into the ascending N-entry list `a`, dropping the largest (keys are unique).
+template <int N>
+__device__ __forceinline__ void insert_n(uint64_t *a, uint64_t k)
 {
 #pragma unroll
-    for (int i = 7; i > 0; --i) a[i] = (k < a[i - 1]) ? a[i - 1] : ((k < a[i]) ? k : a[i]);
+    for (int i = N - 1; i > 0; --i) a[i] = (k < a[i - 1]) ? a[i - 1] : ((k < a[i]) ? k : a[i]);
     a[0] = (k < a[0]) ? k : a[0];
 }
 
@@ -170,7 +171,9 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int C, bool CODES>
+// KM: the group's member count (8, the reference's MAX_K, or 16); a reference writes KM member
+// words and then the group word.
+template <int C, bool CODES, int KM>
 __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p, const uint16_t *__restrict__ codes,
                                                                const float *__restrict__ values,
                                                                const uint32_t *__restrict__ nslots,
@@ -195,15 +198,15 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
     const float *cv = CODES ? nullptr : values + (size_t)p.centre_slot * slot_len;
     const int S = p.spatial_radius;
 
-    // ---- centre search: each lane keeps the best eight (distance, visit order) keys of its share
-    // of the clipped rectangle; eight rounds of a wave-wide minimum merge them in order.
+    // ---- centre search: each lane keeps the best KM (distance, visit order) keys of its share of
+    // the clipped rectangle; KM rounds of a wave-wide minimum merge them in order.
     const int left = clampi(rx - S, 0, maxx), right = clampi(rx + S, 0, maxx);
     const int top = clampi(ry - S, 0, maxy), bottom = clampi(ry + S, 0, maxy);
     const int rw = right - left + 1, rh = bottom - top + 1;
     const int n_live = rw * rh;
-    uint64_t best8[8];
+    uint64_t best8[KM];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) best8[i] = kNoKey;
+    for (int i = 0; i < KM; ++i) best8[i] = kNoKey;
     uint32_t scored = 0;  // candidate patches this lane scores, for the stage's work count
     for (int idx = lane; idx < n_live; idx += 64) {
         const int y = top + idx / rw, x = left + idx % rw;
@@ -211,11 +214,11 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
         scored += self ? 0u : 1u;
         const uint32_t d = self ? 0u : dist_key<C, CODES>(cc, cc, cv, cv, W, rx, ry, x, y);
         const uint64_t key = ((uint64_t)d << 16) | (uint64_t)idx;
-        if (key < best8[7]) insert8(best8, key);
+        if (key < best8[KM - 1]) insert_n<KM>(best8, key);
     }
-    uint32_t mem[8];
+    uint32_t mem[KM];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < KM; ++m) {
         const uint64_t best = wave_min_u64(best8[0]);
         if (best == kNoKey) {
             mem[m] = 0;  // an unfilled slot keeps position word 0
@@ -224,8 +227,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
             mem[m] = pack_pos_t((uint32_t)(left + idx % rw), (uint32_t)(top + idx / rw), 0);
             if (best8[0] == best) {
 #pragma unroll
-                for (int i = 0; i < 7; ++i) best8[i] = best8[i + 1];
-                best8[7] = kNoKey;
+                for (int i = 0; i < KM - 1; ++i) best8[i] = best8[i + 1];
+                best8[KM - 1] = kNoKey;
             }
         }
     }
@@ -236,17 +239,17 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
     const int radius = p.temporal_radius;
     const int gf = radius == 0 ? 1 : (radius == 1 ? 2 : 4);
     bool use_grid = false;
-    if (gf > 1 && p.group_size == 8) {
-        const int volumes = 8 / gf, tail = gf - 1;
+    if (gf > 1) {
+        const int volumes = KM / gf, tail = gf - 1;
         const int bstep = p.blk_step, bsz = p.blksize, bx_n = p.blocks_x, by_n = p.blocks_y;
         const int covers = (bsz + bstep - 1) / bstep;
         const int nrect = covers * covers;
         const size_t nblk = (size_t)bx_n * by_n;
         const int refine = p.refine;
-        uint32_t gpos[8];
-        uint64_t gd[8];
+        uint32_t gpos[KM];
+        uint64_t gd[KM];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < KM; ++i) {
             gpos[i] = 0;
             gd[i] = kNoKey;
         }
@@ -254,11 +257,11 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
             const int first = v * gf;
             uint32_t anchor = 0;
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int i = 0; i < KM; ++i)
                 if (i == v) anchor = mem[i];
             const int ax = (int)(anchor & 0x1FFF), ay = (int)((anchor >> 13) & 0x1FFF);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
+            for (int i = 0; i < KM; ++i) {
                 if (i == first) {
                     gpos[i] = anchor;
                     gd[i] = 0;
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
                                 skipped = true;
                         const uint32_t packed = pack_pos_t((uint32_t)cx, (uint32_t)cy, (uint32_t)t + 1);
 #pragma unroll
-                        for (int k = 0; k < 8; ++k)
+                        for (int k = 0; k < KM; ++k)
                             if (k < held_n && gpos[k] == packed) skipped = true;
                         if (skipped) continue;
                         ++scored;
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
                 wave_sync();
                 // The frame's match joins the volume's ascending list; earlier frames win ties.
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
+                for (int i = 0; i < KM; ++i) {
                     if (i > first && i < first + gf && cd < gd[i]) {
                         const uint64_t hd = gd[i];
                         const uint32_t hp = gpos[i];
@@ -356,14 +359,14 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
                 }
             }
         }
-        use_grid = (k_use == 8);
+        use_grid = (k_use == KM);
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < KM; ++i)
             if (i % gf == tail) use_grid = use_grid && (gd[i] != kNoKey);
         if (use_grid) {
 #pragma unroll
-            for (int m = 0; m < 8; ++m) mem[m] = gpos[m];
-            k_use = 8;
+            for (int m = 0; m < KM; ++m) mem[m] = gpos[m];
+            k_use = KM;
         }
     }
     if (scored_count) {
@@ -372,12 +375,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_search(avd_collab_pass p
         for (int off = 32; off >= 1; off >>= 1) total += (uint32_t)__shfl_xor((int)total, off);
         if (lane == 0) atomicAdd(scored_count, (unsigned long long)total * C);
     }
-    if (live && lane < 9) {
+    if (live && lane < KM + 1) {
         uint32_t word = k_use | (use_grid ? 0x100u : 0u);
 #pragma unroll
-        for (int m = 0; m < 8; ++m)
+        for (int m = 0; m < KM; ++m)
             if (lane == m) word = mem[m];
-        members[(size_t)ref * 9 + lane] = word;
+        members[(size_t)ref * (KM + 1) + lane] = word;
     }
 }
 
@@ -637,12 +640,13 @@ __device__ __forceinline__ uint32_t med3u(uint32_t a, uint32_t b, uint32_t c)
     return r;
 }
 
-// Inserts `k` into the ascending list `a` of eight distinct keys, dropping the largest: slot i
+// Inserts `k` into the ascending list `a` of N distinct keys, dropping the largest: slot i
 // becomes clamp(k, a[i-1], a[i]), one v_med3_u32 per slot.
-__device__ __forceinline__ void insert8_u32(uint32_t *a, uint32_t k)
+template <int N>
+__device__ __forceinline__ void insert_n_u32(uint32_t *a, uint32_t k)
 {
 #pragma unroll
-    for (int i = 7; i > 0; --i) a[i] = med3u(a[i - 1], k, a[i]);
+    for (int i = N - 1; i > 0; --i) a[i] = med3u(a[i - 1], k, a[i]);
     a[0] = min(a[0], k);
 }
 
@@ -686,12 +690,12 @@ __device__ __forceinline__ void add_scored(unsigned long long *scored_count, uin
     }
 }
 
-// Centre search (centre.rs:37-300) of every reference: the best eight of the clipped rectangle by
-// (distance, row-major visit), the self-match pinned first.  Writes the eight positions and the
+// Centre search (centre.rs:37-300) of every reference: the best KM of the clipped rectangle by
+// (distance, row-major visit), the self-match pinned first.  Writes the KM positions and the
 // stack word k_use; the trajectory kernel may then replace them with a volume grid.
 // NARROW: codes of at most 10 bits and spatial_radius <= 15, so a lane's keys are
 // (distance << 5 | candidate) in 32 bits; otherwise (distance << 32 | visit index) in 64.
-template <int C, bool NARROW>
+template <int C, bool NARROW, int KM>
 __device__ __forceinline__ uint32_t centre_search_wave(const avd_collab_pass &p, const uint16_t *__restrict__ ring,
                                                        uint32_t *__restrict__ members, int ref_raw)
 {
@@ -718,10 +722,10 @@ __device__ __forceinline__ uint32_t centre_search_wave(const avd_collab_pass &p,
     const int k = (rh + per_col - 1) / per_col;
     const int tasks = rw * ((rh + k - 1) / k);  // <= 64 for rw <= 33
     const int col = lane % rw, y0 = top + (lane / rw) * k;
-    uint32_t L[8];
-    uint64_t LW[8];
+    uint32_t L[KM];
+    uint64_t LW[KM];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < KM; ++i) {
         L[i] = ~0u;
         LW[i] = kNoKey;
     }
@@ -731,13 +735,13 @@ __device__ __forceinline__ uint32_t centre_search_wave(const avd_collab_pass &p,
         slide<C>(ring, cframe, W, H, x, y0, k, kk, P, A2, [&](int j, uint32_t ssd) {
             const bool self = x == rx && y0 + j == ry;
             const uint32_t d = self ? 0u : ssd + 1u;
-            if (NARROW) insert8_u32(L, (d << 5) | (uint32_t)j);
-            else insert8(LW, ((uint64_t)d << 32) | (uint32_t)((y0 + j - top) * rw + col));
+            if (NARROW) insert_n_u32<KM>(L, (d << 5) | (uint32_t)j);
+            else insert_n<KM>(LW, ((uint64_t)d << 32) | (uint32_t)((y0 + j - top) * rw + col));
         });
     }
-    uint32_t word = 0;  // lane m < 8 keeps member m, lane 8 the group word
+    uint32_t word = 0;  // lane m < KM keeps member m, lane KM the group word
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < KM; ++m) {
         uint64_t head;
         if (NARROW)
             head = L[0] == ~0u ? kNoKey
@@ -755,29 +759,29 @@ __device__ __forceinline__ uint32_t centre_search_wave(const avd_collab_pass &p,
             if (lane == m) word = pack_pos_t((uint32_t)(left + idx % rw), (uint32_t)(top + idx / rw), 0);
             if (head == best) {
 #pragma unroll
-                for (int i = 0; i < 7; ++i) {
+                for (int i = 0; i < KM - 1; ++i) {
                     L[i] = L[i + 1];
                     LW[i] = LW[i + 1];
                 }
-                L[7] = ~0u;
-                LW[7] = kNoKey;
+                L[KM - 1] = ~0u;
+                LW[KM - 1] = kNoKey;
             }
         }
     }
     uint32_t k_use = 1;
     while (k_use * 2 <= (uint32_t)n_live && k_use * 2 <= p.group_size) k_use *= 2;
-    if (lane == 8) word = k_use;
-    if (live && lane < 9) members[(size_t)ref * 9 + lane] = word;
+    if (lane == KM) word = k_use;
+    if (live && lane < KM + 1) members[(size_t)ref * (KM + 1) + lane] = word;
     return live && lane == 0 ? (uint32_t)(n_live - 1) : 0u;  // the self-match is not scored
 }
 
-template <int C, bool NARROW>
+template <int C, bool NARROW, int KM>
 __global__ __launch_bounds__(kWaves * 64) void k_centre_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                                                               uint32_t *__restrict__ members,
                                                               unsigned long long *__restrict__ scored_count)
 {
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint32_t scored = centre_search_wave<C, NARROW>(p, ring, members, xcd_block() * kWaves + wave);
+    const uint32_t scored = centre_search_wave<C, NARROW, KM>(p, ring, members, xcd_block() * kWaves + wave);
     add_scored(scored_count, scored, true, C);
 }
 
@@ -796,12 +800,13 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_codes(avd_collab_pass p,
 // cy, two tiles covering every column any of the 8 references searches, and the wave walks cy down
 // the union of their row ranges, so the 16 lanes of a k-block read consecutive codes of one frame
 // row.  Lane (n, g) receives rows 4g..4g+3 of each product, both halves of references 2g and 2g+1
-// (grid row g) for candidate column n of the tile, forms their keys and keeps a sorted top 8 per
-// reference; the 16 lanes of a row then merge each reference's lists by DPP.
+// (grid row g) for candidate column n of the tile, forms their keys and keeps a sorted top KM per
+// reference (KM = 8 or 16 group members); the 16 lanes of a row then merge each reference's lists
+// by DPP.
 //
-// Keys are (min(d, kKeyCap) << 10 | row-major index) in 32 bits.  A reference whose eighth key
-// reaches the cap (a distance that large could tie-break differently) is listed and searched again
-// with the exact 64-bit keys of the vector search (k_centre_redo).
+// Keys are (min(d, kKeyCap) << 10 | row-major index) in 32 bits.  A reference whose last chosen
+// key reaches the cap (a distance that large could tie-break differently) is listed and searched
+// again with the exact 64-bit keys of the vector search (k_centre_redo).
 // ---------------------------------------------------------------------------------------------
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
@@ -874,7 +879,7 @@ __device__ __forceinline__ half8_t load_b8(const uint16_t *__restrict__ plane, u
     return __builtin_bit_cast(half8_t, w);
 }
 
-template <int C>
+template <int C, int KM>
 __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, const uint16_t *__restrict__ ring,
                                                              const uint16_t *__restrict__ plane, uint32_t odd_at,
                                                              const uint32_t *__restrict__ energy,
@@ -982,11 +987,11 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
     }
     const int cy_lo = clampi(pos_y(ref_row(0)) - S, 0, maxy), cy_hi = clampi(pos_y(ref_row(3)) + S, 0, maxy);
 
-    uint32_t L[2][8];
+    uint32_t L[2][KM];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) L[r][i] = ~0u;
+        for (int i = 0; i < KM; ++i) L[r][i] = ~0u;
     // The next row's operands load while this row's products and keys are formed (two operand
     // sets in turn, so no copy waits on a load in flight).  cy <= maxy, so every patch row exists.
     auto load_row_ops = [&](int cy, half8_t (&B)[KS][2], uint32_t (&eb)[2]) {
@@ -1016,7 +1021,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
                 const uint32_t ssd = kc_[r] + eb[t] - 2u * cross;
                 const uint32_t d = (selfrow && selfcol[t][r]) ? 0u : ssd + 1u;
                 const uint32_t key = (min(d, kKeyCap) << 10) | (uint32_t)(rowbase + colidx[t][r]);
-                insert8_u32(L[r], key | (ok ? 0u : ~0u));
+                insert_n_u32<KM>(L[r], key | (ok ? 0u : ~0u));
             }
         }
     };
@@ -1031,7 +1036,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
         score_row(cy + 1, By, ey);
     }
 
-    // ---- merge each reference's 16 lists (one DPP row) into its best eight.
+    // ---- merge each reference's 16 lists (one DPP row) into its best KM.
     uint32_t word[2] = {0u, 0u};
     bool redo[2];
 #pragma unroll
@@ -1039,7 +1044,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
         const uint32_t inv = (65536u + (uint32_t)rw_[r] - 1u) / (uint32_t)rw_[r];  // idx / rw exact below 1024
         uint32_t last = ~0u;
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {
+        for (int m = 0; m < KM; ++m) {
             const uint32_t head = L[r][0];
             const uint32_t best = row16_min_u32(head);
             if (best != ~0u) {
@@ -1049,8 +1054,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
                 if (n == m) word[r] = pack_pos_t((uint32_t)left_[r] + qx, (uint32_t)top_[r] + qy, 0);
                 if (head == best) {
 #pragma unroll
-                    for (int i = 0; i < 7; ++i) L[r][i] = L[r][i + 1];
-                    L[r][7] = ~0u;
+                    for (int i = 0; i < KM - 1; ++i) L[r][i] = L[r][i + 1];
+                    L[r][KM - 1] = ~0u;
                 }
             }
         }
@@ -1059,8 +1064,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
         redo[r] = last != ~0u && (last >> 10) >= kKeyCap;
         uint32_t k_use = 1;
         while (k_use * 2 <= (uint32_t)nlive_[r] && k_use * 2 <= p.group_size) k_use *= 2;
-        if (n == 8) word[r] = k_use;
-        if (live_[r] && !redo[r] && n < 9) members[(size_t)ref_[r] * 9 + n] = word[r];
+        // Lane n < KM writes member n; the group word follows them (lane KM for eight members,
+        // lane 0 again for sixteen, which fill the row).
+        if (live_[r] && !redo[r]) {
+            if (n < KM) members[(size_t)ref_[r] * (KM + 1) + n] = word[r];
+            if (n == (KM < 16 ? KM : 0)) members[(size_t)ref_[r] * (KM + 1) + KM] = k_use;
+        }
     }
 
     // ---- references whose keys reached the cap go to the exact vector search (k_centre_redo).
@@ -1078,7 +1087,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
 
 // The exact vector centre search (64-bit keys) of the references in `redo_list` (count, then
 // references); every wave walks the list with the grid's stride.
-template <int C>
+template <int C, int KM>
 __global__ __launch_bounds__(kWaves * 64) void k_centre_redo(avd_collab_pass p, const uint16_t *__restrict__ ring,
                                                              const uint32_t *__restrict__ redo_list,
                                                              uint32_t *__restrict__ members)
@@ -1086,7 +1095,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_redo(avd_collab_pass p, 
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const uint32_t count = __builtin_amdgcn_readfirstlane(redo_list[0]);
     for (uint32_t i = blockIdx.x * kWaves + wave; i < count; i += gridDim.x * kWaves)
-        centre_search_wave<C, false>(p, ring, members, (int)__builtin_amdgcn_readfirstlane(redo_list[1 + i]));
+        centre_search_wave<C, false, KM>(p, ring, members, (int)__builtin_amdgcn_readfirstlane(redo_list[1 + i]));
 }
 
 // Trajectory search (trajectory.rs:142-450) and the grid decision (fused/mod.rs:407-553) of every
@@ -1108,9 +1117,15 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_redo(avd_collab_pass p, 
 // counters' per-wave sums (static).
 constexpr size_t kLdsBudget = 64 * 1024;
 
-__host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR)
+// Volumes of a group at most (two frames per volume) and picks of earlier volumes a candidate can
+// equal at most, for KM group members.
+__host__ __device__ constexpr int trajectory_vols(int KM) { return KM / 2; }
+__host__ __device__ constexpr int trajectory_held(int KM) { return KM == 16 ? 9 : 3; }
+
+__host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR, int KM)
 {
-    return (size_t)kWaves * (NR * 256 * 2 + 16 * 4 + NR * 16 * 8 + NR * 4 * 8 * 4 * C * 4 + NR * 4 * 4 + NR * 3 * 4 + 4);
+    return (size_t)kWaves * (NR * 256 * 2 + 16 * 4 + NR * 16 * 8 + NR * trajectory_vols(KM) * 8 * 4 * C * 4 +
+                             NR * trajectory_vols(KM) * 4 + NR * trajectory_held(KM) * 4 + 4);
 }
 
 __device__ __forceinline__ uint32_t lane_mask_lt(uint64_t bal)
@@ -1125,10 +1140,10 @@ __host__ __device__ __forceinline__ int trajectory_entries(const avd_collab_pass
     const int gf = p.temporal_radius == 1 ? 2 : 4;
     const int covers = ((int)p.blksize + (int)p.blk_step - 1) / (int)p.blk_step;
     const int cshift = covers <= 1 ? 0 : (covers <= 2 ? 1 : 2);
-    return (8 / gf) * ((2 * (int)p.temporal_radius) << (2 * cshift));
+    return ((int)p.group_size / gf) * ((2 * (int)p.temporal_radius) << (2 * cshift));
 }
 
-template <int C, int K, int NR, bool SHORT>
+template <int C, int K, int NR, bool SHORT, int KM>
 __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                                                                   const uint32_t *__restrict__ nslots,
                                                                   const int32_t *__restrict__ mv,
@@ -1137,13 +1152,15 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                                                                   unsigned long long *__restrict__ scored_count)
 {
     constexpr int LPR = 64 / NR;  // lanes per reference
+    constexpr int VOLS = trajectory_vols(KM), HELD = trajectory_held(KM);
+    static_assert(LPR >= KM + 1, "a reference's lanes write its member words and group word");
     extern __shared__ uint32_t s_dyn[];  // [kWaves][NR * entries][2]: (left | right << 16), (top | bottom << 16)
     __shared__ uint16_t s_list[kWaves][NR * 256];  // non-empty entries of the current volume: e | r << 8
     __shared__ uint32_t s_slot[kWaves][16];
     __shared__ unsigned long long s_fbest[kWaves][NR][16];
-    __shared__ uint32_t s_patch[kWaves][NR][4][8][4 * C];  // every volume's anchor patch
-    __shared__ uint32_t s_energy[kWaves][NR][4];
-    __shared__ uint32_t s_held[kWaves][NR][3];
+    __shared__ uint32_t s_patch[kWaves][NR][VOLS][8][4 * C];  // every volume's anchor patch
+    __shared__ uint32_t s_energy[kWaves][NR][VOLS];
+    __shared__ uint32_t s_held[kWaves][NR][HELD];
     constexpr int refine = (K - 1) / 2;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int hr = lane / LPR, hl = lane % LPR, hbase = hr * LPR;
@@ -1156,15 +1173,15 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     const size_t slot_len = (size_t)W * H * C;
     const size_t cframe = (size_t)p.centre_slot * slot_len;
     const uint32_t ring_bytes = (uint32_t)(slot_len * p.slots * 2);
-    const uint32_t mine = members[(size_t)my_ref * 9 + min(hl, 8)];
-    const uint32_t k_use = (uint32_t)__shfl((int)mine, hbase + 8);
-    uint32_t anchors[4];
+    const uint32_t mine = members[(size_t)my_ref * (KM + 1) + min(hl, KM - 1)];
+    const uint32_t k_use = members[(size_t)my_ref * (KM + 1) + KM];
+    uint32_t anchors[VOLS];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) anchors[v] = (uint32_t)__shfl((int)mine, hbase + v);
+    for (int v = 0; v < VOLS; ++v) anchors[v] = (uint32_t)__shfl((int)mine, hbase + v);
 
     const int radius = p.temporal_radius;
     const int gf = radius == 1 ? 2 : 4;
-    const int volumes = 8 / gf;
+    const int volumes = KM / gf;
     const int bstep = p.blk_step, bsz = p.blksize, bx_n = p.blocks_x, by_n = p.blocks_y;
     const int covers = (bsz + bstep - 1) / bstep;
     const int cshift = covers <= 1 ? 0 : (covers <= 2 ? 1 : 2);
@@ -1179,7 +1196,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
         const int v = i >> 3, r = i & 7;
         uint32_t anchor = anchors[0];
 #pragma unroll
-        for (int j = 1; j < 4; ++j)
+        for (int j = 1; j < VOLS; ++j)
             if (v == j) anchor = anchors[j];
         uint32_t row[4 * C];
         load_row<C>(ring, cframe + ((size_t)(((anchor >> 13) & 0x1FFF) + r) * W + (anchor & 0x1FFF)) * C, row);
@@ -1193,7 +1210,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
         const int t = e >> nshift, b = e & ((1 << nshift) - 1);
         uint32_t anchor = anchors[0];
 #pragma unroll
-        for (int i = 1; i < 4; ++i)
+        for (int i = 1; i < VOLS; ++i)
             if (v == i) anchor = anchors[i];
         const int ax = (int)(anchor & 0x1FFF), ay = (int)((anchor >> 13) & 0x1FFF);
         const int bx_hi = min(ax / bstep, bx_n - 1), by_hi = min(ay / bstep, by_n - 1);
@@ -1228,21 +1245,30 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
 
     // final[] collects the grid's members.  A volume's list holds its gf - 1 frame picks and their
     // distances (SSD + 1 fits 32 bits; ~0u marks an empty slot).  Only earlier volumes' picks can
-    // equal a candidate (anchors sit in the centre frame), at most three of them.  All of this is
-    // the same on every lane of one reference.
+    // equal a candidate (anchors sit in the centre frame), at most HELD of them (3 for eight
+    // members, 9 for sixteen; position word 0, in the centre frame, matches no candidate).  All of
+    // this is the same on every lane of one reference.
     uint32_t scored = 0;
-    uint32_t final_pos[8], held[3] = {0u, 0u, 0u};
+    uint32_t final_pos[KM], held[HELD];
     bool filled = true;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) final_pos[i] = 0;
+    for (int i = 0; i < KM; ++i) final_pos[i] = 0;
+#pragma unroll
+    for (int i = 0; i < HELD; ++i) held[i] = 0u;
     for (int v = 0; v < volumes; ++v) {
         const int first = v * gf;
         uint32_t anchor = anchors[0];
 #pragma unroll
-        for (int i = 1; i < 4; ++i)
+        for (int i = 1; i < VOLS; ++i)
             if (v == i) anchor = anchors[i];
         uint32_t vd[3] = {~0u, ~0u, ~0u}, vp[3] = {0u, 0u, 0u};
-        if (hl < 3) s_held[wave][hr][hl] = hl == 0 ? held[0] : (hl == 1 ? held[1] : held[2]);
+        if (hl < HELD) {
+            uint32_t h = held[0];
+#pragma unroll
+            for (int i = 1; i < HELD; ++i)
+                if (hl == i) h = held[i];
+            s_held[wave][hr][hl] = h;
+        }
         // Compact the rectangles that hold a position no earlier rectangle of their frame covers
         // (an empty or fully covered one adds nothing but its share of the walk index), in
         // (reference, entry) order.
@@ -1295,7 +1321,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                 }
             }
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
+            for (int i = 0; i < HELD; ++i) {
                 const uint32_t g = s_held[wave][tr][i];
                 const int gy = (int)((g >> 13) & 0x1FFF);
                 if ((int)(g >> 26) == t + 1 && (int)(g & 0x1FFF) == cx && gy >= tp && gy <= bt)
@@ -1366,7 +1392,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
         }
         filled = filled && (gf == 2 ? vd[0] : vd[2]) != ~0u;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < KM; ++i) {
             const int j = i - first - 1;
             if (i == first) final_pos[i] = anchor;
             else if (j >= 0 && j < gf - 1) final_pos[i] = j == 0 ? vp[0] : (j == 1 ? vp[1] : vp[2]);
@@ -1374,20 +1400,20 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             if (j < gf - 1) {  // this volume's picks are held from the next volume on
-                held[2] = held[1];
-                held[1] = held[0];
+#pragma unroll
+                for (int i = HELD - 1; i > 0; --i) held[i] = held[i - 1];
                 held[0] = vp[j];
             }
         }
         wave_sync();
     }
-    const bool use_grid = k_use == 8 && filled;
-    if (use_grid && live && hl < 9) {
-        uint32_t word = 8u | 0x100u;
+    const bool use_grid = k_use == KM && filled;
+    if (use_grid && live && hl < KM + 1) {
+        uint32_t word = (uint32_t)KM | 0x100u;
 #pragma unroll
-        for (int m = 0; m < 8; ++m)
+        for (int m = 0; m < KM; ++m)
             if (hl == m) word = final_pos[m];
-        members[(size_t)my_ref * 9 + hl] = word;
+        members[(size_t)my_ref * (KM + 1) + hl] = word;
     }
     add_scored(scored_count, scored, true, C);
 }
@@ -1455,34 +1481,39 @@ __device__ __forceinline__ void haar_level(float (&v)[8], bool active, bool low)
     }
 }
 
-// The group Haar over the member lanes (grid.rs:119-220, stack_haar_*, fused/mod.rs:854-892).
-// Level i pairs members 2^i apart (lanes 8 << i apart) in every layout: a stack of k_use members
-// runs log2(k_use) levels; a grid of gf frames per volume runs the time levels (pairs inside a
-// volume) and then the volume levels (pairs of volumes), three in all for eight members.
+// The group Haar over the members (grid.rs:119-220, stack_haar_*, fused/mod.rs:854-892).  Level i
+// pairs members 2^i apart in every layout: lanes 8 << i apart for i < 3, and for sixteen members
+// (held as two halves per lane, members m and m + 8) level 3 pairs the two halves inside the lane.
+// A stack of k_use members runs log2(k_use) levels; a grid of gf frames per volume runs the time
+// levels (pairs inside a volume, only approximations continue) and then the volume levels (pairs
+// of volumes, every time coefficient), log2(members) in all.
 struct HaarPlan {
     int levels;
-    bool act[3], low[3];
+    bool act[4], low[4];
 };
 
 __device__ __forceinline__ HaarPlan haar_plan(int m, int k_use, bool grid, int gf)
 {
     HaarPlan h;
-    if (grid) {
-        h.levels = 3;
+    if (grid) {  // k_use members: gf frames per volume, k_use / gf volumes
         const int f = m % gf, vol = m / gf;
-        if (gf == 4) {  // time levels 1, 2; one volume level
-            h.act[0] = true, h.low[0] = (f & 1) == 0;
-            h.act[1] = (f & 1) == 0, h.low[1] = (f & 2) == 0;
-            h.act[2] = true, h.low[2] = (vol & 1) == 0;
-        } else {  // gf == 2: one time level; volume levels 1, 2
-            h.act[0] = true, h.low[0] = f == 0;
-            h.act[1] = true, h.low[1] = (vol & 1) == 0;
-            h.act[2] = (vol & 1) == 0, h.low[2] = (vol & 2) == 0;
+        const int tl = gf == 4 ? 2 : 1;  // time levels
+        h.levels = k_use >= 16 ? 4 : 3;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i < tl) {
+                h.act[i] = (f & ((1 << i) - 1)) == 0;
+                h.low[i] = ((f >> i) & 1) == 0;
+            } else {
+                const int j = i - tl;  // volume level j pairs volumes 2^j apart
+                h.act[i] = (vol & ((1 << j) - 1)) == 0;
+                h.low[i] = ((vol >> j) & 1) == 0;
+            }
         }
     } else {
-        h.levels = k_use >= 8 ? 3 : (k_use >= 4 ? 2 : (k_use >= 2 ? 1 : 0));
+        h.levels = k_use >= 16 ? 4 : (k_use >= 8 ? 3 : (k_use >= 4 ? 2 : (k_use >= 2 ? 1 : 0)));
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < 4; ++i) {
             h.act[i] = (m & ((1 << i) - 1)) == 0 && m < k_use;
             h.low[i] = ((m >> i) & 1) == 0;
         }
@@ -1490,18 +1521,41 @@ __device__ __forceinline__ HaarPlan haar_plan(int m, int k_use, bool grid, int g
     return h;
 }
 
-__device__ __forceinline__ void group_forward(float (&v)[8], const HaarPlan &h)
+// Level 3 of sixteen members: member m (lo) and m + 8 (hi) in one lane.
+__device__ __forceinline__ void haar_level_halves(float (&lo)[8], float (&hi)[8], bool active)
 {
-    if (h.levels > 0) haar_level<8>(v, h.act[0], h.low[0]);
-    if (h.levels > 1) haar_level<16>(v, h.act[1], h.low[1]);
-    if (h.levels > 2) haar_level<32>(v, h.act[2], h.low[2]);
+    const float f = 0.70710678118654752440f;
+    if (!active) return;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const float a = lo[u], b = hi[u];
+        lo[u] = (a + b) * f;
+        hi[u] = (a - b) * f;
+    }
 }
 
-__device__ __forceinline__ void group_inverse(float (&v)[8], const HaarPlan &h)
+template <int HM>
+__device__ __forceinline__ void group_forward(float (&v)[HM][8], const HaarPlan (&h)[HM])
 {
-    if (h.levels > 2) haar_level<32>(v, h.act[2], h.low[2]);
-    if (h.levels > 1) haar_level<16>(v, h.act[1], h.low[1]);
-    if (h.levels > 0) haar_level<8>(v, h.act[0], h.low[0]);
+#pragma unroll
+    for (int s = 0; s < HM; ++s) {
+        if (h[s].levels > 0) haar_level<8>(v[s], h[s].act[0], h[s].low[0]);
+        if (h[s].levels > 1) haar_level<16>(v[s], h[s].act[1], h[s].low[1]);
+        if (h[s].levels > 2) haar_level<32>(v[s], h[s].act[2], h[s].low[2]);
+    }
+    if (HM == 2 && h[0].levels > 3) haar_level_halves(v[0], v[HM - 1], h[0].act[3]);
+}
+
+template <int HM>
+__device__ __forceinline__ void group_inverse(float (&v)[HM][8], const HaarPlan (&h)[HM])
+{
+    if (HM == 2 && h[0].levels > 3) haar_level_halves(v[0], v[HM - 1], h[0].act[3]);
+#pragma unroll
+    for (int s = 0; s < HM; ++s) {
+        if (h[s].levels > 2) haar_level<32>(v[s], h[s].act[2], h[s].low[2]);
+        if (h[s].levels > 1) haar_level<16>(v[s], h[s].act[1], h[s].low[1]);
+        if (h[s].levels > 0) haar_level<8>(v[s], h[s].act[0], h[s].low[0]);
+    }
 }
 
 __device__ __forceinline__ int32_t to_fixed(float v, float scale)
@@ -1534,8 +1588,8 @@ __device__ __forceinline__ void unpack_aggregate(unsigned long long v, int32_t &
 // STAGED: instead of atomics into the accumulators, each member's fixed-point values go to
 // `stage` ([ref][member][c][row][column] i32: a member's patch is 256 contiguous bytes per channel)
 // and the member enters the list of every 32x32 tile its patch overlaps; k_tile_gather then sums
-// them per tile.
-template <int C, bool STAGED>
+// them per tile.  KM group members: lane (m, q) holds column q of members m + 8 h, h < KM / 8.
+template <int C, bool STAGED, int KM>
 __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p, const float *__restrict__ values,
                                                                const uint32_t *__restrict__ nslots,
                                                                const uint32_t *__restrict__ members,
@@ -1543,6 +1597,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
                                                                int32_t *__restrict__ accum_second,
                                                                float *__restrict__ group_weight, TileBins tiles)
 {
+    constexpr int HM = KM / 8;
+    constexpr int KSHIFT = KM == 16 ? 4 : 3;
     __shared__ float s_t[kWaves][8][8][9];  // [wave][member][row][col], padded
     // Per channel and coefficient (v, u): the variance sigma^2 prof[u] prof[v] and the hard
     // threshold lambda sqrt(variance), the same f32 expressions for every reference, once per
@@ -1563,38 +1619,49 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
     int ref = xcd_block() * kWaves + wave;
     const bool live = ref < refs;
     if (!live) ref = refs - 1;
-    const uint32_t *mw = members + (size_t)ref * 9;
-    const uint32_t gword = mw[8];
+    const uint32_t *mw = members + (size_t)ref * (KM + 1);
+    const uint32_t gword = mw[KM];
     const int k_use = (int)(gword & 0xFF);
     const bool grid = (gword & 0x100u) != 0;
     const int gf = p.temporal_radius == 0 ? 1 : (p.temporal_radius == 1 ? 2 : 4);
-    const HaarPlan plan = haar_plan(m, k_use, grid, gf);
-    const uint32_t pos = mw[m];
-    const uint32_t t = pos >> 26;
-    const int slot = t > 0 ? (int)nslots[t - 1] : (int)p.centre_slot;
-    const int mx = (int)(pos & 0x1FFF), my = (int)((pos >> 13) & 0x1FFF);
     const size_t px = (size_t)W * H;
-    const float *src = values + (size_t)slot * px * C;
+    HaarPlan plan[HM];
+    int slot[HM], mx[HM], my[HM];
+    bool in_group[HM];
+#pragma unroll
+    for (int h = 0; h < HM; ++h) {
+        const int mm = m + 8 * h;
+        plan[h] = haar_plan(mm, k_use, grid, gf);
+        const uint32_t pos = mw[mm];
+        const uint32_t t = pos >> 26;
+        slot[h] = t > 0 ? (int)nslots[t - 1] : (int)p.centre_slot;
+        mx[h] = (int)(pos & 0x1FFF);
+        my[h] = (int)((pos >> 13) & 0x1FFF);
+        in_group[h] = mm < k_use;
+    }
     const float scale = p.accum_scale;
-    const bool in_group = m < k_use;
     const int v = q;  // after the first transpose lane (m, q) holds vertical frequency v = q
     float scaled_weight = 0.0f;
 
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        float col[8];
+        float row[HM][8];  // lane (m, v): coefficients (v, u) of members m + 8 h, u = 0..7
 #pragma unroll
-        for (int r = 0; r < 8; ++r) col[r] = src[((size_t)(my + r) * W + mx + q) * C + c];
-        dct8_fwd(col);  // vertical: col[v]
+        for (int h = 0; h < HM; ++h) {
+            const float *src = values + (size_t)slot[h] * px * C;
+            float col[8];
 #pragma unroll
-        for (int vv = 0; vv < 8; ++vv) s_t[wave][m][vv][q] = col[vv];
-        wave_sync();
-        float row[8];  // lane (m, v): coefficients (v, u) for u = 0..7
+            for (int r = 0; r < 8; ++r) col[r] = src[((size_t)(my[h] + r) * W + mx[h] + q) * C + c];
+            dct8_fwd(col);  // vertical: col[v]
 #pragma unroll
-        for (int u = 0; u < 8; ++u) row[u] = s_t[wave][m][q][u];
-        wave_sync();
-        dct8_fwd(row);
-        group_forward(row, plan);
+            for (int vv = 0; vv < 8; ++vv) s_t[wave][m][vv][q] = col[vv];
+            wave_sync();
+#pragma unroll
+            for (int u = 0; u < 8; ++u) row[h][u] = s_t[wave][m][q][u];
+            wave_sync();
+            dct8_fwd(row[h]);
+        }
+        group_forward<HM>(row, plan);
 
         const float sig2 = p.sigma[c] * p.sigma[c];
         const float lam = p.lambda[c];
@@ -1608,10 +1675,6 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
             // takes its own test at lambda and the group DC is always kept.
             const float lv = sig2 * pv;
             const float li = 1.0f / fmaxf(lv, 1e-20f);
-            float e[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) e[u] = row[u] * row[u] * li * (1.0f / fmaxf(p.profile[u], 1e-20f));
-            if (v == 0) e[0] = 0.0f;
             const bool has_up = v > 0, has_down = v < 7;
             const float vertical = 1.0f + (has_up ? 1.0f : 0.0f) + (has_down ? 1.0f : 0.0f);
             const float threshold = lam * p.pool_ratio;
@@ -1619,50 +1682,67 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
             const float bar_first = bar * (vertical + (v == 1 ? 0.0f : 1.0f));
             const float bar_second = bar * (vertical + (v == 0 ? 1.0f : 2.0f));
             const float bar_middle = bar * (vertical + 2.0f), bar_last = bar * (vertical + 1.0f);
-            float kept_profile = 0.0f;
+            float kept_profile[HM];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const float above = __shfl_up(e[u], 1), below = __shfl_down(e[u], 1);
-                float sum = e[u] + (has_down ? below : 0.0f) + (has_up ? above : 0.0f);
-                if (u > 0) sum += e[u - 1];
-                if (u < 7) sum += e[u + 1];
-                const float bar_u = u == 0 ? bar_first : (u == 1 ? bar_second : (u == 7 ? bar_last : bar_middle));
-                bool keep = sum >= bar_u;
-                if (u == 0) {
-                    const bool own = fabsf(row[0]) >= lam * sqrtf(lv * p.profile[0]);
-                    if (v == 0) keep = own;
-                    if (m == 0) keep = keep || v == 0;
+            for (int h = 0; h < HM; ++h) {
+                float e[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    e[u] = row[h][u] * row[h][u] * li * (1.0f / fmaxf(p.profile[u], 1e-20f));
+                if (v == 0) e[0] = 0.0f;
+                kept_profile[h] = 0.0f;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const float above = __shfl_up(e[u], 1), below = __shfl_down(e[u], 1);
+                    float sum = e[u] + (has_down ? below : 0.0f) + (has_up ? above : 0.0f);
+                    if (u > 0) sum += e[u - 1];
+                    if (u < 7) sum += e[u + 1];
+                    const float bar_u = u == 0 ? bar_first : (u == 1 ? bar_second : (u == 7 ? bar_last : bar_middle));
+                    bool keep = sum >= bar_u;
+                    if (u == 0) {
+                        const bool own = fabsf(row[h][0]) >= lam * sqrtf(lv * p.profile[0]);
+                        if (v == 0) keep = own;
+                        if (m == 0 && h == 0) keep = keep || v == 0;
+                    }
+                    kept_profile[h] += keep ? p.profile[u] : 0.0f;
+                    if (in_group[h] && !keep) row[h][u] = 0.0f;
                 }
-                kept_profile += keep ? p.profile[u] : 0.0f;
-                if (in_group && !keep) row[u] = 0.0f;
             }
             // The reference's lane v adds each member's kept profile times its variance, in member
-            // order; lane (j, v) holds member j's.
+            // order; lane (j % 8, v) holds member j's in half j / 8.
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const float kp = __shfl(kept_profile, (j << 3) | v);
+            for (int j = 0; j < KM; ++j) {
+                const float kp = __shfl(kept_profile[j / 8], ((j & 7) << 3) | v);
                 if (j < k_use) part += kp * lv;
             }
         } else {
-            uint32_t counts = 0;  // kept coefficients per u, one nibble each
+            // Kept coefficients per u, one nibble each, per half (at most eight members a half).
+            uint32_t counts[HM];
             float cvs[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                cvs[u] = s_cvs[c][v * 8 + u];
-                bool keep = fabsf(row[u]) >= s_thr[c][v * 8 + u];
-                if (u == 0 && m == 0 && v == 0) keep = true;
-                keep = keep && in_group;
-                if (keep) counts += 1u << (4 * u);
-                else if (in_group) row[u] = 0.0f;
+            for (int u = 0; u < 8; ++u) cvs[u] = s_cvs[c][v * 8 + u];
+#pragma unroll
+            for (int h = 0; h < HM; ++h) {
+                counts[h] = 0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    bool keep = fabsf(row[h][u]) >= s_thr[c][v * 8 + u];
+                    if (u == 0 && m == 0 && h == 0 && v == 0) keep = true;
+                    keep = keep && in_group[h];
+                    if (keep) counts[h] += 1u << (4 * u);
+                    else if (in_group[h]) row[h][u] = 0.0f;
+                }
+                counts[h] += (uint32_t)__shfl_xor((int)counts[h], 8);
+                counts[h] += (uint32_t)__shfl_xor((int)counts[h], 16);
+                counts[h] += (uint32_t)__shfl_xor((int)counts[h], 32);
             }
-            counts += (uint32_t)__shfl_xor((int)counts, 8);
-            counts += (uint32_t)__shfl_xor((int)counts, 16);
-            counts += (uint32_t)__shfl_xor((int)counts, 32);
             // Retained variance: each lane (m, v) adds, for every u, the variance of (u, v) once per
             // member that kept it; the eight v lanes then sum by butterfly.
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int n = (counts >> (4 * u)) & 15;
+                int n = 0;
+#pragma unroll
+                for (int h = 0; h < HM; ++h) n += (counts[h] >> (4 * u)) & 15;
                 for (int i = 0; i < n; ++i) part += cvs[u];
             }
         }
@@ -1676,76 +1756,95 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
             scaled_weight = weight * p.weight_scale;
         }
 
-        group_inverse(row, plan);
-        dct8_inv(row);  // lane (m, v): values at columns k for vertical frequency v
+        group_inverse<HM>(row, plan);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) s_t[wave][m][v][k] = row[k];
-        wave_sync();
+        for (int h = 0; h < HM; ++h) {
+            const int mm = m + 8 * h;
+            dct8_inv(row[h]);  // lane (m, v): values at columns k for vertical frequency v
 #pragma unroll
-        for (int vv = 0; vv < 8; ++vv) col[vv] = s_t[wave][m][vv][q];
-        wave_sync();
-        dct8_inv(col);  // rows of column q
+            for (int k = 0; k < 8; ++k) s_t[wave][m][v][k] = row[h][k];
+            wave_sync();
+            float col[8];
+#pragma unroll
+            for (int vv = 0; vv < 8; ++vv) col[vv] = s_t[wave][m][vv][q];
+            wave_sync();
+            dct8_inv(col);  // rows of column q
 
-        if (STAGED) {
-            if (live) {
-                int32_t *st = tiles.stage + (((size_t)ref * 8 + m) * C + c) * 64 + q;
+            if (STAGED) {
+                if (live) {
+                    int32_t *st = tiles.stage + (((size_t)ref * KM + mm) * C + c) * 64 + q;
+                    const float kq = p.kaiser[q];
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        const float w = scaled_weight * (p.kaiser[r] * kq);
+                        st[r * 8] = in_group[h] ? to_fixed(col[r] * w, scale) : 0;
+                    }
+                }
+            } else if (live && in_group[h]) {
+                const size_t base = (size_t)slot[h] * px;
                 const float kq = p.kaiser[q];
 #pragma unroll
                 for (int r = 0; r < 8; ++r) {
                     const float w = scaled_weight * (p.kaiser[r] * kq);
-                    st[r * 8] = in_group ? to_fixed(col[r] * w, scale) : 0;
+                    const size_t pix = base + (size_t)(my[h] + r) * W + mx[h] + q;
+                    const int32_t a = to_fixed(col[r] * w, scale);
+                    if (c == 0) atomicAdd(&aggregate[pix], pack_aggregate(a, to_fixed_weight(w, scale)));
+                    else atomicAdd(&accum_second[pix], a);
                 }
             }
-            if (c == 0) {
-                // Enter each member in the list of every tile its patch overlaps.  Lane (m, q < 4)
-                // holds claim q of member m.  The workgroup counts its claims per tile in an LDS
-                // hash table (rank = the bucket's count before the claim), one atomic per
-                // distinct tile reserves the entries, and a full list sends the (member, tile)
-                // pair to the overflow list.
-                constexpr int kBuckets = kWaves * 64;
-                __shared__ uint32_t s_key[kBuckets], s_cnt[kBuckets], s_base[kBuckets];
-                s_key[threadIdx.x] = ~0u;
-                s_cnt[threadIdx.x] = 0;
-                __syncthreads();
-                const int tx0 = mx >> kTileShift, ty0 = my >> kTileShift;
-                const int nx = ((mx + 7) >> kTileShift) - tx0 + 1, ny = ((my + 7) >> kTileShift) - ty0 + 1;
-                uint32_t tile = ~0u, bucket = 0, rank = 0;
-                if (live && in_group && q < nx * ny) {
-                    tile = (uint32_t)slot * tiles.per_slot + (ty0 + q / nx) * tiles.tiles_x + tx0 + q % nx;
-                    bucket = (tile * 2654435761u) >> 24;
-                    for (;;) {  // at most 128 keys in 256 buckets: a free or equal bucket exists
-                        const uint32_t old = atomicCAS(&s_key[bucket], ~0u, tile);
-                        if (old == ~0u || old == tile) break;
-                        bucket = (bucket + 1) & (kBuckets - 1);
+        }
+        if (STAGED && c == 0) {
+            // Enter each member in the list of every tile its patch overlaps.  Lane (m, q < 4)
+            // holds claim q of members m + 8 h.  The workgroup counts its claims per tile in an LDS
+            // hash table (rank = the bucket's count before the claim), one atomic per distinct tile
+            // reserves the entries, and a full list sends the (member, tile) pair to the overflow
+            // list.
+            constexpr int kBuckets = kWaves * 64 * HM;
+            __shared__ uint32_t s_key[kBuckets], s_cnt[kBuckets], s_base[kBuckets];
+#pragma unroll
+            for (int h = 0; h < HM; ++h) {
+                s_key[threadIdx.x + h * kWaves * 64] = ~0u;
+                s_cnt[threadIdx.x + h * kWaves * 64] = 0;
+            }
+            __syncthreads();
+            uint32_t tile[HM], bucket[HM], rank[HM];
+#pragma unroll
+            for (int h = 0; h < HM; ++h) {
+                const int tx0 = mx[h] >> kTileShift, ty0 = my[h] >> kTileShift;
+                const int nx = ((mx[h] + 7) >> kTileShift) - tx0 + 1, ny = ((my[h] + 7) >> kTileShift) - ty0 + 1;
+                tile[h] = ~0u, bucket[h] = 0, rank[h] = 0;
+                if (live && in_group[h] && q < nx * ny) {
+                    tile[h] = (uint32_t)slot[h] * tiles.per_slot + (ty0 + q / nx) * tiles.tiles_x + tx0 + q % nx;
+                    uint32_t b = (tile[h] * 2654435761u) >> (KM == 16 ? 23 : 24);
+                    for (;;) {  // at most half as many keys as buckets: a free or equal bucket exists
+                        const uint32_t old = atomicCAS(&s_key[b], ~0u, tile[h]);
+                        if (old == ~0u || old == tile[h]) break;
+                        b = (b + 1) & (kBuckets - 1);
                     }
-                    rank = atomicAdd(&s_cnt[bucket], 1u);
+                    bucket[h] = b;
+                    rank[h] = atomicAdd(&s_cnt[b], 1u);
                 }
-                __syncthreads();
-                if (s_key[threadIdx.x] != ~0u)
-                    s_base[threadIdx.x] = atomicAdd(&tiles.count[s_key[threadIdx.x]], s_cnt[threadIdx.x]);
-                __syncthreads();
-                if (tile != ~0u) {
-                    const uint32_t k = s_base[bucket] + rank;
-                    const uint32_t id = ((uint32_t)ref << 3) | (uint32_t)m;
-                    if (k < kTileCap) {
-                        tiles.list[(size_t)tile * kTileCap + k] = id;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < HM; ++h) {
+                const int i = threadIdx.x + h * kWaves * 64;
+                if (s_key[i] != ~0u) s_base[i] = atomicAdd(&tiles.count[s_key[i]], s_cnt[i]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < HM; ++h) {
+                if (tile[h] != ~0u) {
+                    const uint32_t k = s_base[bucket[h]] + rank[h];
+                    const uint32_t id = ((uint32_t)ref << KSHIFT) | (uint32_t)(m + 8 * h);
+                    if (k < tiles.cap) {
+                        tiles.list[(size_t)tile[h] * tiles.cap + k] = id;
                     } else {
                         const uint32_t o = atomicAdd(tiles.overflow_count, 1u);
                         tiles.overflow[2 * (size_t)o] = id;
-                        tiles.overflow[2 * (size_t)o + 1] = tile;
+                        tiles.overflow[2 * (size_t)o + 1] = tile[h];
                     }
                 }
-            }
-        } else if (live && in_group) {
-            const size_t base = (size_t)slot * px;
-            const float kq = p.kaiser[q];
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const float w = scaled_weight * (p.kaiser[r] * kq);
-                const size_t pix = base + (size_t)(my + r) * W + mx + q;
-                const int32_t a = to_fixed(col[r] * w, scale);
-                if (c == 0) atomicAdd(&aggregate[pix], pack_aggregate(a, to_fixed_weight(w, scale)));
-                else atomicAdd(&accum_second[pix], a);
             }
         }
     }
@@ -1755,13 +1854,15 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter(avd_collab_pass p
 // filter's own arithmetic) for the pixels of row r that fall in [x0, x0 + T) x [y0, y0 + T).
 // MIXED: the stage holds the filtered samples as f16; the window and the group weight are applied
 // here, as the FP32 form's filter applies them.
-template <int C, bool MIXED, typename Add>
+// Member ids are (reference << log2 KM) | member, KM the group size.
+template <int C, bool MIXED, int KM, typename Add>
 __device__ __forceinline__ void member_row(const avd_collab_pass &p, const uint32_t *__restrict__ members,
                                            const float *__restrict__ group_weight, const TileBins &tiles,
                                            uint32_t id, int r, int x0, int y0, int T, Add &&add)
 {
-    const uint32_t ref = id >> 3, m = id & 7;
-    const uint32_t pos = members[(size_t)ref * 9 + m];
+    constexpr int KSHIFT = KM == 16 ? 4 : 3;
+    const uint32_t ref = id >> KSHIFT, m = id & (KM - 1);
+    const uint32_t pos = members[(size_t)ref * (KM + 1) + m];
     const int y = (int)((pos >> 13) & 0x1FFF) + r - y0;
     if (y < 0 || y >= T) return;
     const int xb = (int)(pos & 0x1FFF) - x0;
@@ -1854,27 +1955,28 @@ __device__ __forceinline__ void split_frag(const V &acc, int base, half8_t &hi, 
 
 // One Haar butterfly level, scaled by f, applied to four unit-vector columns held across the
 // member lanes (a reference's members are 8 consecutive lanes; partners are XOR apart).
-template <int XOR>
-__device__ __forceinline__ void haar_columns(float (&v)[4], bool active, bool low, float f)
+template <int XOR, int N>
+__device__ __forceinline__ void haar_columns(float (&v)[N], bool active, bool low, float f)
 {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < N; ++k) {
         const float other = __shfl_xor(v[k], XOR);
         if (active) v[k] = low ? (v[k] + other) * f : (other - v[k]) * f;
     }
 }
 
-// The filter of one group of four references (the wave's), from its samples converted to f16
-// (`pa[c][s]`: rows 2 s + h of member n, channel c), writing the group weights and the staged
-// samples.
-template <int C>
+// The filter of the wave's 32 members (four references of eight, or two of sixteen: KM), from
+// their samples converted to f16 (`pa[c][s]`: rows 2 s + h of member n, channel c), writing the
+// group weights and the staged samples.
+template <int C, int KM>
 __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint4 (*s_frag)[64],
                                             const float (*s_cvs)[64], const float (*s_thr)[64],
                                             const half8_t (&pa)[C][4], float inv_s, int first, int refs,
-                                            const int (&k_use)[4], uint32_t gword, int lane,
+                                            const int (&k_use)[32 / KM], uint32_t gword, int lane,
                                             float *__restrict__ group_weight, const TileBins &tiles)
 {
-    const int n = lane & 31, h = lane >> 5, m = n & 7, jn = n >> 3;
+    constexpr int NREF = 32 / KM, KSHIFT = KM == 16 ? 4 : 3;
+    const int n = lane & 31, h = lane >> 5, m = n & (KM - 1), jn = n >> KSHIFT;
     const int ref = min(first + jn, refs - 1);
     const bool live = first + jn < refs;
     const int k_use_n = (int)(gword & 0xFF);
@@ -1886,22 +1988,27 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
     // forward transform as f^L (f = 1/sqrt 2) times a sum of its group's coefficients with signs
     // +-1, so H = D S with S of +-1 and 0 (exact in f16) and D = diag(f^L); the inverse is
     // H^T = S^T D.  Step 2 multiplies by S only, step 3 compares |S X1| f^L with the threshold, and
-    // step 4 multiplies by S^T D^2, whose entries +-2^-L are exact too.  Lane (n, h) holds
-    // S[m][4 h + k] and (S^T D^2)[m][4 h + k] of its reference (the butterflies run on unit vectors
-    // with factors 1 and 1/2), which sit in K step jn / 2 at slots 4 (jn % 2) + k of the step-2 A
-    // and step-4 B operands.
+    // step 4 multiplies by S^T D^2, whose entries +-2^-L are exact too.  K slot i of lane half h in
+    // K step s is wave member 16 s + 8 (i / 4) + 4 h + i % 4: for eight members that is member
+    // 4 h + i % 4 of reference 2 s + i / 4, for sixteen member 8 (i / 4) + 4 h + i % 4 of reference
+    // s.  Lane (n, h) holds S[m][col] and (S^T D^2)[m][col] of its reference for those columns (the
+    // butterflies run on unit vectors with factors 1 and 1/2), in the step-2 A and step-4 B
+    // operands.
+    constexpr int NC = KM / 2;  // columns of its reference's blocks a lane holds
     half8_t hfrag[2], gfrag[2];
-    // f^L of member 4 h + i of reference j (the rows this lane holds after step 2), from the lane
-    // of that member.
+    // f^L of wave member 8 j + 4 h + i (the rows this lane holds after step 2), from the lane of
+    // that member.
     float row_scale[4][4];
     {
         const HaarPlan plan = haar_plan(m, k_use_n, grid, gf);
-        float hs[4], hg[4];
+        float hs[NC], hg[NC];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) hs[k] = hg[k] = m == 4 * h + k ? 1.0f : 0.0f;
+        for (int k = 0; k < NC; ++k) hs[k] = hg[k] = m == 8 * (k >> 2) + 4 * h + (k & 3) ? 1.0f : 0.0f;
         haar_columns<1>(hs, plan.levels > 0 && plan.act[0], plan.low[0], 1.0f);
         haar_columns<2>(hs, plan.levels > 1 && plan.act[1], plan.low[1], 1.0f);
         haar_columns<4>(hs, plan.levels > 2 && plan.act[2], plan.low[2], 1.0f);
+        if (KM == 16) haar_columns<8>(hs, plan.levels > 3 && plan.act[3], plan.low[3], 1.0f);
+        if (KM == 16) haar_columns<8>(hg, plan.levels > 3 && plan.act[3], plan.low[3], 0.5f);
         haar_columns<4>(hg, plan.levels > 2 && plan.act[2], plan.low[2], 0.5f);
         haar_columns<2>(hg, plan.levels > 1 && plan.act[1], plan.low[1], 0.5f);
         haar_columns<1>(hg, plan.levels > 0 && plan.act[0], plan.low[0], 0.5f);
@@ -1909,13 +2016,14 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
         for (int s = 0; s < 2; ++s)
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const bool mine = s == (jn >> 1) && (i >> 2) == (jn & 1);
-                hfrag[s][i] = (_Float16)(mine ? hs[i & 3] : 0.0f);
-                gfrag[s][i] = (_Float16)(mine ? hg[i & 3] : 0.0f);
+                const bool mine = KM == 16 ? s == jn : (s == (jn >> 1) && (i >> 2) == (jn & 1));
+                const int k = KM == 16 ? i : (i & 3);
+                hfrag[s][i] = (_Float16)(mine ? hs[k] : 0.0f);
+                gfrag[s][i] = (_Float16)(mine ? hg[k] : 0.0f);
             }
         const int L = (plan.levels > 0 && plan.act[0]) + (plan.levels > 1 && plan.act[1]) +
-                      (plan.levels > 2 && plan.act[2]);
-        const float own = ((L & 1) ? 0.70710678118654752440f : 1.0f) * (L >= 2 ? 0.5f : 1.0f);
+                      (plan.levels > 2 && plan.act[2]) + (plan.levels > 3 && plan.act[3]);
+        const float own = ((L & 1) ? 0.70710678118654752440f : 1.0f) * __builtin_ldexpf(1.0f, -(L >> 1));
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -1950,8 +2058,8 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
             }
         }
         // ---- 3. hard threshold (the group DC always kept) and the retained variance of each
-        // reference, from the first channel.  Groups hold 1, 2, 4 or 8 members and the Haar pairs
-        // members inside the group, so the members past k_use (zeroed here like any dropped
+        // reference, from the first channel.  Groups hold 1, 2, 4, 8 or 16 members and the Haar
+        // pairs members inside the group, so the members past k_use (zeroed here like any dropped
         // coefficient) never reach the group's; they only leave the variance sum.
         float y[2][16];
         float kept[4][4];  // [reference j][register i]: retained variance over both tiles
@@ -1964,22 +2072,29 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float v = x2[mt][4 * j + i];
-                    const bool keep = fabsf(v) * row_scale[j][i] >= (i == 0 ? thr_dc : thr);
+                    // Row 8 j (i = 0, lane half 0) is member 0 of a reference for every j with eight
+                    // members, for even j with sixteen.
+                    const bool first_member = i == 0 && (KM == 8 || (j & 1) == 0);
+                    const bool keep = fabsf(v) * row_scale[j][i] >= (first_member ? thr_dc : thr);
                     y[mt][4 * j + i] = keep ? v : 0.0f;
                     const float add = keep ? cvs : 0.0f;
                     kept[j][i] = mt == 0 ? add : kept[j][i] + add;
                 }
         }
         if (c == 0) {
+            // Reference R holds register blocks j = R (KM / 8) + jj, rows 8 jj + 4 h + i of its group.
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int R = 0; R < NREF; ++R) {
                 float part = 0.0f;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) part += 4 * h + i < k_use[j] ? kept[j][i] : 0.0f;
+                for (int jj = 0; jj < KM / 8; ++jj)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        part += 8 * jj + 4 * h + i < k_use[R] ? kept[R * (KM / 8) + jj][i] : 0.0f;
 #pragma unroll
                 for (int o = 1; o < 64; o <<= 1) part += __shfl_xor(part, o);
                 const float weight = (isnan(part) || isinf(part)) ? 0.0f : 1.0f / fmaxf(part, 1e-12f);
-                if (lane == 0 && first + j < refs) group_weight[first + j] = weight;
+                if (lane == 0 && first + R < refs) group_weight[first + R] = weight;
             }
         }
         // ---- 4. inverse Haar (S^T D^2, on hi + lo): X3 tile mt holds coefficients 32 mt + 8 j + 4 h + i of member n.
@@ -2007,7 +2122,7 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
                     0, 0, 0);
         }
         if (live && in_group) {
-            _Float16 *st = reinterpret_cast<_Float16 *>(tiles.stage) + (((size_t)ref * 8 + m) * C + c) * 64 + 4 * h;
+            _Float16 *st = reinterpret_cast<_Float16 *>(tiles.stage) + (((size_t)ref * KM + m) * C + c) * 64 + 4 * h;
 #pragma unroll
             for (int pt = 0; pt < 2; ++pt)
 #pragma unroll
@@ -2021,40 +2136,42 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
     }
 }
 
-// One group of four references per wave.  The workgroup first enters each member in the list of
+// One group of 32 members per wave (four references of eight, two of sixteen).  The workgroup
+// first enters each member in the list of
 // every tile its patch overlaps (lane (n, h) holds claims 2 h, 2 h + 1 of member n): its claims are
 // counted per tile in an LDS hash table (rank = the bucket's count before the claim) and one
 // atomic per distinct tile reserves the entries.  Those atomics, the sample loads and the table
 // copy are then in flight together, and the reserved positions are only read after the filter,
 // when the list entries are written (a full list sends the (member, tile) pair to the overflow
 // list).
-template <int C>
+template <int C, int KM>
 __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_pass p, const float *__restrict__ values,
                                                                      const uint32_t *__restrict__ nslots,
                                                                      const uint32_t *__restrict__ members,
                                                                      float *__restrict__ group_weight, TileBins tiles)
 {
     constexpr int kBuckets = kWaves * 256;
+    constexpr int NREF = 32 / KM, KSHIFT = KM == 16 ? 4 : 3;  // references per wave
     __shared__ uint4 s_frag[16][64];
     __shared__ float s_cvs[C][64], s_thr[C][64];
     __shared__ uint32_t s_key[kBuckets], s_cnt[kBuckets];
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int n = lane & 31, h = lane >> 5, m = n & 7, jn = n >> 3;
+    const int n = lane & 31, h = lane >> 5, m = n & (KM - 1), jn = n >> KSHIFT;
     const int W = p.width, H = p.height;
     const int refs_x = (W - 8 + p.step - 1) / p.step + 1, refs_y = (H - 8 + p.step - 1) / p.step + 1;
     const int refs = refs_x * refs_y;
-    const int first = (xcd_block() * kWaves + wave) * 4;  // the wave's first reference
+    const int first = (xcd_block() * kWaves + wave) * NREF;  // the wave's first reference
     for (int i = threadIdx.x; i < kBuckets; i += kWaves * 64) {
         s_key[i] = ~0u;
         s_cnt[i] = 0;
     }
-    int k_use[4];
+    int k_use[NREF];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) k_use[j] = (int)(members[(size_t)min(first + j, refs - 1) * 9 + 8] & 0xFF);
+    for (int j = 0; j < NREF; ++j) k_use[j] = (int)(members[(size_t)min(first + j, refs - 1) * (KM + 1) + KM] & 0xFF);
     const uint32_t ref = (uint32_t)min(first + jn, refs - 1);
     const bool live = first + jn < refs;
-    const uint32_t *mw = members + (size_t)ref * 9;
-    const uint32_t gword = mw[8], pos = mw[m];
+    const uint32_t *mw = members + (size_t)ref * (KM + 1);
+    const uint32_t gword = mw[KM], pos = mw[m];
     const bool in_group = m < (int)(gword & 0xFF);
     // The ring slot of trajectory index t (0: the centre frame) from the 2 r neighbour slots.
     const uint32_t t = pos >> 26;
@@ -2122,7 +2239,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
                 const float v = raw[s][(i * C + c) >> 2][(i * C + c) & 3];
                 pa[c][s][i] = (_Float16)(exact ? rintf(v * maxv) * pow2 : v);
             }
-    mixed_group<C>(p, s_frag, s_cvs, s_thr, pa, inv_s, first, refs, k_use, gword, lane, group_weight, tiles);
+    mixed_group<C, KM>(p, s_frag, s_cvs, s_thr, pa, inv_s, first, refs, k_use, gword, lane, group_weight, tiles);
 
     // ---- list entries, at the reserved positions (s_cnt now holds the bases).
 #pragma unroll
@@ -2132,9 +2249,9 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
     for (int k = 0; k < 2; ++k) {
         if (tile[k] == ~0u) continue;
         const uint32_t e = s_cnt[bucket[k]] + rank[k];
-        const uint32_t id = (ref << 3) | (uint32_t)m;
-        if (e < kTileCap) {
-            tiles.list[(size_t)tile[k] * kTileCap + e] = id;
+        const uint32_t id = (ref << KSHIFT) | (uint32_t)m;
+        if (e < tiles.cap) {
+            tiles.list[(size_t)tile[k] * tiles.cap + e] = id;
         } else {
             const uint32_t o = atomicAdd(tiles.overflow_count, 1u);
             tiles.overflow[2 * (size_t)o] = id;
@@ -2149,7 +2266,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
 // filter's own arithmetic) to the tile in LDS.  The tile then adds to the accumulators once, as
 // the only writer of those pixels this pass (overflowing pairs are added by k_tile_overflow).
 // MIXED: the stage holds f16 samples, weighted and converted to fixed point here.
-template <int C, bool MIXED>
+template <int C, bool MIXED, int KM>
 __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const uint32_t *__restrict__ members,
                                                      const float *__restrict__ group_weight, TileBins tiles,
                                                      unsigned long long *__restrict__ aggregate,
@@ -2171,11 +2288,11 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
         s_agg[i] = 0;
         if (C == 2) s_second[i] = 0;
     }
-    const uint32_t *list = tiles.list + (size_t)tile * kTileCap;
-    // The first entries are read with the count (a list always holds kTileCap entries, so the read
-    // stays in bounds; entries past the count are ignored).
+    const uint32_t *list = tiles.list + (size_t)tile * tiles.cap;
+    // The first entries are read with the count (a list always holds cap >= 256 entries, so the
+    // read stays in bounds; entries past the count are ignored).
     const uint32_t id0 = list[threadIdx.x];
-    const uint32_t n = min(tiles.count[tile], (uint32_t)kTileCap);
+    const uint32_t n = min(tiles.count[tile], tiles.cap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane >> 3, q = lane & 7;
     const float kr_kq = p.kaiser[r] * p.kaiser[q];
@@ -2187,8 +2304,9 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
         if (k < n) {
             const uint32_t id = base == 0 ? id0 : list[k];
             s_id[threadIdx.x] = id;
-            s_pos[threadIdx.x] = members[(size_t)(id >> 3) * 9 + (id & 7)];
-            s_sw[threadIdx.x] = group_weight[id >> 3] * p.weight_scale;
+            constexpr int KSHIFT = KM == 16 ? 4 : 3;
+            s_pos[threadIdx.x] = members[(size_t)(id >> KSHIFT) * (KM + 1) + (id & (KM - 1))];
+            s_sw[threadIdx.x] = group_weight[id >> KSHIFT] * p.weight_scale;
         }
         __syncthreads();
         const uint32_t batch = min(256u, n - base);
@@ -2251,7 +2369,7 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
 
 // The (member, tile) pairs that did not fit a tile's list: thread (pair, row) adds its row's pixels
 // inside the pair's tile with global atomics.
-template <int C, bool MIXED>
+template <int C, bool MIXED, int KM>
 __global__ __launch_bounds__(256) void k_tile_overflow(avd_collab_pass p, const uint32_t *__restrict__ members,
                                                        const float *__restrict__ group_weight, TileBins tiles,
                                                        unsigned long long *__restrict__ aggregate,
@@ -2267,7 +2385,7 @@ __global__ __launch_bounds__(256) void k_tile_overflow(avd_collab_pass p, const 
         const uint32_t slot = tile / tiles.per_slot, in_slot = tile - slot * tiles.per_slot;
         const int ty = (int)(in_slot / tiles.tiles_x), tx = (int)(in_slot - ty * tiles.tiles_x);
         const int x0 = tx * T, y0 = ty * T;
-        member_row<C, MIXED>(p, members, group_weight, tiles, id, r, x0, y0, T,
+        member_row<C, MIXED, KM>(p, members, group_weight, tiles, id, r, x0, y0, T,
                       [&](int i, unsigned long long v, int32_t v2) {
                           const size_t pix = (size_t)slot * px + (size_t)(y0 + (i >> kTileShift)) * p.width +
                                              x0 + (i & (T - 1));
@@ -2366,7 +2484,8 @@ inline int ref_count(const avd_collab_pass &p)
 bool pass_ok(const avd_collab_pass &p)
 {
     return p.width >= 8 && p.height >= 8 && p.width < 8192 && p.height < 8192 && (p.channels == 1 || p.channels == 2) &&
-           p.step >= 1 && p.step <= 8 && p.spatial_radius >= 1 && p.spatial_radius <= kMaxS && p.group_size == 8 &&
+           p.step >= 1 && p.step <= 8 && p.spatial_radius >= 1 && p.spatial_radius <= kMaxS &&
+           (p.group_size == 8 || p.group_size == 16) &&
            p.temporal_radius <= 8 && p.refine >= 1 && p.refine <= 4 && p.blk_step >= 1 && p.blksize >= 1 &&
            (p.blksize + p.blk_step - 1) / p.blk_step <= 4;
 }
@@ -2396,10 +2515,12 @@ bool centre_on_matrix_cores(const avd_collab_pass &p)
     return p.code_bits >= 1 && p.code_bits <= 10 && p.step + 2 * p.spatial_radius <= 31;
 }
 
-hipError_t launch_collab_centre(const avd_collab_pass &p, const uint16_t *codes, uint32_t *members, hipStream_t s,
-                                const CentreScratch *centre)
+namespace {
+
+template <int KM>
+hipError_t launch_centre_km(const avd_collab_pass &p, const uint16_t *codes, uint32_t *members, hipStream_t s,
+                            const CentreScratch *centre)
 {
-    if (!pass_ok(p) || !codes) return hipErrorInvalidValue;
     const dim3 blk(kWaves * 64);
     if (centre && centre_on_matrix_cores(p)) {
         if (!centre->plane || !centre->energy || !centre->redo || (p.channels == 1 && !centre->plane_odd))
@@ -2413,14 +2534,15 @@ hipError_t launch_collab_centre(const avd_collab_pass &p, const uint16_t *codes,
         if (e != hipSuccess) return e;
         if (p.channels == 1) {
             k_centre_prep<1><<<pg, 256, 0, s>>>(frame, W, H, centre->plane, centre->plane_odd, centre->energy);
-            k_centre_mfma<1><<<gm, blk, 0, s>>>(p, codes, centre->plane, (uint32_t)(centre->plane_odd - centre->plane),
-                                                centre->energy, members, centre->redo, nullptr);
-            k_centre_redo<1><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
+            k_centre_mfma<1, KM><<<gm, blk, 0, s>>>(p, codes, centre->plane,
+                                                    (uint32_t)(centre->plane_odd - centre->plane), centre->energy,
+                                                    members, centre->redo, nullptr);
+            k_centre_redo<1, KM><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
         } else {
             k_centre_prep<2><<<pg, 256, 0, s>>>(frame, W, H, centre->plane, nullptr, centre->energy);
-            k_centre_mfma<2><<<gm, blk, 0, s>>>(p, codes, centre->plane, 0u, centre->energy, members, centre->redo,
-                                                nullptr);
-            k_centre_redo<2><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
+            k_centre_mfma<2, KM><<<gm, blk, 0, s>>>(p, codes, centre->plane, 0u, centre->energy, members,
+                                                    centre->redo, nullptr);
+            k_centre_redo<2, KM><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
         }
         return hipGetLastError();
     }
@@ -2429,13 +2551,85 @@ hipError_t launch_collab_centre(const avd_collab_pass &p, const uint16_t *codes,
     const bool narrow = p.code_bits >= 1 && p.code_bits <= 10 && p.spatial_radius <= 15;
     const unsigned g = div_up(ref_count(p), kWaves);
     if (p.channels == 1) {
-        if (narrow) k_centre_codes<1, true><<<g, blk, 0, s>>>(p, codes, members, nullptr);
-        else k_centre_codes<1, false><<<g, blk, 0, s>>>(p, codes, members, nullptr);
+        if (narrow) k_centre_codes<1, true, KM><<<g, blk, 0, s>>>(p, codes, members, nullptr);
+        else k_centre_codes<1, false, KM><<<g, blk, 0, s>>>(p, codes, members, nullptr);
     } else {
-        if (narrow) k_centre_codes<2, true><<<g, blk, 0, s>>>(p, codes, members, nullptr);
-        else k_centre_codes<2, false><<<g, blk, 0, s>>>(p, codes, members, nullptr);
+        if (narrow) k_centre_codes<2, true, KM><<<g, blk, 0, s>>>(p, codes, members, nullptr);
+        else k_centre_codes<2, false, KM><<<g, blk, 0, s>>>(p, codes, members, nullptr);
     }
     return hipGetLastError();
+}
+
+// One trajectory launch: NR references per wave (4 or 2 for eight members, 2 or 1 for sixteen,
+// whose 17 words need more lanes per reference), SHORT for frames shorter than K + 7 rows or rings
+// past a buffer load's offset.
+template <int C, int K, int KM>
+hipError_t launch_trajectory_ck(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
+                                const int32_t *mv, const float *conf, uint32_t *members, hipStream_t s,
+                                unsigned long long *scored_count, int nr, bool short_frame, size_t ents)
+{
+    const dim3 blk(kWaves * 64);
+    const size_t lds = (size_t)nr * ents;
+    const unsigned g = div_up(ref_count(p), kWaves * nr);
+    if constexpr (KM == 8) {
+        if (short_frame)
+            k_trajectory_codes<C, K, 2, true, 8><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
+        else if (nr == 4)
+            k_trajectory_codes<C, K, 4, false, 8><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
+        else
+            k_trajectory_codes<C, K, 2, false, 8><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
+    } else {
+        if (short_frame)
+            k_trajectory_codes<C, K, 1, true, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
+        else if (nr == 2)
+            k_trajectory_codes<C, K, 2, false, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
+        else
+            k_trajectory_codes<C, K, 1, false, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
+    }
+    return hipGetLastError();
+}
+
+template <int KM>
+hipError_t launch_trajectory_km(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
+                                const int32_t *mv, const float *conf, uint32_t *members, hipStream_t s,
+                                unsigned long long *scored_count)
+{
+    // References per wave: the most whose rectangle tables fit the LDS budget.
+    const size_t ents = (size_t)trajectory_entries(p) * 2 * sizeof(uint32_t) * kWaves;
+    const int C = (int)p.channels;
+    const int nr_hi = KM == 8 ? 4 : 2, nr_lo = KM == 8 ? 2 : 1;
+    const size_t st_hi = C == 1 ? trajectory_static_lds(1, nr_hi, KM) : trajectory_static_lds(2, nr_hi, KM);
+    int nr = st_hi + nr_hi * ents <= kLdsBudget ? nr_hi : nr_lo;
+    // Frames shorter than K + 7 rows, and rings past 2^31 bytes (beyond a buffer load's offset),
+    // take the clamping slide (at two references per wave for eight members, one for sixteen).
+    const bool short_frame = (int)p.height < 2 * (int)p.refine + 8 ||
+                             (uint64_t)p.width * p.height * p.channels * p.slots * 2 >= (1ull << 31);
+    if (short_frame) nr = nr_lo;
+    const size_t st = C == 1 ? trajectory_static_lds(1, nr, KM) : trajectory_static_lds(2, nr, KM);
+    if (st + nr * ents > 160 * 1024) return hipErrorInvalidValue;  // past the LDS of a workgroup
+#define AVD_TRAJ(Cn, Kn) \
+    launch_trajectory_ck<Cn, Kn, KM>(p, codes, nslots, mv, conf, members, s, scored_count, nr, short_frame, ents)
+    switch (p.refine * 2 + 1 + (p.channels == 2 ? 100 : 0)) {
+    case 3: return AVD_TRAJ(1, 3);
+    case 5: return AVD_TRAJ(1, 5);
+    case 7: return AVD_TRAJ(1, 7);
+    case 9: return AVD_TRAJ(1, 9);
+    case 103: return AVD_TRAJ(2, 3);
+    case 105: return AVD_TRAJ(2, 5);
+    case 107: return AVD_TRAJ(2, 7);
+    default: return AVD_TRAJ(2, 9);
+    }
+#undef AVD_TRAJ
+}
+
+}  // namespace
+
+hipError_t launch_collab_centre(const avd_collab_pass &p, const uint16_t *codes, uint32_t *members, hipStream_t s,
+                                const CentreScratch *centre)
+{
+    if (!pass_ok(p) || !codes) return hipErrorInvalidValue;
+    return p.group_size == 16 ? launch_centre_km<16>(p, codes, members, s, centre)
+                              : launch_centre_km<8>(p, codes, members, s, centre);
 }
 
 hipError_t launch_collab_trajectory(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
@@ -2443,44 +2637,9 @@ hipError_t launch_collab_trajectory(const avd_collab_pass &p, const uint16_t *co
                                     unsigned long long *scored_count)
 {
     if (!pass_ok(p) || !codes) return hipErrorInvalidValue;
-    if (p.temporal_radius < 1 || p.group_size != 8) return hipSuccess;
-    const dim3 blk(kWaves * 64);
-    // Four references per wave where their tables fit the LDS budget, else two (the largest
-    // table, 512 entries, fits at two).
-    const size_t ents = (size_t)trajectory_entries(p) * 2 * sizeof(uint32_t) * kWaves;
-    const int C = (int)p.channels;
-    const bool four = trajectory_static_lds(C, 4) + 4 * ents <= kLdsBudget;
-    // Frames shorter than K + 7 rows, and rings past 2^31 bytes (beyond a buffer load's offset),
-    // take the clamping slide (at two references per wave).
-    const bool short_frame = (int)p.height < 2 * (int)p.refine + 8 ||
-                             (uint64_t)p.width * p.height * p.channels * p.slots * 2 >= (1ull << 31);
-    const bool four_rows = four && !short_frame;
-    const size_t lds_at = (four_rows ? 4 : 2) * ents;
-    const unsigned g_at = div_up(ref_count(p), kWaves * (four_rows ? 4 : 2));
-#define AVD_TRAJ(Cn, Kn)                                                                                       \
-    do {                                                                                                       \
-        if (short_frame)                                                                                       \
-            k_trajectory_codes<Cn, Kn, 2, true><<<g_at, blk, lds_at, s>>>(p, codes, nslots, mv, conf, members,  \
-                                                                          scored_count);                       \
-        else if (four_rows)                                                                                    \
-            k_trajectory_codes<Cn, Kn, 4, false><<<g_at, blk, lds_at, s>>>(p, codes, nslots, mv, conf, members, \
-                                                                           scored_count);                      \
-        else                                                                                                   \
-            k_trajectory_codes<Cn, Kn, 2, false><<<g_at, blk, lds_at, s>>>(p, codes, nslots, mv, conf, members, \
-                                                                           scored_count);                      \
-    } while (0)
-    switch (p.refine * 2 + 1 + (p.channels == 2 ? 100 : 0)) {
-    case 3: AVD_TRAJ(1, 3); break;
-    case 5: AVD_TRAJ(1, 5); break;
-    case 7: AVD_TRAJ(1, 7); break;
-    case 9: AVD_TRAJ(1, 9); break;
-    case 103: AVD_TRAJ(2, 3); break;
-    case 105: AVD_TRAJ(2, 5); break;
-    case 107: AVD_TRAJ(2, 7); break;
-    default: AVD_TRAJ(2, 9); break;
-    }
-#undef AVD_TRAJ
-    return hipGetLastError();
+    if (p.temporal_radius < 1) return hipSuccess;
+    return p.group_size == 16 ? launch_trajectory_km<16>(p, codes, nslots, mv, conf, members, s, scored_count)
+                              : launch_trajectory_km<8>(p, codes, nslots, mv, conf, members, s, scored_count);
 }
 
 hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes, const float *values,
@@ -2493,10 +2652,17 @@ hipError_t launch_collab_search(const avd_collab_pass &p, const uint16_t *codes,
         return e != hipSuccess ? e : launch_collab_trajectory(p, codes, nslots, mv, conf, members, s, scored_count);
     }
     const unsigned g = div_up(ref_count(p), kWaves);
-    if (p.channels == 1)
-        k_collab_search<1, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
-    else
-        k_collab_search<2, false><<<g, kWaves * 64, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count);
+    const dim3 blk(kWaves * 64);
+#define AVD_F32_SEARCH(Cn, KMn) \
+    k_collab_search<Cn, false, KMn><<<g, blk, 0, s>>>(p, codes, values, nslots, mv, conf, members, scored_count)
+    if (p.group_size == 16) {
+        if (p.channels == 1) AVD_F32_SEARCH(1, 16);
+        else AVD_F32_SEARCH(2, 16);
+    } else {
+        if (p.channels == 1) AVD_F32_SEARCH(1, 8);
+        else AVD_F32_SEARCH(2, 8);
+    }
+#undef AVD_F32_SEARCH
     return hipGetLastError();
 }
 
@@ -2509,14 +2675,44 @@ hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, c
     if (e != hipSuccess) return e;
     const unsigned g = div_up(ref_count(p), kWaves);
     const TileBins none{};
-    if (p.channels == 1)
-        k_collab_filter<1, false><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
-                                                             group_weight, none);
-    else
-        k_collab_filter<2, false><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
-                                                             group_weight, none);
+    const dim3 blk(kWaves * 64);
+#define AVD_DIRECT(Cn, KMn)                                                                                 \
+    k_collab_filter<Cn, false, KMn><<<g, blk, 0, s>>>(p, values, nslots, members, aggregate, accum_second, \
+                                                      group_weight, none)
+    if (p.group_size == 16) {
+        if (p.channels == 1) AVD_DIRECT(1, 16);
+        else AVD_DIRECT(2, 16);
+    } else {
+        if (p.channels == 1) AVD_DIRECT(1, 8);
+        else AVD_DIRECT(2, 8);
+    }
+#undef AVD_DIRECT
     return hipGetLastError();
 }
+
+namespace {
+
+template <int C, int KM>
+void binned_launches(const avd_collab_pass &p, const float *values, const uint32_t *nslots, const uint32_t *members,
+                     const TileBins &tiles, unsigned long long *aggregate, int32_t *accum_second,
+                     float *group_weight, hipStream_t s, bool mixed, uint32_t refs, uint32_t n_tiles)
+{
+    const dim3 blk(kWaves * 64);
+    if (mixed) {
+        const unsigned gm = div_up(refs, kWaves * (32 / KM));  // 32 members per wave
+        k_collab_filter_mixed<C, KM><<<gm, blk, 0, s>>>(p, values, nslots, members, group_weight, tiles);
+        k_tile_gather<C, true, KM><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+        k_tile_overflow<C, true, KM><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+    } else {
+        const unsigned g = div_up(refs, kWaves);
+        k_collab_filter<C, true, KM><<<g, blk, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
+                                                       group_weight, tiles);
+        k_tile_gather<C, false, KM><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+        k_tile_overflow<C, false, KM><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+    }
+}
+
+}  // namespace
 
 hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
                                        const uint32_t *members, const TileBins &tiles,
@@ -2525,7 +2721,7 @@ hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *va
 {
     if (mixed && p.pool_ratio > 0.0f) return hipErrorInvalidValue;  // the mixed form thresholds hard
     if (!pass_ok(p) || (p.channels == 2 && !accum_second) || !tiles.stage || !tiles.count || !tiles.list ||
-        !tiles.overflow || !tiles.overflow_count ||
+        !tiles.overflow || !tiles.overflow_count || tiles.cap < kTileCapMin ||
         tiles.tiles_x != (p.width + (1u << kTileShift) - 1) >> kTileShift ||
         tiles.per_slot != tiles.tiles_x * ((p.height + (1u << kTileShift) - 1) >> kTileShift))
         return hipErrorInvalidValue;
@@ -2536,28 +2732,16 @@ hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *va
     if (e == hipSuccess) e = launch_fill_u32(tiles.overflow_count, 0, 1, s);
     if (e != hipSuccess) return e;
     const uint32_t refs = (uint32_t)ref_count(p);
-    const unsigned g = div_up(refs, kWaves);
-    const unsigned gm = div_up(refs, kWaves * 4);
-    if (mixed) {
-        if (p.channels == 1) {
-            k_collab_filter_mixed<1><<<gm, kWaves * 64, 0, s>>>(p, values, nslots, members, group_weight, tiles);
-            k_tile_gather<1, true><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
-            k_tile_overflow<1, true><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
-        } else {
-            k_collab_filter_mixed<2><<<gm, kWaves * 64, 0, s>>>(p, values, nslots, members, group_weight, tiles);
-            k_tile_gather<2, true><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
-            k_tile_overflow<2, true><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
-        }
-    } else if (p.channels == 1) {
-        k_collab_filter<1, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
-                                                            group_weight, tiles);
-        k_tile_gather<1, false><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
-        k_tile_overflow<1, false><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+    if (p.group_size == 16) {
+        if (p.channels == 1) binned_launches<1, 16>(p, values, nslots, members, tiles, aggregate, accum_second,
+                                                    group_weight, s, mixed, refs, n_tiles);
+        else binned_launches<2, 16>(p, values, nslots, members, tiles, aggregate, accum_second, group_weight, s,
+                                    mixed, refs, n_tiles);
     } else {
-        k_collab_filter<2, true><<<g, kWaves * 64, 0, s>>>(p, values, nslots, members, aggregate, accum_second,
-                                                            group_weight, tiles);
-        k_tile_gather<2, false><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
-        k_tile_overflow<2, false><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
+        if (p.channels == 1) binned_launches<1, 8>(p, values, nslots, members, tiles, aggregate, accum_second,
+                                                   group_weight, s, mixed, refs, n_tiles);
+        else binned_launches<2, 8>(p, values, nslots, members, tiles, aggregate, accum_second, group_weight, s,
+                                   mixed, refs, n_tiles);
     }
     return hipGetLastError();
 }

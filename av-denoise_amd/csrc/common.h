@@ -10,7 +10,6 @@
 namespace avd {
 
 constexpr int kPatch = 8;
-constexpr int kMaxK = 8;
 
 inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
@@ -100,14 +99,18 @@ hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum
 
 // Binned aggregation: the filter stages each member's fixed-point values (`stage`, [ref][member][c]
 // [64] i32) and enters the member in the list of every 32x32 tile of its ring slot that its patch
-// overlaps (`count` / `list`: kTileCap entries per tile; pairs past that go to `overflow`, up to
-// refs * 8 * 4 pairs); each tile then sums its members in LDS and adds to the accumulators once.
+// overlaps (`count` / `list`: `cap` entries per tile; pairs past that go to `overflow`, up to
+// refs * members * 4 pairs); each tile then sums its members in LDS and adds to the accumulators
+// once.  Member ids are (reference << log2 members) | member.
 constexpr int kTileShift = 5;
-constexpr uint32_t kTileCap = 768;
+// Entries per tile list: groups of 16 put twice the members of groups of 8 in each tile.
+inline uint32_t tile_cap_for(uint32_t group_size) { return group_size == 16 ? 2048u : 768u; }
+constexpr uint32_t kTileCapMin = 256;  // the gather reads a list's first 256 entries with its count
 struct TileBins {
     int32_t *stage;
     uint32_t *count, *list, *overflow, *overflow_count;
     uint32_t tiles_x, per_slot;  // tiles per row, tiles per ring slot
+    uint32_t cap;                // entries per tile list (at least kTileCapMin)
 };
 hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
                                        const uint32_t *members, const TileBins &tiles,

@@ -158,10 +158,12 @@ uint64_t motion_ops(const MotionGeom &g, int neighbours)
 // Per-candidate NLM operations (5C + 2(2P + 1) + 7).
 uint64_t nlm_candidate_ops(int C, int P) { return (uint64_t)(5 * C + 2 * (2 * P + 1) + 7); }
 
-float cross_frame_accum_scale(uint32_t spatial_radius, uint32_t temporal_radius, uint32_t step)
+// cross_frame_accum_scale (aggregate.rs:36-55) with the reference's STEP and MAX_K as the grid
+// stride and the group size, so the stride-3, 16-member grid gets the bound of its own worst case.
+float cross_frame_accum_scale(uint32_t spatial_radius, uint32_t temporal_radius, uint32_t step, uint32_t group_size)
 {
     const uint32_t per_axis = ((8 - 1) + 2 * spatial_radius) / step + 1;
-    const double contribs = (double)per_axis * (double)per_axis * 8.0;
+    const double contribs = (double)per_axis * (double)per_axis * (double)group_size;
     const double passes = (double)(4 * temporal_radius + 1);
     const double max_raw = contribs * passes * 5.0;
     const double budget = (double)INT32_MAX / 2.0;
@@ -621,8 +623,11 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     if (o.transform_form == 1 && o.pooled_threshold)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "the mixed transform form thresholds hard: set pooled_threshold = 0");
     mixed_ = o.transform_form == 1;
-    if (o.step != 4 || o.group_size != 8)
-        throw Failure(AVD_ERR_INVALID_OPTIONS, "this build runs the reference grid: step 4, group size 8");
+    // The reference's grid is stride 4 with groups of 8 (collab/mod.rs:16-20); this build also runs
+    // any stride up to 8 and groups of 16 (volume grids of 4 frames x 4 volumes, 2 x 8).
+    if (o.step < 1 || o.step > 8) throw Failure(AVD_ERR_INVALID_OPTIONS, "step must be in 1..=8");
+    if (o.group_size != 8 && o.group_size != 16)
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "group_size must be 8 (the reference's) or 16");
     if (W < 8 || H < 8) throw Failure(AVD_ERR_INVALID_GEOMETRY, "frames must be at least 8x8 for the patch grid");
     if (W >= 8192 || H >= 8192) throw Failure(AVD_ERR_INVALID_GEOMETRY, "frames must be below 8192 on each axis");
     if (in.kind == AVD_FORMAT_U16 && in.depth > 12)
@@ -654,7 +659,7 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     pass_.sigma[0] = pass_.sigma[1] = sigma_;
     for (int i = 0; i < 8; ++i) pass_.profile[i] = 1.0f;  // dct_noise_profile(0)
     pass_.weight_scale = weight_scale(sigma_, 1.0f);
-    pass_.accum_scale = cross_frame_accum_scale(o.spatial_radius, r_, o.step);
+    pass_.accum_scale = cross_frame_accum_scale(o.spatial_radius, r_, o.step, o.group_size);
     kaiser_window(o.kaiser_beta, pass_.kaiser);
     pass_.code_bits = in.kind == AVD_FORMAT_U8 ? 8u : in.depth;
     // nl4d_pool_ratio (nl4d/options.rs:109-117): the calibrated 2.42 over the plane's default lambda.
@@ -665,19 +670,21 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     refs_ = (size_t)((W - 8 + o.step - 1) / o.step + 1) * ((H - 8 + o.step - 1) / o.step + 1);
     aggregate_.allocate(N_ * px * sizeof(unsigned long long));
     if (C == 2) accum_second_.allocate(N_ * px * sizeof(int32_t));
-    members_.allocate(refs_ * 9 * sizeof(uint32_t));
+    const size_t K = o.group_size;
+    members_.allocate(refs_ * (K + 1) * sizeof(uint32_t));  // K member words, then the group word
     group_weight_.allocate(refs_ * sizeof(float));
     {
         const uint32_t side = 1u << kTileShift;
         const uint32_t tiles_x = (W + side - 1) / side, per_slot = tiles_x * ((H + side - 1) / side);
         const size_t n_tiles = (size_t)per_slot * N_;
-        stage_.allocate(refs_ * C * 512 * sizeof(int32_t));
+        const uint32_t cap = tile_cap_for(o.group_size);
+        stage_.allocate(refs_ * K * C * 64 * sizeof(int32_t));
         tile_count_.allocate((n_tiles + 1) * sizeof(uint32_t));  // the last word counts overflow pairs
-        tile_list_.allocate(n_tiles * kTileCap * sizeof(uint32_t));
-        tile_overflow_.allocate(refs_ * 8 * 4 * 2 * sizeof(uint32_t));
+        tile_list_.allocate(n_tiles * cap * sizeof(uint32_t));
+        tile_overflow_.allocate(refs_ * K * 4 * 2 * sizeof(uint32_t));  // up to four tiles per member
         uint32_t *counts = tile_count_.as<uint32_t>();
         bins_ = TileBins{stage_.as<int32_t>(), counts, tile_list_.as<uint32_t>(), tile_overflow_.as<uint32_t>(),
-                         counts + n_tiles, tiles_x, per_slot};
+                         counts + n_tiles, tiles_x, per_slot, cap};
     }
     // The centre search scores every position of each reference's clipped rectangle but its own.
     centre_ops_ = 0;
@@ -753,12 +760,20 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
         return launch_collab_trajectory(p, ring_.code_ptr(), nslots, mv_.as<int32_t>(), conf_.as<float>(),
                                         members_.as<uint32_t>(), stream_, counter);
     });
-    const uint64_t filter_ops = (uint64_t)refs_ * 8 * ((uint64_t)C_ * (4096 + 576 + 256) + 64 * (C_ + 1));
+    const uint64_t filter_ops =
+        (uint64_t)refs_ * pass_.group_size * ((uint64_t)C_ * (4096 + 576 + 256) + 64 * (C_ + 1));
     timed(AVD_STAGE_COLLAB_FILTER, filter_ops, stream_, "collab filter", [&] {
         return launch_collab_filter_binned(p, ring_.value_ptr(), nslots, members_.as<uint32_t>(), bins_,
                                            aggregate_.as<unsigned long long>(), accum_second_.as<int32_t>(),
                                            group_weight_.as<float>(), stream_, mixed_);
     });
+    if (watch_overflow_) {  // test hook: count the (member, tile) pairs past the lists
+        uint32_t pairs = 0;
+        check_hip(hipMemcpyAsync(&pairs, bins_.overflow_count, sizeof(pairs), hipMemcpyDeviceToHost, stream_),
+                  "read overflow count");
+        check_hip(hipStreamSynchronize(stream_), "read overflow count");
+        overflow_seen_ += pairs;
+    }
     ++passes_;
 }
 
@@ -1128,6 +1143,33 @@ avd_timing Engine::timing()
         timer_.harvest();
         return timer_.totals();
     });
+}
+
+void Nl4dPass::set_tile_cap_for_test(uint32_t cap)
+{
+    if (cap < kTileCapMin || cap > tile_cap_for(pass_.group_size))
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "tile list capacity out of range");
+    bins_.cap = cap;
+    watch_overflow_ = true;
+}
+
+void Engine::set_tile_cap_for_test(uint32_t cap)
+{
+    bool any = false;
+    for (Pass *pass : {luma_.get(), chroma_.get()})
+        if (auto *n = dynamic_cast<Nl4dPass *>(pass)) {
+            n->set_tile_cap_for_test(cap);
+            any = true;
+        }
+    if (!any) throw Failure(AVD_ERR_INVALID_OPTIONS, "only NL4D engines bin their aggregation");
+}
+
+uint64_t Engine::tile_overflows_for_test() const
+{
+    uint64_t total = 0;
+    for (const Pass *pass : {luma_.get(), chroma_.get()})
+        if (auto *n = dynamic_cast<const Nl4dPass *>(pass)) total += n->overflow_pairs_seen();
+    return total;
 }
 
 void Engine::fail_for_test()

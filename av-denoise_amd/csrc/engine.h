@@ -45,7 +45,8 @@ float noise_offset(float sigma, int channels, uint32_t patch_radius);
 float sad_noise_floor(uint32_t blksize, float sigma);
 float thsad(uint32_t blksize, float scale);
 void kaiser_window(float beta, float out[8]);
-float cross_frame_accum_scale(uint32_t spatial_radius, uint32_t temporal_radius, uint32_t step);
+float cross_frame_accum_scale(uint32_t spatial_radius, uint32_t temporal_radius, uint32_t step,
+                              uint32_t group_size);
 float weight_scale(float sigma, float g_max);
 
 // The 2r+1 ingested frames of one plane group: f32 values, their u16 codes and, for motion, the
@@ -186,6 +187,9 @@ class Nl4dPass : public Pass {
     void reset_stream() override;
     size_t max_held() const override;
     bool has_pending() const override;
+    // Test hooks: shrink the tile lists so the overflow path runs, and count its pairs.
+    void set_tile_cap_for_test(uint32_t cap);
+    uint64_t overflow_pairs_seen() const { return overflow_seen_; }
 
   private:
     enum class Clear { Whole, Newest, None };
@@ -217,6 +221,8 @@ class Nl4dPass : public Pass {
     std::vector<bool> zeroed_;
     uint32_t passes_ = 0;
     bool continuation_ = false, tail_emitting_ = false;
+    bool watch_overflow_ = false;
+    uint64_t overflow_seen_ = 0;
 };
 
 class Engine {
@@ -232,6 +238,8 @@ class Engine {
     size_t max_held_frames() const;
     avd_window_span window_span() const;
     void fail_for_test();
+    void set_tile_cap_for_test(uint32_t cap);
+    uint64_t tile_overflows_for_test() const;
     void sample_device_time(uint32_t every);
     avd_timing timing();
 

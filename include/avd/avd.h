@@ -120,7 +120,9 @@ typedef struct avd_nlmeans_options {
 } avd_nlmeans_options;
 
 /* Nl4dOptions.  `step` and `group_size` are this build's reference-grid stride and group size;
- * the reference fixes them at 4 and 8 (collab/mod.rs:16-20). */
+ * the reference fixes them at 4 and 8 (collab/mod.rs:16-20).  This build runs any stride in 1..8
+ * and groups of 8 or 16 members (16: volume grids of 4 frames x 4 volumes, or 2 x 8 at temporal
+ * radius 1; stacks of up to 16 otherwise). */
 typedef struct avd_nl4d_options {
     avd_motion_search motion;
     uint32_t has_sigma;
@@ -210,6 +212,12 @@ avd_status avd_engine_max_held_frames(const avd_engine *engine, uint64_t *out);
 avd_status avd_engine_synchronize(avd_engine *engine);
 /* Poisons the engine as a GPU failure would (fail_through_guard_for_test, nlmeans/engine.rs:195). */
 avd_status avd_engine_fail_for_test(avd_engine *engine);
+/* Test hooks with no reference counterpart, for the binned aggregation of NL4D engines: shrink
+ * every tile list to `cap` entries (256 up to the engine's own, 768 for groups of 8 and 2048 for
+ * 16) so that small frames take the overflow path, and read how many (member, tile) pairs
+ * overflowed since (each pass then waits for its filter to read the count). */
+avd_status avd_engine_set_tile_cap_for_test(avd_engine *engine, uint32_t cap);
+avd_status avd_engine_tile_overflows_for_test(const avd_engine *engine, uint64_t *pairs);
 
 /* Device time per stage (EngineTiming / DeviceTimer, engine/timing.rs:13-200): with sampling on,
  * every launch of a stage is bracketed by HIP events on the engine's stream.  NL4D's grouping

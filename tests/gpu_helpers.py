@@ -43,12 +43,12 @@ def words_host(t: torch.Tensor) -> np.ndarray:
 
 
 def collab_pass(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, centre, slots, geom, profile=None,
-                code_bits=0, pooled=False, step=4):
+                code_bits=0, pooled=False, step=4, group_size=8):
     p = L.CollabPass()
     p.code_bits = code_bits
     p.pool_ratio = O.pool_ratio(C) if pooled else 0.0
     p.width, p.height, p.channels = W, H, C
-    p.step, p.spatial_radius, p.temporal_radius, p.refine, p.group_size = step, S, radius, refine, 8
+    p.step, p.spatial_radius, p.temporal_radius, p.refine, p.group_size = step, S, radius, refine, group_size
     p.blk_step, p.blksize, p.blocks_x, p.blocks_y = geom.step, geom.blksize, geom.blocks_x, geom.blocks_y
     p.centre_slot, p.slots = centre, slots
     p.c_min = c_min
@@ -56,7 +56,7 @@ def collab_pass(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, cent
     p.sigma[0] = p.sigma[1] = sigma
     prof = O.dct_noise_profile(0.0) if profile is None else profile
     p.weight_scale = float(O.weight_scale(sigma, float(prof.max())))
-    p.accum_scale = float(O.cross_frame_accum_scale(S, radius, step))
+    p.accum_scale = float(O.cross_frame_accum_scale(S, radius, step, group_size))
     k = O.kaiser_window(kaiser_beta)
     for i in range(8):
         p.kaiser[i] = float(k[i])
@@ -65,8 +65,9 @@ def collab_pass(W, H, C, radius, S, refine, c_min, lam, sigma, kaiser_beta, cent
 
 
 def site_collab_search(p, codes=None, values=None, nslots=None, mv=None, conf=None):
-    refs = O.refs_along(p.width, p.step) * O.refs_along(p.height, p.step)  # the kernels write refs * 9 words
-    members = torch.zeros(refs * 9, dtype=torch.int32, device="cuda")
+    refs = O.refs_along(p.width, p.step) * O.refs_along(p.height, p.step)
+    # the kernels write group_size member words and the group word per reference
+    members = torch.zeros(refs * (p.group_size + 1), dtype=torch.int32, device="cuda")
     ns = dev(np.asarray(nslots if nslots else [0], np.uint32).view(np.int32))
     mvd = dev(mv if mv is not None else np.zeros(2, np.int32))
     cfd = dev(conf if conf is not None else np.ones(1, np.float32))
@@ -84,7 +85,7 @@ def unpack_aggregate(agg: np.ndarray, second: np.ndarray | None, C: int):
 
 
 def site_collab_filter(p, values, nslots, members, frames):
-    refs = O.refs_along(p.width, p.step) * O.refs_along(p.height, p.step)  # the kernels write refs * 9 words
+    refs = O.refs_along(p.width, p.step) * O.refs_along(p.height, p.step)
     px = p.width * p.height
     agg = torch.zeros(frames * px, dtype=torch.int64, device="cuda")
     second = torch.zeros(frames * px, dtype=torch.int32, device="cuda") if p.channels == 2 else None
@@ -127,13 +128,13 @@ def run_engine(engine, frames_planes, depth_out, dims):
     return outs
 
 
-def assert_members_equal(got: np.ndarray, want: np.ndarray, what: str = "") -> None:
+def assert_members_equal(got: np.ndarray, want: np.ndarray, what: str = "", group_size: int = 8) -> None:
     """Member words match; on a mismatch names the first differing reference and its two groups."""
-    g, w = got.reshape(-1, 9), want.reshape(-1, 9)
+    g, w = got.reshape(-1, group_size + 1), want.reshape(-1, group_size + 1)
     bad = np.nonzero(np.any(g != w, axis=1))[0]
     if len(bad):
         i = int(bad[0])
 
         def fmt(row):
-            return [f"({v & 0x1FFF},{(v >> 13) & 0x1FFF},t{v >> 26})" for v in row[:8]] + [hex(int(row[8]))]
+            return [f"({v & 0x1FFF},{(v >> 13) & 0x1FFF},t{v >> 26})" for v in row[:-1]] + [hex(int(row[-1]))]
         raise AssertionError(f"{what}: {len(bad)} references differ; first {i}: got {fmt(g[i])} want {fmt(w[i])}")
