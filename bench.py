@@ -1,4 +1,5 @@
-"""NL4D throughput on one MI355X: 3840x2160 10-bit 4:2:0, temporal radius 3, spatial radius 12.
+"""NL4D throughput on one MI355X: 3840x2160 10-bit 4:2:0, temporal radius 3, spatial radius 12,
+8x8 patches at stride 3 in groups of up to 16.
 
 A step is one frame through the NL4D engine in steady state: ingest of Y, U and V, the motion
 search of the six neighbours, grouping, transforms, threshold and aggregation of the luma and
@@ -23,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "av-denoise_amd"))
 
 METRIC = "NL4D 4K 10-bit frames/s: MFMA mixed vs FP32 form at ΔPSNR≤0.05 dB; %roofline"
 W, H, DEPTH, RADIUS, SPATIAL, REFINE = 3840, 2160, 10, 3, 12, 2
+STEP, GROUP = 3, 16              # reference grid stride, members per group
 SIGMA_Y, SIGMA_C = 16.0, 8.0     # noise of the clip, in 10-bit codes
 LAMBDA = 2.7
 CLIP_FRAMES = 16
@@ -54,13 +56,14 @@ def parse():
     ap.add_argument("--form", choices=["both", "mixed", "fp32"], default="both",
                     help="transform form(s) to run; the headline value is the mixed form's when it runs")
     ap.add_argument("--no-quality", action="store_true", help="skip the PSNR comparison against the clean scene")
+    ap.add_argument("--no-nlm", action="store_true", help="skip the NLMeans workloads")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, one plain device copy of this many MiB (a known-size kernel that "
                          "hardware-counter runs calibrate FETCH_SIZE / WRITE_SIZE against)")
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "nl4d_4k_radius3_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "nl4d_4k_stride3_k16_traffic.json")
 
 
 def stage_traffic():
@@ -81,7 +84,8 @@ FORMS = {"fp32": 0, "mixed": 1}
 def make_engine(avd, form="mixed"):
     geom = avd.Geometry(W, H, 1, avd.SampleFormat.u16(DEPTH), avd.SampleFormat.u16(DEPTH), 0)
     common = dict(temporal_radius=RADIUS, spatial_radius=SPATIAL, refine=REFINE, lambda_ht=LAMBDA,
-                  pooled_threshold=0, noise_map=0, motion={"estimation": 1}, field_lambda=0.0, transform_form=FORMS[form])
+                  pooled_threshold=0, noise_map=0, motion={"estimation": 1}, field_lambda=0.0, step=STEP,
+                  group_size=GROUP, transform_form=FORMS[form])
     luma = avd.nl4d_options(sigma=SIGMA_Y / 1023.0, **common)
     chroma = avd.nl4d_options(sigma=SIGMA_C / 1023.0, **common)
     return avd.build_engine("nl4d", geom, luma=luma, chroma=chroma), geom
@@ -170,10 +174,13 @@ def run_steps(eng, frames, outs, start, count):
     return done
 
 
-def cpu_baseline(clip_codes, threads):
-    """One steady-state frame (luma + chroma passes) of the top-left quarter of the same clip
-    through the oracle's FP32 restatement, on `threads` host threads; returns seconds per full
-    frame (the quarter's time times four)."""
+CPU_CROP = 4  # the CPU baseline's sample: the top-left 1/CPU_CROP of each axis (1/16 of the pixels)
+
+
+def cpu_baseline(clip_codes, threads, runs=5):
+    """One steady-state frame (luma + chroma passes) of the top-left 960x540 sixteenth of the same
+    clip through the oracle's FP32 restatement, on `threads` host threads: one warm-up, then the
+    median of `runs` timed runs; returns seconds per full frame (the crop's time times sixteen)."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -181,26 +188,101 @@ def cpu_baseline(clip_codes, threads):
 
     O.set_threads(threads)
     N = 2 * RADIUS + 1
-    t_total = 0.0
-    for plane_group in ("luma", "chroma"):
-        if plane_group == "luma":
-            w, h, C, sig = W // 2, H // 2, 1, SIGMA_Y / 1023.0
-            planes = [[f[0][:h, :w]] for f in clip_codes[:N]]
-        else:
-            w, h, C, sig = W // 4, H // 4, 2, SIGMA_C / 1023.0
-            planes = [[f[1][:h, :w], f[2][:h, :w]] for f in clip_codes[:N]]
-        spec = O.Nl4dSpec(temporal_radius=RADIUS, refine=REFINE, spatial_radius=SPATIAL, lambda_ht=LAMBDA,
-                          c_min=0.05, kaiser_beta=2.0, sigma=sig, motion=O.MotionGeom(0, 0, 16, 8, 4, 2))
-        p = O.Nl4dPass(spec, w, h, C, DEPTH, DEPTH)
-        for fr in planes:  # fill the ring without running passes
-            p.ring.write([O.pack_plane(np.ascontiguousarray(x), DEPTH) for x in fr])
-        t0 = time.perf_counter()
-        p._run_pass(RADIUS, "whole")
-        slot = p.ring.phys(0)
-        out = p._read_region(slot)
-        O.egress(out, w * h, C, DEPTH)
-        t_total += time.perf_counter() - t0
-    return 4.0 * t_total
+
+    def one_frame():
+        t_total = 0.0
+        for plane_group in ("luma", "chroma"):
+            if plane_group == "luma":
+                w, h, C, sig = W // CPU_CROP, H // CPU_CROP, 1, SIGMA_Y / 1023.0
+                planes = [[f[0][:h, :w]] for f in clip_codes[:N]]
+            else:
+                w, h, C, sig = W // (2 * CPU_CROP), H // (2 * CPU_CROP), 2, SIGMA_C / 1023.0
+                planes = [[f[1][:h, :w], f[2][:h, :w]] for f in clip_codes[:N]]
+            spec = O.Nl4dSpec(temporal_radius=RADIUS, refine=REFINE, spatial_radius=SPATIAL, lambda_ht=LAMBDA,
+                              c_min=0.05, kaiser_beta=2.0, sigma=sig, motion=O.MotionGeom(0, 0, 16, 8, 4, 2),
+                              step=STEP, group_size=GROUP)
+            p = O.Nl4dPass(spec, w, h, C, DEPTH, DEPTH)
+            for fr in planes:  # fill the ring without running passes
+                p.ring.write([O.pack_plane(np.ascontiguousarray(x), DEPTH) for x in fr])
+            t0 = time.perf_counter()
+            p._run_pass(RADIUS, "whole")
+            slot = p.ring.phys(0)
+            out = p._read_region(slot)
+            O.egress(out, w * h, C, DEPTH)
+            t_total += time.perf_counter() - t0
+        return t_total
+
+    one_frame()  # warm-up
+    times = sorted(one_frame() for _ in range(runs))
+    return float(CPU_CROP * CPU_CROP) * times[runs // 2]
+
+
+NLM_CONFIGS = {
+    # Plain NLMeans, spatial, luma only, 8-bit 640x360, patch radius 2, search radius 7.
+    "spatial_640x360_y8": dict(W=640, H=360, depth=8, chroma="gray", frames=8, r=0, P=2, S=7, sigma=(10.0, 0.0),
+                               steps=64),
+    # Temporal NLMeans, 1920x1080 10-bit 4:2:0, temporal radius 2, patch radius 3, search radius 4.
+    "temporal_1080p_420_10bit": dict(W=1920, H=1080, depth=10, chroma="420", frames=32, r=2, P=3, S=4,
+                                     sigma=(12.0, 8.0), steps=32),
+}
+
+
+def nlm_candidate_flops(C, P):
+    """Operations per candidate pixel of the NLM window (squared differences, channel scale,
+    separable box sum, weight and accumulation): 5C + 2(2P + 1) + 7; the exp is counted apart."""
+    return 5 * C + 2 * (2 * P + 1) + 7
+
+
+def nlm_leg(avd, torch, name, cfg, sample_steps=4):
+    """One NLMeans workload in steady state: frames/s, MPix/s and the window stages' achieved rate
+    against the FP32 vector roofline, with the exp count reported apart."""
+    W_, H_, depth, r, P, S = cfg["W"], cfg["H"], cfg["depth"], cfg["r"], cfg["P"], cfg["S"]
+    chroma = cfg["chroma"]
+    layout = {"gray": 0, "420": 1}[chroma]
+    fmt = avd.SampleFormat.u8() if depth == 8 else avd.SampleFormat.u16(depth)
+    geom = avd.Geometry(W_, H_, layout, fmt, fmt, 0)
+    opts = dict(hq=0, temporal=1 if r else 0, temporal_radius=r, patch_radius=P, search_radius=S, strength=1.2,
+                self_weight=1.0)
+    eng = avd.build_engine("nlmeans", geom, luma=avd.nlmeans_options(**opts),
+                           chroma=avd.nlmeans_options(**opts) if layout else None)
+    top = (1 << depth) - 1
+    clip = avd.synth.clip(cfg["frames"], W_, H_, depth, 0xA5D00000 + (1 if r == 0 else 2), chroma=chroma,
+                          sigma_luma=cfg["sigma"][0], sigma_chroma=cfg["sigma"][1], device="cuda",
+                          lo_hi=(0, top) if depth == 8 else (64, 940))
+    frames = [[avd.synth.pack_words(p, depth) for p in f] for f in clip]
+    dims = avd.plane_dims(geom)
+    outs = [torch.zeros(avd.plane_bytes(fmt, w, h), dtype=torch.uint8, device="cuda") for (w, h) in dims]
+    warm = max(2 * r + 1, 4)
+    run_steps(eng, frames, outs, 0, warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    emitted = run_steps(eng, frames, outs, warm, cfg["steps"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert emitted == cfg["steps"], (emitted, cfg["steps"])
+    eng.sample_device_time(1)
+    run_steps(eng, frames, outs, warm + cfg["steps"], sample_steps)
+    st = eng.timing()
+    eng.sample_device_time(0)
+    window_ms = (st["nlm_pair"]["ms"] + st["nlm_single"]["ms"]) / sample_steps
+    flops = (st["nlm_pair"]["work"] + st["nlm_single"]["work"]) / sample_steps
+    n_q = (2 * S + 1) ** 2
+    cands = n_q * (2 * r + 1) - 1
+    exps = sum(w * h * cands for (w, h) in dims[:1]) + (dims[1][0] * dims[1][1] * cands if layout else 0)
+    tflops = flops / (window_ms * 1e-3) / 1e12
+    fps = cfg["steps"] / elapsed
+    return {
+        "workload": f"NLMeans {'temporal radius %d' % r if r else 'spatial'}, {W_}x{H_} {depth}-bit "
+                    f"{'luma' if layout == 0 else '4:2:0'}, patch radius {P}, search radius {S}, FP32; "
+                    "plain form (one launch per stage and plane group; the batched form is not built)",
+        "frames_per_s": round(fps, 2), "mpix_per_s": round(fps * W_ * H_ / 1e6, 1),
+        "ms_per_frame": round(1000.0 / fps, 3),
+        "window_ms_per_frame": round(window_ms, 4), "flops_per_frame": int(flops),
+        "roofline": {"kernel": "k_nlm (pair + single windows)", "bound": "valu", "achieved": round(tflops, 3),
+                     "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s", "frac": round(tflops / VALU_PEAK_TOPS, 4)},
+        "exp_per_frame": int(exps), "exp_per_s": round(exps / (window_ms * 1e-3), 1),
+        "stages_ms_per_frame": {k: round(v["ms"] / sample_steps, 4) for k, v in st.items() if v["launches"]},
+    }
 
 
 def main():
@@ -249,7 +331,7 @@ def main():
 
     # Per-stage device time over a few more steps, with HIP events on the engine's stream; the
     # engine counts each stage's algorithmic operations (both search stages: 192 per candidate
-    # patch and channel; motion: 3 per pixel of every scored block offset; filter: 8 members x
+    # patch and channel; motion: 3 per pixel of every scored block offset; filter: 16 members x
     # (C x 4928 + 64 x (C + 1)) per reference).
     if not args.sample_all:
         eng.sample_device_time(1)
@@ -296,19 +378,23 @@ def main():
                    else "f32 filter (FP32 form)")),
         "data": "synthetic (seeded moving sinusoid/gradient scene, Gaussian noise 16/8 codes, generated on GPU)",
         "config": {
-            "workload": "NL4D 3840x2160 10-bit 4:2:0, temporal radius 3, spatial radius 12, refine 2, 8x8 patches "
-                        "at stride 4, groups of 8, hard threshold 2.7 sigma (sigma fixed: 16/1023 luma, 8/1023 "
-                        "chroma), motion 16x16/8 blocks radius 4 two-level Direct; " + headline + " form",
+            "workload": "NL4D 3840x2160 10-bit 4:2:0, 16 frames, temporal radius 3, spatial radius 12, refine 2, "
+                        "8x8 patches at stride 3, groups of up to 16 (volume grids of 4 frames x 4 volumes), "
+                        "hard threshold 2.7 sigma (sigma fixed at the clip's noise, 16/1023 luma and 8/1023 "
+                        "chroma: noise estimation is not built), motion 16x16/8 blocks radius 4 two-level "
+                        "Direct, field_lambda 0 (no truemotion stage); " + headline + " form",
             "global_batch": 1,
             "frames_in_clip": CLIP_FRAMES,
             "mpix_per_s": round(fps * W * H / 1e6, 3),
         },
         "roofline": {
             "kernel": f"{dominant} ({dom['kernels']})",
-            "bound": "mfma",
+            # The centre search's products run on the matrix cores (f16); every other stage is
+            # vector work (integer v_dot2 / v_sad operations, f32 FMAs) priced at the FP32 rate.
+            "bound": "mfma" if dominant == "collab_centre" else "valu",
             "achieved": dom["achieved"],
             "peak": dom["peak"],
-            "unit": "TFLOP/s",
+            "unit": "TFLOP/s" if dominant == "collab_centre" else "TOP/s",
             "frac": dom["frac"],
             "traffic": traffic,
             "avg_launch_ms": dom["avg_launch_ms"],
@@ -321,6 +407,8 @@ def main():
         "forms": {f: {"frames_per_s": forms[f]["value"], "psnr_db_vs_clean": psnr.get(f)} for f in forms},
     }
     result["next_rows"] = {"noise_spatial": noise_spatial_measure(avd, torch, clip)}
+    if not args.no_nlm:
+        result["nlm"] = {name: nlm_leg(avd, torch, name, cfg) for name, cfg in NLM_CONFIGS.items()}
     if "mixed" in forms and "fp32" in forms:
         result["mixed_vs_fp32"] = round(forms["mixed"]["value"] / forms["fp32"]["value"], 4)
         if psnr:
@@ -337,9 +425,10 @@ def main():
             "unit": "frames/s",
             "cores": threads,
             "kind": "port",
-            "sample": "one steady-state frame (luma + chroma passes: motion of 6 neighbours, grouping, filter, "
-                      "normalise, egress) over the top-left 1920x1080 quarter of the same clip through the "
-                      "oracle's FP32 restatement, time x4 for the full 3840x2160 frame",
+            "sample": "one steady-state frame (luma + chroma passes: motion of 6 neighbours, grouping at stride 3 "
+                      "in groups of up to 16, filter, normalise, egress) over the top-left 960x540 sixteenth of "
+                      "the same clip through the oracle's FP32 restatement: median of 5 runs after one warm-up, "
+                      "time x16 for the full 3840x2160 frame",
         }
     print(json.dumps(result))
 
