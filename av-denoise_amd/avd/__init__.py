@@ -11,15 +11,18 @@ from __future__ import annotations
 import ctypes
 
 from . import synth  # noqa: F401  (seeded clips)
+from . import y4m  # noqa: F401  (YUV4MPEG2 streams)
 from ._lib import (CollabPass, ContextAfterPush, DevicePlane, EngineSpec, Error, Geometry, Gpu, InvalidGeometry,
                    InvalidOptions, MotionPass, MotionSearch, NeedsReset, Nl4dOptions, NlmeansOptions, NlmWindow,
-                   NothingToEmit, NullArgument, OutputsPending, PlaneMismatch, SampleFormat, WindowSpan, check, load)
+                   NothingToEmit, NullArgument, OutputsPending, PlaneMismatch, Poisoned, QueueFull, SampleFormat,
+                   WindowSpan, check, load)
 
 __all__ = [
     "Engine", "build_engine", "EngineSpec", "Geometry", "SampleFormat", "MotionSearch", "NlmeansOptions",
     "Nl4dOptions", "nlmeans_options", "nl4d_options", "nlmeans_options_for_preset", "nl4d_options_for_preset",
     "plane_bytes", "chroma_dims", "Error", "InvalidOptions", "InvalidGeometry", "PlaneMismatch", "OutputsPending",
-    "NothingToEmit", "ContextAfterPush", "NeedsReset", "Gpu", "NullArgument", "load", "PRESETS",
+    "NothingToEmit", "ContextAfterPush", "NeedsReset", "Gpu", "NullArgument", "QueueFull", "Poisoned", "load",
+    "PRESETS", "HostDenoiser", "engine_spec", "y4m",
 ]
 
 PRESETS = {"veryfast": 0, "fast": 1, "base": 2, "slow": 3, "veryslow": 4}
@@ -197,8 +200,8 @@ class Engine:
             pass
 
 
-def build_engine(kind: str, geometry: Geometry, luma=None, chroma=None, stream=None) -> Engine:
-    """build_engine (tune/setup.rs:79-92): ``kind`` is "nlmeans" or "nl4d"; a plane group left
+def engine_spec(kind: str, geometry: Geometry, luma=None, chroma=None, stream=None) -> EngineSpec:
+    """EngineSpec (engine/spec.rs:146-157): ``kind`` is "nlmeans" or "nl4d"; a plane group left
     ``None`` passes through unchanged."""
     spec = EngineSpec()
     spec.kind = 0 if kind == "nlmeans" else 1
@@ -212,4 +215,89 @@ def build_engine(kind: str, geometry: Geometry, luma=None, chroma=None, stream=N
         spec.nl4d_luma = luma if luma is not None else nl4d_options()
         spec.nl4d_chroma = chroma if chroma is not None else nl4d_options()
     spec.stream = stream
-    return Engine(spec)
+    return spec
+
+
+def build_engine(kind: str, geometry: Geometry, luma=None, chroma=None, stream=None) -> Engine:
+    """build_engine (tune/setup.rs:79-92): ``kind`` is "nlmeans" or "nl4d"; a plane group left
+    ``None`` passes through unchanged."""
+    return Engine(engine_spec(kind, geometry, luma, chroma, stream))
+
+
+class HostDenoiser:
+    """Frames in and out as host bytes (the reference's HostDenoiser, av-denoise/src/host/mod.rs):
+    ``push`` takes one bytes-like buffer per plane (Y, then U and V), ``recv`` returns the oldest
+    finished frame (or None), ``flush`` drains the rest and the temporal tail.  At most
+    ``MAX_PENDING`` frames wait to be collected; a push past that raises QueueFull."""
+
+    MAX_PENDING = 2
+
+    def __init__(self, kind: str, geometry: Geometry, luma=None, chroma=None):
+        self._lib = load()
+        self._h = ctypes.c_void_p()
+        check(self._lib.avd_host_denoiser_create(ctypes.byref(engine_spec(kind, geometry, luma, chroma)),
+                                                 ctypes.byref(self._h)))
+        n = ctypes.c_uint32()
+        ins, outs = (ctypes.c_uint64 * 3)(), (ctypes.c_uint64 * 3)()
+        check(self._lib.avd_host_denoiser_plane_lengths(self._h, ins, outs, 3, ctypes.byref(n)))
+        self.input_lengths = [int(ins[i]) for i in range(n.value)]
+        self.output_lengths = [int(outs[i]) for i in range(n.value)]
+
+    @staticmethod
+    def _pointers(buffers):
+        import numpy as np
+
+        arrays = [np.ascontiguousarray(np.frombuffer(b, np.uint8) if isinstance(b, (bytes, bytearray)) else b)
+                  for b in buffers]
+        ptrs = (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+        lens = (ctypes.c_uint64 * len(arrays))(*[a.nbytes for a in arrays])
+        return arrays, ptrs, lens
+
+    def push(self, planes) -> None:
+        keep, ptrs, lens = self._pointers(planes)
+        check(self._lib.avd_host_denoiser_push(self._h, ptrs, lens, len(keep)))
+
+    def push_priming(self, planes) -> None:
+        keep, ptrs, lens = self._pointers(planes)
+        check(self._lib.avd_host_denoiser_push_priming(self._h, ptrs, lens, len(keep)))
+
+    def recv(self):
+        """The oldest frame in flight as one bytes object per plane, or None when none is."""
+        bufs = [ctypes.create_string_buffer(n) for n in self.output_lengths]
+        ptrs = (ctypes.c_void_p * len(bufs))(*[ctypes.addressof(b) for b in bufs])
+        caps = (ctypes.c_uint64 * len(bufs))(*self.output_lengths)
+        got = ctypes.c_uint32()
+        check(self._lib.avd_host_denoiser_recv(self._h, ptrs, caps, len(bufs), ctypes.byref(got)))
+        return [b.raw for b in bufs] if got.value else None
+
+    def flush(self):
+        """Every remaining frame, in order; the denoiser then takes a fresh stream."""
+        from ._lib import FRAME_SINK
+
+        frames = []
+
+        def sink(_user, planes, lengths, count):
+            frames.append([ctypes.string_at(planes[i], lengths[i]) for i in range(count)])
+
+        cb = FRAME_SINK(sink)
+        check(self._lib.avd_host_denoiser_flush(self._h, cb, None))
+        return frames
+
+    def reset_stream(self) -> None:
+        check(self._lib.avd_host_denoiser_reset_stream(self._h))
+
+    def pending(self) -> int:
+        n = ctypes.c_uint32()
+        check(self._lib.avd_host_denoiser_pending(self._h, ctypes.byref(n)))
+        return n.value
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.avd_host_denoiser_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

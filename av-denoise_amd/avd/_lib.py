@@ -155,11 +155,25 @@ class NullArgument(Error):
     status = 9
 
 
+class QueueFull(Error):
+    """DenoiserError::QueueFull (av-denoise/src/host/mod.rs:40-44)."""
+    status = 10
+
+
+class Poisoned(Error):
+    """DenoiserError::Poisoned (av-denoise/src/host/mod.rs:45-49)."""
+    status = 11
+
+
 _BY_STATUS = {c.status: c for c in (InvalidOptions, InvalidGeometry, PlaneMismatch, OutputsPending, NothingToEmit,
-                                     ContextAfterPush, NeedsReset, Gpu, NullArgument)}
+                                     ContextAfterPush, NeedsReset, Gpu, NullArgument, QueueFull, Poisoned)}
 
 # Every exported symbol and its ctypes signature (restype, argtypes).
 P = ctypes.c_void_p
+# avd_frame_sink: (user, planes, lengths, count).
+FRAME_SINK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64),
+                              ctypes.c_uint32)
+
 SIGNATURES = {
     "avd_abi_version": (u32, []),
     "avd_last_error_message": (ctypes.c_char_p, []),
@@ -194,6 +208,19 @@ SIGNATURES = {
     "avd_site_nlm_pair": (ctypes.c_int, [ctypes.POINTER(NlmWindow), P, P, P, P, P, P, P, P, P]),
     "avd_site_nlm_finish": (ctypes.c_int, [P, P, P, P, f32, u32, u32, P, P]),
     "avd_site_noise_spatial": (ctypes.c_int, [P, u32, u32, u32, u32, P, P, u32, P]),
+    "avd_host_denoiser_create": (ctypes.c_int, [ctypes.POINTER(EngineSpec), ctypes.POINTER(P)]),
+    "avd_host_denoiser_destroy": (None, [P]),
+    "avd_host_denoiser_plane_lengths": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64),
+                                                       ctypes.POINTER(ctypes.c_uint64), u32,
+                                                       ctypes.POINTER(ctypes.c_uint32)]),
+    "avd_host_denoiser_push": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(ctypes.c_uint64), u32]),
+    "avd_host_denoiser_push_priming": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(ctypes.c_uint64), u32]),
+    "avd_host_denoiser_recv": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(ctypes.c_uint64), u32,
+                                              ctypes.POINTER(ctypes.c_uint32)]),
+    "avd_host_denoiser_flush": (ctypes.c_int, [P, FRAME_SINK, P]),
+    "avd_host_denoiser_reset_stream": (ctypes.c_int, [P]),
+    "avd_host_denoiser_pending": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint32)]),
+    "avd_host_denoiser_window_span": (ctypes.c_int, [P, ctypes.POINTER(WindowSpan)]),
     "avd_site_noise_spatial_noalloc": (ctypes.c_int, [P, u32, u32, u32, u32, P, u32, P]),
     "avd_noise_partials_len": (u32, [u32, u32]),
     "avd_noise_sigma_from_abs_sum": (f32, [f32, u32, u32]),

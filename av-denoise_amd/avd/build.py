@@ -34,6 +34,7 @@ SOURCES = {
     "nlm.hip": [],
     "noise.hip": EXACT,
     "engine.cpp": EXACT,
+    "host.cpp": EXACT,
     "abi.cpp": EXACT,
 }
 

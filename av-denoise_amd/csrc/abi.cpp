@@ -7,10 +7,16 @@
 #include <string>
 
 #include "engine.h"
+#include "host.h"
 
 struct avd_engine {
     avd::Engine impl;
     explicit avd_engine(const avd_engine_spec &s) : impl(s) {}
+};
+
+struct avd_host_denoiser {
+    avd::HostDenoiser impl;
+    explicit avd_host_denoiser(const avd_engine_spec &s) : impl(s) {}
 };
 
 namespace {
@@ -230,6 +236,79 @@ avd_status avd_engine_timing(avd_engine *e, avd_timing *out)
 }
 
 // ---- launch sites ----------------------------------------------------------------------------
+
+avd_status avd_host_denoiser_create(const avd_engine_spec *spec, avd_host_denoiser **out)
+{
+    if (!spec || !out) return fail(AVD_ERR_NULL_ARGUMENT, "spec or out is null");
+    *out = nullptr;
+    return guard([&] { *out = new avd_host_denoiser(*spec); });
+}
+
+void avd_host_denoiser_destroy(avd_host_denoiser *d) { delete d; }
+
+avd_status avd_host_denoiser_plane_lengths(const avd_host_denoiser *d, uint64_t *input, uint64_t *output,
+                                           uint32_t count, uint32_t *planes)
+{
+    if (!d || !planes) return fail(AVD_ERR_NULL_ARGUMENT, "denoiser or planes is null");
+    *planes = d->impl.planes();
+    for (uint32_t i = 0; i < count && i < d->impl.planes(); ++i) {
+        if (input) input[i] = d->impl.input_length(i);
+        if (output) output[i] = d->impl.output_length(i);
+    }
+    return AVD_OK;
+}
+
+avd_status avd_host_denoiser_push(avd_host_denoiser *d, const void *const *planes, const uint64_t *lengths,
+                                  uint32_t count)
+{
+    if (!d || !planes || !lengths) return fail(AVD_ERR_NULL_ARGUMENT, "denoiser, planes or lengths is null");
+    return guard([&] { d->impl.push(planes, lengths, count); });
+}
+
+avd_status avd_host_denoiser_push_priming(avd_host_denoiser *d, const void *const *planes, const uint64_t *lengths,
+                                          uint32_t count)
+{
+    if (!d || !planes || !lengths) return fail(AVD_ERR_NULL_ARGUMENT, "denoiser, planes or lengths is null");
+    return guard([&] { d->impl.push_priming(planes, lengths, count); });
+}
+
+avd_status avd_host_denoiser_recv(avd_host_denoiser *d, void *const *planes, const uint64_t *capacities,
+                                  uint32_t count, uint32_t *received)
+{
+    if (!d || !planes || !capacities || !received)
+        return fail(AVD_ERR_NULL_ARGUMENT, "denoiser, planes, capacities or received is null");
+    *received = 0;
+    return guard([&] { *received = d->impl.recv(planes, capacities, count) ? 1u : 0u; });
+}
+
+avd_status avd_host_denoiser_flush(avd_host_denoiser *d, avd_frame_sink sink, void *user)
+{
+    if (!d || !sink) return fail(AVD_ERR_NULL_ARGUMENT, "denoiser or sink is null");
+    return guard([&] {
+        d->impl.flush([&](const void *const *planes, const uint64_t *lengths, uint32_t count) {
+            sink(user, planes, lengths, count);
+        });
+    });
+}
+
+avd_status avd_host_denoiser_reset_stream(avd_host_denoiser *d)
+{
+    if (!d) return fail(AVD_ERR_NULL_ARGUMENT, "denoiser is null");
+    return guard([&] { d->impl.reset_stream(); });
+}
+
+avd_status avd_host_denoiser_pending(const avd_host_denoiser *d, uint32_t *pending)
+{
+    if (!d || !pending) return fail(AVD_ERR_NULL_ARGUMENT, "denoiser or pending is null");
+    *pending = d->impl.pending();
+    return AVD_OK;
+}
+
+avd_status avd_host_denoiser_window_span(const avd_host_denoiser *d, avd_window_span *out)
+{
+    if (!d || !out) return fail(AVD_ERR_NULL_ARGUMENT, "denoiser or out is null");
+    return guard([&] { *out = d->impl.window_span(); });
+}
 
 avd_status avd_site_ingest(const void *plane0, const void *plane1, float *slot, uint16_t *codes, uint32_t pixels,
                            uint32_t channels, avd_sample_format format, void *stream)

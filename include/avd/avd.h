@@ -49,7 +49,10 @@ typedef enum avd_status {
     AVD_ERR_CONTEXT_AFTER_PUSH = 6,
     AVD_ERR_NEEDS_RESET = 7,
     AVD_ERR_GPU = 8,
-    AVD_ERR_NULL_ARGUMENT = 9
+    AVD_ERR_NULL_ARGUMENT = 9,
+    /* The host wire API's own errors (DenoiserError, av-denoise/src/host/mod.rs:36-50). */
+    AVD_ERR_QUEUE_FULL = 10,
+    AVD_ERR_POISONED = 11
 } avd_status;
 
 /* SampleFormat: U8, or U16 holding `depth` significant bits (9..16). */
@@ -256,6 +259,44 @@ typedef struct avd_timing {
 avd_status avd_engine_sample_device_time(avd_engine *engine, uint32_t every);
 /* Waits for the sampled launches and returns the totals since sampling began (Engine::timing). */
 avd_status avd_engine_timing(avd_engine *engine, avd_timing *out);
+
+/* ---- host wire API (HostDenoiser, av-denoise/src/host/mod.rs:27-345) ------------------------ */
+
+/* A denoiser over frames held as host bytes: one buffer per plane (Y, then U and V for a chroma
+ * layout), each plane `width * height` samples of 1 byte (U8) or 2 bytes little-endian (U16).  It
+ * drives an engine of `spec` (whose `stream` is ignored: the denoiser runs on a stream of its own)
+ * and keeps at most AVD_HOST_MAX_PENDING readbacks in flight (MAX_PENDING, host/mod.rs:27). */
+#define AVD_HOST_MAX_PENDING 2u
+typedef struct avd_host_denoiser avd_host_denoiser;
+/* Receives one output frame: `count` planes of `lengths[i]` bytes, valid for the call only. */
+typedef void (*avd_frame_sink)(void *user, const void *const *planes, const uint64_t *lengths, uint32_t count);
+
+avd_status avd_host_denoiser_create(const avd_engine_spec *spec, avd_host_denoiser **out);
+void avd_host_denoiser_destroy(avd_host_denoiser *denoiser);
+/* Planes per frame and the wire bytes of each input and output plane (`count` entries each). */
+avd_status avd_host_denoiser_plane_lengths(const avd_host_denoiser *denoiser, uint64_t *input, uint64_t *output,
+                                           uint32_t count, uint32_t *planes);
+/* HostDenoiser::push: uploads one frame and starts denoising the frames it completes.
+ * AVD_ERR_QUEUE_FULL once AVD_HOST_MAX_PENDING frames wait to be collected (no state changes; recv,
+ * then push the same frame again); AVD_ERR_PLANE_MISMATCH for a wrong plane count or length (no
+ * state changes); any other failure poisons the stream (AVD_ERR_POISONED until reset_stream). */
+avd_status avd_host_denoiser_push(avd_host_denoiser *denoiser, const void *const *planes, const uint64_t *lengths,
+                                  uint32_t count);
+/* HostDenoiser::push_priming: one frame of context only (Engine::push_context). */
+avd_status avd_host_denoiser_push_priming(avd_host_denoiser *denoiser, const void *const *planes,
+                                          const uint64_t *lengths, uint32_t count);
+/* HostDenoiser::recv: waits for the oldest frame in flight and copies it into `planes`
+ * (`capacities[i]` bytes each); *received = 0 when nothing is in flight. */
+avd_status avd_host_denoiser_recv(avd_host_denoiser *denoiser, void *const *planes, const uint64_t *capacities,
+                                  uint32_t count, uint32_t *received);
+/* HostDenoiser::flush: hands the frames in flight and the temporal tail to `sink` in order; the
+ * denoiser then takes a fresh stream. */
+avd_status avd_host_denoiser_flush(avd_host_denoiser *denoiser, avd_frame_sink sink, void *user);
+/* HostDenoiser::reset_stream: drops the frames in flight and the stream, and clears the poison. */
+avd_status avd_host_denoiser_reset_stream(avd_host_denoiser *denoiser);
+/* Frames in flight. */
+avd_status avd_host_denoiser_pending(const avd_host_denoiser *denoiser, uint32_t *pending);
+avd_status avd_host_denoiser_window_span(const avd_host_denoiser *denoiser, avd_window_span *out);
 
 /* ---- launch sites: one stage of one pass over caller-owned device buffers ----------------- */
 
