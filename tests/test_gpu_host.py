@@ -49,8 +49,9 @@ def test_host_denoiser_matches_the_engine_and_keeps_its_queue():
     queue_full_seen = False
     for f in clip:
         planes = [_wire(p.numpy(), depth) for p in f]
-        with pytest.raises(avd.PlaneMismatch):  # a short plane changes nothing
-            den.push([planes[0][:-2]] + planes[1:])
+        if den.pending() < avd.HostDenoiser.MAX_PENDING:  # (a full queue is reported first)
+            with pytest.raises(avd.PlaneMismatch):  # a short plane changes nothing
+                den.push([planes[0][:-2]] + planes[1:])
         while True:
             try:
                 den.push(planes)
