@@ -1117,15 +1117,15 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_redo(avd_collab_pass p, 
 // counters' per-wave sums (static).
 constexpr size_t kLdsBudget = 64 * 1024;
 
-// Volumes of a group at most (two frames per volume) and picks of earlier volumes a candidate can
-// equal at most, for KM group members.
+// Volumes of a group at most (two frames per volume) for KM group members; an earlier volume
+// holds at most one pick per neighbour frame, so a frame holds at most VOLS - 1 picks.
 __host__ __device__ constexpr int trajectory_vols(int KM) { return KM / 2; }
-__host__ __device__ constexpr int trajectory_held(int KM) { return KM == 16 ? 9 : 3; }
 
 __host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR, int KM)
 {
     return (size_t)kWaves * (NR * 256 * 2 + 16 * 4 + NR * 16 * 8 + NR * trajectory_vols(KM) * 8 * 4 * C * 4 +
-                             NR * trajectory_vols(KM) * 4 + NR * trajectory_held(KM) * 4 + 4);
+                             NR * trajectory_vols(KM) * 4 + NR * 16 * (trajectory_vols(KM) - 1) * 4 + NR * 16 * 4 +
+                             4);
 }
 
 __device__ __forceinline__ uint32_t lane_mask_lt(uint64_t bal)
@@ -1152,15 +1152,16 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                                                                   unsigned long long *__restrict__ scored_count)
 {
     constexpr int LPR = 64 / NR;  // lanes per reference
-    constexpr int VOLS = trajectory_vols(KM), HELD = trajectory_held(KM);
-    static_assert(LPR >= KM + 1, "a reference's lanes write its member words and group word");
+    constexpr int VOLS = trajectory_vols(KM);
+    static_assert(LPR >= KM, "a reference's lanes write its member words");
     extern __shared__ uint32_t s_dyn[];  // [kWaves][NR * entries][2]: (left | right << 16), (top | bottom << 16)
     __shared__ uint16_t s_list[kWaves][NR * 256];  // non-empty entries of the current volume: e | r << 8
     __shared__ uint32_t s_slot[kWaves][16];
     __shared__ unsigned long long s_fbest[kWaves][NR][16];
     __shared__ uint32_t s_patch[kWaves][NR][VOLS][8][4 * C];  // every volume's anchor patch
     __shared__ uint32_t s_energy[kWaves][NR][VOLS];
-    __shared__ uint32_t s_held[kWaves][NR][HELD];
+    __shared__ uint32_t s_held[kWaves][NR][16][VOLS - 1];  // earlier volumes' picks, by neighbour frame
+    __shared__ uint32_t s_hcnt[kWaves][NR][16];
     constexpr int refine = (K - 1) / 2;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int hr = lane / LPR, hl = lane % LPR, hbase = hr * LPR;
@@ -1231,6 +1232,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
         rect_tab[hr * ents + idx][0] = xr;
         rect_tab[hr * ents + idx][1] = yr;
     }
+    if (hl < frames_n) s_hcnt[wave][hr][hl] = 0;
     wave_sync();
     if (hl < volumes) {
         uint32_t e = 0;
@@ -1245,16 +1247,13 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
 
     // final[] collects the grid's members.  A volume's list holds its gf - 1 frame picks and their
     // distances (SSD + 1 fits 32 bits; ~0u marks an empty slot).  Only earlier volumes' picks can
-    // equal a candidate (anchors sit in the centre frame), at most HELD of them (3 for eight
-    // members, 9 for sixteen; position word 0, in the centre frame, matches no candidate).  All of
-    // this is the same on every lane of one reference.
+    // equal a candidate (anchors sit in the centre frame); they are kept by frame in s_held.  All
+    // of this is the same on every lane of one reference.
     uint32_t scored = 0;
-    uint32_t final_pos[KM], held[HELD];
+    uint32_t final_pos[KM];
     bool filled = true;
 #pragma unroll
     for (int i = 0; i < KM; ++i) final_pos[i] = 0;
-#pragma unroll
-    for (int i = 0; i < HELD; ++i) held[i] = 0u;
     for (int v = 0; v < volumes; ++v) {
         const int first = v * gf;
         uint32_t anchor = anchors[0];
@@ -1262,13 +1261,6 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
         for (int i = 1; i < VOLS; ++i)
             if (v == i) anchor = anchors[i];
         uint32_t vd[3] = {~0u, ~0u, ~0u}, vp[3] = {0u, 0u, 0u};
-        if (hl < HELD) {
-            uint32_t h = held[0];
-#pragma unroll
-            for (int i = 1; i < HELD; ++i)
-                if (hl == i) h = held[i];
-            s_held[wave][hr][hl] = h;
-        }
         // Compact the rectangles that hold a position no earlier rectangle of their frame covers
         // (an empty or fully covered one adds nothing but its share of the walk index), in
         // (reference, entry) order.
@@ -1320,12 +1312,11 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                     if (ylo <= yhi) skip |= ((2u << (yhi - ylo)) - 1u) << (ylo - tp);
                 }
             }
-#pragma unroll
-            for (int i = 0; i < HELD; ++i) {
-                const uint32_t g = s_held[wave][tr][i];
+            const int n_held = (int)s_hcnt[wave][tr][t];
+            for (int i = 0; i < n_held; ++i) {
+                const uint32_t g = s_held[wave][tr][t][i];
                 const int gy = (int)((g >> 13) & 0x1FFF);
-                if ((int)(g >> 26) == t + 1 && (int)(g & 0x1FFF) == cx && gy >= tp && gy <= bt)
-                    skip |= 1u << (gy - tp);
+                if ((int)(g & 0x1FFF) == cx && gy >= tp && gy <= bt) skip |= 1u << (gy - tp);
             }
             const uint32_t live_rows = ((1u << h) - 1u) & ~skip;
             if (!live_rows) continue;  // every position of this column was searched or is held
@@ -1397,23 +1388,29 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
             if (i == first) final_pos[i] = anchor;
             else if (j >= 0 && j < gf - 1) final_pos[i] = j == 0 ? vp[0] : (j == 1 ? vp[1] : vp[2]);
         }
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            if (j < gf - 1) {  // this volume's picks are held from the next volume on
-#pragma unroll
-                for (int i = HELD - 1; i > 0; --i) held[i] = held[i - 1];
-                held[0] = vp[j];
+        // This volume's picks are held from the next volume on, each in its own frame's list (the
+        // picks of one volume sit in distinct frames, so lane j appends pick j alone).
+        if (hl < gf - 1) {
+            const uint32_t pick = hl == 0 ? vp[0] : (hl == 1 ? vp[1] : vp[2]);
+            if (pick != 0u) {
+                const int tf = (int)(pick >> 26) - 1;
+                const uint32_t at = s_hcnt[wave][hr][tf];
+                s_held[wave][hr][tf][at] = pick;
+                s_hcnt[wave][hr][tf] = at + 1;
             }
         }
         wave_sync();
     }
     const bool use_grid = k_use == KM && filled;
-    if (use_grid && live && hl < KM + 1) {
-        uint32_t word = (uint32_t)KM | 0x100u;
+    if (use_grid && live) {
+        // Lane hl < KM writes member hl; the group word follows (lane KM, or lane 0 again when the
+        // reference has only KM lanes).
+        uint32_t word = 0;
 #pragma unroll
         for (int m = 0; m < KM; ++m)
             if (hl == m) word = final_pos[m];
-        members[(size_t)my_ref * (KM + 1) + hl] = word;
+        if (hl < KM) members[(size_t)my_ref * (KM + 1) + hl] = word;
+        if (hl == (LPR > KM ? KM : 0)) members[(size_t)my_ref * (KM + 1) + KM] = (uint32_t)KM | 0x100u;
     }
     add_scored(scored_count, scored, true, C);
 }
@@ -2560,9 +2557,9 @@ hipError_t launch_centre_km(const avd_collab_pass &p, const uint16_t *codes, uin
     return hipGetLastError();
 }
 
-// One trajectory launch: NR references per wave (4 or 2 for eight members, 2 or 1 for sixteen,
-// whose 17 words need more lanes per reference), SHORT for frames shorter than K + 7 rows or rings
-// past a buffer load's offset.
+// One trajectory launch: NR references per wave (4, or fewer where the rectangle tables outgrow
+// the LDS budget: 2 for eight members, 2 then 1 for sixteen), SHORT for frames shorter than K + 7
+// rows or rings past a buffer load's offset.
 template <int C, int K, int KM>
 hipError_t launch_trajectory_ck(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
                                 const int32_t *mv, const float *conf, uint32_t *members, hipStream_t s,
@@ -2581,6 +2578,8 @@ hipError_t launch_trajectory_ck(const avd_collab_pass &p, const uint16_t *codes,
     } else {
         if (short_frame)
             k_trajectory_codes<C, K, 1, true, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
+        else if (nr == 4)
+            k_trajectory_codes<C, K, 4, false, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
         else if (nr == 2)
             k_trajectory_codes<C, K, 2, false, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
         else
@@ -2597,9 +2596,12 @@ hipError_t launch_trajectory_km(const avd_collab_pass &p, const uint16_t *codes,
     // References per wave: the most whose rectangle tables fit the LDS budget.
     const size_t ents = (size_t)trajectory_entries(p) * 2 * sizeof(uint32_t) * kWaves;
     const int C = (int)p.channels;
-    const int nr_hi = KM == 8 ? 4 : 2, nr_lo = KM == 8 ? 2 : 1;
-    const size_t st_hi = C == 1 ? trajectory_static_lds(1, nr_hi, KM) : trajectory_static_lds(2, nr_hi, KM);
-    int nr = st_hi + nr_hi * ents <= kLdsBudget ? nr_hi : nr_lo;
+    auto fits = [&](int nr) {
+        const size_t st = C == 1 ? trajectory_static_lds(1, nr, KM) : trajectory_static_lds(2, nr, KM);
+        return st + nr * ents <= kLdsBudget;
+    };
+    const int nr_lo = KM == 8 ? 2 : 1;
+    int nr = fits(4) ? 4 : (KM == 16 && fits(2) ? 2 : nr_lo);
     // Frames shorter than K + 7 rows, and rings past 2^31 bytes (beyond a buffer load's offset),
     // take the clamping slide (at two references per wave for eight members, one for sixteen).
     const bool short_frame = (int)p.height < 2 * (int)p.refine + 8 ||
