@@ -22,7 +22,7 @@ __all__ = [
     "Nl4dOptions", "nlmeans_options", "nl4d_options", "nlmeans_options_for_preset", "nl4d_options_for_preset",
     "plane_bytes", "chroma_dims", "Error", "InvalidOptions", "InvalidGeometry", "PlaneMismatch", "OutputsPending",
     "NothingToEmit", "ContextAfterPush", "NeedsReset", "Gpu", "NullArgument", "QueueFull", "Poisoned", "load",
-    "PRESETS", "HostDenoiser", "engine_spec", "y4m",
+    "PRESETS", "HostDenoiser", "engine_spec", "y4m", "set_tuning", "tuning_entries",
 ]
 
 PRESETS = {"veryfast": 0, "fast": 1, "base": 2, "slow": 3, "veryslow": 4}
@@ -222,6 +222,23 @@ def build_engine(kind: str, geometry: Geometry, luma=None, chroma=None, stream=N
     """build_engine (tune/setup.rs:79-92): ``kind`` is "nlmeans" or "nl4d"; a plane group left
     ``None`` passes through unchanged."""
     return Engine(engine_spec(kind, geometry, luma, chroma, stream))
+
+
+def set_tuning(enabled: bool) -> None:
+    """Turns the launch-site tuner on (the default) or off (every site runs its baseline)."""
+    load().avd_tuning_set_enabled(1 if enabled else 0)
+
+
+def tuning_entries() -> list:
+    """The tuner's choices so far: site, shape key, chosen candidate and each candidate's time."""
+    from ._lib import TuneEntry
+
+    n = ctypes.c_uint32()
+    check(load().avd_tuning_entries(None, 0, ctypes.byref(n)))
+    arr = (TuneEntry * max(n.value, 1))()
+    check(load().avd_tuning_entries(arr, n.value, ctypes.byref(n)))
+    return [{"site": e.site.decode(), "key": e.key.decode(), "device": e.device, "choice": e.choice,
+             "ms": [round(e.ms[i], 4) for i in range(e.candidates)]} for e in arr[:n.value]]
 
 
 class HostDenoiser:

@@ -93,6 +93,12 @@ class Timing(ctypes.Structure):
                 for i, name in enumerate(STAGES)}
 
 
+class TuneEntry(ctypes.Structure):
+    """avd_tune_entry: one launch site's tuned choice."""
+    _fields_ = [("site", ctypes.c_char * 32), ("key", ctypes.c_char * 96), ("device", u32), ("choice", u32),
+                ("candidates", u32), ("ms", ctypes.c_double * 8)]
+
+
 class CollabPass(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("channels", u32), ("step", u32), ("spatial_radius", u32),
                 ("temporal_radius", u32), ("refine", u32), ("group_size", u32), ("blk_step", u32),
@@ -221,6 +227,8 @@ SIGNATURES = {
     "avd_host_denoiser_reset_stream": (ctypes.c_int, [P]),
     "avd_host_denoiser_pending": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint32)]),
     "avd_host_denoiser_window_span": (ctypes.c_int, [P, ctypes.POINTER(WindowSpan)]),
+    "avd_tuning_set_enabled": (None, [u32]),
+    "avd_tuning_entries": (ctypes.c_int, [ctypes.POINTER(TuneEntry), u32, ctypes.POINTER(ctypes.c_uint32)]),
     "avd_site_noise_spatial_noalloc": (ctypes.c_int, [P, u32, u32, u32, u32, P, u32, P]),
     "avd_noise_partials_len": (u32, [u32, u32]),
     "avd_noise_sigma_from_abs_sum": (f32, [f32, u32, u32]),

@@ -35,6 +35,7 @@ SOURCES = {
     "noise.hip": EXACT,
     "engine.cpp": EXACT,
     "host.cpp": EXACT,
+    "tune.cpp": EXACT,
     "abi.cpp": EXACT,
 }
 

@@ -8,6 +8,7 @@
 
 #include "engine.h"
 #include "host.h"
+#include "tune.h"
 
 struct avd_engine {
     avd::Engine impl;
@@ -308,6 +309,27 @@ avd_status avd_host_denoiser_window_span(const avd_host_denoiser *d, avd_window_
 {
     if (!d || !out) return fail(AVD_ERR_NULL_ARGUMENT, "denoiser or out is null");
     return guard([&] { *out = d->impl.window_span(); });
+}
+
+void avd_tuning_set_enabled(uint32_t enabled) { avd::Tuner::instance().set_enabled(enabled != 0); }
+
+avd_status avd_tuning_entries(avd_tune_entry *out, uint32_t capacity, uint32_t *count)
+{
+    if (!count || (capacity && !out)) return fail(AVD_ERR_NULL_ARGUMENT, "out or count is null");
+    return guard([&] {
+        const std::vector<avd::TuneEntry> all = avd::Tuner::instance().entries();
+        *count = (uint32_t)all.size();
+        for (uint32_t i = 0; i < capacity && i < all.size(); ++i) {
+            avd_tune_entry &e = out[i];
+            std::memset(&e, 0, sizeof(e));
+            std::strncpy(e.site, all[i].site.c_str(), sizeof(e.site) - 1);
+            std::strncpy(e.key, all[i].key.c_str(), sizeof(e.key) - 1);
+            e.device = (uint32_t)all[i].device;
+            e.choice = (uint32_t)all[i].choice;
+            e.candidates = (uint32_t)all[i].candidates;
+            for (size_t k = 0; k < all[i].ms.size() && k < 8; ++k) e.ms[k] = all[i].ms[k];
+        }
+    });
 }
 
 avd_status avd_site_ingest(const void *plane0, const void *plane1, float *slot, uint16_t *codes, uint32_t pixels,

@@ -28,11 +28,15 @@ struct NlmWindow {
     float offset;        // pair: the flat noise offset
     int step, blocks_x, blocks_y;
 };
+// `variant` picks the rows each wave walks (0: 16, 1: 8, 2: 4); every variant gives bit-identical
+// sums, so the tuner may choose any (kNlmVariants of them).
+constexpr int kNlmVariants = 3;
+int nlm_variant_rows(int variant);
 hipError_t launch_nlm_single(const NlmWindow &w, const float *frame, const float *lut, float *accum, float *wsum,
-                             float *maxw, hipStream_t s);
+                             float *maxw, hipStream_t s, int variant = 0);
 hipError_t launch_nlm_pair(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd,
                            const float *conf_f, const float *conf_b, float *accum, float *wsum, float *maxw,
-                           hipStream_t s);
+                           hipStream_t s, int variant = 0);
 hipError_t launch_nlm_finish(const float *frame, const float *accum, const float *wsum, const float *maxw,
                              float self_weight, uint32_t pixels, int C, float *out, hipStream_t s);
 bool nlm_supported(int C, int P, int S);

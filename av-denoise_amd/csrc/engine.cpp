@@ -9,6 +9,7 @@
 // nlmeans/params.rs:229-268, nlmeans/options.rs:106-137, nl4d/options.rs:118-185,
 // collab/kernels/aggregate.rs:36-125 and motion/confidence.rs:17-29.
 #include "engine.h"
+#include "tune.h"
 
 #include <algorithm>
 #include <cmath>
@@ -434,6 +435,61 @@ NlmPass::NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, av
         mv_.allocate(2 * r_ * blocks * 2 * sizeof(int32_t));
         conf_.allocate(2 * r_ * blocks * sizeof(float));
     }
+    tune_windows();
+}
+
+NlmWindow NlmPass::window() const
+{
+    NlmWindow w{(int)W_, (int)H_, C_, (int)P_, (int)S_, h2_ * 1.4426950408889634f, offset_, 1, 1, 1};
+    if (motion_ || conf_only_) {
+        w.step = mg_.step;
+        w.blocks_x = mg_.blocks_x;
+        w.blocks_y = mg_.blocks_y;
+    }
+    return w;
+}
+
+// The window launch sites' rows per wave, picked by the tuner on this pass's own buffers: ring
+// slots of flat mid-grey (the window walks every offset whatever the content, so the time is the
+// content's) and, with confidence, blocks of confidence 1.  The stream then starts from scratch.
+void NlmPass::tune_windows()
+{
+    const size_t px = (size_t)W_ * H_;
+    check_hip(launch_fill_u32(ring_.values.ptr, 0x3f000000u, px * C_ * ring_.N, stream_), "tuning frames");
+    if (confidence_)
+        check_hip(launch_fill_u32(conf_.ptr, 0x3f800000u, conf_.bytes / sizeof(float), stream_), "tuning confidence");
+    const NlmWindow w = window();
+    auto clear = [&]() -> hipError_t {
+        hipError_t e = launch_fill_u32(accum_.ptr, 0, px * C_, stream_);
+        if (e == hipSuccess) e = launch_fill_u32(wsum_.ptr, 0, px, stream_);
+        if (e == hipSuccess) e = launch_fill_u32(maxw_.ptr, 0, px, stream_);
+        return e;
+    };
+    auto read = [&]() {
+        std::vector<uint8_t> out(px * (C_ + 2) * sizeof(float));
+        check_hip(hipMemcpy(out.data(), accum_.ptr, px * C_ * sizeof(float), hipMemcpyDeviceToHost), "tuning read");
+        check_hip(hipMemcpy(out.data() + px * C_ * 4, wsum_.ptr, px * sizeof(float), hipMemcpyDeviceToHost),
+                  "tuning read");
+        check_hip(hipMemcpy(out.data() + px * (C_ + 1) * 4, maxw_.ptr, px * sizeof(float), hipMemcpyDeviceToHost),
+                  "tuning read");
+        return out;
+    };
+    const std::string key = "C" + std::to_string(C_) + " P" + std::to_string(P_) + " S" + std::to_string(S_) + " " +
+                            std::to_string(W_) + "x" + std::to_string(H_);
+    const float *f = ring_.value_slot(0);
+    variant_single_ = Tuner::instance().choose(
+        "nlm_single", key, kNlmVariants, stream_,
+        [&](int v) { return launch_nlm_single(w, f, lut_.as<float>(), accum_.as<float>(), wsum_.as<float>(),
+                                              maxw_.as<float>(), stream_, v); },
+        clear, read);
+    if (r_ > 0) {
+        const float *c = confidence_ ? conf_.as<float>() : nullptr;
+        variant_pair_ = Tuner::instance().choose(
+            "nlm_pair", key + (c ? " confidence" : ""), kNlmVariants, stream_,
+            [&](int v) { return launch_nlm_pair(w, f, f, f, c, c, accum_.as<float>(), wsum_.as<float>(),
+                                                maxw_.as<float>(), stream_, v); },
+            clear, read);
+    }
 }
 
 size_t NlmPass::max_held() const { return r_; }
@@ -512,12 +568,7 @@ void NlmPass::denoise()
     check_hip(launch_fill_u32(accum_.ptr, 0, px * C, stream_), "zero");
     check_hip(launch_fill_u32(wsum_.ptr, 0, px, stream_), "zero");
     check_hip(launch_fill_u32(maxw_.ptr, 0, px, stream_), "zero");
-    NlmWindow w{(int)W_, (int)H_, C, (int)P_, (int)S_, h2_ * 1.4426950408889634f, offset_, 1, 1, 1};
-    if (motion_ || conf_only_) {
-        w.step = mg_.step;
-        w.blocks_x = mg_.blocks_x;
-        w.blocks_y = mg_.blocks_y;
-    }
+    const NlmWindow w = window();
     const size_t blocks = (motion_ || conf_only_) ? (size_t)mg_.blocks_x * mg_.blocks_y : 0;
     auto conf_of = [&](int k) -> const float * {
         if (!confidence_) return nullptr;
@@ -528,11 +579,11 @@ void NlmPass::denoise()
     for (int qk = -(int)r_; qk < 0; ++qk)
         timed(AVD_STAGE_NLM_PAIR, px * window * 2 * nlm_candidate_ops(C, P_), stream_, "nlm pair", [&] {
             return launch_nlm_pair(w, cf, nb[qk + r_], nb[-qk + r_], conf_of(qk), conf_of(-qk), accum_.as<float>(),
-                                   wsum_.as<float>(), maxw_.as<float>(), stream_);
+                                   wsum_.as<float>(), maxw_.as<float>(), stream_, variant_pair_);
         });
     timed(AVD_STAGE_NLM_SINGLE, px * (window - 1) * nlm_candidate_ops(C, P_), stream_, "nlm single", [&] {
         return launch_nlm_single(w, cf, lut_.as<float>(), accum_.as<float>(), wsum_.as<float>(), maxw_.as<float>(),
-                                 stream_);
+                                 stream_, variant_single_);
     });
     float *out = outputs_[next_output_].as<float>();
     timed(AVD_STAGE_NLM_FINISH, px, stream_, "nlm finish", [&] {

@@ -159,6 +159,8 @@ class NlmPass : public Pass {
   private:
     void ingest(const avd_device_plane *planes);
     void denoise();
+    NlmWindow window() const;
+    void tune_windows();
     void egress(const float *frame, const avd_device_plane *planes);
 
     uint32_t W_, H_;
@@ -173,6 +175,7 @@ class NlmPass : public Pass {
     DeviceBuffer outputs_[2];
     bool ready_ = false;
     int ready_output_ = 0, next_output_ = 0;
+    int variant_single_ = 0, variant_pair_ = 0;  // the tuner's rows per wave of the window sites
     uint32_t tail_remaining_ = 0;
 };
 

@@ -14,7 +14,9 @@
 // X0 - 2P + l.  The horizontal sum is a chain of 2P DPP wave_shr:1 adds (each a single v_add_f32
 // with a DPP source), so it needs no LDS and no barrier.  Each lane walks ROWS output rows of its
 // column: the centre values of its ROWS + 2P distance rows stay in registers for the whole
-// window, and the vertical box sum adds 2P+1 of the row sums kept in registers.  The neighbour
+// window, and the vertical box sum adds 2P+1 of the row sums kept in registers.  ROWS (16, 8 or 4)
+// trades the 2P halo rows each wave recomputes against the number of waves a small frame spreads
+// over; every choice gives bit-identical sums, and the tuner picks one per window shape.  The neighbour
 // frame(s) of the workgroup are staged once in LDS with their (P+s) halo, so the offset loop runs
 // with no barrier at all.  exp is v_exp_f32 (exp2) with log2(e) folded into h2 on the host.
 #include "common.h"
@@ -23,7 +25,6 @@ namespace avd {
 
 namespace {
 
-constexpr int kRows = 16;
 constexpr int kMaxP = 8;
 constexpr int kMaxS = 8;
 
@@ -50,7 +51,7 @@ __device__ void stage_tile(float *tile, const float *__restrict__ frame, int W, 
     }
 }
 
-template <int C, int P, int WV, bool PAIR, bool CONF>
+template <int C, int P, int WV, int ROWS, bool PAIR, bool CONF>
 __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__restrict__ centre,
                                                      const float *__restrict__ fwd, const float *__restrict__ bwd,
                                                      const float *__restrict__ lut, const float *__restrict__ conf_f,
@@ -58,15 +59,15 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
                                                      float *__restrict__ wsum, float *__restrict__ maxw)
 {
     extern __shared__ float lds[];
-    constexpr int DR = kRows + 2 * P;  // distance rows per wave
+    constexpr int DR = ROWS + 2 * P;  // distance rows per wave
     constexpr int OW = 64 - 2 * P;     // output columns per wave
     const int S = w.S, W = w.W, H = w.H;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int X0 = blockIdx.x * OW;
-    const int Y0 = blockIdx.y * (WV * kRows);
+    const int Y0 = blockIdx.y * (WV * ROWS);
 
-    // Neighbour tiles: columns X0-P-S .. X0-P+63+S, rows Y0-P-S .. Y0+WV*kRows+P-1+S.
-    const int tcols = 64 + 2 * S, trows = WV * kRows + 2 * P + 2 * S;
+    // Neighbour tiles: columns X0-P-S .. X0-P+63+S, rows Y0-P-S .. Y0+WV*ROWS+P-1+S.
+    const int tcols = 64 + 2 * S, trows = WV * ROWS + 2 * P + 2 * S;
     const int tx0 = X0 - P - S, ty0 = Y0 - P - S;
     float *tf = lds;
     float *tb = lds + (size_t)tcols * trows * C;
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
 
     // Centre values of this lane's distance column, one per distance row, in registers.
     const int xd = X0 - P + lane;
-    const int ybase = Y0 + wave * kRows - P;
+    const int ybase = Y0 + wave * ROWS - P;
     float cen[DR][C];
 #pragma unroll
     for (int r = 0; r < DR; ++r) {
@@ -88,21 +89,21 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
     const float scale = (C == 1) ? 3.0f : 1.5f;
     const float h2l = w.h2_log2e;
     const int ox = X0 - 2 * P + lane;  // output column of this lane (valid for lane >= 2P)
-    float conf_fv[kRows], conf_bv[kRows];
+    float conf_fv[ROWS], conf_bv[ROWS];
     if (CONF) {
         const int bcol = min(clampi(ox, W - 1) / w.step, w.blocks_x - 1);
 #pragma unroll
-        for (int j = 0; j < kRows; ++j) {
-            const int oy = clampi(Y0 + wave * kRows + j, H - 1);
+        for (int j = 0; j < ROWS; ++j) {
+            const int oy = clampi(Y0 + wave * ROWS + j, H - 1);
             const int bidx = min(oy / w.step, w.blocks_y - 1) * w.blocks_x + bcol;
             conf_fv[j] = conf_f[bidx];
             conf_bv[j] = conf_b[bidx];
         }
     }
 
-    float acc[kRows][C], ws[kRows], mw[kRows];
+    float acc[ROWS][C], ws[ROWS], mw[ROWS];
 #pragma unroll
-    for (int j = 0; j < kRows; ++j) {
+    for (int j = 0; j < ROWS; ++j) {
 #pragma unroll
         for (int c = 0; c < C; ++c) acc[j][c] = 0.0f;
         ws[j] = 0.0f;
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
 
     const int side = 2 * S + 1;
     // Tile coordinates of this lane's distance position at offset 0.
-    const int trow0 = wave * kRows + S;  // + r + qy
+    const int trow0 = wave * ROWS + S;  // + r + qy
     const int tcol0 = lane + S;          // + qx
     // Gather column of the output pixel: lane - P + S (+/- qx).  Lanes below 2P own no output and
     // read a clamped column so every LDS read stays inside the tile.
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
                 }
             }
 #pragma unroll
-            for (int j = 0; j < kRows; ++j) {
+            for (int j = 0; j < ROWS; ++j) {
                 float Df = 0.0f;
 #pragma unroll
                 for (int k = 0; k <= 2 * P; ++k) Df += hf[j + k];
@@ -182,8 +183,8 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
 
     if (lane < 2 * P || ox >= W) return;
 #pragma unroll
-    for (int j = 0; j < kRows; ++j) {
-        const int oy = Y0 + wave * kRows + j;
+    for (int j = 0; j < ROWS; ++j) {
+        const int oy = Y0 + wave * ROWS + j;
         if (oy >= H) break;
         const size_t p = (size_t)oy * W + ox;
 #pragma unroll
@@ -216,10 +217,10 @@ __global__ __launch_bounds__(256) void k_finish(const float *__restrict__ frame,
 // take half the rows to keep the LDS footprint near that of a luma pass.
 constexpr int waves_for(int C, bool pair) { return (pair && C == 2) ? 2 : 4; }
 
-size_t tile_bytes(int C, int P, int S, bool pair)
+size_t tile_bytes(int C, int P, int S, bool pair, int rows)
 {
     const int wv = waves_for(C, pair);
-    const size_t one = (size_t)(64 + 2 * S) * (wv * kRows + 2 * P + 2 * S) * C * sizeof(float);
+    const size_t one = (size_t)(64 + 2 * S) * (wv * rows + 2 * P + 2 * S) * C * sizeof(float);
     return pair ? 2 * one : one;
 }
 
@@ -236,41 +237,56 @@ int device_lds_limit()
     return limit;
 }
 
+template <int C, int P, int ROWS>
+hipError_t dispatch_rows(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd, const float *lut,
+                         const float *cf, const float *cb, float *accum, float *wsum, float *maxw, bool pair, bool conf,
+                         hipStream_t s)
+{
+    constexpr int WS = waves_for(C, false), WP = waves_for(C, true);
+    const size_t lds = tile_bytes(C, P, w.S, pair, ROWS);
+    if (!pair) {
+        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WS * ROWS));
+        k_nlm<C, P, WS, ROWS, false, false><<<grid, WS * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum,
+                                                                      maxw);
+    } else {
+        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WP * ROWS));
+        if (conf)
+            k_nlm<C, P, WP, ROWS, true, true><<<grid, WP * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum,
+                                                                        maxw);
+        else
+            k_nlm<C, P, WP, ROWS, true, false><<<grid, WP * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum,
+                                                                         wsum, maxw);
+    }
+    return hipGetLastError();
+}
+
 template <int C, int P>
 hipError_t dispatch(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd, const float *lut,
                     const float *cf, const float *cb, float *accum, float *wsum, float *maxw, bool pair, bool conf,
-                    hipStream_t s)
+                    int variant, hipStream_t s)
 {
-    constexpr int WS = waves_for(C, false), WP = waves_for(C, true);
-    const size_t lds = tile_bytes(C, P, w.S, pair);
-    if (!pair) {
-        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WS * kRows));
-        k_nlm<C, P, WS, false, false><<<grid, WS * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw);
-    } else {
-        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WP * kRows));
-        if (conf)
-            k_nlm<C, P, WP, true, true><<<grid, WP * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw);
-        else
-            k_nlm<C, P, WP, true, false><<<grid, WP * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw);
+    switch (variant) {
+    case 1: return dispatch_rows<C, P, 8>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 2: return dispatch_rows<C, P, 4>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    default: return dispatch_rows<C, P, 16>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
     }
-    return hipGetLastError();
 }
 
 template <int C>
 hipError_t dispatch_p(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd, const float *lut,
                       const float *cf, const float *cb, float *accum, float *wsum, float *maxw, bool pair, bool conf,
-                      hipStream_t s)
+                      int variant, hipStream_t s)
 {
     switch (w.P) {
-    case 0: return dispatch<C, 0>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
-    case 1: return dispatch<C, 1>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
-    case 2: return dispatch<C, 2>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
-    case 3: return dispatch<C, 3>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
-    case 4: return dispatch<C, 4>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
-    case 5: return dispatch<C, 5>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
-    case 6: return dispatch<C, 6>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
-    case 7: return dispatch<C, 7>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
-    case 8: return dispatch<C, 8>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 0: return dispatch<C, 0>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, variant, s);
+    case 1: return dispatch<C, 1>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, variant, s);
+    case 2: return dispatch<C, 2>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, variant, s);
+    case 3: return dispatch<C, 3>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, variant, s);
+    case 4: return dispatch<C, 4>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, variant, s);
+    case 5: return dispatch<C, 5>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, variant, s);
+    case 6: return dispatch<C, 6>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, variant, s);
+    case 7: return dispatch<C, 7>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, variant, s);
+    case 8: return dispatch<C, 8>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, variant, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -280,26 +296,29 @@ hipError_t dispatch_p(const NlmWindow &w, const float *centre, const float *fwd,
 bool nlm_supported(int C, int P, int S)
 {
     return (C == 1 || C == 2) && P >= 0 && P <= kMaxP && S >= 0 && S <= kMaxS &&
-           tile_bytes(C, P, S, true) <= (size_t)device_lds_limit() &&
-           tile_bytes(C, P, S, false) <= (size_t)device_lds_limit();
+           tile_bytes(C, P, S, true, 16) <= (size_t)device_lds_limit() &&
+           tile_bytes(C, P, S, false, 16) <= (size_t)device_lds_limit();
 }
+int nlm_variant_rows(int variant) { return variant == 1 ? 8 : (variant == 2 ? 4 : 16); }
 
 hipError_t launch_nlm_single(const NlmWindow &w, const float *frame, const float *lut, float *accum, float *wsum,
-                             float *maxw, hipStream_t s)
+                             float *maxw, hipStream_t s, int variant)
 {
-    if (!nlm_supported(w.C, w.P, w.S)) return hipErrorInvalidValue;
-    if (w.C == 1) return dispatch_p<1>(w, frame, frame, frame, lut, nullptr, nullptr, accum, wsum, maxw, false, false, s);
-    return dispatch_p<2>(w, frame, frame, frame, lut, nullptr, nullptr, accum, wsum, maxw, false, false, s);
+    if (!nlm_supported(w.C, w.P, w.S) || variant < 0 || variant >= kNlmVariants) return hipErrorInvalidValue;
+    if (w.C == 1)
+        return dispatch_p<1>(w, frame, frame, frame, lut, nullptr, nullptr, accum, wsum, maxw, false, false, variant, s);
+    return dispatch_p<2>(w, frame, frame, frame, lut, nullptr, nullptr, accum, wsum, maxw, false, false, variant, s);
 }
 
 hipError_t launch_nlm_pair(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd,
                            const float *conf_f, const float *conf_b, float *accum, float *wsum, float *maxw,
-                           hipStream_t s)
+                           hipStream_t s, int variant)
 {
-    if (!nlm_supported(w.C, w.P, w.S)) return hipErrorInvalidValue;
+    if (!nlm_supported(w.C, w.P, w.S) || variant < 0 || variant >= kNlmVariants) return hipErrorInvalidValue;
     const bool conf = conf_f != nullptr && conf_b != nullptr;
-    if (w.C == 1) return dispatch_p<1>(w, centre, fwd, bwd, nullptr, conf_f, conf_b, accum, wsum, maxw, true, conf, s);
-    return dispatch_p<2>(w, centre, fwd, bwd, nullptr, conf_f, conf_b, accum, wsum, maxw, true, conf, s);
+    if (w.C == 1)
+        return dispatch_p<1>(w, centre, fwd, bwd, nullptr, conf_f, conf_b, accum, wsum, maxw, true, conf, variant, s);
+    return dispatch_p<2>(w, centre, fwd, bwd, nullptr, conf_f, conf_b, accum, wsum, maxw, true, conf, variant, s);
 }
 
 hipError_t launch_nlm_finish(const float *frame, const float *accum, const float *wsum, const float *maxw,

@@ -282,6 +282,7 @@ def nlm_leg(avd, torch, name, cfg, sample_steps=4):
                      "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s", "frac": round(tflops / VALU_PEAK_TOPS, 4)},
         "exp_per_frame": int(exps), "exp_per_s": round(exps / (window_ms * 1e-3), 1),
         "stages_ms_per_frame": {k: round(v["ms"] / sample_steps, 4) for k, v in st.items() if v["launches"]},
+        "tuned": [e for e in avd.tuning_entries() if e["key"].endswith(f"{W_}x{H_}") or f"{W_}x{H_} " in e["key"]],
     }
 
 

@@ -298,6 +298,26 @@ avd_status avd_host_denoiser_reset_stream(avd_host_denoiser *denoiser);
 avd_status avd_host_denoiser_pending(const avd_host_denoiser *denoiser, uint32_t *pending);
 avd_status avd_host_denoiser_window_span(const avd_host_denoiser *denoiser, avd_window_span *out);
 
+/* ---- launch-site tuner (KernelSelector, tune/selector.rs:109; VariantSite, tune/site.rs:56-157) -- */
+
+/* An engine's tunable launch sites (the NLM windows' rows per wave; the NL4D trajectory search's
+ * references per wave) offer candidate launches that compute bit-identical outputs.  With tuning on
+ * (the default) the first engine of a shape runs every candidate on its own buffers, drops any whose
+ * outputs differ from the baseline's (candidate 0) by a bit, times the rest with HIP events and
+ * keeps the fastest for that device, site and shape for the rest of the process.  Off, every site
+ * runs its baseline. */
+void avd_tuning_set_enabled(uint32_t enabled);
+typedef struct avd_tune_entry {
+    char site[32];
+    char key[96];
+    uint32_t device;
+    uint32_t choice;
+    uint32_t candidates;
+    double ms[8]; /* best time of each candidate; negative when its outputs differed */
+} avd_tune_entry;
+/* Copies up to `capacity` of the tuner's choices so far; *count = how many exist. */
+avd_status avd_tuning_entries(avd_tune_entry *out, uint32_t capacity, uint32_t *count);
+
 /* ---- launch sites: one stage of one pass over caller-owned device buffers ----------------- */
 
 /* Plane ingest (engine/kernels.rs:25-63): u32 words of `channels` planes into an interleaved f32
