@@ -114,6 +114,11 @@ class MotionPass(ctypes.Structure):
                 ("max_value", f32), ("sad_floor", f32), ("thsad", f32)]
 
 
+class FieldLayout(ctypes.Structure):
+    """FieldLayout (motion/chroma_map.rs:18-53): a motion field's block grid and i32 field stride."""
+    _fields_ = [("blocks_x", u32), ("blocks_y", u32), ("step", u32), ("blksize", u32), ("stride", u32)]
+
+
 class NlmWindow(ctypes.Structure):
     _fields_ = [("width", u32), ("height", u32), ("channels", u32), ("patch_radius", u32), ("search_radius", u32),
                 ("h2_inv_norm", f32), ("noise_offset", f32), ("step", u32), ("blocks_x", u32), ("blocks_y", u32)]
@@ -210,6 +215,9 @@ SIGNATURES = {
     "avd_site_collab_filter": (ctypes.c_int, [ctypes.POINTER(CollabPass), P, P, P, P, P, P, P]),
     "avd_site_collab_normalise": (ctypes.c_int, [P, P, P, u32, u32, P]),
     "avd_site_motion": (ctypes.c_int, [ctypes.POINTER(MotionPass), P, P, P, P, P]),
+    "avd_site_motion_refine": (ctypes.c_int, [ctypes.POINTER(MotionPass), P, P, P, u32, P, P, P]),
+    "avd_site_map_luma_fields": (ctypes.c_int, [ctypes.POINTER(FieldLayout), P, ctypes.POINTER(FieldLayout), P, u32,
+                                                u32, u32, P]),
     "avd_site_nlm_single": (ctypes.c_int, [ctypes.POINTER(NlmWindow), P, P, P, P, P, P]),
     "avd_site_nlm_pair": (ctypes.c_int, [ctypes.POINTER(NlmWindow), P, P, P, P, P, P, P, P, P]),
     "avd_site_nlm_finish": (ctypes.c_int, [P, P, P, P, f32, u32, u32, P, P]),

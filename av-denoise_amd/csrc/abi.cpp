@@ -487,6 +487,44 @@ avd_status avd_site_motion(const avd_motion_pass *p, const uint16_t *ring_codes,
     return hip_status(e, "motion");
 }
 
+avd_status avd_site_motion_refine(const avd_motion_pass *p, const uint16_t *ring_codes,
+                                  const uint32_t *neighbour_slots, const int32_t *seeds, uint32_t refine_radius,
+                                  int32_t *mv_field, float *confidence, void *stream)
+{
+    if (!p || !ring_codes || !neighbour_slots || !seeds || !mv_field) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (p->overlap >= p->blksize || p->width == 0 || p->height == 0 || p->channels < 1 || p->channels > 2 ||
+        refine_radius > 8)
+        return fail(AVD_ERR_INVALID_OPTIONS, "invalid motion geometry");
+    avd::MotionGeom g{(int)p->width, (int)p->height, (int)p->channels, (int)p->blksize, (int)(p->blksize - p->overlap),
+                      (int)refine_radius, 1, 0, 0};
+    g.blocks_x = std::max((int)avd::div_up(p->width, g.step), 1);
+    g.blocks_y = std::max((int)avd::div_up(p->height, g.step), 1);
+    return hip_status(avd::launch_motion_refine(g, ring_codes, (int)p->centre_slot, neighbour_slots, (int)p->neighbours,
+                                                seeds, mv_field, confidence, p->max_value, p->sad_floor, p->thsad,
+                                                (int)refine_radius, static_cast<hipStream_t>(stream)),
+                      "motion refine");
+}
+
+avd_status avd_site_map_luma_fields(const avd_field_layout *luma, const int32_t *luma_fields,
+                                    const avd_field_layout *chroma, int32_t *chroma_fields, uint32_t shift_x,
+                                    uint32_t shift_y, uint32_t fields, void *stream)
+{
+    if (!luma || !chroma || !luma_fields || !chroma_fields) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    auto bad = [](const avd_field_layout *l) {
+        return l->blocks_x == 0 || l->blocks_y == 0 || l->step == 0 || l->blksize == 0 || l->blocks_x > 65535 ||
+               l->blocks_y > 65535 || (uint64_t)l->stride < (uint64_t)l->blocks_x * l->blocks_y * 2 ||
+               l->stride > (1u << 30);
+    };
+    if (bad(luma) || bad(chroma) || shift_x > 1 || shift_y > 1)
+        return fail(AVD_ERR_INVALID_OPTIONS, "invalid field layout");
+    const avd::FieldMap m{(int)luma->blocks_x,   (int)luma->blocks_y,   (int)luma->step,    (int)luma->blksize,
+                          (int)luma->stride,     (int)chroma->blocks_x, (int)chroma->blocks_y, (int)chroma->step,
+                          (int)chroma->blksize,  (int)chroma->stride,   (int)shift_x,       (int)shift_y};
+    return hip_status(avd::launch_map_luma_fields(m, luma_fields, chroma_fields, (int)fields,
+                                                  static_cast<hipStream_t>(stream)),
+                      "map luma fields");
+}
+
 avd_status avd_site_nlm_single(const avd_nlm_window *w, const float *frame, const float *lut, float *accum,
                                float *wsum, float *maxw, void *stream)
 {

@@ -55,6 +55,20 @@ hipError_t launch_motion(const MotionGeom &g, const uint16_t *codes, const int32
                          float sad_floor, float thsad, hipStream_t s);
 hipError_t launch_warp(const float *src, float *dst, const int32_t *mv, int W, int H, int C, int step,
                        int blocks_x, int blocks_y, hipStream_t s);
+// The fine pass alone at `radius` around per-block seeds (`seeds`, laid out as `mv`), on level 0:
+// the chroma pass's refine of the luma pass's mapped vectors.  `seeds` may be `mv`.
+hipError_t launch_motion_refine(const MotionGeom &g, const uint16_t *codes, int centre_slot, const uint32_t *nslots_dev,
+                                int neighbours, const int32_t *seeds, int32_t *mv, float *conf, float max_value,
+                                float sad_floor, float thsad, int radius, hipStream_t s);
+// Two motion block grids, luma and chroma, with the i32 stride between fields of each and the
+// chroma subsampling as shifts per axis.
+struct FieldMap {
+    int luma_bx, luma_by, luma_step, luma_blk, luma_stride;
+    int chroma_bx, chroma_by, chroma_step, chroma_blk, chroma_stride;
+    int shift_x, shift_y;
+};
+// `fields` luma motion fields onto the chroma block grid.
+hipError_t launch_map_luma_fields(const FieldMap &m, const int32_t *luma, int32_t *chroma, int fields, hipStream_t s);
 
 // ---- collab.hip -------------------------------------------------------------------------
 // `scored_count`, when non-null, points at kScoreSlots counters kScoreStride apart; their sum gains

@@ -351,6 +351,81 @@ static void check_estimation(const avd_motion_search &m, uint32_t temporal_radiu
 }
 
 // ---------------------------------------------------------------------------------------------
+// Luma motion shared with the chroma pass (motion/luma_ring.rs, motion/chroma_map.rs)
+// ---------------------------------------------------------------------------------------------
+
+bool same_motion_search(const avd_motion_search &a, const avd_motion_search &b)
+{
+    return a.blksize == b.blksize && a.overlap == b.overlap && a.search_radius == b.search_radius &&
+           a.max_search_escalation == b.max_search_escalation && a.pyramid_levels == b.pyramid_levels &&
+           a.estimation == b.estimation && a.refine_radius == b.refine_radius;
+}
+
+void Pass::search_motion(const MotionGeom &g, const Ring &ring, uint32_t centre, const uint32_t *nslots,
+                         int neighbours, int32_t *mv, float *conf, float mx, float sad_floor, float th, bool shares)
+{
+    const uint32_t frame = ring.slot_frames[centre];
+    if (share_ && reads_motion_ && shares) {
+        const int32_t *seeds = share_->read(frame, stream_);
+        MotionGeom rg = g;
+        rg.radius = kMappedRefineRadius;
+        rg.levels = 1;
+        timed(AVD_STAGE_MOTION, motion_ops(rg, neighbours), stream_, "motion refine", [&] {
+            return launch_motion_refine(g, ring.code_ptr(), (int)centre, nslots, neighbours, seeds, mv, conf, mx,
+                                        sad_floor, th, kMappedRefineRadius, stream_);
+        });
+        return;
+    }
+    timed(AVD_STAGE_MOTION, motion_ops(g, neighbours), stream_, "motion", [&] {
+        return launch_motion(g, ring.code_ptr(), ring.l1_ptr(), (int)centre, nslots, neighbours, mv, conf, mx,
+                             sad_floor, th, stream_);
+    });
+    if (share_ && shares)
+        timed(AVD_STAGE_MOTION, 0, stream_, "share motion", [&] { return share_->write(frame, mv, stream_); });
+}
+
+LumaMotionShare::LumaMotionShare(const MotionGeom &luma, const MotionGeom &chroma, int shift_x, int shift_y,
+                                 uint32_t radius)
+{
+    map_ = FieldMap{luma.blocks_x,  luma.blocks_y,  luma.step,   luma.blksize, luma.blocks_x * luma.blocks_y * 2,
+                    chroma.blocks_x, chroma.blocks_y, chroma.step, chroma.blksize,
+                    chroma.blocks_x * chroma.blocks_y * 2, shift_x, shift_y};
+    slots_ = 2 * radius + 1;
+    fields_ = 2 * radius;
+    buf_.allocate((size_t)slots_ * fields_ * map_.chroma_stride * sizeof(int32_t));
+    ready_.assign(slots_, nullptr);
+    for (auto &e : ready_) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create motion share event");
+    tags_.assign(slots_, -1);
+}
+
+LumaMotionShare::~LumaMotionShare()
+{
+    for (hipEvent_t e : ready_)
+        if (e) (void)hipEventDestroy(e);
+}
+
+hipError_t LumaMotionShare::write(uint32_t frame, const int32_t *luma_mv, hipStream_t s)
+{
+    const uint32_t slot = frame % slots_;
+    int32_t *dst = buf_.as<int32_t>() + (size_t)slot * fields_ * map_.chroma_stride;
+    hipError_t e = launch_map_luma_fields(map_, luma_mv, dst, (int)fields_, s);
+    if (e == hipSuccess) e = hipEventRecord(ready_[slot], s);
+    if (e == hipSuccess) tags_[slot] = frame;
+    return e;
+}
+
+const int32_t *LumaMotionShare::read(uint32_t frame, hipStream_t s)
+{
+    const uint32_t slot = frame % slots_;
+    if (tags_[slot] != (int64_t)frame)
+        throw Failure(AVD_ERR_GPU, "the luma pass has not shared the motion of stream frame " + std::to_string(frame));
+    check_hip(hipStreamWaitEvent(s, ready_[slot], 0), "wait for luma motion");
+    return buf_.as<int32_t>() + (size_t)slot * fields_ * map_.chroma_stride;
+}
+
+void LumaMotionShare::clear() { std::fill(tags_.begin(), tags_.end(), -1); }
+
+// ---------------------------------------------------------------------------------------------
 // NLMeans pass
 // ---------------------------------------------------------------------------------------------
 
@@ -406,6 +481,7 @@ NlmPass::NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, av
             throw Failure(AVD_ERR_INVALID_OPTIONS, "search escalation is not built yet");
         if (in.kind == AVD_FORMAT_U16 && in.depth > 12)
             throw Failure(AVD_ERR_INVALID_GEOMETRY, "motion search on more than 12-bit samples is not supported");
+        motion_opts_ = m;
         mg_ = MotionGeom{(int)W, (int)H, C, (int)m.blksize, (int)(m.blksize - m.overlap), (int)m.search_radius,
                          (int)m.pyramid_levels, 0, 0};
         mg_.blocks_x = std::max((int)div_up(W, mg_.step), 1);
@@ -436,6 +512,21 @@ NlmPass::NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, av
         conf_.allocate(2 * r_ * blocks * sizeof(float));
     }
     tune_windows();
+}
+
+void NlmPass::set_motion_share(LumaMotionShare *share, bool reads)
+{
+    Pass::set_motion_share(share, reads);
+    if (reads) ring_.pyr1.release();
+}
+
+// The window's motion: the pass's own search (run_match, nlmeans/dispatch/motion.rs:180-224), shared
+// when the pass writes luma motion (share_window_motion, :226-252), or the refine of the luma
+// pass's mapped vectors (refine_luma_motion, :254-313).
+void NlmPass::window_motion(uint32_t centre, const uint32_t *nslots, int neighbours, float mx)
+{
+    search_motion(mg_, ring_, centre, nslots, neighbours, mv_.as<int32_t>(), conf_.as<float>(), mx,
+                        sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), motion_);
 }
 
 NlmWindow NlmPass::window() const
@@ -550,11 +641,7 @@ void NlmPass::denoise()
             if (k != 0) slots.push_back(ring_.phys((int)r_ + k));
         const float mx = max_value(in_);
         const size_t blocks = (size_t)mg_.blocks_x * mg_.blocks_y;
-        timed(AVD_STAGE_MOTION, motion_ops(mg_, (int)slots.size()), stream_, "motion", [&] {
-            return launch_motion(mg_, ring_.code_ptr(), ring_.l1_ptr(), (int)centre, ring_.neighbours_of(r_),
-                                 (int)slots.size(), mv_.as<int32_t>(), conf_.as<float>(), mx,
-                                 sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), stream_);
-        });
+        window_motion(centre, ring_.neighbours_of(r_), (int)slots.size(), mx);
         for (size_t i = 0; i < slots.size() && motion_; ++i) {
             const int k = (int)i < (int)r_ ? (int)i - (int)r_ : (int)i - (int)r_ + 1;
             float *dst = comp_.as<float>() + (size_t)slots[i] * px * C;
@@ -686,6 +773,7 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
 
     sigma_ = o.sigma;
     thsad_scale_ = o.thsad_scale;
+    motion_opts_ = m;
     mg_ = MotionGeom{(int)W, (int)H, C, (int)m.blksize, (int)step, (int)m.search_radius, (int)m.pyramid_levels, 0, 0};
     mg_.blocks_x = std::max((int)div_up(W, step), 1);
     mg_.blocks_y = std::max((int)div_up(H, step), 1);
@@ -766,6 +854,12 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
 
 size_t Nl4dPass::max_held() const { return 2 * r_; }
 
+void Nl4dPass::set_motion_share(LumaMotionShare *share, bool reads)
+{
+    Pass::set_motion_share(share, reads);
+    if (reads) ring_.pyr1.release();
+}
+
 void Nl4dPass::reset_stream()
 {
     ring_.reset();
@@ -795,11 +889,8 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
     }
     std::fill(zeroed_.begin(), zeroed_.end(), false);
     const uint32_t *nslots = ring_.neighbours_of(centre_logical);
-    timed(AVD_STAGE_MOTION, motion_ops(mg_, (int)(N_ - 1)), stream_, "motion", [&] {
-        return launch_motion(mg_, ring_.code_ptr(), ring_.l1_ptr(), (int)centre, nslots, (int)(N_ - 1),
-                             mv_.as<int32_t>(), conf_.as<float>(), max_value(in_),
-                             sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), stream_);
-    });
+    search_motion(mg_, ring_, centre, nslots, (int)(N_ - 1), mv_.as<int32_t>(), conf_.as<float>(), max_value(in_),
+                  sad_noise_floor(mg_.blksize, sigma_), thsad(mg_.blksize, thsad_scale_), true);
     avd_collab_pass p = pass_;
     p.centre_slot = centre;
     unsigned long long *counter = timer_ ? timer_->scored_counter() : nullptr;
@@ -935,6 +1026,32 @@ bool Nl4dPass::has_pending() const { return !pending_.empty(); }
 // Planar engine
 // ---------------------------------------------------------------------------------------------
 
+// EngineSpec::validate (engine/spec.rs:320-341) for a spec that cleans both plane groups: the
+// passes read the same window span and, since the chroma pass takes its motion from the luma pass,
+// either neither searches motion or both run the same search.
+static void check_same_motion(const avd_engine_spec &spec)
+{
+    struct Motion {
+        bool active;
+        uint32_t span;
+        avd_motion_search m;
+    };
+    auto nlm = [](const avd_nlmeans_options &o) {
+        const uint32_t r = o.temporal ? o.temporal_radius : 0;
+        return Motion{o.motion_compensation != 0 && r > 0, r, o.motion};
+    };
+    auto nl4d = [](const avd_nl4d_options &o) { return Motion{true, 2 * o.temporal_radius, o.motion}; };
+    const bool is_nl4d = spec.kind == AVD_ENGINE_NL4D;
+    const Motion l = is_nl4d ? nl4d(spec.nl4d_luma) : nlm(spec.nlm_luma);
+    const Motion c = is_nl4d ? nl4d(spec.nl4d_chroma) : nlm(spec.nlm_chroma);
+    if (l.span != c.span)
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "the luma pass holds " + std::to_string(l.span) +
+                                                   " frames back but the chroma pass holds " + std::to_string(c.span));
+    if (l.active != c.active || (l.active && !same_motion_search(l.m, c.m)))
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "the chroma pass takes its motion from the luma pass, so both need "
+                                               "the same motion search and temporal radius");
+}
+
 Engine::Engine(const avd_engine_spec &spec) : spec_(spec)
 {
     stream_ = static_cast<hipStream_t>(spec.stream);
@@ -951,6 +1068,7 @@ Engine::Engine(const avd_engine_spec &spec) : spec_(spec)
     const bool passes_through = !gray && (spec.clean_luma != 0) != (spec.clean_chroma != 0);
     if (passes_through && (g.input.kind != g.output.kind || g.input.depth != g.output.depth))
         throw Failure(AVD_ERR_INVALID_GEOMETRY, "an uncleaned plane needs the same input and output format");
+    if (spec.clean_luma && spec.clean_chroma && !gray) check_same_motion(spec);
     planes_ = gray ? 1 : 3;
     chroma_w_ = g.width;
     chroma_h_ = g.height;
@@ -980,6 +1098,15 @@ Engine::Engine(const avd_engine_spec &spec) : spec_(spec)
     if (chroma_) chroma_->set_timer(&timer_);
     if (luma_ && chroma_ && luma_->max_held() != chroma_->max_held())
         throw Failure(AVD_ERR_INVALID_OPTIONS, "the luma and chroma passes read different window spans");
+    // luma_motion_ring (engine/spec.rs:350-365): when both passes search motion, the chroma pass
+    // refines the luma pass's instead of searching its own.
+    if (luma_ && chroma_ && luma_->searches_motion() && chroma_->searches_motion()) {
+        const int shift_x = g.chroma == AVD_CHROMA_444 ? 0 : 1, shift_y = g.chroma == AVD_CHROMA_420 ? 1 : 0;
+        share_.reset(new LumaMotionShare(luma_->motion_geom(), chroma_->motion_geom(), shift_x, shift_y,
+                                         luma_->temporal_radius()));
+        luma_->set_motion_share(share_.get(), false);
+        chroma_->set_motion_share(share_.get(), true);
+    }
     if (passes_through) {
         pt_slots_ = lead().max_held() + 1;
         const bool luma_uncleaned = !spec.clean_luma;
@@ -1145,6 +1272,7 @@ uint32_t Engine::finish()
 
 void Engine::restart()
 {
+    if (share_) share_->clear();
     next_frame_ = 0;
     pushed_ = false;
     tail_left_ = 0;

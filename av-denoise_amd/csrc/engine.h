@@ -35,6 +35,8 @@ struct DeviceBuffer {
     }
 };
 
+class LumaMotionShare;
+
 uint32_t samples_per_word(avd_sample_format f);
 float max_value(avd_sample_format f);
 uint64_t plane_bytes(avd_sample_format f, uint64_t pixels);
@@ -125,12 +127,29 @@ class Pass {
     virtual bool has_pending() const = 0;
     uint32_t emitted_frame() const { return emitted_frame_; }
     void set_timer(StageTimer *t) { timer_ = t; }
+    // The motion search the pass runs (none for a pass that searches no motion), and its block grid.
+    virtual bool searches_motion() const = 0;
+    virtual uint32_t temporal_radius() const = 0;
+    virtual const avd_motion_search &motion_search() const = 0;
+    virtual const MotionGeom &motion_geom() const = 0;
+    // The luma pass writes its motion into `share`; the chroma pass (`reads`) refines it instead of
+    // searching its own, so it drops its coarse pyramid level.
+    virtual void set_motion_share(LumaMotionShare *share, bool reads)
+    {
+        share_ = share;
+        reads_motion_ = reads;
+    }
     // The stream the pass enqueues on (the engine moves the chroma pass between its own stream and
     // the caller's; the engine's fork and join order the switch).
     void set_stream(hipStream_t s) { stream_ = s; }
 
   protected:
     explicit Pass(hipStream_t s) : stream_(s) {}
+    // The window's motion of `neighbours` slots against `centre` into mv/conf: the pass's own
+    // search, written into the share when the pass writes one (and `shares`), or, on the pass that
+    // reads it, the refine of the luma pass's mapped vectors.
+    void search_motion(const MotionGeom &g, const Ring &ring, uint32_t centre, const uint32_t *nslots, int neighbours,
+                       int32_t *mv, float *conf, float mx, float sad_floor, float th, bool shares);
     hipStream_t stream_;
     // Runs `launch` (returning hipError_t) as one timed launch of `stage`.
     template <typename F>
@@ -142,7 +161,44 @@ class Pass {
     }
     uint32_t emitted_frame_ = 0;
     StageTimer *timer_ = nullptr;
+    LumaMotionShare *share_ = nullptr;
+    bool reads_motion_ = false;
 };
+
+// The luma pass's motion, mapped onto the chroma block grid for the chroma pass of the same centre
+// frame (the reference's LumaMotionRing, motion/luma_ring.rs:10-150).  One slot per window frame,
+// chosen and tagged by the centre's stream frame: luma never runs more passes ahead of chroma than
+// that, since the engine runs a call's luma passes, then its chroma passes, and each call's end
+// orders the chroma stream before the next call's luma work.  The luma stream maps into a slot and
+// records its event; the chroma stream waits for that event before it reads the slot.
+class LumaMotionShare {
+  public:
+    LumaMotionShare(const MotionGeom &luma, const MotionGeom &chroma, int shift_x, int shift_y, uint32_t radius);
+    ~LumaMotionShare();
+    LumaMotionShare(const LumaMotionShare &) = delete;
+    LumaMotionShare &operator=(const LumaMotionShare &) = delete;
+    // Maps the 2r window fields `luma_mv` of centre stream frame `frame` into its slot, on `s`.
+    hipError_t write(uint32_t frame, const int32_t *luma_mv, hipStream_t s);
+    // The mapped fields of centre stream frame `frame`, once `s` has waited for them; fails when
+    // the slot holds another frame.
+    const int32_t *read(uint32_t frame, hipStream_t s);
+    void clear();
+
+  private:
+    FieldMap map_{};
+    uint32_t slots_ = 0, fields_ = 0;
+    DeviceBuffer buf_;
+    std::vector<hipEvent_t> ready_;
+    std::vector<int64_t> tags_;
+};
+
+// How far the chroma pass searches around each mapped luma vector (MAPPED_REFINE_RADIUS,
+// nlmeans/dispatch/motion.rs:42-43).
+constexpr int kMappedRefineRadius = 1;
+
+// Whether two passes run the same motion search (the reference compares MotionCompensationMode and
+// the temporal radius, engine/spec.rs:320-341).
+bool same_motion_search(const avd_motion_search &a, const avd_motion_search &b);
 
 class NlmPass : public Pass {
   public:
@@ -155,10 +211,16 @@ class NlmPass : public Pass {
     void reset_stream() override;
     size_t max_held() const override;
     bool has_pending() const override;
+    bool searches_motion() const override { return motion_; }
+    uint32_t temporal_radius() const override { return r_; }
+    const avd_motion_search &motion_search() const override { return motion_opts_; }
+    const MotionGeom &motion_geom() const override { return mg_; }
+    void set_motion_share(LumaMotionShare *share, bool reads) override;
 
   private:
     void ingest(const avd_device_plane *planes);
     void denoise();
+    void window_motion(uint32_t centre, const uint32_t *nslots, int neighbours, float mx);
     NlmWindow window() const;
     void tune_windows();
     void egress(const float *frame, const avd_device_plane *planes);
@@ -169,6 +231,7 @@ class NlmPass : public Pass {
     uint32_t r_ = 0, S_ = 2, P_ = 4;
     bool hq_ = false, has_sigma_ = false, motion_ = false, confidence_ = false, conf_only_ = false, pushed_ = false;
     float sigma_ = 0.0f, self_weight_ = 1.0f, h2_ = 0.0f, offset_ = 0.0f, thsad_scale_ = 1.0f;
+    avd_motion_search motion_opts_{};
     MotionGeom mg_{};
     Ring ring_;
     DeviceBuffer accum_, wsum_, maxw_, lut_, comp_, mv_, conf_;
@@ -190,6 +253,11 @@ class Nl4dPass : public Pass {
     void reset_stream() override;
     size_t max_held() const override;
     bool has_pending() const override;
+    bool searches_motion() const override { return true; }
+    uint32_t temporal_radius() const override { return r_; }
+    const avd_motion_search &motion_search() const override { return motion_opts_; }
+    const MotionGeom &motion_geom() const override { return mg_; }
+    void set_motion_share(LumaMotionShare *share, bool reads) override;
     // Test hooks: shrink the tile lists so the overflow path runs, and count its pairs.
     void set_tile_cap_for_test(uint32_t cap);
     uint64_t overflow_pairs_seen() const { return overflow_seen_; }
@@ -205,6 +273,7 @@ class Nl4dPass : public Pass {
     avd_sample_format in_, out_;
     uint32_t r_ = 1, N_ = 3;
     float sigma_ = 0.0f, thsad_scale_ = 1.0f;
+    avd_motion_search motion_opts_{};
     MotionGeom mg_{};
     avd_collab_pass pass_{};
     size_t refs_ = 0;
@@ -266,6 +335,7 @@ class Engine {
     hipStream_t stream_ = nullptr, side_ = nullptr;
     hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
     StageTimer timer_;
+    std::unique_ptr<LumaMotionShare> share_;  // outlives the passes that point at it
     std::unique_ptr<Pass> luma_, chroma_;
     uint32_t planes_ = 1, chroma_w_ = 0, chroma_h_ = 0;
     DeviceBuffer passthrough_;

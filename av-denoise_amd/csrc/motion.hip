@@ -364,9 +364,9 @@ __global__ __launch_bounds__(kBmThreads) void k_bm_coarse(const int32_t *__restr
 // Fine pass (block_match.rs:306-369) on level 0 around the seed, then the confidence
 // (:489-535): the level-0 SAD over max, one f32 division.
 __global__ __launch_bounds__(kBmThreads) void k_bm_fine(const uint16_t *__restrict__ codes, int W, int H, int C,
-                                                        int blk, int step, int R, int use_seed, int centre_slot,
-                                                        const uint32_t *__restrict__ nslots, int neighbours,
-                                                        int32_t *__restrict__ mv, float *__restrict__ conf,
+                                                        int blk, int step, int R, const int32_t *seeds,
+                                                        int centre_slot, const uint32_t *__restrict__ nslots,
+                                                        int neighbours, int32_t *mv, float *__restrict__ conf,
                                                         float max_value, float sad_floor, float thsad)
 {
     extern __shared__ uint32_t lds[];
@@ -382,8 +382,8 @@ __global__ __launch_bounds__(kBmThreads) void k_bm_fine(const uint16_t *__restri
         cen, nb, sp, nslots, neighbours, bc * step, br * step, blk, R,
         [&](int n, int &sdx, int &sdy) {
             const size_t b = (size_t)n * per_n + blkid;
-            sdx = use_seed ? mv[b * 2] : 0;
-            sdy = use_seed ? mv[b * 2 + 1] : 0;
+            sdx = seeds ? seeds[b * 2] : 0;
+            sdy = seeds ? seeds[b * 2 + 1] : 0;
         },
         [&](int n, int dx, int dy, uint32_t sad) {
             const size_t b = (size_t)n * per_n + blkid;
@@ -434,9 +434,9 @@ __global__ __launch_bounds__(64) void k_bm_coarse_rows(const int32_t *__restrict
 
 template <int BLK, int R>
 __global__ __launch_bounds__(64) void k_bm_fine_rows(const uint16_t *__restrict__ codes, int W, int H, int C, int step,
-                                                     int use_seed, int centre_slot,
+                                                     const int32_t *seeds, int centre_slot,
                                                      const uint32_t *__restrict__ nslots, int neighbours,
-                                                     int32_t *__restrict__ mv, float *__restrict__ conf,
+                                                     int32_t *mv, float *__restrict__ conf,
                                                      float max_value, float sad_floor, float thsad)
 {
     extern __shared__ __align__(16) uint32_t lds_rows[];
@@ -452,8 +452,8 @@ __global__ __launch_bounds__(64) void k_bm_fine_rows(const uint16_t *__restrict_
         cen, nb, sp, nslots, neighbours, bc * step, br * step,
         [&](int n, int &sdx, int &sdy) {
             const size_t b = (size_t)n * per_n + blkid;
-            sdx = use_seed ? mv[b * 2] : 0;
-            sdy = use_seed ? mv[b * 2 + 1] : 0;
+            sdx = seeds ? seeds[b * 2] : 0;
+            sdy = seeds ? seeds[b * 2 + 1] : 0;
         },
         [&](int n, int dx, int dy, uint32_t sad) {
             const size_t b = (size_t)n * per_n + blkid;
@@ -470,18 +470,22 @@ __global__ __launch_bounds__(64) void k_bm_fine_rows(const uint16_t *__restrict_
         lds_rows);
 }
 
-// Specialised forms exist for these block sizes and radii.
-inline bool rows_form(int blk, int R) { return (blk == 8 || blk == 16 || blk == 32) && (R == 2 || R == 4 || R == 8); }
+// Specialised forms exist for these block sizes and radii (radius 1 is the chroma refine's).
+inline bool rows_form(int blk, int R)
+{
+    return (blk == 8 || blk == 16 || blk == 32) && (R == 1 || R == 2 || R == 4 || R == 8);
+}
 
+#define AVD_ROWS_RADII(B, RV, CALL)                                                                         \
+    do {                                                                                                    \
+        if ((RV) == 1) CALL(B, 1); else if ((RV) == 2) CALL(B, 2); else if ((RV) == 4) CALL(B, 4);          \
+        else CALL(B, 8);                                                                                    \
+    } while (0)
 #define AVD_ROWS_DISPATCH(BLKV, RV, CALL)                                                                   \
     do {                                                                                                    \
-        if ((BLKV) == 8) {                                                                                  \
-            if ((RV) == 2) CALL(8, 2); else if ((RV) == 4) CALL(8, 4); else CALL(8, 8);                     \
-        } else if ((BLKV) == 16) {                                                                          \
-            if ((RV) == 2) CALL(16, 2); else if ((RV) == 4) CALL(16, 4); else CALL(16, 8);                  \
-        } else {                                                                                            \
-            if ((RV) == 2) CALL(32, 2); else if ((RV) == 4) CALL(32, 4); else CALL(32, 8);                  \
-        }                                                                                                   \
+        if ((BLKV) == 8) AVD_ROWS_RADII(8, RV, CALL);                                                       \
+        else if ((BLKV) == 16) AVD_ROWS_RADII(16, RV, CALL);                                                \
+        else AVD_ROWS_RADII(32, RV, CALL);                                                                  \
     } while (0)
 
 __global__ __launch_bounds__(256) void k_warp(const float *__restrict__ src, float *__restrict__ dst,
@@ -502,6 +506,82 @@ inline unsigned grid_for(uint64_t items)
 {
     uint64_t g = (items + 255) / 256;
     return (unsigned)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
+}
+
+// The fine pass on level 0 at radius R around `seeds` (zero when null), into mv and conf.
+hipError_t launch_fine(const MotionGeom &g, int R, const uint16_t *codes, int centre_slot, const uint32_t *nslots_dev,
+                       int neighbours, const int32_t *seeds, int32_t *mv, float *conf, float max_value,
+                       float sad_floor, float thsad, hipStream_t s)
+{
+    const dim3 grid(g.blocks_x, g.blocks_y);
+    if (rows_form(g.blksize, R)) {
+#define AVD_FINE(B, RR)                                                                                      \
+    k_bm_fine_rows<B, RR><<<grid, 64, match_rows_lds_bytes(B, RR, neighbours), s>>>(                         \
+        codes, g.W, g.H, g.C, g.step, seeds, centre_slot, nslots_dev, neighbours, mv, conf, max_value,       \
+        sad_floor, thsad)
+        AVD_ROWS_DISPATCH(g.blksize, R, AVD_FINE);
+#undef AVD_FINE
+    } else {
+        k_bm_fine<<<grid, kBmThreads, match_lds_bytes(g.blksize, R, neighbours), s>>>(
+            codes, g.W, g.H, g.C, g.blksize, g.step, R, seeds, centre_slot, nslots_dev, neighbours, mv, conf,
+            max_value, sad_floor, thsad);
+    }
+    return hipGetLastError();
+}
+
+// The lower median of `n` values `stride` words apart: the value with at most (n - 1) / 2 of the
+// others below it and more than that at or below it (chroma_map.rs:22-44 ranks the same way).
+__device__ __forceinline__ int lower_median(const int32_t *v, int cols, int rows, int row_pitch)
+{
+    const int n = cols * rows, mid = (n - 1) / 2;
+    for (int i = 0; i < n; ++i) {
+        const int vi = v[((i / cols) * row_pitch + i % cols) * 2];
+        int below = 0, equal = 0;
+        for (int j = 0; j < n; ++j) {
+            const int vj = v[((j / cols) * row_pitch + j % cols) * 2];
+            below += vj < vi;
+            equal += vj == vi;
+        }
+        if (below <= mid && mid < below + equal) return vi;
+    }
+    return v[0];
+}
+
+// Divides by 1 << shift, rounding half away from zero (chroma_map.rs:6-16).
+__device__ __forceinline__ int scale_down(int v, int shift)
+{
+    if (shift == 0) return v;
+    const int m = v < 0 ? -v : v, r = (m + (1 << (shift - 1))) >> shift;
+    return v < 0 ? -r : r;
+}
+
+// One thread per (field, chroma block): the chroma block covers (chroma_blk << shift) luma samples
+// a side from (col * chroma_step) << shift, so covers = (area - luma_blk) / luma_step + 1 luma blocks
+// a side from start = origin / luma_step; those past the luma grid drop out (the first one never
+// does).  The per-component lower median of the rest, scaled down by the subsampling, is the
+// chroma block's vector.  The candidates are re-read from the (cached) luma field, so any cover
+// size runs without per-thread arrays.
+__global__ __launch_bounds__(256) void k_map_luma_fields(FieldMap m, const int32_t *__restrict__ luma,
+                                                         int32_t *__restrict__ chroma, int fields)
+{
+    const int area_x = m.chroma_blk << m.shift_x, area_y = m.chroma_blk << m.shift_y;
+    const int covers_x = max(area_x - m.luma_blk, 0) / m.luma_step + 1;
+    const int covers_y = max(area_y - m.luma_blk, 0) / m.luma_step + 1;
+    const int blocks = m.chroma_bx * m.chroma_by;
+    const int total = blocks * fields;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
+        const int f = i / blocks, b = i - f * blocks;
+        const int row = b / m.chroma_bx, col = b - row * m.chroma_bx;
+        const int sc = ((col * m.chroma_step) << m.shift_x) / m.luma_step;
+        const int sr = ((row * m.chroma_step) << m.shift_y) / m.luma_step;
+        const int last_c = min(sc + covers_x - 1, m.luma_bx - 1), last_r = min(sr + covers_y - 1, m.luma_by - 1);
+        const int first_c = min(sc, last_c), first_r = min(sr, last_r);
+        const int32_t *src = luma + (size_t)f * m.luma_stride + ((size_t)first_r * m.luma_bx + first_c) * 2;
+        const int cols = last_c - first_c + 1, rows = last_r - first_r + 1;
+        int32_t *dst = chroma + (size_t)f * m.chroma_stride + (size_t)b * 2;
+        dst[0] = scale_down(lower_median(src, cols, rows, m.luma_bx), m.shift_x);
+        dst[1] = scale_down(lower_median(src + 1, cols, rows, m.luma_bx), m.shift_y);
+    }
 }
 
 }  // namespace
@@ -540,19 +620,34 @@ hipError_t launch_motion(const MotionGeom &g, const uint16_t *codes, const int32
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    const dim3 grid(g.blocks_x, g.blocks_y);
-    if (rows_form(g.blksize, g.radius)) {
-#define AVD_FINE(B, RR)                                                                                      \
-    k_bm_fine_rows<B, RR><<<grid, 64, match_rows_lds_bytes(B, RR, neighbours), s>>>(                         \
-        codes, g.W, g.H, g.C, g.step, use_seed, centre_slot, nslots_dev, neighbours, mv, conf, max_value,    \
-        sad_floor, thsad)
-        AVD_ROWS_DISPATCH(g.blksize, g.radius, AVD_FINE);
-#undef AVD_FINE
-    } else {
-        k_bm_fine<<<grid, kBmThreads, match_lds_bytes(g.blksize, g.radius, neighbours), s>>>(
-            codes, g.W, g.H, g.C, g.blksize, g.step, g.radius, use_seed, centre_slot, nslots_dev, neighbours, mv,
-            conf, max_value, sad_floor, thsad);
-    }
+    return launch_fine(g, g.radius, codes, centre_slot, nslots_dev, neighbours, use_seed ? mv : nullptr, mv, conf,
+                       max_value, sad_floor, thsad, s);
+}
+
+// run_seeded_refine (motion/analyse.rs:321-367) at the single-level geometry
+// (MotionCtx::single_level, motion/context.rs:108-116): no coarse pass, the fine pass `radius`
+// around each seed.
+hipError_t launch_motion_refine(const MotionGeom &g, const uint16_t *codes, int centre_slot, const uint32_t *nslots_dev,
+                                int neighbours, const int32_t *seeds, int32_t *mv, float *conf, float max_value,
+                                float sad_floor, float thsad, int radius, hipStream_t s)
+{
+    if (neighbours <= 0) return hipSuccess;
+    if (radius < 0 || radius > kMaxR || g.blksize > kMaxBlk || g.blksize < 2 || neighbours > 16 || !seeds)
+        return hipErrorInvalidValue;
+    return launch_fine(g, radius, codes, centre_slot, nslots_dev, neighbours, seeds, mv, conf, max_value, sad_floor,
+                       thsad, s);
+}
+
+// map_luma_fields (motion/chroma_map.rs:56-98) over mc_map_luma_fields (kernels/chroma_map.rs:57-123).
+hipError_t launch_map_luma_fields(const FieldMap &m, const int32_t *luma, int32_t *chroma, int fields, hipStream_t s)
+{
+    if (fields <= 0) return hipSuccess;
+    if (m.luma_bx < 1 || m.luma_by < 1 || m.chroma_bx < 1 || m.chroma_by < 1 || m.luma_step < 1 || m.chroma_step < 1 ||
+        m.shift_x < 0 || m.shift_x > 1 || m.shift_y < 0 || m.shift_y > 1 ||
+        m.luma_stride < m.luma_bx * m.luma_by * 2 || m.chroma_stride < m.chroma_bx * m.chroma_by * 2)
+        return hipErrorInvalidValue;
+    const uint64_t items = (uint64_t)m.chroma_bx * m.chroma_by * fields;
+    k_map_luma_fields<<<grid_for(items), 256, 0, s>>>(m, luma, chroma, fields);
     return hipGetLastError();
 }
 

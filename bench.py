@@ -191,6 +191,7 @@ def cpu_baseline(clip_codes, threads, runs=5):
 
     def one_frame():
         t_total = 0.0
+        passes = []
         for plane_group in ("luma", "chroma"):
             if plane_group == "luma":
                 w, h, C, sig = W // CPU_CROP, H // CPU_CROP, 1, SIGMA_Y / 1023.0
@@ -204,6 +205,10 @@ def cpu_baseline(clip_codes, threads, runs=5):
             p = O.Nl4dPass(spec, w, h, C, DEPTH, DEPTH)
             for fr in planes:  # fill the ring without running passes
                 p.ring.write([O.pack_plane(np.ascontiguousarray(x), DEPTH) for x in fr])
+            passes.append(p)
+        O.link_motion(passes[0], passes[1], "420")  # chroma refines the luma pass's motion
+        for p in passes:
+            w, h, C = p.W, p.H, p.C
             t0 = time.perf_counter()
             p._run_pass(RADIUS, "whole")
             slot = p.ring.phys(0)

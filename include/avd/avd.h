@@ -389,6 +389,28 @@ typedef struct avd_motion_pass {
 avd_status avd_site_motion(const avd_motion_pass *pass, const uint16_t *ring_codes, const uint32_t *neighbour_slots,
                            int32_t *mv_field, float *confidence, void *stream);
 
+/* run_seeded_refine (motion/analyse.rs:321-367) at the single-level refine geometry
+ * (MotionCtx::single_level, motion/context.rs:108-116): the fine block match on level 0 only,
+ * refine_radius around each seed, with confidence.  `seeds` holds one (x, y) per block per
+ * neighbour, laid out as mv_field, and may be mv_field itself.  pass->search_radius and
+ * pyramid_levels are not read. */
+avd_status avd_site_motion_refine(const avd_motion_pass *pass, const uint16_t *ring_codes,
+                                  const uint32_t *neighbour_slots, const int32_t *seeds, uint32_t refine_radius,
+                                  int32_t *mv_field, float *confidence, void *stream);
+
+/* FieldLayout (motion/chroma_map.rs:18-53): a motion field's block grid and the i32 stride
+ * between its fields (at least blocks_x * blocks_y * 2). */
+typedef struct avd_field_layout {
+    uint32_t blocks_x, blocks_y, step, blksize, stride;
+} avd_field_layout;
+
+/* map_luma_fields (motion/chroma_map.rs:56-98): `fields` luma motion fields onto the chroma block
+ * grid, each chroma vector the per-component lower median of the luma vectors its area covers,
+ * divided by the subsampling (shift 0 or 1 per axis) rounding half away from zero. */
+avd_status avd_site_map_luma_fields(const avd_field_layout *luma, const int32_t *luma_fields,
+                                    const avd_field_layout *chroma, int32_t *chroma_fields, uint32_t shift_x,
+                                    uint32_t shift_y, uint32_t fields, void *stream);
+
 /* Parameters of one NLM window launch (nlm_fused_single_window / nlm_fused_pair_accumulate_window,
  * nlmeans/kernels/fused.rs:138-530). */
 typedef struct avd_nlm_window {

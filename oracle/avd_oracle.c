@@ -313,6 +313,66 @@ void avdo_block_match_fine(const int32_t *cen0, const int32_t *nb0, int w, int h
         }
 }
 
+/* Luma motion onto the chroma block grid: map_luma_fields (motion/chroma_map.rs:56-98) and
+ * mc_map_luma_fields (motion/kernels/chroma_map.rs:57-123).  A chroma block covers the luma area
+ * (chroma_blk << shift) from (col * chroma_step) << shift; it reads covers = (area - luma_blk) /
+ * luma_step + 1 luma blocks a side from start = origin / luma_step, those past the luma grid
+ * dropped (the first always kept), takes the per-component lower median of what is left
+ * (chroma_map.rs:22-44) and divides it by 1 << shift, rounding half away from zero (:6-16). */
+static int map_scale_down(int v, int shift)
+{
+    if (shift == 0) return v;
+    int m = v < 0 ? -v : v;
+    int r = (m + (1 << (shift - 1))) / (1 << shift);
+    return v < 0 ? -r : r;
+}
+
+static int map_lower_median(const int *v, int n)
+{
+    /* The (n-1)/2-th smallest: the value with at most that many below it and more at or below. */
+    int mid = (n - 1) / 2;
+    for (int i = 0; i < n; ++i) {
+        int below = 0, equal = 0;
+        for (int j = 0; j < n; ++j) {
+            below += v[j] < v[i];
+            equal += v[j] == v[i];
+        }
+        if (below <= mid && mid < below + equal) return v[i];
+    }
+    return v[0];
+}
+
+void avdo_map_luma_fields(const int32_t *luma, int lbx, int lby, int lstep, int lblk, int lstride, int32_t *chroma,
+                          int cbx, int cby, int cstep, int cblk, int cstride, int shift_x, int shift_y, int fields)
+{
+    int area_x = cblk << shift_x, area_y = cblk << shift_y;
+    int covers_x = (area_x > lblk ? area_x - lblk : 0) / lstep + 1;
+    int covers_y = (area_y > lblk ? area_y - lblk : 0) / lstep + 1;
+    int *vx = (int *)malloc(sizeof(int) * (size_t)covers_x * covers_y);
+    int *vy = (int *)malloc(sizeof(int) * (size_t)covers_x * covers_y);
+    for (int f = 0; f < fields; ++f)
+        for (int row = 0; row < cby; ++row)
+            for (int col = 0; col < cbx; ++col) {
+                int sc = ((col * cstep) << shift_x) / lstep, sr = ((row * cstep) << shift_y) / lstep;
+                int last_c = sc + covers_x - 1 < lbx - 1 ? sc + covers_x - 1 : lbx - 1;
+                int last_r = sr + covers_y - 1 < lby - 1 ? sr + covers_y - 1 : lby - 1;
+                int first_c = sc < last_c ? sc : last_c, first_r = sr < last_r ? sr : last_r;
+                int n = 0;
+                for (int r = first_r; r <= last_r && r < first_r + covers_y; ++r)
+                    for (int c = first_c; c <= last_c && c < first_c + covers_x; ++c) {
+                        const int32_t *src = luma + (size_t)f * lstride + ((size_t)r * lbx + c) * 2;
+                        vx[n] = src[0];
+                        vy[n] = src[1];
+                        ++n;
+                    }
+                int32_t *dst = chroma + (size_t)f * cstride + ((size_t)row * cbx + col) * 2;
+                dst[0] = map_scale_down(map_lower_median(vx, n), shift_x);
+                dst[1] = map_scale_down(map_lower_median(vy, n), shift_y);
+            }
+    free(vx);
+    free(vy);
+}
+
 /* mc_warp: dst(x,y) = src(clamp(x+mv.x), clamp(y+mv.y)) with block min(x/step, bx-1). */
 void avdo_warp(const float *src, float *dst, const int32_t *mv, int W, int H, int C, int step,
                int blocks_x, int blocks_y)

@@ -330,6 +330,58 @@ def motion_search(geom: MotionGeom, pyr_centre, pyr_nb, maxv, sad_floor, th, wan
     return mv, conf
 
 
+MAPPED_REFINE_RADIUS = 1  # nlmeans/dispatch/motion.rs:42-43
+
+
+def map_luma_fields(luma_mv, luma: MotionGeom, chroma: MotionGeom, shift_x: int, shift_y: int, fields: int):
+    """map_luma_fields (motion/chroma_map.rs:56-98, kernels/chroma_map.rs:57-123): ``fields`` luma
+    fields, one after another, onto the chroma block grid."""
+    src = np.ascontiguousarray(luma_mv, np.int32)
+    lstride, cstride = luma.blocks * 2, chroma.blocks * 2
+    assert src.size >= fields * lstride
+    out = np.zeros(max(fields * cstride, 2), np.int32)
+    lib().avdo_map_luma_fields(_p(src, I32P), luma.blocks_x, luma.blocks_y, luma.step, luma.blksize, lstride,
+                               _p(out, I32P), chroma.blocks_x, chroma.blocks_y, chroma.step, chroma.blksize, cstride,
+                               shift_x, shift_y, fields)
+    return out[: fields * cstride]
+
+
+def motion_refine(geom: MotionGeom, pyr_centre, pyr_nb, seeds, maxv, sad_floor, th, radius=MAPPED_REFINE_RADIUS):
+    """run_seeded_refine (motion/analyse.rs:321-367) of one neighbour at the single-level refine
+    geometry (MotionCtx::single_level, motion/context.rs:108-116): the fine match on level 0,
+    ``radius`` around each seed, with its confidence."""
+    mv = np.array(seeds, np.int32, copy=True)
+    conf = np.zeros(geom.blocks, np.float32)
+    lib().avdo_block_match_fine(_p(pyr_centre[0], I32P), _p(pyr_nb[0], I32P), geom.W, geom.H, geom.blksize,
+                                geom.step, radius, geom.blocks_x, geom.blocks_y, 1, _p(mv, I32P), _p(conf, F32P),
+                                ctypes.c_float(maxv), ctypes.c_float(sad_floor), ctypes.c_float(th))
+    return mv, conf
+
+
+class LumaMotionShare:
+    """LumaMotionRing (motion/luma_ring.rs:10-150) between a luma pass and the chroma pass of the
+    same stream: the luma pass leaves each window's fields under the centre's stream frame
+    (share_window_motion, nlmeans/dispatch/motion.rs:226-252) and the chroma pass of that frame maps
+    them onto its block grid and refines them instead of searching (refine_luma_motion, :254-313).
+    ``shift_x``/``shift_y`` are the chroma subsampling (1, 1 for 4:2:0; 1, 0 for 4:2:2; 0, 0 for
+    4:4:4).  Frames are kept rather than rotated through 2r+1 slots: the luma stream may be run to
+    its end before the chroma stream starts."""
+
+    def __init__(self, shift_x: int, shift_y: int):
+        self.shift_x, self.shift_y = shift_x, shift_y
+        self.luma_geom = None
+        self.fields = {}
+
+    def put(self, frame: int, geom: MotionGeom, mv) -> None:
+        self.luma_geom = geom
+        self.fields[frame] = np.array(mv, np.int32, copy=True)
+
+    def mapped(self, frame: int, chroma: MotionGeom, neighbours: int):
+        if frame not in self.fields:
+            raise RuntimeError(f"the luma pass has not shared the motion of stream frame {frame}")
+        return map_luma_fields(self.fields[frame], self.luma_geom, chroma, self.shift_x, self.shift_y, neighbours)
+
+
 def warp(src, mv, W, H, C, geom: MotionGeom):
     dst = np.zeros_like(src)
     lib().avdo_warp(_p(src, F32P), _p(dst, F32P), _p(mv, I32P), W, H, C, geom.step, geom.blocks_x, geom.blocks_y)
@@ -505,10 +557,11 @@ class _Ring:
 class NlmPass:
     """One NLMeans plane group over a stream (Repeat edges)."""
 
-    def __init__(self, spec: NlmSpec, W, H, C, depth_in, depth_out, precision="f32"):
+    def __init__(self, spec: NlmSpec, W, H, C, depth_in, depth_out, precision="f32", share=None, reads_motion=False):
         self.s, self.W, self.H, self.C = spec, W, H, C
         self.depth_out = depth_out
         self.precision = precision
+        self.share, self.reads_motion = share, reads_motion
         self.r = spec.temporal_radius
         self.ring = _Ring(W, H, C, depth_in, 2 * self.r + 1)
         self.out = []
@@ -572,13 +625,23 @@ class NlmPass:
             sigma = s.sigma if s.sigma is not None else 0.0
             floor = float(sad_noise_floor(geom.blksize, sigma))
             th = float(thsad(geom.blksize, s.thsad_scale))
-            nidx = 0
-            for k in list(range(-r, 0)) + list(range(1, r + 1)):
+            frame = self.ring.slot_frames[centre]
+            ks = list(range(-r, 0)) + list(range(1, r + 1))
+            per = geom.blocks * 2
+            mapped = self.share.mapped(frame, geom, len(ks)) if self.reads_motion else None
+            shared = []
+            for i, k in enumerate(ks):
                 slot = self.ring.phys(r + k)
-                mv, conf = motion_search(geom, self.ring.pyr[centre], self.ring.pyr[slot], self.maxv, floor, th)
+                if mapped is not None:
+                    mv, conf = motion_refine(geom, self.ring.pyr[centre], self.ring.pyr[slot],
+                                             mapped[i * per:(i + 1) * per], self.maxv, floor, th)
+                else:
+                    mv, conf = motion_search(geom, self.ring.pyr[centre], self.ring.pyr[slot], self.maxv, floor, th)
+                shared.append(mv)
                 frames[k] = warp(self.ring.values[slot], mv, W, H, C, geom)
                 confs[k] = conf if (s.hq and s.temporal_confidence) else None
-                nidx += 1
+            if self.share is not None and not self.reads_motion:
+                self.share.put(frame, geom, np.concatenate(shared))
         else:
             for k in range(-r, r + 1):
                 if k:
@@ -615,10 +678,11 @@ class Nl4dSpec:
 class Nl4dPass:
     """One NL4D plane group over a stream (Shifted edges, head/middle/tail passes)."""
 
-    def __init__(self, spec: Nl4dSpec, W, H, C, depth_in, depth_out, precision="f32"):
+    def __init__(self, spec: Nl4dSpec, W, H, C, depth_in, depth_out, precision="f32", share=None, reads_motion=False):
         self.s, self.W, self.H, self.C = spec, W, H, C
         self.depth_out = depth_out
         self.precision = precision
+        self.share, self.reads_motion = share, reads_motion
         self.r = spec.temporal_radius
         self.N = 2 * self.r + 1
         self.ring = _Ring(W, H, C, depth_in, self.N)
@@ -652,10 +716,19 @@ class Nl4dPass:
         floor = float(sad_noise_floor(self.geom.blksize, sig))
         th = float(thsad(self.geom.blksize, self.s.thsad_scale))
         mvs, confs = [], []
-        for slot in nslots:
-            mv, conf = motion_search(self.geom, self.ring.pyr[centre], self.ring.pyr[slot], self.maxv, floor, th)
+        frame = self.ring.slot_frames[centre]
+        per = self.geom.blocks * 2
+        mapped = self.share.mapped(frame, self.geom, len(nslots)) if self.reads_motion else None
+        for i, slot in enumerate(nslots):
+            if mapped is not None:
+                mv, conf = motion_refine(self.geom, self.ring.pyr[centre], self.ring.pyr[slot],
+                                         mapped[i * per:(i + 1) * per], self.maxv, floor, th)
+            else:
+                mv, conf = motion_search(self.geom, self.ring.pyr[centre], self.ring.pyr[slot], self.maxv, floor, th)
             mvs.append(mv)
             confs.append(conf)
+        if self.share is not None and not self.reads_motion and mvs:
+            self.share.put(frame, self.geom, np.concatenate(mvs))
         mv = np.concatenate(mvs) if mvs else np.zeros(2, np.int32)
         conf = np.concatenate(confs) if confs else np.zeros(1, np.float32)
         members = collab_search(self.ring.codes, self.params, centre, nslots, mv, conf)
@@ -714,6 +787,21 @@ class Nl4dPass:
         self.ring = _Ring(self.W, self.H, self.C, self.ring.depth, self.N)
         self.passes = 0
         return emit
+
+
+def chroma_shifts(chroma: str):
+    """The subsampling shifts of a chroma layout name ("420", "422", "444")."""
+    return {"420": (1, 1), "422": (1, 0), "444": (0, 0)}[chroma]
+
+
+def link_motion(luma_pass, chroma_pass, chroma: str = "420"):
+    """Has ``chroma_pass`` refine ``luma_pass``'s motion instead of searching its own, as the
+    reference's planar engine does whenever both passes search motion (engine/spec.rs:350-365,
+    :474-492).  Run the luma stream before (or interleaved ahead of) the chroma stream."""
+    share = LumaMotionShare(*chroma_shifts(chroma))
+    luma_pass.share, luma_pass.reads_motion = share, False
+    chroma_pass.share, chroma_pass.reads_motion = share, True
+    return share
 
 
 def run_stream(pass_obj, frames_words):

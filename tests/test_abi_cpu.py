@@ -90,6 +90,18 @@ def geom(chroma=0, fmt_in=None, fmt_out=None, w=64, h=48):
      avd.InvalidOptions),
     ("nlmeans", dict(hq=1, sigma=0.02, temporal=1, temporal_radius=3, motion_compensation=1,
                      motion={"estimation": 0}), None, geom(), avd.InvalidOptions),
+    # The chroma pass takes its motion from the luma pass (engine/spec.rs:320-341): both search the
+    # same way at the same temporal radius, or neither searches.
+    ("nl4d", dict(sigma=0.02, pooled_threshold=0),
+     dict(sigma=0.02, pooled_threshold=0, motion={"estimation": 1, "blksize": 8, "overlap": 4}), geom(1),
+     avd.InvalidOptions),
+    ("nl4d", dict(sigma=0.02, pooled_threshold=0), dict(sigma=0.02, pooled_threshold=0, temporal_radius=1), geom(1),
+     avd.InvalidOptions),
+    ("nlmeans", dict(temporal=1, temporal_radius=1, motion_compensation=1, motion={"estimation": 1}),
+     dict(temporal=1, temporal_radius=1, motion_compensation=0), geom(1), avd.InvalidOptions),
+    ("nlmeans", dict(temporal=1, temporal_radius=1, motion_compensation=1, motion={"estimation": 1}),
+     dict(temporal=1, temporal_radius=1, motion_compensation=1, motion={"estimation": 1, "search_radius": 2}),
+     geom(1), avd.InvalidOptions),
     ("nlmeans", dict(patch_radius=17), None, geom(), avd.InvalidOptions),
     ("nlmeans", dict(search_radius=9), None, geom(), avd.InvalidOptions),
     ("nlmeans", dict(hq=1), None, geom(), avd.InvalidOptions),  # HQ needs a fixed sigma here

@@ -156,6 +156,7 @@ def test_sixteen_member_engine_420_is_bit_exact_and_mixed_within_0_05_db():
         outs[form] = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f] for f in clip], depth, dims)
     pl = O.Nl4dPass(_spec(r, S, sig_y / 1023, step, 16), W, H, 1, depth, depth)
     pc = O.Nl4dPass(_spec(r, S, sig_c / 1023, step, 16), W // 2, H // 2, 2, depth, depth)
+    O.link_motion(pl, pc, "420")
     want_l, _ = O.run_stream(pl, [[O.pack_plane(f[0].numpy(), depth)] for f in clip])
     want_c, _ = O.run_stream(pc, [[O.pack_plane(f[1].numpy(), depth), O.pack_plane(f[2].numpy(), depth)] for f in clip])
     for i in range(frames):

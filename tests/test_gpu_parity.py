@@ -332,6 +332,7 @@ def test_nl4d_engine_420_luma_and_chroma_are_bit_exact():
     got = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f] for f in clip], depth, dims)
     pl = O.Nl4dPass(_nl4d_spec(r, S, sigma), W, H, 1, depth, depth)
     pc = O.Nl4dPass(_nl4d_spec(r, S, sigma), W // 2, H // 2, 2, depth, depth)
+    O.link_motion(pl, pc, "420")  # the chroma pass refines the luma pass's motion
     want_l, _ = O.run_stream(pl, [[O.pack_plane(f[0].numpy(), depth)] for f in clip])
     want_c, _ = O.run_stream(pc, [[O.pack_plane(f[1].numpy(), depth), O.pack_plane(f[2].numpy(), depth)] for f in clip])
     assert len(got) == frames
@@ -355,11 +356,13 @@ def test_nl4d_engine_with_pooled_threshold_is_bit_exact(chroma):
     dims = [(W, H)] if chroma == "gray" else [(W, H), (W // 2, H // 2), (W // 2, H // 2)]
     got = run_engine(eng, [[words_dev(p.numpy(), depth) for p in f[:len(dims)]] for f in clip], depth, dims)
     pl = O.Nl4dPass(_nl4d_spec(r, S, sigma, lam=4.158, pooled=True), W, H, 1, depth, depth)
+    pc = O.Nl4dPass(_nl4d_spec(r, S, sigma, lam=3.234, pooled=True), W // 2, H // 2, 2, depth, depth)
+    if chroma != "gray":
+        O.link_motion(pl, pc, chroma)
     want_l, _ = O.run_stream(pl, [[O.pack_plane(f[0].numpy(), depth)] for f in clip])
     for i in range(frames):
         assert np.array_equal(got[i][0], want_l[i][0]), f"luma {i}"
     if chroma != "gray":
-        pc = O.Nl4dPass(_nl4d_spec(r, S, sigma, lam=3.234, pooled=True), W // 2, H // 2, 2, depth, depth)
         want_c, _ = O.run_stream(pc, [[O.pack_plane(f[1].numpy(), depth), O.pack_plane(f[2].numpy(), depth)]
                                       for f in clip])
         for i in range(frames):

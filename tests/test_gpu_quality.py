@@ -51,10 +51,11 @@ def test_forms_against_the_fp64_oracle_on_the_benchmark_method():
         return O.Nl4dSpec(temporal_radius=r, refine=2, spatial_radius=S, lambda_ht=2.7, c_min=0.05, kaiser_beta=2.0,
                           sigma=sigma, motion=O.MotionGeom(0, 0, 16, 8, 4, 2), step=step, group_size=K)
 
-    f64_l, _ = O.run_stream(O.Nl4dPass(spec(sig_y / peak), W, H, 1, depth, depth, precision="f64"),
-                            [[O.pack_plane(f[0].numpy(), depth)] for f in noisy])
-    f64_c, _ = O.run_stream(O.Nl4dPass(spec(sig_c / peak), W // 2, H // 2, 2, depth, depth, precision="f64"),
-                            [[O.pack_plane(f[1].numpy(), depth), O.pack_plane(f[2].numpy(), depth)] for f in noisy])
+    pl = O.Nl4dPass(spec(sig_y / peak), W, H, 1, depth, depth, precision="f64")
+    pc = O.Nl4dPass(spec(sig_c / peak), W // 2, H // 2, 2, depth, depth, precision="f64")
+    O.link_motion(pl, pc, "420")
+    f64_l, _ = O.run_stream(pl, [[O.pack_plane(f[0].numpy(), depth)] for f in noisy])
+    f64_c, _ = O.run_stream(pc, [[O.pack_plane(f[1].numpy(), depth), O.pack_plane(f[2].numpy(), depth)] for f in noisy])
     f64 = [[f64_l[i][0], f64_c[i][0], f64_c[i][1]] for i in range(frames)]
     for k, (w, h) in enumerate(dims):
         ref = np.concatenate([clean[i][k].numpy().reshape(-1) for i in range(frames)])
