@@ -350,6 +350,28 @@ static void check_estimation(const avd_motion_search &m, uint32_t temporal_radiu
                       "yet; use Direct");
 }
 
+// MotionCompensationMode::validate (motion/search.rs:226-290, MotionEstimation::validate :143-155),
+// then what this build runs of it: two pyramid levels at most, Direct estimation, no escalation.
+static void validate_motion_search(const avd_motion_search &m, uint32_t temporal_radius)
+{
+    auto invalid = [](const std::string &what) { throw Failure(AVD_ERR_INVALID_OPTIONS, "motion-compensation " + what); };
+    if (m.blksize < 4) invalid("blksize=" + std::to_string(m.blksize) + " is too small, the minimum is 4 pixels per side");
+    if (m.blksize > 32) invalid("blksize=" + std::to_string(m.blksize) + " exceeds the supported maximum of 32");
+    if (m.blksize % 2 != 0)
+        invalid("blksize=" + std::to_string(m.blksize) + " must be even so the /2 coarse level is well-defined");
+    if (m.overlap >= m.blksize) invalid("overlap must be strictly less than blksize");
+    if (m.search_radius == 0 || m.search_radius > 8) invalid("search_radius must be in 1..=8");
+    if (m.max_search_escalation < m.search_radius || m.max_search_escalation > 8)
+        invalid("max_search_escalation must be between search_radius and 8");
+    if (m.pyramid_levels == 0 || m.pyramid_levels > 3) invalid("pyramid_levels must be in 1..=3");
+    if (m.estimation == 2 && (m.refine_radius == 0 || m.refine_radius > 8))
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "motion-estimation refine_radius must be in 1..=8");
+    if (m.pyramid_levels > 2) throw Failure(AVD_ERR_INVALID_OPTIONS, "three-level motion pyramids are not built yet");
+    check_estimation(m, temporal_radius);
+    if (m.max_search_escalation > m.search_radius)
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "search escalation is not built yet");
+}
+
 // ---------------------------------------------------------------------------------------------
 // Luma motion shared with the chroma pass (motion/luma_ring.rs, motion/chroma_map.rs)
 // ---------------------------------------------------------------------------------------------
@@ -473,12 +495,7 @@ NlmPass::NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, av
     }
     if (motion_) {
         const avd_motion_search &m = o.motion;
-        if (m.overlap >= m.blksize || m.blksize > 32 || m.blksize < 2 || m.search_radius > 8 || m.pyramid_levels < 1 ||
-            m.pyramid_levels > 2)
-            throw Failure(AVD_ERR_INVALID_OPTIONS, "unsupported motion search geometry");
-        check_estimation(m, r_);
-        if (m.max_search_escalation > m.search_radius)
-            throw Failure(AVD_ERR_INVALID_OPTIONS, "search escalation is not built yet");
+        validate_motion_search(m, r_);
         if (in.kind == AVD_FORMAT_U16 && in.depth > 12)
             throw Failure(AVD_ERR_INVALID_GEOMETRY, "motion search on more than 12-bit samples is not supported");
         motion_opts_ = m;
@@ -741,15 +758,10 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     if (!(o.kaiser_beta >= 0.0f && o.kaiser_beta <= 8.0f))
         throw Failure(AVD_ERR_INVALID_OPTIONS, "kaiser_beta must be in 0..=8");
     const avd_motion_search &m = o.motion;
-    if (m.overlap >= m.blksize) throw Failure(AVD_ERR_INVALID_OPTIONS, "overlap must be below blksize");
+    validate_motion_search(m, r_);
     const uint32_t step = m.blksize - m.overlap;
     if ((m.blksize + step - 1) / step > 4)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "more than 4 motion blocks would cover a patch on each axis");
-    if (m.blksize > 32 || m.blksize < 2 || m.search_radius > 8 || m.pyramid_levels < 1 || m.pyramid_levels > 2)
-        throw Failure(AVD_ERR_INVALID_OPTIONS, "unsupported motion search geometry");
-    check_estimation(m, r_);
-    if (m.max_search_escalation > m.search_radius)
-        throw Failure(AVD_ERR_INVALID_OPTIONS, "search escalation is not built yet");
     // The truemotion regulariser (nl4d/denoiser.rs:235, truemotion/stage.rs:61) runs whenever
     // field_lambda is above 0; it is not built, so only field_lambda = 0 keeps the reference's output.
     if (o.field_lambda != 0.0f)

@@ -609,6 +609,19 @@ static inline int covering_lo(int p, int blksize, int step)
                 collab_search_one##SFX(ring, slot_len, pp, centre_slot, nslots, mv, conf, ix, iy,          \
                                        members + ((size_t)iy * pp->refs_x + ix) * (pp->k_max + 1));      \
         }                                                                                                \
+    }                                                                                                    \
+    /* The same for the `n` references listed in `refs` (row-major ids), into members[n][K + 1]. */     \
+    void avdo_collab_search_refs##SFX(const T *ring, const avdo_collab_params *pp, int centre_slot,       \
+                                      const uint32_t *nslots, const int32_t *mv, const float *conf,       \
+                                      const uint32_t *refs, int n, uint32_t *members)                     \
+    {                                                                                                    \
+        size_t slot_len = (size_t)pp->W * pp->H * pp->C;                                                 \
+        _Pragma("omp parallel for schedule(dynamic, 8)") for (int i = 0; i < n; ++i)                      \
+        {                                                                                                \
+            const int ix = (int)(refs[i] % (uint32_t)pp->refs_x), iy = (int)(refs[i] / (uint32_t)pp->refs_x); \
+            collab_search_one##SFX(ring, slot_len, pp, centre_slot, nslots, mv, conf, ix, iy,              \
+                                   members + (size_t)i * (pp->k_max + 1));                               \
+        }                                                                                                \
     }
 
 /* Codes: the self-match is distance 0 and every other candidate its SSD plus one. */

@@ -442,6 +442,22 @@ def collab_search(ring, params: CollabParams, centre_slot, nslots, mv, conf):
     return members
 
 
+def collab_search_refs(ring, params: CollabParams, centre_slot, nslots, mv, conf, refs):
+    """collab_search for the listed references only (row-major ids): members[len(refs)][k_max + 1]."""
+    ids = np.ascontiguousarray(refs, np.uint32)
+    members = np.zeros(max(ids.size, 1) * (params.k_max + 1), np.uint32)
+    ns = np.ascontiguousarray(nslots, np.uint32) if len(nslots) else np.zeros(1, np.uint32)
+    mvv = np.ascontiguousarray(mv, np.int32) if mv is not None and mv.size else np.zeros(2, np.int32)
+    cf = np.ascontiguousarray(conf, np.float32) if conf is not None and conf.size else np.zeros(1, np.float32)
+    if ring.dtype == np.uint16:
+        fn, rp = lib().avdo_collab_search_refs_u16, _p(ring, U16P)
+    else:
+        fn, rp = lib().avdo_collab_search_refs_f32, _p(np.ascontiguousarray(ring, np.float32), F32P)
+    fn(rp, ctypes.byref(params), centre_slot, _p(ns, U32P), _p(mvv, I32P), _p(cf, F32P), _p(ids, U32P), int(ids.size),
+       _p(members, U32P))
+    return members[: ids.size * (params.k_max + 1)]
+
+
 def collab_filter(values_ring, params: CollabParams, centre_slot, nslots, members, accum, wsum, precision="f32"):
     refs = params.refs_x * params.refs_y
     gw = np.zeros(refs, np.float32)

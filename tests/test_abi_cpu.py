@@ -102,6 +102,15 @@ def geom(chroma=0, fmt_in=None, fmt_out=None, w=64, h=48):
     ("nlmeans", dict(temporal=1, temporal_radius=1, motion_compensation=1, motion={"estimation": 1}),
      dict(temporal=1, temporal_radius=1, motion_compensation=1, motion={"estimation": 1, "search_radius": 2}),
      geom(1), avd.InvalidOptions),
+    # MotionCompensationMode::validate (motion/search.rs:226-290): block size 4..=32 and even, search
+    # radius 1..=8, escalation between the radius and 8, 1..=3 pyramid levels (3 is not built here).
+    *[("nl4d", dict(sigma=0.02, pooled_threshold=0, motion={"estimation": 1, **m}), None, geom(), avd.InvalidOptions)
+      for m in ({"blksize": 2, "overlap": 0}, {"blksize": 34, "overlap": 8}, {"blksize": 9, "overlap": 4},
+                {"search_radius": 0, "max_search_escalation": 0}, {"max_search_escalation": 2},
+                {"max_search_escalation": 9}, {"pyramid_levels": 0}, {"pyramid_levels": 3})],
+    ("nlmeans", dict(hq=1, sigma=0.02, temporal=1, temporal_radius=1, motion_compensation=1,
+                     motion={"estimation": 1, "search_radius": 9, "max_search_escalation": 9}), None, geom(),
+     avd.InvalidOptions),
     ("nlmeans", dict(patch_radius=17), None, geom(), avd.InvalidOptions),
     ("nlmeans", dict(search_radius=9), None, geom(), avd.InvalidOptions),
     ("nlmeans", dict(hq=1), None, geom(), avd.InvalidOptions),  # HQ needs a fixed sigma here
