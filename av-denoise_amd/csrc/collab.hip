@@ -1119,13 +1119,12 @@ constexpr size_t kLdsBudget = 64 * 1024;
 
 // Volumes of a group at most (two frames per volume) for KM group members; an earlier volume
 // holds at most one pick per neighbour frame, so a frame holds at most VOLS - 1 picks.
-__host__ __device__ constexpr int trajectory_vols(int KM) { return KM / 2; }
-
-__host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR, int KM)
+// The static LDS of the trajectory kernel for NR references per wave in groups of KM with volume
+// grids of GF frames (KM / GF volumes).
+__host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR, int KM, int GF)
 {
-    return (size_t)kWaves * (NR * 256 * 2 + 16 * 4 + NR * 16 * 8 + NR * trajectory_vols(KM) * 8 * 4 * C * 4 +
-                             NR * trajectory_vols(KM) * 4 + NR * 16 * (trajectory_vols(KM) - 1) * 4 + NR * 16 * 4 +
-                             4);
+    return (size_t)kWaves * (16 * 4 + NR * 16 * 8 + NR * (KM / GF) * 8 * 4 * C * 4 + NR * (KM / GF) * 4 +
+                             NR * 16 * (KM / GF - 1) * 4 + NR * 16 * 4 + 4);
 }
 
 __device__ __forceinline__ uint32_t lane_mask_lt(uint64_t bal)
@@ -1133,18 +1132,33 @@ __device__ __forceinline__ uint32_t lane_mask_lt(uint64_t bal)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
 }
 
-// Entries of one reference's rectangle table, and the dynamic LDS the kernel needs for NR
-// references per wave.
+// Entries of one reference's rectangle table per volume (neighbour frames times covering blocks),
+// and over all its volumes.
+__host__ __device__ __forceinline__ int trajectory_volume_entries(const avd_collab_pass &p)
+{
+    const int covers = ((int)p.blksize + (int)p.blk_step - 1) / (int)p.blk_step;
+    const int cshift = covers <= 1 ? 0 : (covers <= 2 ? 1 : 2);
+    return (2 * (int)p.temporal_radius) << (2 * cshift);
+}
+
 __host__ __device__ __forceinline__ int trajectory_entries(const avd_collab_pass &p)
 {
     const int gf = p.temporal_radius == 1 ? 2 : 4;
-    const int covers = ((int)p.blksize + (int)p.blk_step - 1) / (int)p.blk_step;
-    const int cshift = covers <= 1 ? 0 : (covers <= 2 ? 1 : 2);
-    return ((int)p.group_size / gf) * ((2 * (int)p.temporal_radius) << (2 * cshift));
+    return ((int)p.group_size / gf) * trajectory_volume_entries(p);
 }
 
-template <int C, int K, int NR, bool SHORT, int KM>
-__global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
+// The dynamic LDS of NR references per wave: each wave's rectangle tables ([NR * entries][2] u32)
+// and its list of one volume's non-empty entries ([NR * volume entries] u16).
+__host__ __device__ __forceinline__ size_t trajectory_dynamic_lds(const avd_collab_pass &p, int NR)
+{
+    const size_t rects = (size_t)kWaves * NR * trajectory_entries(p) * 2 * sizeof(uint32_t);
+    const size_t list = ((size_t)kWaves * NR * trajectory_volume_entries(p) * sizeof(uint16_t) + 15) & ~(size_t)15;
+    return rects + list;
+}
+
+template <int C, int K, int NR, bool SHORT, int KM, int GF>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(C == 1 ? 4 : 3)))
+void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                                                                   const uint32_t *__restrict__ nslots,
                                                                   const int32_t *__restrict__ mv,
                                                                   const float *__restrict__ conf,
@@ -1152,10 +1166,11 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                                                                   unsigned long long *__restrict__ scored_count)
 {
     constexpr int LPR = 64 / NR;  // lanes per reference
-    constexpr int VOLS = trajectory_vols(KM);
+    constexpr int VOLS = KM / GF;
     static_assert(LPR >= KM, "a reference's lanes write its member words");
-    extern __shared__ uint32_t s_dyn[];  // [kWaves][NR * entries][2]: (left | right << 16), (top | bottom << 16)
-    __shared__ uint16_t s_list[kWaves][NR * 256];  // non-empty entries of the current volume: e | r << 8
+    // Dynamic: [kWaves][NR * entries][2] rectangles, (left | right << 16), (top | bottom << 16); then
+    // [kWaves][NR * volume entries] the current volume's non-empty entries, e | r << 8.
+    extern __shared__ uint32_t s_dyn[];
     __shared__ uint32_t s_slot[kWaves][16];
     __shared__ unsigned long long s_fbest[kWaves][NR][16];
     __shared__ uint32_t s_patch[kWaves][NR][VOLS][8][4 * C];  // every volume's anchor patch
@@ -1181,8 +1196,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     for (int v = 0; v < VOLS; ++v) anchors[v] = (uint32_t)__shfl((int)mine, hbase + v);
 
     const int radius = p.temporal_radius;
-    const int gf = radius == 1 ? 2 : 4;
-    const int volumes = KM / gf;
+    constexpr int gf = GF;
+    constexpr int volumes = VOLS;
     const int bstep = p.blk_step, bsz = p.blksize, bx_n = p.blocks_x, by_n = p.blocks_y;
     const int covers = (bsz + bstep - 1) / bstep;
     const int cshift = covers <= 1 ? 0 : (covers <= 2 ? 1 : 2);
@@ -1192,6 +1207,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
     const int ents = volumes * per_vol;
     const size_t nblk = (size_t)bx_n * by_n;
     uint32_t(*const rect_tab)[2] = reinterpret_cast<uint32_t(*)[2]>(s_dyn) + (size_t)wave * NR * ents;
+    uint16_t *const s_list = reinterpret_cast<uint16_t *>(s_dyn + (size_t)kWaves * NR * ents * 2) +
+                             (size_t)wave * NR * per_vol;
     if (lane < frames_n) s_slot[wave][lane] = nslots[lane];
     for (int i = hl; i < volumes * 8; i += LPR) {  // lane (v, r) stages row r of volume v's anchor patch
         const int v = i >> 3, r = i & 7;
@@ -1281,14 +1298,14 @@ __global__ __launch_bounds__(kWaves * 64) void k_trajectory_codes(avd_collab_pas
                 }
             }
             const uint64_t bal = __ballot(valid);
-            if (valid) s_list[wave][n_valid + lane_mask_lt(bal)] = (uint16_t)(e | (r_of << 8));
+            if (valid) s_list[n_valid + lane_mask_lt(bal)] = (uint16_t)(e | (r_of << 8));
             n_valid += __popcll(bal);
         }
         if (hl < frames_n) s_fbest[wave][hr][hl] = kNoKey;
         wave_sync();
         for (int task = lane; task < n_valid * K; task += 64) {
             const int k = task / K, col = task - k * K;
-            const uint32_t ent = s_list[wave][k];
+            const uint32_t ent = s_list[k];
             const int tr = (int)(ent >> 8), e = (int)(ent & 0xFFu);
             const uint32_t(*rects)[2] = rect_tab + tr * ents + v * per_vol;
             const int t = e >> nshift, b = e & ((1 << nshift) - 1);
@@ -2560,57 +2577,51 @@ hipError_t launch_centre_km(const avd_collab_pass &p, const uint16_t *codes, uin
 // One trajectory launch: NR references per wave (4, or fewer where the rectangle tables outgrow
 // the LDS budget: 2 for eight members, 2 then 1 for sixteen), SHORT for frames shorter than K + 7
 // rows or rings past a buffer load's offset.
-template <int C, int K, int KM>
+template <int C, int K, int KM, int GF>
 hipError_t launch_trajectory_ck(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
                                 const int32_t *mv, const float *conf, uint32_t *members, hipStream_t s,
-                                unsigned long long *scored_count, int nr, bool short_frame, size_t ents)
+                                unsigned long long *scored_count, int nr, bool short_frame, size_t lds)
 {
     const dim3 blk(kWaves * 64);
-    const size_t lds = (size_t)nr * ents;
     const unsigned g = div_up(ref_count(p), kWaves * nr);
+#define AVD_TRAJ_LAUNCH(NRn, SHORTn) \
+    k_trajectory_codes<C, K, NRn, SHORTn, KM, GF><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count)
     if constexpr (KM == 8) {
-        if (short_frame)
-            k_trajectory_codes<C, K, 2, true, 8><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
-        else if (nr == 4)
-            k_trajectory_codes<C, K, 4, false, 8><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
-        else
-            k_trajectory_codes<C, K, 2, false, 8><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
+        if (short_frame) AVD_TRAJ_LAUNCH(2, true);
+        else if (nr == 4) AVD_TRAJ_LAUNCH(4, false);
+        else AVD_TRAJ_LAUNCH(2, false);
     } else {
-        if (short_frame)
-            k_trajectory_codes<C, K, 1, true, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
-        else if (nr == 4)
-            k_trajectory_codes<C, K, 4, false, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
-        else if (nr == 2)
-            k_trajectory_codes<C, K, 2, false, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
-        else
-            k_trajectory_codes<C, K, 1, false, 16><<<g, blk, lds, s>>>(p, codes, nslots, mv, conf, members, scored_count);
+        if (short_frame) AVD_TRAJ_LAUNCH(1, true);
+        else if (nr == 4) AVD_TRAJ_LAUNCH(4, false);
+        else if (nr == 2) AVD_TRAJ_LAUNCH(2, false);
+        else AVD_TRAJ_LAUNCH(1, false);
     }
+#undef AVD_TRAJ_LAUNCH
     return hipGetLastError();
 }
 
-template <int KM>
-hipError_t launch_trajectory_km(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
+template <int KM, int GF>
+hipError_t launch_trajectory_gf(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
                                 const int32_t *mv, const float *conf, uint32_t *members, hipStream_t s,
                                 unsigned long long *scored_count)
 {
-    // References per wave: the most whose rectangle tables fit the LDS budget.
-    const size_t ents = (size_t)trajectory_entries(p) * 2 * sizeof(uint32_t) * kWaves;
+    // References per wave: the most whose LDS (static arrays and rectangle tables) fits the budget.
     const int C = (int)p.channels;
-    auto fits = [&](int nr) {
-        const size_t st = C == 1 ? trajectory_static_lds(1, nr, KM) : trajectory_static_lds(2, nr, KM);
-        return st + nr * ents <= kLdsBudget;
+    auto lds_of = [&](int nr) {
+        return (C == 1 ? trajectory_static_lds(1, nr, KM, GF) : trajectory_static_lds(2, nr, KM, GF)) +
+               trajectory_dynamic_lds(p, nr);
     };
     const int nr_lo = KM == 8 ? 2 : 1;
-    int nr = fits(4) ? 4 : (KM == 16 && fits(2) ? 2 : nr_lo);
+    int nr = lds_of(4) <= kLdsBudget ? 4 : (KM == 16 && lds_of(2) <= kLdsBudget ? 2 : nr_lo);
     // Frames shorter than K + 7 rows, and rings past 2^31 bytes (beyond a buffer load's offset),
     // take the clamping slide (at two references per wave for eight members, one for sixteen).
     const bool short_frame = (int)p.height < 2 * (int)p.refine + 8 ||
                              (uint64_t)p.width * p.height * p.channels * p.slots * 2 >= (1ull << 31);
     if (short_frame) nr = nr_lo;
-    const size_t st = C == 1 ? trajectory_static_lds(1, nr, KM) : trajectory_static_lds(2, nr, KM);
-    if (st + nr * ents > 160 * 1024) return hipErrorInvalidValue;  // past the LDS of a workgroup
+    if (lds_of(nr) > 160 * 1024) return hipErrorInvalidValue;  // past the LDS of a workgroup
+    const size_t lds = trajectory_dynamic_lds(p, nr);
 #define AVD_TRAJ(Cn, Kn) \
-    launch_trajectory_ck<Cn, Kn, KM>(p, codes, nslots, mv, conf, members, s, scored_count, nr, short_frame, ents)
+    launch_trajectory_ck<Cn, Kn, KM, GF>(p, codes, nslots, mv, conf, members, s, scored_count, nr, short_frame, lds)
     switch (p.refine * 2 + 1 + (p.channels == 2 ? 100 : 0)) {
     case 3: return AVD_TRAJ(1, 3);
     case 5: return AVD_TRAJ(1, 5);
@@ -2622,6 +2633,16 @@ hipError_t launch_trajectory_km(const avd_collab_pass &p, const uint16_t *codes,
     default: return AVD_TRAJ(2, 9);
     }
 #undef AVD_TRAJ
+}
+
+// Volume grids of 2 frames at temporal radius 1 and 4 above it (collab/mod.rs:45-51).
+template <int KM>
+hipError_t launch_trajectory_km(const avd_collab_pass &p, const uint16_t *codes, const uint32_t *nslots,
+                                const int32_t *mv, const float *conf, uint32_t *members, hipStream_t s,
+                                unsigned long long *scored_count)
+{
+    return p.temporal_radius == 1 ? launch_trajectory_gf<KM, 2>(p, codes, nslots, mv, conf, members, s, scored_count)
+                                  : launch_trajectory_gf<KM, 4>(p, codes, nslots, mv, conf, members, s, scored_count);
 }
 
 }  // namespace
