@@ -1124,7 +1124,7 @@ constexpr size_t kLdsBudget = 64 * 1024;
 __host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR, int KM, int GF)
 {
     return (size_t)kWaves * (16 * 4 + NR * 16 * 8 + NR * (KM / GF) * 8 * 4 * C * 4 + NR * (KM / GF) * 4 +
-                             NR * 16 * (KM / GF - 1) * 4 + NR * 16 * 4 + 4);
+                             NR * 16 * (KM / GF - 1) * 4 + NR * 16 * 4 + NR * KM * 4 + 4);
 }
 
 __device__ __forceinline__ uint32_t lane_mask_lt(uint64_t bal)
@@ -1177,6 +1177,7 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     __shared__ uint32_t s_energy[kWaves][NR][VOLS];
     __shared__ uint32_t s_held[kWaves][NR][16][VOLS - 1];  // earlier volumes' picks, by neighbour frame
     __shared__ uint32_t s_hcnt[kWaves][NR][16];
+    __shared__ uint32_t s_final[kWaves][NR][KM];  // the grid's members, volume by volume
     constexpr int refine = (K - 1) / 2;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int hr = lane / LPR, hl = lane % LPR, hbase = hr * LPR;
@@ -1250,6 +1251,7 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
         rect_tab[hr * ents + idx][1] = yr;
     }
     if (hl < frames_n) s_hcnt[wave][hr][hl] = 0;
+    if (hl < KM) s_final[wave][hr][hl] = 0;
     wave_sync();
     if (hl < volumes) {
         uint32_t e = 0;
@@ -1262,15 +1264,12 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
         s_energy[wave][hr][hl] = e;
     }
 
-    // final[] collects the grid's members.  A volume's list holds its gf - 1 frame picks and their
+    // s_final collects the grid's members.  A volume's list holds its gf - 1 frame picks and their
     // distances (SSD + 1 fits 32 bits; ~0u marks an empty slot).  Only earlier volumes' picks can
     // equal a candidate (anchors sit in the centre frame); they are kept by frame in s_held.  All
     // of this is the same on every lane of one reference.
     uint32_t scored = 0;
-    uint32_t final_pos[KM];
     bool filled = true;
-#pragma unroll
-    for (int i = 0; i < KM; ++i) final_pos[i] = 0;
     for (int v = 0; v < volumes; ++v) {
         const int first = v * gf;
         uint32_t anchor = anchors[0];
@@ -1399,12 +1398,8 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
             }
         }
         filled = filled && (gf == 2 ? vd[0] : vd[2]) != ~0u;
-#pragma unroll
-        for (int i = 0; i < KM; ++i) {
-            const int j = i - first - 1;
-            if (i == first) final_pos[i] = anchor;
-            else if (j >= 0 && j < gf - 1) final_pos[i] = j == 0 ? vp[0] : (j == 1 ? vp[1] : vp[2]);
-        }
+        // Lane j < gf of each reference places member first + j: the anchor, then the picks.
+        if (hl < gf) s_final[wave][hr][first + hl] = hl == 0 ? anchor : (hl == 1 ? vp[0] : (hl == 2 ? vp[1] : vp[2]));
         // This volume's picks are held from the next volume on, each in its own frame's list (the
         // picks of one volume sit in distinct frames, so lane j appends pick j alone).
         if (hl < gf - 1) {
@@ -1422,11 +1417,7 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     if (use_grid && live) {
         // Lane hl < KM writes member hl; the group word follows (lane KM, or lane 0 again when the
         // reference has only KM lanes).
-        uint32_t word = 0;
-#pragma unroll
-        for (int m = 0; m < KM; ++m)
-            if (hl == m) word = final_pos[m];
-        if (hl < KM) members[(size_t)my_ref * (KM + 1) + hl] = word;
+        if (hl < KM) members[(size_t)my_ref * (KM + 1) + hl] = s_final[wave][hr][hl];
         if (hl == (LPR > KM ? KM : 0)) members[(size_t)my_ref * (KM + 1) + KM] = (uint32_t)KM | 0x100u;
     }
     add_scored(scored_count, scored, true, C);
