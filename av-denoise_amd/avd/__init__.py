@@ -169,6 +169,14 @@ class Engine:
         """Samples device time of every frame (1) or of none (0) (Engine::sample_device_time)."""
         check(self._lib.avd_engine_sample_device_time(self._h, every))
 
+    NLM_FORMS = {"plain": 0, "batched": 1, "default": 2}
+
+    def set_nlm_form(self, form: str) -> None:
+        """NLMeans engines: one launch per window ("plain"), every window of a frame and the finish
+        in one launch ("batched"), or the default (spatial passes batched, temporal ones plain);
+        bit-identical outputs."""
+        check(self._lib.avd_engine_set_nlm_form(self._h, self.NLM_FORMS[form]))
+
     def timing(self) -> dict:
         """Per-stage device milliseconds, launches and work since sampling began (Engine::timing)."""
         from ._lib import Timing

@@ -80,7 +80,7 @@ class WindowSpan(ctypes.Structure):
 
 
 STAGES = ["ingest", "motion", "collab_search", "collab_filter", "normalise", "egress", "nlm_pair", "nlm_single",
-          "nlm_finish", "warp", "collab_centre"]
+          "nlm_finish", "warp", "collab_centre", "nlm_window"]
 
 
 class Timing(ctypes.Structure):
@@ -208,6 +208,7 @@ SIGNATURES = {
     "avd_engine_set_tile_cap_for_test": (ctypes.c_int, [P, u32]),
     "avd_engine_tile_overflows_for_test": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64)]),
     "avd_engine_sample_device_time": (ctypes.c_int, [P, u32]),
+    "avd_engine_set_nlm_form": (ctypes.c_int, [P, u32]),
     "avd_engine_timing": (ctypes.c_int, [P, ctypes.POINTER(Timing)]),
     "avd_site_ingest": (ctypes.c_int, [P, P, P, P, u32, u32, SampleFormat, P]),
     "avd_site_egress": (ctypes.c_int, [P, P, P, u32, u32, SampleFormat, P]),

@@ -230,6 +230,12 @@ avd_status avd_engine_sample_device_time(avd_engine *e, uint32_t every)
     return guard([&] { e->impl.sample_device_time(every); });
 }
 
+avd_status avd_engine_set_nlm_form(avd_engine *e, uint32_t form)
+{
+    if (!e) return fail(AVD_ERR_NULL_ARGUMENT, "engine is null");
+    return guard([&] { e->impl.set_nlm_form(form); });
+}
+
 avd_status avd_engine_timing(avd_engine *e, avd_timing *out)
 {
     if (!e || !out) return fail(AVD_ERR_NULL_ARGUMENT, "engine or out is null");

@@ -40,6 +40,20 @@ hipError_t launch_nlm_pair(const NlmWindow &w, const float *centre, const float 
 hipError_t launch_nlm_finish(const float *frame, const float *accum, const float *wsum, const float *maxw,
                              float self_weight, uint32_t pixels, int C, float *out, hipStream_t s);
 bool nlm_supported(int C, int P, int S);
+// The temporal pairs of one window, oldest first (q = -r .. -1): the forward frame at -q (the
+// older neighbour), the backward one at +q, and their block confidences (null without).
+constexpr int kNlmMaxPairs = 8;
+struct NlmFrames {
+    int pairs;
+    const float *fwd[kNlmMaxPairs], *bwd[kNlmMaxPairs];
+    const float *conf_f[kNlmMaxPairs], *conf_b[kNlmMaxPairs];
+};
+// The batched form: every pair, the single window and the finish of one frame in one launch, the
+// window sums kept on chip; bit-identical to launch_nlm_pair (each pair) + launch_nlm_single +
+// launch_nlm_finish on zeroed accumulators.  `variant` picks 8 (0) or 4 (1) rows per wave.
+constexpr int kNlmBatchedVariants = 2;
+hipError_t launch_nlm_window(const NlmWindow &w, const NlmFrames &f, const float *centre, const float *lut,
+                             float self_weight, float *out, hipStream_t s, int variant = 0);
 
 // ---- motion.hip -------------------------------------------------------------------------
 struct MotionGeom {

@@ -590,6 +590,24 @@ void NlmPass::tune_windows()
         [&](int v) { return launch_nlm_single(w, f, lut_.as<float>(), accum_.as<float>(), wsum_.as<float>(),
                                               maxw_.as<float>(), stream_, v); },
         clear, read);
+    {
+        NlmFrames fr{};
+        fr.pairs = (int)r_;
+        for (uint32_t k = 0; k < r_; ++k) {
+            fr.fwd[k] = fr.bwd[k] = f;
+            fr.conf_f[k] = fr.conf_b[k] = confidence_ ? conf_.as<float>() : nullptr;
+        }
+        float *out = outputs_[0].as<float>();
+        variant_window_ = Tuner::instance().choose(
+            "nlm_window", key + " r" + std::to_string(r_) + (confidence_ ? " confidence" : ""), kNlmBatchedVariants,
+            stream_, [&](int v) { return launch_nlm_window(w, fr, f, lut_.as<float>(), self_weight_, out, stream_, v); },
+            [&]() -> hipError_t { return launch_fill_u32(out, 0, px * C_, stream_); },
+            [&]() {
+                std::vector<uint8_t> o(px * C_ * sizeof(float));
+                check_hip(hipMemcpy(o.data(), out, o.size(), hipMemcpyDeviceToHost), "tuning read");
+                return o;
+            });
+    }
     if (r_ > 0) {
         const float *c = confidence_ ? conf_.as<float>() : nullptr;
         variant_pair_ = Tuner::instance().choose(
@@ -669,9 +687,6 @@ void NlmPass::denoise()
             nb[k + r_] = dst;
         }
     }
-    check_hip(launch_fill_u32(accum_.ptr, 0, px * C, stream_), "zero");
-    check_hip(launch_fill_u32(wsum_.ptr, 0, px, stream_), "zero");
-    check_hip(launch_fill_u32(maxw_.ptr, 0, px, stream_), "zero");
     const NlmWindow w = window();
     const size_t blocks = (motion_ || conf_only_) ? (size_t)mg_.blocks_x * mg_.blocks_y : 0;
     auto conf_of = [&](int k) -> const float * {
@@ -680,6 +695,30 @@ void NlmPass::denoise()
         return conf_.as<float>() + (size_t)idx * blocks;
     };
     const uint64_t window = (uint64_t)(2 * S_ + 1) * (2 * S_ + 1);
+    if (batched_) {
+        // Pairs q = -r .. -1 (the forward frame the older neighbour), the single window and the
+        // finish in one launch.
+        NlmFrames fr{};
+        fr.pairs = (int)r_;
+        for (int qk = -(int)r_; qk < 0; ++qk) {
+            const int k = qk + (int)r_;
+            fr.fwd[k] = nb[qk + r_];
+            fr.bwd[k] = nb[-qk + r_];
+            fr.conf_f[k] = conf_of(qk);
+            fr.conf_b[k] = conf_of(-qk);
+        }
+        float *out = outputs_[next_output_].as<float>();
+        const uint64_t candidates = px * (window * 2 * r_ + window - 1);
+        timed(AVD_STAGE_NLM_WINDOW, candidates * nlm_candidate_ops(C, P_), stream_, "nlm window", [&] {
+            return launch_nlm_window(w, fr, cf, lut_.as<float>(), self_weight_, out, stream_, variant_window_);
+        });
+        ready_output_ = next_output_;
+        next_output_ = (next_output_ + 1) % 2;
+        return;
+    }
+    check_hip(launch_fill_u32(accum_.ptr, 0, px * C, stream_), "zero");
+    check_hip(launch_fill_u32(wsum_.ptr, 0, px, stream_), "zero");
+    check_hip(launch_fill_u32(maxw_.ptr, 0, px, stream_), "zero");
     for (int qk = -(int)r_; qk < 0; ++qk)
         timed(AVD_STAGE_NLM_PAIR, px * window * 2 * nlm_candidate_ops(C, P_), stream_, "nlm pair", [&] {
             return launch_nlm_pair(w, cf, nb[qk + r_], nb[-qk + r_], conf_of(qk), conf_of(-qk), accum_.as<float>(),
@@ -1108,6 +1147,7 @@ Engine::Engine(const avd_engine_spec &spec) : spec_(spec)
     }
     if (luma_) luma_->set_timer(&timer_);
     if (chroma_) chroma_->set_timer(&timer_);
+    set_nlm_form(AVD_NLM_FORM_DEFAULT);
     if (luma_ && chroma_ && luma_->max_held() != chroma_->max_held())
         throw Failure(AVD_ERR_INVALID_OPTIONS, "the luma and chroma passes read different window spans");
     // luma_motion_ring (engine/spec.rs:350-365): when both passes search motion, the chroma pass
@@ -1280,6 +1320,17 @@ uint32_t Engine::finish()
         if (n == 0) restart();
         return n;
     });
+}
+
+void Engine::set_nlm_form(uint32_t form)
+{
+    if (form > AVD_NLM_FORM_DEFAULT) throw Failure(AVD_ERR_INVALID_OPTIONS, "unknown NLMeans launch form");
+    if (spec_.kind != AVD_ENGINE_NLMEANS) return;
+    for (Pass *p : {luma_.get(), chroma_.get()}) {
+        if (!p) continue;
+        NlmPass &n = static_cast<NlmPass &>(*p);
+        n.set_batched(form == AVD_NLM_FORM_DEFAULT ? n.temporal_radius() == 0 : form == AVD_NLM_FORM_BATCHED);
+    }
 }
 
 void Engine::restart()

@@ -213,6 +213,10 @@ class NlmPass : public Pass {
     bool has_pending() const override;
     bool searches_motion() const override { return motion_; }
     uint32_t temporal_radius() const override { return r_; }
+    // The batched form (all windows of a frame and the finish in one launch) or the plain one
+    // (one launch per pair, the single window and the finish); the outputs are bit-identical.
+    // By default a spatial pass (one window) is batched and a temporal one plain.
+    void set_batched(bool on) { batched_ = on; }
     const avd_motion_search &motion_search() const override { return motion_opts_; }
     const MotionGeom &motion_geom() const override { return mg_; }
     void set_motion_share(LumaMotionShare *share, bool reads) override;
@@ -239,6 +243,8 @@ class NlmPass : public Pass {
     bool ready_ = false;
     int ready_output_ = 0, next_output_ = 0;
     int variant_single_ = 0, variant_pair_ = 0;  // the tuner's rows per wave of the window sites
+    int variant_window_ = 0;                     // and of the batched form's window
+    bool batched_ = false;                       // every window of a frame in one launch
     uint32_t tail_remaining_ = 0;
 };
 
@@ -314,6 +320,9 @@ class Engine {
     uint64_t tile_overflows_for_test() const;
     void sample_device_time(uint32_t every);
     avd_timing timing();
+    // NLMeans engines: the plain (0), batched (1) or default (2) launch form (NL4D engines have one
+    // form).
+    void set_nlm_form(uint32_t form);
 
   private:
     Pass &lead();

@@ -51,57 +51,21 @@ __device__ void stage_tile(float *tile, const float *__restrict__ frame, int W, 
     }
 }
 
-template <int C, int P, int WV, int ROWS, bool PAIR, bool CONF>
-__global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__restrict__ centre,
-                                                     const float *__restrict__ fwd, const float *__restrict__ bwd,
-                                                     const float *__restrict__ lut, const float *__restrict__ conf_f,
-                                                     const float *__restrict__ conf_b, float *__restrict__ accum,
-                                                     float *__restrict__ wsum, float *__restrict__ maxw)
+// The offset loop of one window for this lane's ROWS output rows, into acc / ws / mw (zeroed
+// here): every offset q (q != 0 for a single window) against the staged tile `tf` (the forward
+// frame of a pair, or the frame itself) and, for a pair, `tb` (the backward frame at -q).  `cen`
+// holds the lane's centre column of ROWS + 2P distance rows; `tcols` is the tiles' row pitch.
+template <int C, int P, int ROWS, bool PAIR, bool CONF>
+__device__ __forceinline__ void window_sums(const NlmWindow &w, const float *tf, const float *tb, int tcols, int wave,
+                                            int lane, const float (&cen)[ROWS + 2 * P][C],
+                                            const float *__restrict__ lut, const float (&conf_fv)[ROWS],
+                                            const float (&conf_bv)[ROWS], float (&acc)[ROWS][C], float (&ws)[ROWS],
+                                            float (&mw)[ROWS])
 {
-    extern __shared__ float lds[];
-    constexpr int DR = ROWS + 2 * P;  // distance rows per wave
-    constexpr int OW = 64 - 2 * P;     // output columns per wave
-    const int S = w.S, W = w.W, H = w.H;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int X0 = blockIdx.x * OW;
-    const int Y0 = blockIdx.y * (WV * ROWS);
-
-    // Neighbour tiles: columns X0-P-S .. X0-P+63+S, rows Y0-P-S .. Y0+WV*ROWS+P-1+S.
-    const int tcols = 64 + 2 * S, trows = WV * ROWS + 2 * P + 2 * S;
-    const int tx0 = X0 - P - S, ty0 = Y0 - P - S;
-    float *tf = lds;
-    float *tb = lds + (size_t)tcols * trows * C;
-    stage_tile<C, WV>(tf, fwd, W, H, tx0, ty0, tcols, trows);
-    if (PAIR) stage_tile<C, WV>(tb, bwd, W, H, tx0, ty0, tcols, trows);
-
-    // Centre values of this lane's distance column, one per distance row, in registers.
-    const int xd = X0 - P + lane;
-    const int ybase = Y0 + wave * ROWS - P;
-    float cen[DR][C];
-#pragma unroll
-    for (int r = 0; r < DR; ++r) {
-        const float *src = centre + ((size_t)clampi(ybase + r, H - 1) * W + clampi(xd, W - 1)) * C;
-#pragma unroll
-        for (int c = 0; c < C; ++c) cen[r][c] = src[c];
-    }
-    __syncthreads();
-
+    constexpr int DR = ROWS + 2 * P;
+    const int S = w.S;
     const float scale = (C == 1) ? 3.0f : 1.5f;
     const float h2l = w.h2_log2e;
-    const int ox = X0 - 2 * P + lane;  // output column of this lane (valid for lane >= 2P)
-    float conf_fv[ROWS], conf_bv[ROWS];
-    if (CONF) {
-        const int bcol = min(clampi(ox, W - 1) / w.step, w.blocks_x - 1);
-#pragma unroll
-        for (int j = 0; j < ROWS; ++j) {
-            const int oy = clampi(Y0 + wave * ROWS + j, H - 1);
-            const int bidx = min(oy / w.step, w.blocks_y - 1) * w.blocks_x + bcol;
-            conf_fv[j] = conf_f[bidx];
-            conf_bv[j] = conf_b[bidx];
-        }
-    }
-
-    float acc[ROWS][C], ws[ROWS], mw[ROWS];
 #pragma unroll
     for (int j = 0; j < ROWS; ++j) {
 #pragma unroll
@@ -109,7 +73,6 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
         ws[j] = 0.0f;
         mw[j] = 0.0f;
     }
-
     const int side = 2 * S + 1;
     // Tile coordinates of this lane's distance position at offset 0.
     const int trow0 = wave * ROWS + S;  // + r + qy
@@ -180,6 +143,68 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
             }
         }
     }
+}
+
+// The centre values of this lane's distance column, one per distance row, and the confidence of
+// the blocks its output rows fall in.
+template <int C, int P, int ROWS>
+__device__ __forceinline__ void load_centre(const NlmWindow &w, const float *__restrict__ centre, int X0, int Y0,
+                                            int wave, int lane, float (&cen)[ROWS + 2 * P][C])
+{
+    const int xd = X0 - P + lane;
+    const int ybase = Y0 + wave * ROWS - P;
+#pragma unroll
+    for (int r = 0; r < ROWS + 2 * P; ++r) {
+        const float *src = centre + ((size_t)clampi(ybase + r, w.H - 1) * w.W + clampi(xd, w.W - 1)) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) cen[r][c] = src[c];
+    }
+}
+
+template <int ROWS>
+__device__ __forceinline__ void load_conf(const NlmWindow &w, const float *__restrict__ conf, int ox, int Y0, int wave,
+                                          float (&out)[ROWS])
+{
+    const int bcol = min(clampi(ox, w.W - 1) / w.step, w.blocks_x - 1);
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) {
+        const int oy = clampi(Y0 + wave * ROWS + j, w.H - 1);
+        out[j] = conf[min(oy / w.step, w.blocks_y - 1) * w.blocks_x + bcol];
+    }
+}
+
+template <int C, int P, int WV, int ROWS, bool PAIR, bool CONF>
+__global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__restrict__ centre,
+                                                     const float *__restrict__ fwd, const float *__restrict__ bwd,
+                                                     const float *__restrict__ lut, const float *__restrict__ conf_f,
+                                                     const float *__restrict__ conf_b, float *__restrict__ accum,
+                                                     float *__restrict__ wsum, float *__restrict__ maxw)
+{
+    extern __shared__ float lds[];
+    const int S = w.S, W = w.W, H = w.H;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int X0 = blockIdx.x * (64 - 2 * P);
+    const int Y0 = blockIdx.y * (WV * ROWS);
+
+    // Neighbour tiles: columns X0-P-S .. X0-P+63+S, rows Y0-P-S .. Y0+WV*ROWS+P-1+S.
+    const int tcols = 64 + 2 * S, trows = WV * ROWS + 2 * P + 2 * S;
+    const int tx0 = X0 - P - S, ty0 = Y0 - P - S;
+    float *tf = lds;
+    float *tb = lds + (size_t)tcols * trows * C;
+    stage_tile<C, WV>(tf, fwd, W, H, tx0, ty0, tcols, trows);
+    if (PAIR) stage_tile<C, WV>(tb, bwd, W, H, tx0, ty0, tcols, trows);
+    float cen[ROWS + 2 * P][C];
+    load_centre<C, P, ROWS>(w, centre, X0, Y0, wave, lane, cen);
+    __syncthreads();
+
+    const int ox = X0 - 2 * P + lane;  // output column of this lane (valid for lane >= 2P)
+    float conf_fv[ROWS], conf_bv[ROWS];
+    if (CONF) {
+        load_conf<ROWS>(w, conf_f, ox, Y0, wave, conf_fv);
+        load_conf<ROWS>(w, conf_b, ox, Y0, wave, conf_bv);
+    }
+    float acc[ROWS][C], ws[ROWS], mw[ROWS];
+    window_sums<C, P, ROWS, PAIR, CONF>(w, tf, tb, tcols, wave, lane, cen, lut, conf_fv, conf_bv, acc, ws, mw);
 
     if (lane < 2 * P || ox >= W) return;
 #pragma unroll
@@ -191,6 +216,88 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
         for (int c = 0; c < C; ++c) accum[p * C + c] += acc[j][c];
         wsum[p] += ws[j];
         maxw[p] = fmaxf(maxw[p], mw[j]);
+    }
+}
+
+// The batched form: one launch per frame and plane group walks every temporal pair (oldest first),
+// then the single window, then the finish, with the window sums of each tile in registers.  Each
+// window's sums start from zero and join the running totals in the plain form's order (the plain
+// form adds every launch's sums to accumulators that start at zero), so the output is bit-identical
+// to the plain launches' while the accumulators never reach memory.  The pair tiles are re-staged
+// per pair; the single window stages the centre frame into the first of them.
+template <int C, int P, int WV, int ROWS, bool CONF>
+__global__ __launch_bounds__(WV * 64) void k_nlm_window(NlmWindow w, NlmFrames f, const float *__restrict__ centre,
+                                                        const float *__restrict__ lut, float self_weight,
+                                                        float *__restrict__ out)
+{
+    extern __shared__ float lds[];
+    const int S = w.S, W = w.W, H = w.H;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int X0 = blockIdx.x * (64 - 2 * P);
+    const int Y0 = blockIdx.y * (WV * ROWS);
+    const int tcols = 64 + 2 * S, trows = WV * ROWS + 2 * P + 2 * S;
+    const int tx0 = X0 - P - S, ty0 = Y0 - P - S;
+    float *tf = lds;
+    float *tb = lds + (size_t)tcols * trows * C;
+    float cen[ROWS + 2 * P][C];
+    load_centre<C, P, ROWS>(w, centre, X0, Y0, wave, lane, cen);
+    const int ox = X0 - 2 * P + lane;
+
+    float tacc[ROWS][C], tws[ROWS], tmw[ROWS];
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) tacc[j][c] = 0.0f;
+        tws[j] = 0.0f;
+        tmw[j] = 0.0f;
+    }
+    float acc[ROWS][C], ws[ROWS], mw[ROWS];
+    float conf_fv[ROWS], conf_bv[ROWS];
+    for (int k = 0; k < f.pairs; ++k) {
+        if (k > 0) __syncthreads();  // every wave is done with the previous pair's tiles
+        stage_tile<C, WV>(tf, f.fwd[k], W, H, tx0, ty0, tcols, trows);
+        stage_tile<C, WV>(tb, f.bwd[k], W, H, tx0, ty0, tcols, trows);
+        __syncthreads();
+        if (CONF) {
+            load_conf<ROWS>(w, f.conf_f[k], ox, Y0, wave, conf_fv);
+            load_conf<ROWS>(w, f.conf_b[k], ox, Y0, wave, conf_bv);
+        }
+        window_sums<C, P, ROWS, true, CONF>(w, tf, tb, tcols, wave, lane, cen, nullptr, conf_fv, conf_bv, acc, ws, mw);
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) tacc[j][c] = tacc[j][c] + acc[j][c];
+            tws[j] = tws[j] + ws[j];
+            tmw[j] = fmaxf(tmw[j], mw[j]);
+        }
+    }
+    if (f.pairs > 0) __syncthreads();
+    stage_tile<C, WV>(tf, centre, W, H, tx0, ty0, tcols, trows);
+    __syncthreads();
+    window_sums<C, P, ROWS, false, false>(w, tf, tf, tcols, wave, lane, cen, lut, conf_fv, conf_bv, acc, ws, mw);
+
+    if (lane < 2 * P || ox >= W) return;
+    // The finish (accumulate.rs:54-101), as k_finish computes it.
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) {
+        const int oy = Y0 + wave * ROWS + j;
+        if (oy >= H) break;
+        float a[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) a[c] = tacc[j][c] + acc[j][c];
+        const float wsum = tws[j] + ws[j];
+        const float maxw = fmaxf(tmw[j], mw[j]);
+        const size_t p = (size_t)oy * W + ox;
+        const float sw = self_weight * maxw;
+        const float den = sw + wsum;
+        if (den > 1e-30f) {
+            const float inv = 1.0f / den;
+#pragma unroll
+            for (int c = 0; c < C; ++c) out[p * C + c] = (centre[p * C + c] * sw + a[c]) * inv;
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; ++c) out[p * C + c] = centre[p * C + c];
+        }
     }
 }
 
@@ -291,6 +398,44 @@ hipError_t dispatch_p(const NlmWindow &w, const float *centre, const float *fwd,
     }
 }
 
+template <int C, int P, int ROWS>
+hipError_t dispatch_window(const NlmWindow &w, const NlmFrames &f, const float *centre, const float *lut,
+                           float self_weight, float *out, hipStream_t s)
+{
+    constexpr int WV = waves_for(C, true);
+    const size_t lds = tile_bytes(C, P, w.S, true, ROWS);
+    const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WV * ROWS));
+    const bool conf = f.pairs > 0 && f.conf_f[0] != nullptr;
+    if (conf)
+        k_nlm_window<C, P, WV, ROWS, true><<<grid, WV * 64, lds, s>>>(w, f, centre, lut, self_weight, out);
+    else
+        k_nlm_window<C, P, WV, ROWS, false><<<grid, WV * 64, lds, s>>>(w, f, centre, lut, self_weight, out);
+    return hipGetLastError();
+}
+
+template <int C>
+hipError_t dispatch_window_p(const NlmWindow &w, const NlmFrames &f, const float *centre, const float *lut,
+                             float self_weight, float *out, int variant, hipStream_t s)
+{
+#define AVD_WINDOW_P(PP)                                                                                      \
+    case PP:                                                                                                  \
+        return variant == 1 ? dispatch_window<C, PP, 4>(w, f, centre, lut, self_weight, out, s)              \
+                            : dispatch_window<C, PP, 8>(w, f, centre, lut, self_weight, out, s);
+    switch (w.P) {
+        AVD_WINDOW_P(0)
+        AVD_WINDOW_P(1)
+        AVD_WINDOW_P(2)
+        AVD_WINDOW_P(3)
+        AVD_WINDOW_P(4)
+        AVD_WINDOW_P(5)
+        AVD_WINDOW_P(6)
+        AVD_WINDOW_P(7)
+        AVD_WINDOW_P(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef AVD_WINDOW_P
+}
+
 }  // namespace
 
 bool nlm_supported(int C, int P, int S)
@@ -319,6 +464,19 @@ hipError_t launch_nlm_pair(const NlmWindow &w, const float *centre, const float 
     if (w.C == 1)
         return dispatch_p<1>(w, centre, fwd, bwd, nullptr, conf_f, conf_b, accum, wsum, maxw, true, conf, variant, s);
     return dispatch_p<2>(w, centre, fwd, bwd, nullptr, conf_f, conf_b, accum, wsum, maxw, true, conf, variant, s);
+}
+
+hipError_t launch_nlm_window(const NlmWindow &w, const NlmFrames &f, const float *centre, const float *lut,
+                             float self_weight, float *out, hipStream_t s, int variant)
+{
+    if (!nlm_supported(w.C, w.P, w.S) || variant < 0 || variant >= kNlmBatchedVariants || f.pairs < 0 ||
+        f.pairs > kNlmMaxPairs || !centre || !lut || !out)
+        return hipErrorInvalidValue;
+    for (int k = 0; k < f.pairs; ++k)
+        if (!f.fwd[k] || !f.bwd[k] || (f.conf_f[0] != nullptr) != (f.conf_f[k] != nullptr && f.conf_b[k] != nullptr))
+            return hipErrorInvalidValue;
+    if (w.C == 1) return dispatch_window_p<1>(w, f, centre, lut, self_weight, out, variant, s);
+    return dispatch_window_p<2>(w, f, centre, lut, self_weight, out, variant, s);
 }
 
 hipError_t launch_nlm_finish(const float *frame, const float *accum, const float *wsum, const float *maxw,

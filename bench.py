@@ -248,8 +248,6 @@ def nlm_leg(avd, torch, name, cfg, sample_steps=4):
     geom = avd.Geometry(W_, H_, layout, fmt, fmt, 0)
     opts = dict(hq=0, temporal=1 if r else 0, temporal_radius=r, patch_radius=P, search_radius=S, strength=1.2,
                 self_weight=1.0)
-    eng = avd.build_engine("nlmeans", geom, luma=avd.nlmeans_options(**opts),
-                           chroma=avd.nlmeans_options(**opts) if layout else None)
     top = (1 << depth) - 1
     clip = avd.synth.clip(cfg["frames"], W_, H_, depth, 0xA5D00000 + (1 if r == 0 else 2), chroma=chroma,
                           sigma_luma=cfg["sigma"][0], sigma_chroma=cfg["sigma"][1], device="cuda",
@@ -258,36 +256,54 @@ def nlm_leg(avd, torch, name, cfg, sample_steps=4):
     dims = avd.plane_dims(geom)
     outs = [torch.zeros(avd.plane_bytes(fmt, w, h), dtype=torch.uint8, device="cuda") for (w, h) in dims]
     warm = max(2 * r + 1, 4)
-    run_steps(eng, frames, outs, 0, warm)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    emitted = run_steps(eng, frames, outs, warm, cfg["steps"])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    assert emitted == cfg["steps"], (emitted, cfg["steps"])
-    eng.sample_device_time(1)
-    run_steps(eng, frames, outs, warm + cfg["steps"], sample_steps)
-    st = eng.timing()
-    eng.sample_device_time(0)
-    window_ms = (st["nlm_pair"]["ms"] + st["nlm_single"]["ms"]) / sample_steps
-    flops = (st["nlm_pair"]["work"] + st["nlm_single"]["work"]) / sample_steps
     n_q = (2 * S + 1) ** 2
     cands = n_q * (2 * r + 1) - 1
     exps = sum(w * h * cands for (w, h) in dims[:1]) + (dims[1][0] * dims[1][1] * cands if layout else 0)
-    tflops = flops / (window_ms * 1e-3) / 1e12
-    fps = cfg["steps"] / elapsed
+    forms = {}
+    for form in ("plain", "batched"):
+        eng = avd.build_engine("nlmeans", geom, luma=avd.nlmeans_options(**opts),
+                               chroma=avd.nlmeans_options(**opts) if layout else None)
+        eng.set_nlm_form(form)
+        run_steps(eng, frames, outs, 0, warm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        emitted = run_steps(eng, frames, outs, warm, cfg["steps"])
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        assert emitted == cfg["steps"], (emitted, cfg["steps"])
+        eng.sample_device_time(1)
+        run_steps(eng, frames, outs, warm + cfg["steps"], sample_steps)
+        st = eng.timing()
+        eng.sample_device_time(0)
+        window = ("nlm_window",) if form == "batched" else ("nlm_pair", "nlm_single")
+        window_ms = sum(st[k]["ms"] for k in window) / sample_steps
+        flops = sum(st[k]["work"] for k in window) / sample_steps
+        tflops = flops / (window_ms * 1e-3) / 1e12
+        fps = cfg["steps"] / elapsed
+        forms[form] = {
+            "frames_per_s": round(fps, 2), "mpix_per_s": round(fps * W_ * H_ / 1e6, 1),
+            "ms_per_frame": round(1000.0 / fps, 3), "window_ms_per_frame": round(window_ms, 4),
+            "flops_per_frame": int(flops),
+            "roofline": {"kernel": "k_nlm_window (every window and the finish)" if form == "batched"
+                         else "k_nlm (pair + single windows)", "bound": "valu", "achieved": round(tflops, 3),
+                         "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s", "frac": round(tflops / VALU_PEAK_TOPS, 4)},
+            "exp_per_s": round(exps / (window_ms * 1e-3), 1),
+            "stages_ms_per_frame": {k: round(v["ms"] / sample_steps, 4) for k, v in st.items() if v["launches"]},
+        }
+    # The engine's default form: batched for a spatial pass, plain for a temporal one.
+    b = forms["batched" if r == 0 else "plain"]
     return {
         "workload": f"NLMeans {'temporal radius %d' % r if r else 'spatial'}, {W_}x{H_} {depth}-bit "
-                    f"{'luma' if layout == 0 else '4:2:0'}, patch radius {P}, search radius {S}, FP32; "
-                    "plain form (one launch per stage and plane group; the batched form is not built)",
-        "frames_per_s": round(fps, 2), "mpix_per_s": round(fps * W_ * H_ / 1e6, 1),
-        "ms_per_frame": round(1000.0 / fps, 3),
-        "window_ms_per_frame": round(window_ms, 4), "flops_per_frame": int(flops),
-        "roofline": {"kernel": "k_nlm (pair + single windows)", "bound": "valu", "achieved": round(tflops, 3),
-                     "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s", "frac": round(tflops / VALU_PEAK_TOPS, 4)},
-        "exp_per_frame": int(exps), "exp_per_s": round(exps / (window_ms * 1e-3), 1),
-        "stages_ms_per_frame": {k: round(v["ms"] / sample_steps, 4) for k, v in st.items() if v["launches"]},
-        "tuned": [e for e in avd.tuning_entries() if e["key"].endswith(f"{W_}x{H_}") or f"{W_}x{H_} " in e["key"]],
+                    f"{'luma' if layout == 0 else '4:2:0'}, patch radius {P}, search radius {S}, FP32; plain form "
+                    "(one launch per pair, single window and finish, per plane group) and batched form (every "
+                    "window and the finish of a plane group in one launch), bit-identical outputs; headline figures "
+                    "are the engine's default form (batched for spatial, plain for temporal passes)",
+        "frames_per_s": b["frames_per_s"], "mpix_per_s": b["mpix_per_s"], "ms_per_frame": b["ms_per_frame"],
+        "roofline": b["roofline"], "exp_per_frame": int(exps),
+        "batched_vs_plain": round(forms["batched"]["frames_per_s"] / forms["plain"]["frames_per_s"], 3),
+        "forms": forms,
+        "tuned": [e for e in avd.tuning_entries() if e["key"].endswith(f"{W_}x{H_}") or f"{W_}x{H_} " in e["key"]
+                  or f"{W_}x{H_} r" in e["key"]],
     }
 
 

@@ -231,9 +231,9 @@ avd_status avd_engine_tile_overflows_for_test(const avd_engine *engine, uint64_t
  * block offset (subtract, absolute value, add) on both pyramid levels; AVD_STAGE_COLLAB_FILTER per
  * reference 8 members x (C x (4096 + 576 + 256) + 64 x (C + 1)) (forward and inverse 2-D DCT as
  * 8x8x8 products, three Haar levels of 3 operations, 4 per threshold, one fixed-point add per
- * accumulator); AVD_STAGE_NLM_PAIR and AVD_STAGE_NLM_SINGLE (5C + 2(2P + 1) + 7) per candidate
- * pixel (squared differences, channel scale, separable box sum, weight and accumulation);
- * pixels for the rest. */
+ * accumulator); AVD_STAGE_NLM_PAIR, AVD_STAGE_NLM_SINGLE and AVD_STAGE_NLM_WINDOW (5C + 2(2P + 1)
+ * + 7) per candidate pixel (squared differences, channel scale, separable box sum, weight and
+ * accumulation); pixels for the rest. */
 enum {
     AVD_STAGE_INGEST = 0,
     AVD_STAGE_MOTION = 1,
@@ -246,7 +246,8 @@ enum {
     AVD_STAGE_NLM_FINISH = 8,
     AVD_STAGE_WARP = 9,
     AVD_STAGE_COLLAB_CENTRE = 10,
-    AVD_STAGE_COUNT = 11
+    AVD_STAGE_NLM_WINDOW = 11,  /* the batched form's one launch per frame: every window and the finish */
+    AVD_STAGE_COUNT = 12
 };
 
 typedef struct avd_timing {
@@ -259,6 +260,16 @@ typedef struct avd_timing {
 avd_status avd_engine_sample_device_time(avd_engine *engine, uint32_t every);
 /* Waits for the sampled launches and returns the totals since sampling began (Engine::timing). */
 avd_status avd_engine_timing(avd_engine *engine, avd_timing *out);
+
+/* No counterpart in the reference, whose NLMeans launches once per temporal pair, once for the
+ * single window and once for the finish (nlmeans/dispatch/mod.rs:665-679): an NLMeans engine
+ * runs each frame's windows that way (AVD_NLM_FORM_PLAIN) or batched (AVD_NLM_FORM_BATCHED:
+ * every pair, the single window and the finish of a plane group in one launch, the window sums
+ * kept on chip).  The outputs are bit-identical.  AVD_NLM_FORM_DEFAULT, the form an engine starts
+ * in, batches spatial passes and keeps temporal ones plain (the faster of the two on an MI355X at
+ * 640x360 spatial and 1920x1080 temporal).  NL4D engines have one form and ignore the call. */
+enum { AVD_NLM_FORM_PLAIN = 0, AVD_NLM_FORM_BATCHED = 1, AVD_NLM_FORM_DEFAULT = 2 };
+avd_status avd_engine_set_nlm_form(avd_engine *engine, uint32_t form);
 
 /* ---- host wire API (HostDenoiser, av-denoise/src/host/mod.rs:27-345) ------------------------ */
 

@@ -36,5 +36,6 @@ def test_tuned_nlm_windows_match_the_baseline(chroma, r, P, S):
     assert entries, avd.tuning_entries()
     for e in entries:
         print(e)
-        assert len(e["ms"]) == 3 and all(t >= 0 for t in e["ms"]), e  # every candidate matched the baseline
+        # every candidate (16, 8 or 4 rows per wave; 8 or 4 for the batched window) matched the baseline
+        assert len(e["ms"]) == (2 if e["site"] == "nlm_window" else 3) and all(t >= 0 for t in e["ms"]), e
         assert e["ms"][e["choice"]] == min(e["ms"])
