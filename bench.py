@@ -57,6 +57,8 @@ def parse():
                     help="transform form(s) to run; the headline value is the mixed form's when it runs")
     ap.add_argument("--no-quality", action="store_true", help="skip the PSNR comparison against the clean scene")
     ap.add_argument("--no-nlm", action="store_true", help="skip the NLMeans workloads")
+    ap.add_argument("--nlm-only", action="store_true",
+                    help="run only the NLMeans workloads and print their JSON (for kernel traces of them)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, one plain device copy of this many MiB (a known-size kernel that "
                          "hardware-counter runs calibrate FETCH_SIZE / WRITE_SIZE against)")
@@ -317,6 +319,9 @@ def main():
     import avd
 
     torch.cuda.set_device(0)
+    if args.nlm_only:
+        print(json.dumps({"nlm": {name: nlm_leg(avd, torch, name, cfg) for name, cfg in NLM_CONFIGS.items()}}))
+        return
     clip, frames = gpu_clip(avd, torch)
     warm = max(args.warmup, 2 * RADIUS + 1)
 
