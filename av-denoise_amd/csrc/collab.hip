@@ -30,6 +30,7 @@
 #include "common.h"
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 namespace avd {
 
@@ -2315,38 +2316,49 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
         }
         __syncthreads();
         const uint32_t batch = min(256u, n - base);
-        // kPer members per wave per step: every staged load is issued before the first add.
+        // kPer members per wave per step: every staged load is issued before the first add (the
+        // loads are unconditional, a lane past the batch reading the first member's row again, so
+        // none waits behind a branch).
         constexpr int kPer = 8;
         for (uint32_t e0 = wave; e0 < batch; e0 += 4 * kPer) {
             int32_t a[kPer], a2[kPer], wq[kPer];
             int cell[kPer];
+            float sw[kPer];
+            using Staged = typename std::conditional<MIXED, _Float16, int32_t>::type;
+            Staged v0[kPer], v1[kPer];
 #pragma unroll
             for (int u = 0; u < kPer; ++u) {
                 const uint32_t e = e0 + 4 * u;
-                cell[u] = -1;
-                if (e < batch) {
-                    const uint32_t id = s_id[e], pos = s_pos[e];
-                    const int lx = (int)(pos & 0x1FFF) + q - x0, ly = (int)((pos >> 13) & 0x1FFF) + r - y0;
-                    if (lx >= 0 && lx < T && ly >= 0 && ly < T) {
-                        cell[u] = ly * kRow + lx;
-                        if (MIXED) {
-                            // The FP32 form's weight, clamp and rounding (half away from zero, as
-                            // f32::round): to_fixed(v w) = round(clamp(v w scale, +-5 scale)), and
-                            // the weight term round(min(w scale, scale) 5), w = sw kaiser[r] kaiser[q].
-                            const _Float16 *st =
-                                reinterpret_cast<const _Float16 *>(tiles.stage) + (size_t)id * C * 64 + lane;
-                            const float ws = s_sw[e] * kk_scale;
-                            const float lim = 5.0f * scale;
-                            a[u] = (int32_t)roundf(fminf(fmaxf((float)st[0] * ws, -lim), lim));
-                            if (C == 2) a2[u] = (int32_t)roundf(fminf(fmaxf((float)st[64] * ws, -lim), lim));
-                            wq[u] = (int32_t)roundf(fminf(ws, scale) * 5.0f);
-                        } else {
-                            const int32_t *st = tiles.stage + (size_t)id * C * 64 + lane;
-                            a[u] = st[0];
-                            if (C == 2) a2[u] = st[64];
-                            wq[u] = to_fixed_weight(s_sw[e] * kr_kq, scale);
-                        }
+                const uint32_t ec = e < batch ? e : e0;
+                const uint32_t id = s_id[ec], pos = s_pos[ec];
+                sw[u] = s_sw[ec];
+                const int lx = (int)(pos & 0x1FFF) + q - x0, ly = (int)((pos >> 13) & 0x1FFF) + r - y0;
+                cell[u] = (e < batch && lx >= 0 && lx < T && ly >= 0 && ly < T) ? ly * kRow + lx : -1;
+                const Staged *st = reinterpret_cast<const Staged *>(tiles.stage) + (size_t)id * C * 64 + lane;
+                v0[u] = st[0];
+                if (C == 2) v1[u] = st[64];
+            }
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                if (MIXED) {
+                    // The FP32 form's weight and clamp, to_fixed(v w) = round(clamp(v w scale,
+                    // +-5 scale)) and the weight term round(min(w scale, scale) 5), w = sw kaiser[r]
+                    // kaiser[q], rounded half away from zero by adding +-0.5 and truncating (which
+                    // differs from f32::round only for values within an ulp below a half, one unit in
+                    // ~2^20; this form is not bit-exact anyway).
+                    const float ws = sw[u] * kk_scale;
+                    const float lim = 5.0f * scale;
+                    const float y = __builtin_amdgcn_fmed3f((float)v0[u] * ws, -lim, lim);
+                    a[u] = (int32_t)(y + __builtin_copysignf(0.5f, y));
+                    if (C == 2) {
+                        const float y2 = __builtin_amdgcn_fmed3f((float)v1[u] * ws, -lim, lim);
+                        a2[u] = (int32_t)(y2 + __builtin_copysignf(0.5f, y2));
                     }
+                    wq[u] = (int32_t)(fminf(ws, scale) * 5.0f + 0.5f);
+                } else {
+                    a[u] = (int32_t)v0[u];
+                    if (C == 2) a2[u] = (int32_t)v1[u];
+                    wq[u] = to_fixed_weight(sw[u] * kr_kq, scale);
                 }
             }
 #pragma unroll
