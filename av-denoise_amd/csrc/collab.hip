@@ -483,69 +483,82 @@ __device__ __forceinline__ void slide_fixed(const uint16_t *__restrict__ ring, s
     }
 }
 
-// slide_fixed for the trajectory search, where every lane starts a new column: candidates
-// (x, y0 + j), j < kk <= K, of a frame with H >= K + 7.  The K + 7 rows are read from y0s =
-// min(y0, H - K - 7), so none passes the frame (a rectangle clipped at the bottom slides from
-// higher up and skips its first y0 - y0s candidates).  Rows are read as raw buffer loads from the
-// ring (`ring_bytes` long): a scalar resource and one 32-bit byte offset per row; one-channel rows
-// start on the even code at or below the row's first and are shifted into place by the lane's
-// fixed byte shift of that row parity.
+// slide_fixed for the trajectory search, over the two adjacent columns x0 (even) and x0 + 1 at
+// once: candidates (x0 + c, y0 + j), j < kk <= K, of a frame with H >= K + 7, against patch P.
+// One set of row loads serves both columns: a row of the pair spans the codes of
+// nine pixels, from the first pixel of column x0, which is dword-aligned (one-channel frames of
+// even width, every ring slot and row then starting on an even code).  The K + 7 rows are read
+// from y0s = min(y0, H - K - 7), so none passes the frame (a rectangle clipped at the bottom slides
+// from higher up and skips its first y0 - y0s candidates).  Rows are read as raw buffer loads from
+// the ring (`ring_bytes` long): a scalar resource and one 32-bit byte offset per row; emit(c, j, ok,
+// ssd) runs for every row, `ok` false for a j outside [0, kk).  The ninth
+// pixel is read alone (one code, or one dword of two): past the last column of a row it is the next
+// row's first, and past the ring's end the load returns 0; column x0 + 1 reads it only when it lies
+// on the frame.
 template <int C, int K, typename Emit>
-__device__ __forceinline__ void slide_traj(const uint16_t *__restrict__ ring, uint32_t ring_bytes, size_t frame, int W,
-                                           int H, int x, int y0, int kk, const uint32_t (&P)[8][4 * C], uint32_t A2,
+__device__ __forceinline__ void slide_pair(const uint16_t *__restrict__ ring, uint32_t ring_bytes, size_t frame, int W,
+                                           int H, int x0, int y0, int kk, const uint32_t (&P)[8][4 * C], uint32_t A2,
                                            Emit &&emit)
 {
     const int y0s = min(y0, H - K - 7);
     const int delta = y0 - y0s;
-    const size_t a0 = frame + ((size_t)y0s * W + x) * C;  // first code of row 0
+    const size_t a0 = frame + ((size_t)y0s * W + x0) * C;  // first code of row 0, even
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(ring), (short)0, (int)ring_bytes, 0x00020000);
-    uint32_t acc[8], energy[8];
+    uint32_t acc0[8], acc1[8], en0[8], en1[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) acc[i] = 0;
-    uint32_t window = 0;
-    const uint32_t sh_e = C == 1 ? (uint32_t)(a0 & 1) : 0u, sh_o = C == 1 ? (uint32_t)((a0 + W) & 1) : 0u;
-    const uint32_t be = (uint32_t)((a0 - sh_e) * 2), bo = (uint32_t)((a0 + (size_t)W * C - sh_o) * 2);
-    const uint32_t pitch2 = (uint32_t)(4 * W * C);  // bytes per two rows
+    for (int i = 0; i < 8; ++i) acc0[i] = acc1[i] = 0;
+    uint32_t win0 = 0, win1 = 0;
+    const uint32_t base = (uint32_t)(a0 * 2), pitch = (uint32_t)(2 * W * C);
 #pragma unroll
     for (int t = 0; t < K + 7; ++t) {
-        uint32_t row[4 * C];
-        const bool odd = (t & 1) != 0;
-        const uint32_t off = (odd ? bo : be) + (uint32_t)(t >> 1) * pitch2;
+        const uint32_t off = base + (uint32_t)t * pitch;
+        uint32_t ra[4 * C], rb[4 * C];
         const int4_t v0 = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)off, 0, 0);
         if (C == 1) {
-            // The row's eighth code alone (a dword holding it can pass the ring's end, and a buffer
-            // load drops a dword that is not wholly inside).
             const uint32_t d4 = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc, (int)off + 16, 0, 0);
-            const uint32_t sh = (odd ? sh_o : sh_e) * 2u;
-            row[0] = __builtin_amdgcn_alignbyte((uint32_t)v0[1], (uint32_t)v0[0], sh);
-            row[1] = __builtin_amdgcn_alignbyte((uint32_t)v0[2], (uint32_t)v0[1], sh);
-            row[2] = __builtin_amdgcn_alignbyte((uint32_t)v0[3], (uint32_t)v0[2], sh);
-            row[3] = __builtin_amdgcn_alignbyte(d4, (uint32_t)v0[3], sh);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) ra[i] = (uint32_t)v0[i];
+            rb[0] = __builtin_amdgcn_alignbyte((uint32_t)v0[1], (uint32_t)v0[0], 2);
+            rb[1] = __builtin_amdgcn_alignbyte((uint32_t)v0[2], (uint32_t)v0[1], 2);
+            rb[2] = __builtin_amdgcn_alignbyte((uint32_t)v0[3], (uint32_t)v0[2], 2);
+            rb[3] = __builtin_amdgcn_alignbyte(d4, (uint32_t)v0[3], 2);
         } else {
             const int4_t v1 = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)off + 16, 0, 0);
+            const uint32_t d8 = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)off + 32, 0, 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                row[i] = (uint32_t)v0[i];
-                row[4 + i] = (uint32_t)v1[i];
+                ra[i] = (uint32_t)v0[i];
+                ra[4 + i] = (uint32_t)v1[i];
             }
+#pragma unroll
+            for (int i = 0; i < 7; ++i) rb[i] = ra[i + 1];
+            rb[7] = d8;
         }
-        uint32_t e2 = 0;
+        uint32_t e0 = 0, e1 = 0;
 #pragma unroll
         for (int i = 0; i < 4 * C; ++i) {
-            e2 = __builtin_amdgcn_udot2(as_u2(row[i]), as_u2(row[i]), e2, false);
+            e0 = __builtin_amdgcn_udot2(as_u2(ra[i]), as_u2(ra[i]), e0, false);
+            e1 = __builtin_amdgcn_udot2(as_u2(rb[i]), as_u2(rb[i]), e1, false);
 #pragma unroll
             for (int r = 0; r < 8; ++r)
-                if (t - r >= 0 && t - r < K)
-                    acc[(t - r) & 7] = __builtin_amdgcn_udot2(as_u2(P[r][i]), as_u2(row[i]), acc[(t - r) & 7], false);
+                if (t - r >= 0 && t - r < K) {
+                    acc0[(t - r) & 7] = __builtin_amdgcn_udot2(as_u2(P[r][i]), as_u2(ra[i]), acc0[(t - r) & 7], false);
+                    acc1[(t - r) & 7] = __builtin_amdgcn_udot2(as_u2(P[r][i]), as_u2(rb[i]), acc1[(t - r) & 7], false);
+                }
         }
-        energy[t & 7] = e2;
-        window += e2;
+        en0[t & 7] = e0;
+        en1[t & 7] = e1;
+        win0 += e0;
+        win1 += e1;
         if (t >= 7) {
             const int j = t - 7 - delta;
-            if (j >= 0 && j < kk) emit(j, A2 + window - 2u * acc[(t - 7) & 7]);
-            acc[(t - 7) & 7] = 0;
-            window -= energy[(t - 7) & 7];
+            const bool ok = (unsigned)j < (unsigned)kk;  // no branch: the emit selects
+            emit(0, j, ok, A2 + win0 - 2u * acc0[(t - 7) & 7]);
+            emit(1, j, ok, A2 + win1 - 2u * acc1[(t - 7) & 7]);
+            acc0[(t - 7) & 7] = acc1[(t - 7) & 7] = 0;
+            win0 -= en0[(t - 7) & 7];
+            win1 -= en1[(t - 7) & 7];
         }
     }
 }
@@ -1124,7 +1137,7 @@ constexpr size_t kLdsBudget = 64 * 1024;
 // grids of GF frames (KM / GF volumes).
 __host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR, int KM, int GF)
 {
-    return (size_t)kWaves * (16 * 4 + NR * 16 * 8 + NR * (KM / GF) * 8 * 4 * C * 4 + NR * (KM / GF) * 4 +
+    return (size_t)kWaves * (16 * 4 + 2 * 64 * 4 + NR * (KM / GF) * 8 * 4 * C * 4 + NR * (KM / GF) * 4 +
                              NR * 16 * (KM / GF - 1) * 4 + NR * 16 * 4 + NR * KM * 4 + 4);
 }
 
@@ -1148,17 +1161,25 @@ __host__ __device__ __forceinline__ int trajectory_entries(const avd_collab_pass
     return ((int)p.group_size / gf) * trajectory_volume_entries(p);
 }
 
-// The dynamic LDS of NR references per wave: each wave's rectangle tables ([NR * entries][2] u32)
-// and its list of one volume's non-empty entries ([NR * volume entries] u16).
+// The dynamic LDS of NR references per wave: each wave's best keys ([NR][volumes][frames] u64),
+// rectangle tables ([NR * entries][2] u32), list of non-empty entries ([NR * entries] u16) and
+// list of entries searched again ([NR * volume entries] u16).
 __host__ __device__ __forceinline__ size_t trajectory_dynamic_lds(const avd_collab_pass &p, int NR)
 {
+    const int gf = p.temporal_radius == 1 ? 2 : 4;
+    const size_t bests = (size_t)kWaves * NR * ((int)p.group_size / gf) * 2 * p.temporal_radius * sizeof(uint64_t);
     const size_t rects = (size_t)kWaves * NR * trajectory_entries(p) * 2 * sizeof(uint32_t);
-    const size_t list = ((size_t)kWaves * NR * trajectory_volume_entries(p) * sizeof(uint16_t) + 15) & ~(size_t)15;
-    return rects + list;
+    const size_t lists =
+        (size_t)kWaves * NR * (trajectory_entries(p) + trajectory_volume_entries(p)) * sizeof(uint16_t);
+    return (bests + rects + lists + 15) & ~(size_t)15;
 }
 
+// Waves per SIMD the trajectory kernel is built for: four where its registers fit 128 (one channel
+// up to seven rows of candidates, two channels up to five), three otherwise.
+constexpr int trajectory_waves(int C, int K, bool SHORT) { return !SHORT && K <= (C == 1 ? 7 : 5) ? 4 : 3; }
+
 template <int C, int K, int NR, bool SHORT, int KM, int GF>
-__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(C == 1 ? 4 : 3)))
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(trajectory_waves(C, K, SHORT))))
 void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                                                                   const uint32_t *__restrict__ nslots,
                                                                   const int32_t *__restrict__ mv,
@@ -1169,16 +1190,18 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     constexpr int LPR = 64 / NR;  // lanes per reference
     constexpr int VOLS = KM / GF;
     static_assert(LPR >= KM, "a reference's lanes write its member words");
-    // Dynamic: [kWaves][NR * entries][2] rectangles, (left | right << 16), (top | bottom << 16); then
-    // [kWaves][NR * volume entries] the current volume's non-empty entries, e | r << 8.
-    extern __shared__ uint32_t s_dyn[];
+    // Dynamic: [kWaves][NR][volumes][frames] the best key of each (reference, volume, frame); then
+    // [kWaves][NR * entries][2] rectangles, (left | right << 16), (top | bottom << 16); then
+    // [kWaves][NR * entries] the non-empty entries, e | r << 10; then [kWaves][NR * volume entries]
+    // the entries searched again.
+    extern __shared__ unsigned long long s_dyn[];
     __shared__ uint32_t s_slot[kWaves][16];
-    __shared__ unsigned long long s_fbest[kWaves][NR][16];
     __shared__ uint32_t s_patch[kWaves][NR][VOLS][8][4 * C];  // every volume's anchor patch
     __shared__ uint32_t s_energy[kWaves][NR][VOLS];
     __shared__ uint32_t s_held[kWaves][NR][16][VOLS - 1];  // earlier volumes' picks, by neighbour frame
     __shared__ uint32_t s_hcnt[kWaves][NR][16];
     __shared__ uint32_t s_final[kWaves][NR][KM];  // the grid's members, volume by volume
+    __shared__ uint32_t s_cd[kWaves][64], s_cp[kWaves][64];  // decoded bests across a search again
     constexpr int refine = (K - 1) / 2;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int hr = lane / LPR, hl = lane % LPR, hbase = hr * LPR;
@@ -1208,9 +1231,13 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     const int per_vol = frames_n << nshift;
     const int ents = volumes * per_vol;
     const size_t nblk = (size_t)bx_n * by_n;
-    uint32_t(*const rect_tab)[2] = reinterpret_cast<uint32_t(*)[2]>(s_dyn) + (size_t)wave * NR * ents;
-    uint16_t *const s_list = reinterpret_cast<uint16_t *>(s_dyn + (size_t)kWaves * NR * ents * 2) +
-                             (size_t)wave * NR * per_vol;
+    const int fb_n = volumes * frames_n;  // bests per reference
+    unsigned long long *const s_fbest = s_dyn + (size_t)wave * NR * fb_n;
+    uint32_t *const rect_base = reinterpret_cast<uint32_t *>(s_dyn + (size_t)kWaves * NR * fb_n);
+    uint32_t(*const rect_tab)[2] = reinterpret_cast<uint32_t(*)[2]>(rect_base) + (size_t)wave * NR * ents;
+    uint16_t *const list_base = reinterpret_cast<uint16_t *>(rect_base + (size_t)kWaves * NR * ents * 2);
+    uint16_t *const s_list = list_base + (size_t)wave * NR * ents;
+    uint16_t *const s_redo = list_base + (size_t)kWaves * NR * ents + (size_t)wave * NR * per_vol;
     if (lane < frames_n) s_slot[wave][lane] = nslots[lane];
     for (int i = hl; i < volumes * 8; i += LPR) {  // lane (v, r) stages row r of volume v's anchor patch
         const int v = i >> 3, r = i & 7;
@@ -1224,35 +1251,63 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
         for (int j = 0; j < 4 * C; ++j) s_patch[wave][hr][v][r][j] = row[j];
     }
 
-    // ---- every volume's rectangles, per reference.
-    for (int idx = hl; idx < ents; idx += LPR) {
-        const int v = idx / per_vol, e = idx - v * per_vol;
-        const int t = e >> nshift, b = e & ((1 << nshift) - 1);
-        uint32_t anchor = anchors[0];
+    // ---- every volume's rectangles, per reference.  A lane takes four entries at a time and
+    // issues the loads of all four fields (block index 0 for an entry without a block) before it
+    // reads any, so the wave waits for memory once per four entries.
+    for (int base = hl; base < ents; base += 4 * LPR) {
+        size_t blk[4];
+        bool has[4];
+        int px[4], py[4];
 #pragma unroll
-        for (int i = 1; i < VOLS; ++i)
-            if (v == i) anchor = anchors[i];
-        const int ax = (int)(anchor & 0x1FFF), ay = (int)((anchor >> 13) & 0x1FFF);
-        const int bx_hi = min(ax / bstep, bx_n - 1), by_hi = min(ay / bstep, by_n - 1);
-        const int ovx = max(ax + 8, bsz) - bsz, ovy = max(ay + 8, bsz) - bsz;
-        const int bx_lo = min((ovx + bstep - 1) / bstep, bx_hi), by_lo = min((ovy + bstep - 1) / bstep, by_hi);
-        const int wbx = bx_lo + (b & cmask), wby = by_lo + (b >> cshift);
-        uint32_t xr = 1u, yr = 1u;  // left 1 > right 0: empty
-        if (wbx <= bx_hi && wby <= by_hi) {
-            const size_t blk = (size_t)t * nblk + (size_t)wby * bx_n + wbx;
-            const float c = conf[blk];
-            const int32_t dx = mv[blk * 2], dy = mv[blk * 2 + 1];
-            if (c >= p.c_min) {
-                const int px = ax + dx, py = ay + dy;
-                xr = (uint32_t)clampi(px - refine, 0, maxx) | ((uint32_t)clampi(px + refine, 0, maxx) << 16);
-                yr = (uint32_t)clampi(py - refine, 0, maxy) | ((uint32_t)clampi(py + refine, 0, maxy) << 16);
+        for (int u = 0; u < 4; ++u) {
+            const int idx = base + u * LPR;
+            const int v = idx / per_vol, e = idx - v * per_vol;
+            const int t = e >> nshift, b = e & ((1 << nshift) - 1);
+            uint32_t anchor = anchors[0];
+#pragma unroll
+            for (int i = 1; i < VOLS; ++i)
+                if (v == i) anchor = anchors[i];
+            const int ax = (int)(anchor & 0x1FFF), ay = (int)((anchor >> 13) & 0x1FFF);
+            const int bx_hi = min(ax / bstep, bx_n - 1), by_hi = min(ay / bstep, by_n - 1);
+            const int ovx = max(ax + 8, bsz) - bsz, ovy = max(ay + 8, bsz) - bsz;
+            const int bx_lo = min((ovx + bstep - 1) / bstep, bx_hi), by_lo = min((ovy + bstep - 1) / bstep, by_hi);
+            const int wbx = bx_lo + (b & cmask), wby = by_lo + (b >> cshift);
+            has[u] = idx < ents && wbx <= bx_hi && wby <= by_hi;
+            blk[u] = has[u] ? (size_t)t * nblk + (size_t)wby * bx_n + wbx : 0;
+            px[u] = ax;
+            py[u] = ay;
+        }
+        float c[4];
+        int32_t dx[4], dy[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c[u] = conf[blk[u]];
+            dx[u] = mv[blk[u] * 2];
+            dy[u] = mv[blk[u] * 2 + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = base + u * LPR;
+            uint32_t xr = 1u, yr = 1u;  // left 1 > right 0: empty
+            if (has[u] && c[u] >= p.c_min) {
+                const int qx = px[u] + dx[u], qy = py[u] + dy[u];
+                xr = (uint32_t)clampi(qx - refine, 0, maxx) | ((uint32_t)clampi(qx + refine, 0, maxx) << 16);
+                yr = (uint32_t)clampi(qy - refine, 0, maxy) | ((uint32_t)clampi(qy + refine, 0, maxy) << 16);
+            }
+            if (idx < ents) {
+                rect_tab[hr * ents + idx][0] = xr;
+                rect_tab[hr * ents + idx][1] = yr;
             }
         }
-        rect_tab[hr * ents + idx][0] = xr;
-        rect_tab[hr * ents + idx][1] = yr;
     }
     if (hl < frames_n) s_hcnt[wave][hr][hl] = 0;
-    if (hl < KM) s_final[wave][hr][hl] = 0;
+    if (hl < KM) {  // each volume's anchor first, its picks to come
+        uint32_t anchor = 0;
+#pragma unroll
+        for (int i = 0; i < VOLS; ++i)
+            if (hl == i * GF) anchor = anchors[i];
+        s_final[wave][hr][hl] = anchor;
+    }
     wave_sync();
     if (hl < volumes) {
         uint32_t e = 0;
@@ -1265,154 +1320,252 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
         s_energy[wave][hr][hl] = e;
     }
 
-    // s_final collects the grid's members.  A volume's list holds its gf - 1 frame picks and their
-    // distances (SSD + 1 fits 32 bits; ~0u marks an empty slot).  Only earlier volumes' picks can
-    // equal a candidate (anchors sit in the centre frame); they are kept by frame in s_held.  All
-    // of this is the same on every lane of one reference.
+    // Every volume's frames are searched in one task loop first, with no position excluded for
+    // the group: a volume's list then takes its frames' bests in volume order, and a best that is
+    // already held (an earlier volume picked it in that frame) is searched again, that reference
+    // and frame alone, with the held positions skipped.  The minimum over the positions not held
+    // is the unconstrained minimum whenever that is not held, so the picks are the ones of a search
+    // that skips held positions from the start.  A volume's list holds its gf - 1 frame picks and
+    // their distances (SSD + 1 fits 32 bits; ~0u marks an empty slot); s_final collects the grid's
+    // members.  All of this is the same on every lane of one reference.
     uint32_t scored = 0;
-    bool filled = true;
-    for (int v = 0; v < volumes; ++v) {
-        const int first = v * gf;
-        uint32_t anchor = anchors[0];
-#pragma unroll
-        for (int i = 1; i < VOLS; ++i)
-            if (v == i) anchor = anchors[i];
-        uint32_t vd[3] = {~0u, ~0u, ~0u}, vp[3] = {0u, 0u, 0u};
-        // Compact the rectangles that hold a position no earlier rectangle of their frame covers
-        // (an empty or fully covered one adds nothing but its share of the walk index), in
-        // (reference, entry) order.
-        int n_valid = 0;
-        for (int base = 0; base < NR * per_vol; base += 64) {
-            const int idx = base + lane;
-            const int r_of = min(idx / per_vol, NR - 1), e = idx - r_of * per_vol;
-            const uint32_t(*rects)[2] = rect_tab + r_of * ents + v * per_vol;
-            bool valid = idx < NR * per_vol && (rects[e][0] & 0xFFFFu) <= (rects[e][0] >> 16);
-            if (valid) {
-                const int l = (int)(rects[e][0] & 0xFFFFu), r = (int)(rects[e][0] >> 16);
-                const int tp = (int)(rects[e][1] & 0xFFFFu), bt = (int)(rects[e][1] >> 16);
-                const int f0 = e & ~((1 << nshift) - 1);
-                for (int e2 = f0; e2 < e; ++e2) {
-                    const int l2 = (int)(rects[e2][0] & 0xFFFFu), r2 = (int)(rects[e2][0] >> 16);
-                    const int tp2 = (int)(rects[e2][1] & 0xFFFFu), bt2 = (int)(rects[e2][1] >> 16);
-                    if (l2 <= l && r <= r2 && tp2 <= tp && bt <= bt2) valid = false;
-                }
-            }
-            const uint64_t bal = __ballot(valid);
-            if (valid) s_list[n_valid + lane_mask_lt(bal)] = (uint16_t)(e | (r_of << 8));
-            n_valid += __popcll(bal);
-        }
-        if (hl < frames_n) s_fbest[wave][hr][hl] = kNoKey;
-        wave_sync();
-        for (int task = lane; task < n_valid * K; task += 64) {
-            const int k = task / K, col = task - k * K;
-            const uint32_t ent = s_list[k];
-            const int tr = (int)(ent >> 8), e = (int)(ent & 0xFFu);
-            const uint32_t(*rects)[2] = rect_tab + tr * ents + v * per_vol;
-            const int t = e >> nshift, b = e & ((1 << nshift) - 1);
+    for (int i = lane; i < NR * fb_n; i += 64) s_fbest[i] = kNoKey;
+    // Compact the rectangles that hold a position no earlier rectangle of their (volume, frame)
+    // covers (an empty or fully covered one adds nothing but its share of the walk index), in
+    // (reference, volume, entry) order.
+    int n_valid = 0;
+    for (int base = 0; base < NR * ents; base += 64) {
+        const int idx = base + lane;
+        const int r_of = min(idx / ents, NR - 1), e = idx - r_of * ents;
+        const uint32_t(*rects)[2] = rect_tab + r_of * ents;
+        bool valid = idx < NR * ents && (rects[e][0] & 0xFFFFu) <= (rects[e][0] >> 16);
+        if (valid) {
             const int l = (int)(rects[e][0] & 0xFFFFu), r = (int)(rects[e][0] >> 16);
             const int tp = (int)(rects[e][1] & 0xFFFFu), bt = (int)(rects[e][1] >> 16);
-            const int cx = l + col;
-            if (cx > r) continue;  // a rectangle clipped narrower than K
-            const int w = r - l + 1, h = bt - tp + 1;
-            // Positions inside an earlier rectangle of this frame, or already held by the group,
-            // are skipped; `walk` counts the positions of the earlier rectangles.
-            uint32_t skip = 0;
-            int walk = 0;
-            for (int b2 = 0; b2 < b; ++b2) {
-                const uint32_t x2 = rects[(t << nshift) + b2][0], y2 = rects[(t << nshift) + b2][1];
-                const int l2 = (int)(x2 & 0xFFFFu), r2 = (int)(x2 >> 16);
-                if (l2 > r2) continue;
-                const int tp2 = (int)(y2 & 0xFFFFu), bt2 = (int)(y2 >> 16);
-                walk += (r2 - l2 + 1) * (bt2 - tp2 + 1);
-                if (cx >= l2 && cx <= r2) {
-                    const int ylo = max(tp2, tp), yhi = min(bt2, bt);
-                    if (ylo <= yhi) skip |= ((2u << (yhi - ylo)) - 1u) << (ylo - tp);
-                }
+            const int f0 = e & ~((1 << nshift) - 1);
+            for (int e2 = f0; e2 < e; ++e2) {
+                const int l2 = (int)(rects[e2][0] & 0xFFFFu), r2 = (int)(rects[e2][0] >> 16);
+                const int tp2 = (int)(rects[e2][1] & 0xFFFFu), bt2 = (int)(rects[e2][1] >> 16);
+                if (l2 <= l && r <= r2 && tp2 <= tp && bt <= bt2) valid = false;
             }
+        }
+        const uint64_t bal = __ballot(valid);
+        if (valid) s_list[n_valid + lane_mask_lt(bal)] = (uint16_t)(e | (r_of << 10));
+        n_valid += __popcll(bal);
+    }
+    wave_sync();
+
+    // Column pair `jp` of list entry `ent`'s rectangle, columns x0 = (left & ~1) + 2 jp and x0 + 1
+    // (either may lie outside the rectangle): their best (distance, walk index) key joins the
+    // (reference, volume, frame) best.  Positions inside an earlier rectangle of the frame are
+    // skipped (`walk` counts the positions of those rectangles), and with `held` so are the
+    // positions the group holds in that frame.  Returns the positions scored.
+    auto run_task = [&](uint32_t ent, int jp, bool held) -> uint32_t {
+        const int tr = (int)(ent >> 10), ei = (int)(ent & 0x3FFu);
+        const int v = ei / per_vol, e = ei - v * per_vol;
+        const uint32_t(*rects)[2] = rect_tab + tr * ents + v * per_vol;
+        const int t = e >> nshift, b = e & ((1 << nshift) - 1);
+        const int l = (int)(rects[e][0] & 0xFFFFu), r = (int)(rects[e][0] >> 16);
+        const int tp = (int)(rects[e][1] & 0xFFFFu), bt = (int)(rects[e][1] >> 16);
+        const int x0 = (l & ~1) + 2 * jp;
+        const bool on0 = x0 >= l && x0 <= r, on1 = x0 + 1 <= r;
+        if (!on0 && !on1) return 0u;  // a rectangle clipped narrower than K
+        const int w = r - l + 1, h = bt - tp + 1;
+        uint32_t skip0 = on0 ? 0u : ~0u, skip1 = on1 ? 0u : ~0u;
+        int walk = 0;
+        for (int b2 = 0; b2 < b; ++b2) {
+            const uint32_t x2 = rects[(t << nshift) + b2][0], y2 = rects[(t << nshift) + b2][1];
+            const int l2 = (int)(x2 & 0xFFFFu), r2 = (int)(x2 >> 16);
+            if (l2 > r2) continue;
+            const int tp2 = (int)(y2 & 0xFFFFu), bt2 = (int)(y2 >> 16);
+            walk += (r2 - l2 + 1) * (bt2 - tp2 + 1);
+            const int ylo = max(tp2, tp), yhi = min(bt2, bt);
+            if (ylo <= yhi) {
+                const uint32_t bits = ((2u << (yhi - ylo)) - 1u) << (ylo - tp);
+                if (x0 >= l2 && x0 <= r2) skip0 |= bits;
+                if (x0 + 1 >= l2 && x0 + 1 <= r2) skip1 |= bits;
+            }
+        }
+        if (held) {
             const int n_held = (int)s_hcnt[wave][tr][t];
             for (int i = 0; i < n_held; ++i) {
                 const uint32_t g = s_held[wave][tr][t][i];
-                const int gy = (int)((g >> 13) & 0x1FFF);
-                if ((int)(g & 0x1FFF) == cx && gy >= tp && gy <= bt) skip |= 1u << (gy - tp);
+                const int gx = (int)(g & 0x1FFF), gy = (int)((g >> 13) & 0x1FFF);
+                if (gy >= tp && gy <= bt) {
+                    if (gx == x0) skip0 |= 1u << (gy - tp);
+                    if (gx == x0 + 1) skip1 |= 1u << (gy - tp);
+                }
             }
-            const uint32_t live_rows = ((1u << h) - 1u) & ~skip;
-            if (!live_rows) continue;  // every position of this column was searched or is held
-            if (ref0 + tr < refs) scored += (uint32_t)__popc(live_rows);
-            uint32_t P[8][4 * C];
+        }
+        const uint32_t rows = (1u << h) - 1u;
+        const uint32_t live0 = rows & ~skip0, live1 = rows & ~skip1;
+        if (!(live0 | live1)) return 0u;  // every position of these columns was searched or is held
+        uint32_t P[8][4 * C];
 #pragma unroll
-            for (int rr = 0; rr < 8; ++rr)
+        for (int rr = 0; rr < 8; ++rr)
 #pragma unroll
-                for (int i = 0; i < 4 * C; ++i) P[rr][i] = s_patch[wave][tr][v][rr][i];
-            uint64_t best = kNoKey;
-            // Every lane runs K rows of candidates; a rectangle clipped shorter drops the rest.
-            auto keep = [&](int j, uint32_t ssd) {
-                const uint64_t key = ((uint64_t)(ssd + 1u) << 32) | (uint32_t)(walk + j * w + col);
-                if (!((skip >> j) & 1u) && key < best) best = key;
-            };
-            if (SHORT)  // rows past the frame are clamped
-                slide_fixed<C, K>(ring, (size_t)s_slot[wave][t] * slot_len, W, H, cx, tp, h, P, s_energy[wave][tr][v],
-                                  keep);
-            else
-                slide_traj<C, K>(ring, ring_bytes, (size_t)s_slot[wave][t] * slot_len, W, H, cx, tp, h, P,
-                                 s_energy[wave][tr][v], keep);
-            if (best != kNoKey) atomicMin(&s_fbest[wave][tr][t], (unsigned long long)best);
+            for (int i = 0; i < 4 * C; ++i) P[rr][i] = s_patch[wave][tr][v][rr][i];
+        uint64_t best = kNoKey;
+        const int c0 = x0 - l;  // column x0's index in the rectangle (-1 left of it)
+        // Every lane runs K rows of candidates; a rectangle clipped shorter drops the rest.
+        // Selects only, so the slide stays one basic block and its row loads can all be issued ahead.
+        auto keep = [&](int c, int j, bool ok, uint32_t ssd) {
+            const uint32_t sk = c ? skip1 : skip0;
+            const bool take = ok & !((sk >> (j & 31)) & 1u);
+            const uint64_t key = ((uint64_t)(ssd + 1u) << 32) | (uint32_t)(walk + j * w + c0 + c);
+            const uint64_t k2 = take ? key : kNoKey;
+            best = k2 < best ? k2 : best;
+        };
+        const size_t frame = (size_t)s_slot[wave][t] * slot_len;
+        if (SHORT) {  // rows past the frame are clamped; one column at a time
+            if (on0)
+                slide_fixed<C, K>(ring, frame, W, H, x0, tp, h, P, s_energy[wave][tr][v],
+                                  [&](int j, uint32_t ssd) { keep(0, j, true, ssd); });
+            if (on1)
+                slide_fixed<C, K>(ring, frame, W, H, x0 + 1, tp, h, P, s_energy[wave][tr][v],
+                                  [&](int j, uint32_t ssd) { keep(1, j, true, ssd); });
+        } else {
+            slide_pair<C, K>(ring, ring_bytes, frame, W, H, x0, tp, h, P, s_energy[wave][tr][v], keep);
+        }
+        if (best != kNoKey) atomicMin(&s_fbest[(tr * volumes + v) * frames_n + t], (unsigned long long)best);
+        return ref0 + tr < refs ? (uint32_t)(__popc(live0) + __popc(live1)) : 0u;
+    };
+    constexpr int KP = (K + 2) / 2;  // column pairs per rectangle
+    // One task loop serves the first search (the whole list, nothing held) and each search again
+    // (a volume's clashing entries, held positions skipped), so the slide is inlined once.  Across a
+    // search again, the decoded bests wait in s_cd / s_cp.
+    const uint16_t *tlist = s_list;
+    int tn = n_valid, v = 0;
+    bool held = false;
+    bool filled = true;
+    uint64_t clashes = 0;
+    for (;;) {
+        for (int task = lane; task < tn * KP; task += 64) {
+            const int k = task / KP, jp = task - k * KP;
+            const uint32_t n = run_task(tlist[k], jp, held);
+            if (!held) scored += n;
         }
         wave_sync();
-        // Lane t of each reference's lanes decodes that reference's frame t.
-        uint32_t cd = ~0u;
-        uint32_t cp = 0;
-        if (hl < frames_n) {
+        for (; v < volumes; ++v) {
             const uint32_t(*rects)[2] = rect_tab + hr * ents + v * per_vol;
-            const uint64_t best = s_fbest[wave][hr][hl];
-            if (best != kNoKey) {
-                int i = (int)(best & 0xFFFFFFFFu);
-                for (int b = 0; b < (1 << nshift); ++b) {
-                    const uint32_t x2 = rects[(hl << nshift) + b][0], y2 = rects[(hl << nshift) + b][1];
-                    const int l2 = (int)(x2 & 0xFFFFu), r2 = (int)(x2 >> 16);
-                    if (l2 > r2) continue;
-                    const int tp2 = (int)(y2 & 0xFFFFu), bt2 = (int)(y2 >> 16);
-                    const int w2 = r2 - l2 + 1, n2 = w2 * (bt2 - tp2 + 1);
-                    if (i < n2) {
-                        cp = pack_pos_t((uint32_t)(l2 + i % w2), (uint32_t)(tp2 + i / w2), (uint32_t)hl + 1);
-                        break;
+            // Lane t of each reference's lanes decodes that reference's frame t.
+            auto decode = [&](uint32_t &cd, uint32_t &cp) {
+                cd = ~0u;
+                cp = 0;
+                const uint64_t best = s_fbest[(hr * volumes + v) * frames_n + hl];
+                if (best != kNoKey) {
+                    int i = (int)(best & 0xFFFFFFFFu);
+                    for (int b = 0; b < (1 << nshift); ++b) {
+                        const uint32_t x2 = rects[(hl << nshift) + b][0], y2 = rects[(hl << nshift) + b][1];
+                        const int l2 = (int)(x2 & 0xFFFFu), r2 = (int)(x2 >> 16);
+                        if (l2 > r2) continue;
+                        const int tp2 = (int)(y2 & 0xFFFFu), bt2 = (int)(y2 >> 16);
+                        const int w2 = r2 - l2 + 1, n2 = w2 * (bt2 - tp2 + 1);
+                        if (i < n2) {
+                            cp = pack_pos_t((uint32_t)(l2 + i % w2), (uint32_t)(tp2 + i / w2), (uint32_t)hl + 1);
+                            break;
+                        }
+                        i -= n2;
                     }
-                    i -= n2;
+                    cd = (uint32_t)(best >> 32);
                 }
-                cd = (uint32_t)(best >> 32);
+            };
+            uint32_t cd = ~0u, cp = 0;
+            if (held) {
+                // Back from searching this volume again: the clashing frames decode their new bests.
+                held = false;
+                if (hl < frames_n) {
+                    cd = s_cd[wave][lane];
+                    cp = s_cp[wave][lane];
+                    if ((clashes >> lane) & 1u) decode(cd, cp);
+                }
+            } else {
+                bool clash = false;
+                if (hl < frames_n) {
+                    decode(cd, cp);
+                    // Held positions of this frame: a clash when one is the best; and each one
+                    // inside the frame's rectangles was counted above but is not scored.
+                    const int n_held = (int)s_hcnt[wave][hr][hl];
+                    for (int i = 0; i < n_held; ++i) {
+                        const uint32_t g = s_held[wave][hr][hl][i];
+                        clash = clash || g == cp;
+                        const int gx = (int)(g & 0x1FFF), gy = (int)((g >> 13) & 0x1FFF);
+                        bool inside = false;
+                        for (int b = 0; b < (1 << nshift); ++b) {
+                            const uint32_t x2 = rects[(hl << nshift) + b][0], y2 = rects[(hl << nshift) + b][1];
+                            inside = inside || ((int)(x2 & 0xFFFFu) <= gx && gx <= (int)(x2 >> 16) &&
+                                                (int)(y2 & 0xFFFFu) <= gy && gy <= (int)(y2 >> 16));
+                        }
+                        if (inside && ref0 + hr < refs) scored -= 1u;
+                    }
+                }
+                clashes = __ballot(clash);
+                if (clashes) {
+                    if (hl < frames_n) {
+                        s_cd[wave][lane] = cd;
+                        s_cp[wave][lane] = cp;
+                    }
+                    if (clash) s_fbest[(hr * volumes + v) * frames_n + hl] = kNoKey;
+                    // This volume's entries of the clashing (reference, frame) pairs.
+                    int n_redo = 0;
+                    for (int base = 0; base < n_valid; base += 64) {
+                        const int i = base + lane;
+                        bool take = false;
+                        uint16_t ent = 0;
+                        if (i < n_valid) {
+                            ent = s_list[i];
+                            const int tr = (int)(ent >> 10), ei = (int)(ent & 0x3FFu), vv = ei / per_vol;
+                            const int t = (ei - vv * per_vol) >> nshift;
+                            take = vv == v && ((clashes >> (tr * LPR + t)) & 1u);
+                        }
+                        const uint64_t bal = __ballot(take);
+                        if (take) s_redo[n_redo + lane_mask_lt(bal)] = ent;
+                        n_redo += __popcll(bal);
+                    }
+                    wave_sync();
+                    tlist = s_redo;
+                    tn = n_redo;
+                    held = true;
+                    break;  // search again, then come back to this volume
+                }
             }
-        }
-        // Frames join the volume's ascending list in frame order; a later frame never displaces an
-        // equal one, and a displaced entry moves on past equal ones (the reference's ordered
-        // insertion, which is not a stable sort when distances tie).
-        for (int u = 0; u < frames_n; ++u) {
-            uint32_t d = (uint32_t)__shfl((int)cd, hbase + u);
-            uint32_t q = (uint32_t)__shfl((int)cp, hbase + u);
+            // Frames join the volume's ascending list in frame order; a later frame never displaces
+            // an equal one, and a displaced entry moves on past equal ones (the reference's ordered
+            // insertion, which is not a stable sort when distances tie).
+            uint32_t vd[3] = {~0u, ~0u, ~0u}, vp[3] = {0u, 0u, 0u};
+            for (int u = 0; u < frames_n; ++u) {
+                uint32_t d = (uint32_t)__shfl((int)cd, hbase + u);
+                uint32_t q = (uint32_t)__shfl((int)cp, hbase + u);
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                if (j < gf - 1 && d < vd[j]) {
-                    const uint32_t hd = vd[j], hp = vp[j];
-                    vd[j] = d;
-                    vp[j] = q;
-                    d = hd;
-                    q = hp;
+                for (int j = 0; j < 3; ++j) {
+                    if (j < gf - 1 && d < vd[j]) {
+                        const uint32_t hd = vd[j], hp = vp[j];
+                        vd[j] = d;
+                        vp[j] = q;
+                        d = hd;
+                        q = hp;
+                    }
                 }
             }
-        }
-        filled = filled && (gf == 2 ? vd[0] : vd[2]) != ~0u;
-        // Lane j < gf of each reference places member first + j: the anchor, then the picks.
-        if (hl < gf) s_final[wave][hr][first + hl] = hl == 0 ? anchor : (hl == 1 ? vp[0] : (hl == 2 ? vp[1] : vp[2]));
-        // This volume's picks are held from the next volume on, each in its own frame's list (the
-        // picks of one volume sit in distinct frames, so lane j appends pick j alone).
-        if (hl < gf - 1) {
-            const uint32_t pick = hl == 0 ? vp[0] : (hl == 1 ? vp[1] : vp[2]);
-            if (pick != 0u) {
-                const int tf = (int)(pick >> 26) - 1;
-                const uint32_t at = s_hcnt[wave][hr][tf];
-                s_held[wave][hr][tf][at] = pick;
-                s_hcnt[wave][hr][tf] = at + 1;
+            filled = filled && (gf == 2 ? vd[0] : vd[2]) != ~0u;
+            // Lanes 1 .. gf - 1 of each reference place members first + 1 .. (the anchor is in
+            // place already).
+            const int first = v * gf;
+            if (hl >= 1 && hl < gf) s_final[wave][hr][first + hl] = hl == 1 ? vp[0] : (hl == 2 ? vp[1] : vp[2]);
+            // This volume's picks are held from the next volume on, each in its own frame's list
+            // (the picks of one volume sit in distinct frames, so lane j appends pick j alone).
+            if (hl < gf - 1) {
+                const uint32_t pick = hl == 0 ? vp[0] : (hl == 1 ? vp[1] : vp[2]);
+                if (pick != 0u) {
+                    const int tf = (int)(pick >> 26) - 1;
+                    const uint32_t at = s_hcnt[wave][hr][tf];
+                    s_held[wave][hr][tf][at] = pick;
+                    s_hcnt[wave][hr][tf] = at + 1;
+                }
             }
+            wave_sync();
         }
-        wave_sync();
+        if (v == volumes) break;
     }
     const bool use_grid = k_use == KM && filled;
     if (use_grid && live) {
@@ -2619,7 +2772,8 @@ hipError_t launch_trajectory_gf(const avd_collab_pass &p, const uint16_t *codes,
     // Frames shorter than K + 7 rows, and rings past 2^31 bytes (beyond a buffer load's offset),
     // take the clamping slide (at two references per wave for eight members, one for sixteen).
     const bool short_frame = (int)p.height < 2 * (int)p.refine + 8 ||
-                             (uint64_t)p.width * p.height * p.channels * p.slots * 2 >= (1ull << 31);
+                             (uint64_t)p.width * p.height * p.channels * p.slots * 2 >= (1ull << 31) ||
+                             (C == 1 && (p.width & 1u));
     if (short_frame) nr = nr_lo;
     if (lds_of(nr) > 160 * 1024) return hipErrorInvalidValue;  // past the LDS of a workgroup
     const size_t lds = trajectory_dynamic_lds(p, nr);
