@@ -202,9 +202,39 @@ __device__ __forceinline__ void stage_rect(const Level &l, int x0, int y0, int r
     }
 }
 
-// The COLS x ROWS rectangle at (x0, y0) of a level into u16 rows `stride` samples apart, the
-// wave's lanes sweeping consecutive samples (so a load instruction covers a few cache lines, not
-// one per lane); a rectangle inside the level reads without clamps.
+// One lane's share of the COLS x ROWS rectangle at (x0, y0) of a level: sample lane + 64 k in
+// register k, the wave's lanes sweeping consecutive samples (so a load instruction covers a few
+// cache lines, not one per lane).  Every load of the rectangle is issued before any is used; a
+// rectangle inside the level reads without clamps.
+template <int COLS, int ROWS>
+struct TileRegs {
+    static constexpr int N = COLS * ROWS, PER = (N + 63) / 64;
+    uint32_t v[PER];
+};
+
+template <int COLS, int ROWS>
+__device__ __forceinline__ void load_tile16(const Level &l, int x0, int y0, int lane, TileRegs<COLS, ROWS> &t)
+{
+    using T = TileRegs<COLS, ROWS>;
+    const bool inside = x0 >= 0 && y0 >= 0 && x0 + COLS <= l.w && y0 + ROWS <= l.h;
+    // Sample k's (column, row) offset from (x0, y0), clamped to the level when the rectangle passes
+    // an edge; 32-bit sample offsets from the level's start.
+    auto offset = [&](int k) -> uint32_t {
+        const int i = min(lane + 64 * k, T::N - 1), r = i / COLS, c = i - r * COLS;
+        const int x = inside ? x0 + c : clampi(x0 + c, l.w - 1), y = inside ? y0 + r : clampi(y0 + r, l.h - 1);
+        return (uint32_t)(y * l.w + x);
+    };
+    if (l.codes) {
+#pragma unroll
+        for (int k = 0; k < T::PER; ++k) t.v[k] = l.codes[(size_t)offset(k) * l.C];
+    } else {
+#pragma unroll
+        for (int k = 0; k < T::PER; ++k) t.v[k] = (uint32_t)l.plane[offset(k)];
+    }
+}
+
+// The COLS x ROWS rectangle at (x0, y0) of a level into u16 rows `stride` samples apart, one
+// sample per lane at a time (for rectangles too large to hold in registers).
 template <int COLS, int ROWS>
 __device__ __forceinline__ void stage_tile16(const Level &l, int x0, int y0, uint16_t *dst, int stride, int lane)
 {
@@ -218,6 +248,18 @@ __device__ __forceinline__ void stage_tile16(const Level &l, int x0, int y0, uin
         else
             v = l.at(x0 + c, y0 + r);
         dst[r * stride + c] = (uint16_t)v;
+    }
+}
+
+// The rectangle loaded by load_tile16 into u16 rows `stride` samples apart.
+template <int COLS, int ROWS>
+__device__ __forceinline__ void store_tile16(const TileRegs<COLS, ROWS> &t, uint16_t *dst, int stride, int lane)
+{
+    using T = TileRegs<COLS, ROWS>;
+#pragma unroll
+    for (int k = 0; k < T::PER; ++k) {
+        const int i = lane + 64 * k, r = i / COLS, c = i - r * COLS;
+        if (i < T::N) dst[r * stride + c] = (uint16_t)t.v[k];
     }
 }
 
@@ -261,7 +303,11 @@ __device__ __forceinline__ void match_rows(const Level &cen, const Level &nb, si
     uint32_t *s_slot = (uint32_t *)(s_seed + 32);            // 16
     // batch windows of WW rows of WS u16 samples, rows 16-byte aligned
     uint16_t *s_win = reinterpret_cast<uint16_t *>(s_slot + 16);
-    stage_tile16<BLK, BLK>(cen, ox, oy, s_blk, BLK, lane);
+    {
+        TileRegs<BLK, BLK> tb;
+        load_tile16<BLK, BLK>(cen, ox, oy, lane, tb);
+        store_tile16<BLK, BLK>(tb, s_blk, BLK, lane);
+    }
     for (int n0 = 0; n0 < neighbours; n0 += batch) {
         const int nb_n = min(batch, neighbours - n0);
         if (lane < nb_n) {
@@ -273,12 +319,33 @@ __device__ __forceinline__ void match_rows(const Level &cen, const Level &nb, si
             s_key[lane] = ~0u;
         }
         __syncthreads();
-        // The whole wave stages each neighbour's window in turn.
-        for (int nl = 0; nl < nb_n; ++nl) {
+        // The whole wave stages each neighbour's window in turn, the next window's loads issued
+        // before this one's stores.
+        auto load_win = [&](int nl, TileRegs<WW, WW> &t) {
             Level l = nb;
             if (l.codes) l.codes += (size_t)s_slot[nl] * level_len;
             else l.plane += (size_t)s_slot[nl] * level_len;
-            stage_tile16<WW, WW>(l, ox + s_seed[2 * nl] - R, oy + s_seed[2 * nl + 1] - R, s_win + nl * WIN, WS, lane);
+            load_tile16<WW, WW>(l, ox + s_seed[2 * nl] - R, oy + s_seed[2 * nl + 1] - R, lane, t);
+        };
+        if constexpr (TileRegs<WW, WW>::PER <= 16) {
+            TileRegs<WW, WW> ta, tb;
+            load_win(0, ta);
+            for (int nl = 0; nl < nb_n; nl += 2) {
+                if (nl + 1 < nb_n) load_win(nl + 1, tb);
+                store_tile16<WW, WW>(ta, s_win + nl * WIN, WS, lane);
+                if (nl + 1 < nb_n) {
+                    if (nl + 2 < nb_n) load_win(nl + 2, ta);
+                    store_tile16<WW, WW>(tb, s_win + (nl + 1) * WIN, WS, lane);
+                }
+            }
+        } else {  // windows past 1024 samples: sample by sample, to spare registers
+            for (int nl = 0; nl < nb_n; ++nl) {
+                Level l = nb;
+                if (l.codes) l.codes += (size_t)s_slot[nl] * level_len;
+                else l.plane += (size_t)s_slot[nl] * level_len;
+                stage_tile16<WW, WW>(l, ox + s_seed[2 * nl] - R, oy + s_seed[2 * nl + 1] - R, s_win + nl * WIN, WS,
+                                     lane);
+            }
         }
         __syncthreads();
         for (int task = lane; task < nb_n * SIDE; task += 64) {
