@@ -1239,27 +1239,30 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     uint16_t *const s_list = list_base + (size_t)wave * NR * ents;
     uint16_t *const s_redo = list_base + (size_t)kWaves * NR * ents + (size_t)wave * NR * per_vol;
     if (lane < frames_n) s_slot[wave][lane] = nslots[lane];
-    for (int i = hl; i < volumes * 8; i += LPR) {  // lane (v, r) stages row r of volume v's anchor patch
-        const int v = i >> 3, r = i & 7;
+    // Lane (v, r) stages row r of volume v's anchor patch: its loads go out now, its LDS stores
+    // after the rectangle tables' loads have gone out too.
+    constexpr int PROWS = (VOLS * 8 + LPR - 1) / LPR;
+    uint32_t prow[PROWS][4 * C];
+#pragma unroll
+    for (int u = 0; u < PROWS; ++u) {
+        const int i = min(hl + u * LPR, VOLS * 8 - 1), v = i >> 3, r = i & 7;
         uint32_t anchor = anchors[0];
 #pragma unroll
         for (int j = 1; j < VOLS; ++j)
             if (v == j) anchor = anchors[j];
-        uint32_t row[4 * C];
-        load_row<C>(ring, cframe + ((size_t)(((anchor >> 13) & 0x1FFF) + r) * W + (anchor & 0x1FFF)) * C, row);
-#pragma unroll
-        for (int j = 0; j < 4 * C; ++j) s_patch[wave][hr][v][r][j] = row[j];
+        load_row<C>(ring, cframe + ((size_t)(((anchor >> 13) & 0x1FFF) + r) * W + (anchor & 0x1FFF)) * C, prow[u]);
     }
 
-    // ---- every volume's rectangles, per reference.  A lane takes four entries at a time and
-    // issues the loads of all four fields (block index 0 for an entry without a block) before it
-    // reads any, so the wave waits for memory once per four entries.
-    for (int base = hl; base < ents; base += 4 * LPR) {
-        size_t blk[4];
-        bool has[4];
-        int px[4], py[4];
+    // ---- every volume's rectangles, per reference.  A lane takes eight entries at a time and
+    // issues the loads of all eight fields (block index 0 for an entry without a block) before it
+    // reads any, so the wave waits for memory once per eight entries.
+    constexpr int EPL = 8;
+    for (int base = hl; base < ents; base += EPL * LPR) {
+        size_t blk[EPL];
+        bool has[EPL];
+        int px[EPL], py[EPL];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < EPL; ++u) {
             const int idx = base + u * LPR;
             const int v = idx / per_vol, e = idx - v * per_vol;
             const int t = e >> nshift, b = e & ((1 << nshift) - 1);
@@ -1277,16 +1280,16 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
             px[u] = ax;
             py[u] = ay;
         }
-        float c[4];
-        int32_t dx[4], dy[4];
+        float c[EPL];
+        int32_t dx[EPL], dy[EPL];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < EPL; ++u) {
             c[u] = conf[blk[u]];
             dx[u] = mv[blk[u] * 2];
             dy[u] = mv[blk[u] * 2 + 1];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < EPL; ++u) {
             const int idx = base + u * LPR;
             uint32_t xr = 1u, yr = 1u;  // left 1 > right 0: empty
             if (has[u] && c[u] >= p.c_min) {
@@ -1298,6 +1301,14 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                 rect_tab[hr * ents + idx][0] = xr;
                 rect_tab[hr * ents + idx][1] = yr;
             }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PROWS; ++u) {
+        const int i = hl + u * LPR;
+        if (i < VOLS * 8) {
+#pragma unroll
+            for (int j = 0; j < 4 * C; ++j) s_patch[wave][hr][i >> 3][i & 7][j] = prow[u][j];
         }
     }
     if (hl < frames_n) s_hcnt[wave][hr][hl] = 0;
@@ -2303,8 +2314,11 @@ __device__ __forceinline__ void mixed_group(const avd_collab_pass &p, const uint
 // copy are then in flight together, and the reserved positions are only read after the filter,
 // when the list entries are written (a full list sends the (member, tile) pair to the overflow
 // list).
-template <int C, int KM>
+// CODES: the members' samples come from the ring's u16 code mirror (half the bytes of the values),
+// for codes of at most 11 bits, which enter the transforms exactly either way.
+template <int C, int KM, bool CODES>
 __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_pass p, const float *__restrict__ values,
+                                                                     const uint16_t *__restrict__ codes,
                                                                      const uint32_t *__restrict__ nslots,
                                                                      const uint32_t *__restrict__ members,
                                                                      float *__restrict__ group_weight, TileBins tiles)
@@ -2366,13 +2380,21 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
         base[k] = s_key[i] != ~0u ? atomicAdd(&tiles.count[s_key[i]], s_cnt[i]) : 0u;
     }
 
-    // ---- samples: rows 2 s + h of the member's patch, 8 C values each; the tables.
-    const float *src = values + (size_t)slot * W * H * C + ((size_t)(my + h) * W + mx) * C;
+    // ---- samples: rows 2 s + h of the member's patch, 8 C values (or codes) each; the tables.
+    const size_t at = (size_t)slot * W * H * C + ((size_t)(my + h) * W + mx) * C;
     float4u_t raw[4][2 * C];
+    uint32_t cw[4][4 * C];
+    if constexpr (CODES) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+        for (int s = 0; s < 4; ++s) load_row<C>(codes, at + (size_t)2 * s * W * C, cw[s]);
+    } else {
+        const float *src = values + at;
 #pragma unroll
-        for (int k = 0; k < 2 * C; ++k) raw[s][k] = *reinterpret_cast<const float4u_t *>(src + (size_t)2 * s * W * C + 4 * k);
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int k = 0; k < 2 * C; ++k)
+                raw[s][k] = *reinterpret_cast<const float4u_t *>(src + (size_t)2 * s * W * C + 4 * k);
+    }
     for (int i = threadIdx.x; i < 16 * 64; i += kWaves * 64) s_frag[i >> 6][i & 63] = g_mixed_frags[i >> 6][i & 63];
     if (threadIdx.x < 64 * C) {
         const int c = threadIdx.x >> 6, v = (threadIdx.x >> 3) & 7, u = threadIdx.x & 7;
@@ -2395,8 +2417,14 @@ __global__ __launch_bounds__(kWaves * 64) void k_collab_filter_mixed(avd_collab_
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const float v = raw[s][(i * C + c) >> 2][(i * C + c) & 3];
-                pa[c][s][i] = (_Float16)(exact ? rintf(v * maxv) * pow2 : v);
+                if constexpr (CODES) {
+                    const int j = i * C + c;
+                    const uint16_t code = (uint16_t)(cw[s][j >> 1] >> (16 * (j & 1)));
+                    pa[c][s][i] = (_Float16)code * (_Float16)pow2;
+                } else {
+                    const float v = raw[s][(i * C + c) >> 2][(i * C + c) & 3];
+                    pa[c][s][i] = (_Float16)(exact ? rintf(v * maxv) * pow2 : v);
+                }
             }
     mixed_group<C, KM>(p, s_frag, s_cvs, s_thr, pa, inv_s, first, refs, k_use, gword, lane, group_weight, tiles);
 
@@ -2873,14 +2901,20 @@ hipError_t launch_collab_filter(const avd_collab_pass &p, const float *values, c
 namespace {
 
 template <int C, int KM>
-void binned_launches(const avd_collab_pass &p, const float *values, const uint32_t *nslots, const uint32_t *members,
-                     const TileBins &tiles, unsigned long long *aggregate, int32_t *accum_second,
-                     float *group_weight, hipStream_t s, bool mixed, uint32_t refs, uint32_t n_tiles)
+void binned_launches(const avd_collab_pass &p, const float *values, const uint16_t *codes, const uint32_t *nslots,
+                     const uint32_t *members, const TileBins &tiles, unsigned long long *aggregate,
+                     int32_t *accum_second, float *group_weight, hipStream_t s, bool mixed, uint32_t refs,
+                     uint32_t n_tiles)
 {
     const dim3 blk(kWaves * 64);
     if (mixed) {
         const unsigned gm = div_up(refs, kWaves * (32 / KM));  // 32 members per wave
-        k_collab_filter_mixed<C, KM><<<gm, blk, 0, s>>>(p, values, nslots, members, group_weight, tiles);
+        if (codes && p.code_bits >= 1 && p.code_bits <= 11)
+            k_collab_filter_mixed<C, KM, true><<<gm, blk, 0, s>>>(p, values, codes, nslots, members, group_weight,
+                                                                  tiles);
+        else
+            k_collab_filter_mixed<C, KM, false><<<gm, blk, 0, s>>>(p, values, nullptr, nslots, members,
+                                                                   group_weight, tiles);
         k_tile_gather<C, true, KM><<<n_tiles, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
         k_tile_overflow<C, true, KM><<<64, 256, 0, s>>>(p, members, group_weight, tiles, aggregate, accum_second);
     } else {
@@ -2897,7 +2931,7 @@ void binned_launches(const avd_collab_pass &p, const float *values, const uint32
 hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
                                        const uint32_t *members, const TileBins &tiles,
                                        unsigned long long *aggregate, int32_t *accum_second, float *group_weight,
-                                       hipStream_t s, bool mixed)
+                                       hipStream_t s, bool mixed, const uint16_t *codes)
 {
     if (mixed && p.pool_ratio > 0.0f) return hipErrorInvalidValue;  // the mixed form thresholds hard
     if (!pass_ok(p) || (p.channels == 2 && !accum_second) || !tiles.stage || !tiles.count || !tiles.list ||
@@ -2913,14 +2947,14 @@ hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *va
     if (e != hipSuccess) return e;
     const uint32_t refs = (uint32_t)ref_count(p);
     if (p.group_size == 16) {
-        if (p.channels == 1) binned_launches<1, 16>(p, values, nslots, members, tiles, aggregate, accum_second,
+        if (p.channels == 1) binned_launches<1, 16>(p, values, codes, nslots, members, tiles, aggregate, accum_second,
                                                     group_weight, s, mixed, refs, n_tiles);
-        else binned_launches<2, 16>(p, values, nslots, members, tiles, aggregate, accum_second, group_weight, s,
+        else binned_launches<2, 16>(p, values, codes, nslots, members, tiles, aggregate, accum_second, group_weight, s,
                                     mixed, refs, n_tiles);
     } else {
-        if (p.channels == 1) binned_launches<1, 8>(p, values, nslots, members, tiles, aggregate, accum_second,
+        if (p.channels == 1) binned_launches<1, 8>(p, values, codes, nslots, members, tiles, aggregate, accum_second,
                                                    group_weight, s, mixed, refs, n_tiles);
-        else binned_launches<2, 8>(p, values, nslots, members, tiles, aggregate, accum_second, group_weight, s,
+        else binned_launches<2, 8>(p, values, codes, nslots, members, tiles, aggregate, accum_second, group_weight, s,
                                    mixed, refs, n_tiles);
     }
     return hipGetLastError();

@@ -147,6 +147,6 @@ struct TileBins {
 hipError_t launch_collab_filter_binned(const avd_collab_pass &p, const float *values, const uint32_t *nslots,
                                        const uint32_t *members, const TileBins &tiles,
                                        unsigned long long *aggregate, int32_t *accum_second, float *group_weight,
-                                       hipStream_t s, bool mixed = false);
+                                       hipStream_t s, bool mixed = false, const uint16_t *codes = nullptr);
 
 }  // namespace avd

@@ -958,7 +958,7 @@ void Nl4dPass::run_pass(uint32_t centre_logical, Clear clear)
     timed(AVD_STAGE_COLLAB_FILTER, filter_ops, stream_, "collab filter", [&] {
         return launch_collab_filter_binned(p, ring_.value_ptr(), nslots, members_.as<uint32_t>(), bins_,
                                            aggregate_.as<unsigned long long>(), accum_second_.as<int32_t>(),
-                                           group_weight_.as<float>(), stream_, mixed_);
+                                           group_weight_.as<float>(), stream_, mixed_, ring_.code_ptr());
     });
     if (watch_overflow_) {  // test hook: count the (member, tile) pairs past the lists
         uint32_t pairs = 0;
