@@ -1141,6 +1141,17 @@ __host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR, int KM
                              NR * 16 * (KM / GF - 1) * 4 + NR * 16 * 4 + NR * KM * 4 + 4);
 }
 
+// x / d for x < 2^16 and a divisor fixed for the kernel: one high multiply by m = ceil(2^32 / d)
+// (exact over that range for every d < 2^16; d = 1 has no 32-bit m and passes x through).
+struct FastDiv {
+    uint32_t m;
+    __device__ explicit FastDiv(uint32_t d) : m(d > 1 ? (uint32_t)((0x100000000ull + d - 1) / d) : 0u) {}
+    __device__ __forceinline__ int operator()(int x) const
+    {
+        return m ? (int)__umulhi((uint32_t)x, m) : x;
+    }
+};
+
 __device__ __forceinline__ uint32_t lane_mask_lt(uint64_t bal)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
@@ -1231,6 +1242,7 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     const int per_vol = frames_n << nshift;
     const int ents = volumes * per_vol;
     const size_t nblk = (size_t)bx_n * by_n;
+    const FastDiv div_vol((uint32_t)per_vol), div_ents((uint32_t)ents), div_bstep((uint32_t)bstep);
     const int fb_n = volumes * frames_n;  // bests per reference
     unsigned long long *const s_fbest = s_dyn + (size_t)wave * NR * fb_n;
     uint32_t *const rect_base = reinterpret_cast<uint32_t *>(s_dyn + (size_t)kWaves * NR * fb_n);
@@ -1264,16 +1276,16 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
 #pragma unroll
         for (int u = 0; u < EPL; ++u) {
             const int idx = base + u * LPR;
-            const int v = idx / per_vol, e = idx - v * per_vol;
+            const int v = div_vol(idx), e = idx - v * per_vol;
             const int t = e >> nshift, b = e & ((1 << nshift) - 1);
             uint32_t anchor = anchors[0];
 #pragma unroll
             for (int i = 1; i < VOLS; ++i)
                 if (v == i) anchor = anchors[i];
             const int ax = (int)(anchor & 0x1FFF), ay = (int)((anchor >> 13) & 0x1FFF);
-            const int bx_hi = min(ax / bstep, bx_n - 1), by_hi = min(ay / bstep, by_n - 1);
+            const int bx_hi = min(div_bstep(ax), bx_n - 1), by_hi = min(div_bstep(ay), by_n - 1);
             const int ovx = max(ax + 8, bsz) - bsz, ovy = max(ay + 8, bsz) - bsz;
-            const int bx_lo = min((ovx + bstep - 1) / bstep, bx_hi), by_lo = min((ovy + bstep - 1) / bstep, by_hi);
+            const int bx_lo = min(div_bstep(ovx + bstep - 1), bx_hi), by_lo = min(div_bstep(ovy + bstep - 1), by_hi);
             const int wbx = bx_lo + (b & cmask), wby = by_lo + (b >> cshift);
             has[u] = idx < ents && wbx <= bx_hi && wby <= by_hi;
             blk[u] = has[u] ? (size_t)t * nblk + (size_t)wby * bx_n + wbx : 0;
@@ -1347,7 +1359,7 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     int n_valid = 0;
     for (int base = 0; base < NR * ents; base += 64) {
         const int idx = base + lane;
-        const int r_of = min(idx / ents, NR - 1), e = idx - r_of * ents;
+        const int r_of = min(div_ents(idx), NR - 1), e = idx - r_of * ents;
         const uint32_t(*rects)[2] = rect_tab + r_of * ents;
         bool valid = idx < NR * ents && (rects[e][0] & 0xFFFFu) <= (rects[e][0] >> 16);
         if (valid) {
@@ -1373,7 +1385,7 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     // positions the group holds in that frame.  Returns the positions scored.
     auto run_task = [&](uint32_t ent, int jp, bool held) -> uint32_t {
         const int tr = (int)(ent >> 10), ei = (int)(ent & 0x3FFu);
-        const int v = ei / per_vol, e = ei - v * per_vol;
+        const int v = div_vol(ei), e = ei - v * per_vol;
         const uint32_t(*rects)[2] = rect_tab + tr * ents + v * per_vol;
         const int t = e >> nshift, b = e & ((1 << nshift) - 1);
         const int l = (int)(rects[e][0] & 0xFFFFu), r = (int)(rects[e][0] >> 16);
@@ -1525,7 +1537,7 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                         uint16_t ent = 0;
                         if (i < n_valid) {
                             ent = s_list[i];
-                            const int tr = (int)(ent >> 10), ei = (int)(ent & 0x3FFu), vv = ei / per_vol;
+                            const int tr = (int)(ent >> 10), ei = (int)(ent & 0x3FFu), vv = div_vol(ei);
                             const int t = (ei - vv * per_vol) >> nshift;
                             take = vv == v && ((clashes >> (tr * LPR + t)) & 1u);
                         }
