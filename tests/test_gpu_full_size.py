@@ -1,4 +1,4 @@
-"""The grouping search at the benchmark's full size, on sampled references.
+"""The grouping search and the whole NL4D engine at the benchmark's full size.
 
 3840x2160 10-bit luma and its 1920x1080 4:2:0 chroma, temporal radius 3 (seven ring slots), spatial
 radius 12, stride 3, groups of up to 16: the frame sizes at which the workgroup grids are remapped
@@ -7,6 +7,11 @@ through 32-bit byte offsets, and the matrix-core centre search runs with its red
 motion fields come from the motion site (its parity is tested on its own); the oracle recomputes
 the members of about 1700 references -- the corners, the last row and column, the grid's edges and
 a seeded spread -- from the same ring, fields and confidences, and they must match bit for bit.
+
+The engine test runs the FP32 form over a two-frame 3840x2160 luma stream at temporal radius 2
+(volume grids of 4 frames), stride 3, groups of up to 16, and compares every output frame with the
+oracle's multi-threaded FP32 restatement bit for bit: motion, both searches, the filter, the binned
+aggregation (tile lists of a 120 x 68 tile grid per slot) and the normalise at size.
 """
 from __future__ import annotations
 
@@ -20,7 +25,9 @@ import avd
 from avd import _lib as L
 
 import oracle as O
-from gpu_helpers import assert_members_equal, collab_pass, dev, host, ptr, require_gpu, site_collab_search
+from gpu_helpers import (assert_members_equal, collab_pass, dev, gray_geometry, host, ptr, require_gpu, run_engine,
+                         site_collab_search, words_dev)
+from test_gpu_k16 import _opts, _spec
 
 pytestmark = pytest.mark.gpu
 
@@ -89,3 +96,19 @@ def test_full_size_search_matches_the_oracle_on_sampled_references(plane):
     # The grid groups (volumes filled from every frame) are among them.
     words = got.reshape(-1, K + 1)[:, K]
     assert np.any(words & 0x100)
+
+
+def test_full_size_engine_fp32_stream_matches_the_oracle():
+    W, H, depth, r, S, step = 3840, 2160, 10, 2, 12, 3
+    frames, sigma = 2, 16.0 / 1023
+    print("full-size engine: 3840x2160 luma, r=2, stride 3, groups of 16, FP32 form", flush=True)
+    clip = avd.synth.clip(frames, W, H, depth, 0xA5D00004, chroma="gray", sigma_luma=16.0)
+    codes = [f[0].numpy() for f in clip]
+    eng = avd.build_engine("nl4d", gray_geometry(W, H, depth), luma=_opts(r, S, sigma, step, 16))
+    got = run_engine(eng, [[words_dev(c, depth)] for c in codes], depth, [(W, H)])
+    O.set_threads(min(16, O.max_threads()))
+    want, _ = O.run_stream(O.Nl4dPass(_spec(r, S, sigma, step, 16), W, H, 1, depth, depth),
+                           [[O.pack_plane(c, depth)] for c in codes])
+    assert len(got) == len(want) == frames
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert np.array_equal(g[0], w[0]), f"frame {i}"
