@@ -1429,15 +1429,26 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
 #pragma unroll
             for (int i = 0; i < 4 * C; ++i) P[rr][i] = s_patch[wave][tr][v][rr][i];
         uint64_t best = kNoKey;
+        uint32_t best_d = ~0u, best_i = 0;
         const int c0 = x0 - l;  // column x0's index in the rectangle (-1 left of it)
         // Every lane runs K rows of candidates; a rectangle clipped shorter drops the rest.
         // Selects only, so the slide stays one basic block and its row loads can all be issued ahead.
+        // The paired slide emits in ascending walk index (row by row, column x0 before x0 + 1), so
+        // a strictly smaller distance alone decides there; the clamping slide runs its columns one
+        // after the other and compares whole keys.
         auto keep = [&](int c, int j, bool ok, uint32_t ssd) {
             const uint32_t sk = c ? skip1 : skip0;
             const bool take = ok & !((sk >> (j & 31)) & 1u);
-            const uint64_t key = ((uint64_t)(ssd + 1u) << 32) | (uint32_t)(walk + j * w + c0 + c);
-            const uint64_t k2 = take ? key : kNoKey;
-            best = k2 < best ? k2 : best;
+            const uint32_t idx = (uint32_t)(walk + j * w + c0 + c);
+            if constexpr (SHORT) {
+                const uint64_t key = ((uint64_t)(ssd + 1u) << 32) | idx;
+                const uint64_t k2 = take ? key : kNoKey;
+                best = k2 < best ? k2 : best;
+            } else {
+                const bool better = take & (ssd + 1u < best_d);
+                best_d = better ? ssd + 1u : best_d;
+                best_i = better ? idx : best_i;
+            }
         };
         const size_t frame = (size_t)s_slot[wave][t] * slot_len;
         if (SHORT) {  // rows past the frame are clamped; one column at a time
@@ -1449,6 +1460,7 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                                   [&](int j, uint32_t ssd) { keep(1, j, true, ssd); });
         } else {
             slide_pair<C, K>(ring, ring_bytes, frame, W, H, x0, tp, h, P, s_energy[wave][tr][v], keep);
+            if (best_d != ~0u) best = ((uint64_t)best_d << 32) | best_i;
         }
         if (best != kNoKey) atomicMin(&s_fbest[(tr * volumes + v) * frames_n + t], (unsigned long long)best);
         return ref0 + tr < refs ? (uint32_t)(__popc(live0) + __popc(live1)) : 0u;
@@ -1485,7 +1497,11 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                         const int tp2 = (int)(y2 & 0xFFFFu), bt2 = (int)(y2 >> 16);
                         const int w2 = r2 - l2 + 1, n2 = w2 * (bt2 - tp2 + 1);
                         if (i < n2) {
-                            cp = pack_pos_t((uint32_t)(l2 + i % w2), (uint32_t)(tp2 + i / w2), (uint32_t)hl + 1);
+                            // i / w2 as (i + 0.5) times the reciprocal: exact for i < 2^12, w2 <= 32
+                            // (the half keeps the product clear of an integer by more than the
+                            // reciprocal's error).
+                            const int qy = (int)(((float)i + 0.5f) * __builtin_amdgcn_rcpf((float)w2));
+                            cp = pack_pos_t((uint32_t)(l2 + i - qy * w2), (uint32_t)(tp2 + qy), (uint32_t)hl + 1);
                             break;
                         }
                         i -= n2;
