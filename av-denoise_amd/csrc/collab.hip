@@ -2491,8 +2491,12 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
     // Rows padded to 40 cells: the eight rows of a patch then fall on distinct banks (two passes
     // per 64-bit add, the minimum, instead of eight).
     constexpr int kRow = T + 8;
-    __shared__ unsigned long long s_agg[T * kRow];
-    __shared__ int32_t s_second[C == 2 ? T * kRow : 1];
+    // Past the tile's cells, one trash cell per lane: a member pixel outside the tile adds there, so
+    // the walk has no branch (a branch let the compiler sink a member's staged load into it, where
+    // the lane then waited on it alone).  The trash is never read.
+    constexpr int kTrash = T * kRow;
+    __shared__ unsigned long long s_agg[T * kRow + 64];
+    __shared__ int32_t s_second[C == 2 ? T * kRow + 64 : 1];
     __shared__ uint32_t s_id[256], s_pos[256];
     __shared__ float s_sw[256];
     const uint32_t tile = blockIdx.x;
@@ -2542,7 +2546,8 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
                 const uint32_t id = s_id[ec], pos = s_pos[ec];
                 sw[u] = s_sw[ec];
                 const int lx = (int)(pos & 0x1FFF) + q - x0, ly = (int)((pos >> 13) & 0x1FFF) + r - y0;
-                cell[u] = (e < batch && lx >= 0 && lx < T && ly >= 0 && ly < T) ? ly * kRow + lx : -1;
+                const bool in = e < batch && (unsigned)lx < (unsigned)T && (unsigned)ly < (unsigned)T;
+                cell[u] = in ? ly * kRow + lx : kTrash + lane;
                 const Staged *st = reinterpret_cast<const Staged *>(tiles.stage) + (size_t)id * C * 64 + lane;
                 v0[u] = st[0];
                 if (C == 2) v1[u] = st[64];
@@ -2572,7 +2577,6 @@ __global__ __launch_bounds__(256) void k_tile_gather(avd_collab_pass p, const ui
             }
 #pragma unroll
             for (int u = 0; u < kPer; ++u) {
-                if (cell[u] < 0) continue;
                 atomicAdd(&s_agg[cell[u]], pack_aggregate(a[u], wq[u]));
                 if (C == 2) atomicAdd(&s_second[cell[u]], a2[u]);
             }
