@@ -18,7 +18,9 @@
 // trades the 2P halo rows each wave recomputes against the number of waves a small frame spreads
 // over; every choice gives bit-identical sums, and the tuner picks one per window shape.  The neighbour
 // frame(s) of the workgroup are staged once in LDS with their (P+s) halo, so the offset loop runs
-// with no barrier at all.  exp is v_exp_f32 (exp2) with log2(e) folded into h2 on the host.
+// with no barrier at all.  Each row group's offsets are split over SPLIT waves (2, or 4 for frames up
+// to 2^20 pixels) whose sums then add in LDS in part order, so small frames still fill the chip.
+// exp is v_exp_f32 (exp2) with log2(e) folded into h2 on the host.
 #include "common.h"
 
 namespace avd {
@@ -55,9 +57,9 @@ __device__ void stage_tile(float *tile, const float *__restrict__ frame, int W, 
 // here): every offset q (q != 0 for a single window) against the staged tile `tf` (the forward
 // frame of a pair, or the frame itself) and, for a pair, `tb` (the backward frame at -q).  `cen`
 // holds the lane's centre column of ROWS + 2P distance rows; `tcols` is the tiles' row pitch.
-template <int C, int P, int ROWS, bool PAIR, bool CONF>
+template <int C, int P, int ROWS, bool PAIR, bool CONF, int SPLIT>
 __device__ __forceinline__ void window_sums(const NlmWindow &w, const float *tf, const float *tb, int tcols, int wave,
-                                            int lane, const float (&cen)[ROWS + 2 * P][C],
+                                            int lane, int part, const float (&cen)[ROWS + 2 * P][C],
                                             const float *__restrict__ lut, const float (&conf_fv)[ROWS],
                                             const float (&conf_bv)[ROWS], float (&acc)[ROWS][C], float (&ws)[ROWS],
                                             float (&mw)[ROWS])
@@ -80,7 +82,11 @@ __device__ __forceinline__ void window_sums(const NlmWindow &w, const float *tf,
     // Gather column of the output pixel: lane - P + S (+/- qx).  Lanes below 2P own no output and
     // read a clamped column so every LDS read stays inside the tile.
     const int gcol0 = max(lane, P) - P + S;
-    for (int qy = -S; qy <= S; ++qy) {
+    // This wave's share of the offsets: the rows qy of part `part` of SPLIT consecutive runs of
+    // offset rows (a test inside the loop for a finer split cost the loop its shape and doubled its
+    // registers).
+    const int qy_lo = -S + part * side / SPLIT, qy_hi = -S + (part + 1) * side / SPLIT;
+    for (int qy = qy_lo; qy < qy_hi; ++qy) {
         for (int qx = -S; qx <= S; ++qx) {
             if (!PAIR && qx == 0 && qy == 0) continue;
             const float off = PAIR ? w.offset : lut[(qy + S) * side + (qx + S)];
@@ -145,6 +151,50 @@ __device__ __forceinline__ void window_sums(const NlmWindow &w, const float *tf,
     }
 }
 
+// Adds the window sums of parts 1 .. SPLIT-1 into part 0's, in part order, through `red`: LDS of
+// (SPLIT - 1) * WV * kCombineRows * (C + 2) * 64 floats, kCombineRows output rows at a time (the
+// tiles are done with, so the caller passes their memory).  Every wave of the workgroup calls it;
+// part 0 holds the totals afterwards.
+constexpr int kCombineRows = 4;
+
+template <int C, int ROWS, int WV, int SPLIT>
+__device__ __forceinline__ void combine_parts(float *red, int wave, int lane, int part, float (&acc)[ROWS][C],
+                                              float (&ws)[ROWS], float (&mw)[ROWS])
+{
+    if constexpr (SPLIT > 1) {
+        constexpr int CR = ROWS < kCombineRows ? ROWS : kCombineRows, F = CR * (C + 2);
+#pragma unroll
+        for (int j0 = 0; j0 < ROWS; j0 += CR) {
+            __syncthreads();  // every wave is done with the tiles, or part 0 with the last rows
+            if (part > 0) {
+                float *dst = red + ((size_t)((part - 1) * WV + wave) * F) * 64 + lane;
+#pragma unroll
+                for (int j = 0; j < CR; ++j) {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) dst[(size_t)(j * (C + 2) + c) * 64] = acc[j0 + j][c];
+                    dst[(size_t)(j * (C + 2) + C) * 64] = ws[j0 + j];
+                    dst[(size_t)(j * (C + 2) + C + 1) * 64] = mw[j0 + j];
+                }
+            }
+            __syncthreads();
+            if (part == 0) {
+#pragma unroll
+                for (int p = 1; p < SPLIT; ++p) {
+                    const float *src = red + ((size_t)((p - 1) * WV + wave) * F) * 64 + lane;
+#pragma unroll
+                    for (int j = 0; j < CR; ++j) {
+#pragma unroll
+                        for (int c = 0; c < C; ++c)
+                            acc[j0 + j][c] = acc[j0 + j][c] + src[(size_t)(j * (C + 2) + c) * 64];
+                        ws[j0 + j] = ws[j0 + j] + src[(size_t)(j * (C + 2) + C) * 64];
+                        mw[j0 + j] = fmaxf(mw[j0 + j], src[(size_t)(j * (C + 2) + C + 1) * 64]);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // The centre values of this lane's distance column, one per distance row, and the confidence of
 // the blocks its output rows fall in.
 template <int C, int P, int ROWS>
@@ -173,16 +223,19 @@ __device__ __forceinline__ void load_conf(const NlmWindow &w, const float *__res
     }
 }
 
-template <int C, int P, int WV, int ROWS, bool PAIR, bool CONF>
-__global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__restrict__ centre,
-                                                     const float *__restrict__ fwd, const float *__restrict__ bwd,
-                                                     const float *__restrict__ lut, const float *__restrict__ conf_f,
-                                                     const float *__restrict__ conf_b, float *__restrict__ accum,
-                                                     float *__restrict__ wsum, float *__restrict__ maxw)
+template <int C, int P, int WV, int ROWS, bool PAIR, bool CONF, int SPLIT>
+__global__ __launch_bounds__(WV * SPLIT * 64) void k_nlm(NlmWindow w, const float *__restrict__ centre,
+                                                             const float *__restrict__ fwd, const float *__restrict__ bwd,
+                                                             const float *__restrict__ lut,
+                                                             const float *__restrict__ conf_f,
+                                                             const float *__restrict__ conf_b, float *__restrict__ accum,
+                                                             float *__restrict__ wsum, float *__restrict__ maxw)
 {
     extern __shared__ float lds[];
     const int S = w.S, W = w.W, H = w.H;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // Wave (wave, part): row group `wave` of the workgroup's WV, offset part `part` of SPLIT.
+    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) % WV;
+    const int part = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) / WV);  // uniform per wave
     const int X0 = blockIdx.x * (64 - 2 * P);
     const int Y0 = blockIdx.y * (WV * ROWS);
 
@@ -191,8 +244,8 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
     const int tx0 = X0 - P - S, ty0 = Y0 - P - S;
     float *tf = lds;
     float *tb = lds + (size_t)tcols * trows * C;
-    stage_tile<C, WV>(tf, fwd, W, H, tx0, ty0, tcols, trows);
-    if (PAIR) stage_tile<C, WV>(tb, bwd, W, H, tx0, ty0, tcols, trows);
+    stage_tile<C, WV * SPLIT>(tf, fwd, W, H, tx0, ty0, tcols, trows);
+    if (PAIR) stage_tile<C, WV * SPLIT>(tb, bwd, W, H, tx0, ty0, tcols, trows);
     float cen[ROWS + 2 * P][C];
     load_centre<C, P, ROWS>(w, centre, X0, Y0, wave, lane, cen);
     __syncthreads();
@@ -204,9 +257,11 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
         load_conf<ROWS>(w, conf_b, ox, Y0, wave, conf_bv);
     }
     float acc[ROWS][C], ws[ROWS], mw[ROWS];
-    window_sums<C, P, ROWS, PAIR, CONF>(w, tf, tb, tcols, wave, lane, cen, lut, conf_fv, conf_bv, acc, ws, mw);
+    window_sums<C, P, ROWS, PAIR, CONF, SPLIT>(w, tf, tb, tcols, wave, lane, part, cen, lut, conf_fv, conf_bv, acc, ws,
+                                               mw);
+    combine_parts<C, ROWS, WV, SPLIT>(lds, wave, lane, part, acc, ws, mw);
 
-    if (lane < 2 * P || ox >= W) return;
+    if (part != 0 || lane < 2 * P || ox >= W) return;
 #pragma unroll
     for (int j = 0; j < ROWS; ++j) {
         const int oy = Y0 + wave * ROWS + j;
@@ -225,14 +280,16 @@ __global__ __launch_bounds__(WV * 64) void k_nlm(NlmWindow w, const float *__res
 // form adds every launch's sums to accumulators that start at zero), so the output is bit-identical
 // to the plain launches' while the accumulators never reach memory.  The pair tiles are re-staged
 // per pair; the single window stages the centre frame into the first of them.
-template <int C, int P, int WV, int ROWS, bool CONF>
-__global__ __launch_bounds__(WV * 64) void k_nlm_window(NlmWindow w, NlmFrames f, const float *__restrict__ centre,
-                                                        const float *__restrict__ lut, float self_weight,
-                                                        float *__restrict__ out)
+template <int C, int P, int WV, int ROWS, bool CONF, int SPLIT>
+__global__ __launch_bounds__(WV * SPLIT * 64) void k_nlm_window(NlmWindow w, NlmFrames f,
+                                                                const float *__restrict__ centre,
+                                                                const float *__restrict__ lut, float self_weight,
+                                                                float *__restrict__ out)
 {
     extern __shared__ float lds[];
     const int S = w.S, W = w.W, H = w.H;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) % WV;
+    const int part = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) / WV);  // uniform per wave
     const int X0 = blockIdx.x * (64 - 2 * P);
     const int Y0 = blockIdx.y * (WV * ROWS);
     const int tcols = 64 + 2 * S, trows = WV * ROWS + 2 * P + 2 * S;
@@ -255,14 +312,16 @@ __global__ __launch_bounds__(WV * 64) void k_nlm_window(NlmWindow w, NlmFrames f
     float conf_fv[ROWS], conf_bv[ROWS];
     for (int k = 0; k < f.pairs; ++k) {
         if (k > 0) __syncthreads();  // every wave is done with the previous pair's tiles
-        stage_tile<C, WV>(tf, f.fwd[k], W, H, tx0, ty0, tcols, trows);
-        stage_tile<C, WV>(tb, f.bwd[k], W, H, tx0, ty0, tcols, trows);
+        stage_tile<C, WV * SPLIT>(tf, f.fwd[k], W, H, tx0, ty0, tcols, trows);
+        stage_tile<C, WV * SPLIT>(tb, f.bwd[k], W, H, tx0, ty0, tcols, trows);
         __syncthreads();
         if (CONF) {
             load_conf<ROWS>(w, f.conf_f[k], ox, Y0, wave, conf_fv);
             load_conf<ROWS>(w, f.conf_b[k], ox, Y0, wave, conf_bv);
         }
-        window_sums<C, P, ROWS, true, CONF>(w, tf, tb, tcols, wave, lane, cen, nullptr, conf_fv, conf_bv, acc, ws, mw);
+        window_sums<C, P, ROWS, true, CONF, SPLIT>(w, tf, tb, tcols, wave, lane, part, cen, nullptr, conf_fv, conf_bv,
+                                                   acc, ws, mw);
+        combine_parts<C, ROWS, WV, SPLIT>(lds, wave, lane, part, acc, ws, mw);
 #pragma unroll
         for (int j = 0; j < ROWS; ++j) {
 #pragma unroll
@@ -272,11 +331,13 @@ __global__ __launch_bounds__(WV * 64) void k_nlm_window(NlmWindow w, NlmFrames f
         }
     }
     if (f.pairs > 0) __syncthreads();
-    stage_tile<C, WV>(tf, centre, W, H, tx0, ty0, tcols, trows);
+    stage_tile<C, WV * SPLIT>(tf, centre, W, H, tx0, ty0, tcols, trows);
     __syncthreads();
-    window_sums<C, P, ROWS, false, false>(w, tf, tf, tcols, wave, lane, cen, lut, conf_fv, conf_bv, acc, ws, mw);
+    window_sums<C, P, ROWS, false, false, SPLIT>(w, tf, tf, tcols, wave, lane, part, cen, lut, conf_fv, conf_bv, acc,
+                                                 ws, mw);
+    combine_parts<C, ROWS, WV, SPLIT>(lds, wave, lane, part, acc, ws, mw);
 
-    if (lane < 2 * P || ox >= W) return;
+    if (part != 0 || lane < 2 * P || ox >= W) return;
     // The finish (accumulate.rs:54-101), as k_finish computes it.
 #pragma unroll
     for (int j = 0; j < ROWS; ++j) {
@@ -320,15 +381,31 @@ __global__ __launch_bounds__(256) void k_finish(const float *__restrict__ frame,
     }
 }
 
-// Waves per workgroup: pairs of two-channel frames stage two tiles of twice the width, so they
-// take half the rows to keep the LDS footprint near that of a luma pass.
-constexpr int waves_for(int C, bool pair) { return (pair && C == 2) ? 2 : 4; }
-
-size_t tile_bytes(int C, int P, int S, bool pair, int rows)
+// Row groups per workgroup: pairs of two-channel frames stage two tiles of twice the width, so they
+// take half the rows to keep the LDS footprint near that of a luma pass; and a workgroup holds at
+// most eight waves (row groups x offset parts), four for two channels, whose windows with large
+// patches need more than 256 VGPRs.
+constexpr int waves_for(int C, bool pair, int split)
 {
-    const int wv = waves_for(C, pair);
+    const int base = (pair && C == 2) ? 2 : 4, cap = C == 2 ? 4 : 8;
+    return base * split > cap ? (cap / split > 0 ? cap / split : 1) : base;
+}
+
+// Offset parts per row group: each workgroup runs SPLIT waves per row group, each on a consecutive
+// run of the window's offsets, so a frame puts SPLIT times the waves on the chip (frames up to 2^20
+// pixels, which would leave SIMDs idle otherwise, take four).  The part sums add in part order, the
+// same for every tile shape and both launch forms, which therefore stay bit-identical.
+inline int offset_split(int W, int H) { return (size_t)W * H <= (1u << 20) ? 4 : 2; }
+
+// LDS of a workgroup: its tile(s), or the part sums while they combine, whichever is larger.
+size_t tile_bytes(int C, int P, int S, bool pair, int rows, int split)
+{
+    const int wv = waves_for(C, pair, split);
     const size_t one = (size_t)(64 + 2 * S) * (wv * rows + 2 * P + 2 * S) * C * sizeof(float);
-    return pair ? 2 * one : one;
+    const size_t tiles = pair ? 2 * one : one;
+    const int cr = rows < kCombineRows ? rows : kCombineRows;
+    const size_t parts = (size_t)(split - 1) * wv * cr * (C + 2) * 64 * sizeof(float);
+    return tiles > parts ? tiles : parts;
 }
 
 int device_lds_limit()
@@ -344,27 +421,37 @@ int device_lds_limit()
     return limit;
 }
 
+template <int C, int P, int ROWS, int SPLIT>
+hipError_t dispatch_split(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd,
+                          const float *lut, const float *cf, const float *cb, float *accum, float *wsum, float *maxw,
+                          bool pair, bool conf, hipStream_t s)
+{
+    constexpr int WS = waves_for(C, false, SPLIT), WP = waves_for(C, true, SPLIT);
+    const size_t lds = tile_bytes(C, P, w.S, pair, ROWS, SPLIT);
+    if (!pair) {
+        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WS * ROWS));
+        k_nlm<C, P, WS, ROWS, false, false, SPLIT><<<grid, WS * SPLIT * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf,
+                                                                                     cb, accum, wsum, maxw);
+    } else {
+        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WP * ROWS));
+        if (conf)
+            k_nlm<C, P, WP, ROWS, true, true, SPLIT><<<grid, WP * SPLIT * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf,
+                                                                                       cb, accum, wsum, maxw);
+        else
+            k_nlm<C, P, WP, ROWS, true, false, SPLIT><<<grid, WP * SPLIT * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf,
+                                                                                        cb, accum, wsum, maxw);
+    }
+    return hipGetLastError();
+}
+
 template <int C, int P, int ROWS>
 hipError_t dispatch_rows(const NlmWindow &w, const float *centre, const float *fwd, const float *bwd, const float *lut,
                          const float *cf, const float *cb, float *accum, float *wsum, float *maxw, bool pair, bool conf,
                          hipStream_t s)
 {
-    constexpr int WS = waves_for(C, false), WP = waves_for(C, true);
-    const size_t lds = tile_bytes(C, P, w.S, pair, ROWS);
-    if (!pair) {
-        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WS * ROWS));
-        k_nlm<C, P, WS, ROWS, false, false><<<grid, WS * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum,
-                                                                      maxw);
-    } else {
-        const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WP * ROWS));
-        if (conf)
-            k_nlm<C, P, WP, ROWS, true, true><<<grid, WP * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum,
-                                                                        maxw);
-        else
-            k_nlm<C, P, WP, ROWS, true, false><<<grid, WP * 64, lds, s>>>(w, centre, fwd, bwd, lut, cf, cb, accum,
-                                                                         wsum, maxw);
-    }
-    return hipGetLastError();
+    if (offset_split(w.W, w.H) == 4)
+        return dispatch_split<C, P, ROWS, 4>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    return dispatch_split<C, P, ROWS, 2>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
 }
 
 template <int C, int P>
@@ -398,19 +485,30 @@ hipError_t dispatch_p(const NlmWindow &w, const float *centre, const float *fwd,
     }
 }
 
+template <int C, int P, int ROWS, int SPLIT>
+hipError_t dispatch_window_split(const NlmWindow &w, const NlmFrames &f, const float *centre, const float *lut,
+                                 float self_weight, float *out, hipStream_t s)
+{
+    constexpr int WV = waves_for(C, true, SPLIT);
+    const size_t lds = tile_bytes(C, P, w.S, true, ROWS, SPLIT);
+    const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WV * ROWS));
+    const bool conf = f.pairs > 0 && f.conf_f[0] != nullptr;
+    if (conf)
+        k_nlm_window<C, P, WV, ROWS, true, SPLIT><<<grid, WV * SPLIT * 64, lds, s>>>(w, f, centre, lut, self_weight,
+                                                                                    out);
+    else
+        k_nlm_window<C, P, WV, ROWS, false, SPLIT><<<grid, WV * SPLIT * 64, lds, s>>>(w, f, centre, lut, self_weight,
+                                                                                     out);
+    return hipGetLastError();
+}
+
 template <int C, int P, int ROWS>
 hipError_t dispatch_window(const NlmWindow &w, const NlmFrames &f, const float *centre, const float *lut,
                            float self_weight, float *out, hipStream_t s)
 {
-    constexpr int WV = waves_for(C, true);
-    const size_t lds = tile_bytes(C, P, w.S, true, ROWS);
-    const dim3 grid(div_up(w.W, 64 - 2 * P), div_up(w.H, WV * ROWS));
-    const bool conf = f.pairs > 0 && f.conf_f[0] != nullptr;
-    if (conf)
-        k_nlm_window<C, P, WV, ROWS, true><<<grid, WV * 64, lds, s>>>(w, f, centre, lut, self_weight, out);
-    else
-        k_nlm_window<C, P, WV, ROWS, false><<<grid, WV * 64, lds, s>>>(w, f, centre, lut, self_weight, out);
-    return hipGetLastError();
+    if (offset_split(w.W, w.H) == 4)
+        return dispatch_window_split<C, P, ROWS, 4>(w, f, centre, lut, self_weight, out, s);
+    return dispatch_window_split<C, P, ROWS, 2>(w, f, centre, lut, self_weight, out, s);
 }
 
 template <int C>
@@ -441,8 +539,10 @@ hipError_t dispatch_window_p(const NlmWindow &w, const NlmFrames &f, const float
 bool nlm_supported(int C, int P, int S)
 {
     return (C == 1 || C == 2) && P >= 0 && P <= kMaxP && S >= 0 && S <= kMaxS &&
-           tile_bytes(C, P, S, true, 16) <= (size_t)device_lds_limit() &&
-           tile_bytes(C, P, S, false, 16) <= (size_t)device_lds_limit();
+           tile_bytes(C, P, S, true, 16, 2) <= (size_t)device_lds_limit() &&
+           tile_bytes(C, P, S, false, 16, 2) <= (size_t)device_lds_limit() &&
+           tile_bytes(C, P, S, true, 16, 4) <= (size_t)device_lds_limit() &&
+           tile_bytes(C, P, S, false, 16, 4) <= (size_t)device_lds_limit();
 }
 int nlm_variant_rows(int variant) { return variant == 1 ? 8 : (variant == 2 ? 4 : 16); }
 
