@@ -18,8 +18,8 @@
 // trades the 2P halo rows each wave recomputes against the number of waves a small frame spreads
 // over; every choice gives bit-identical sums, and the tuner picks one per window shape.  The neighbour
 // frame(s) of the workgroup are staged once in LDS with their (P+s) halo, so the offset loop runs
-// with no barrier at all.  Each row group's offsets are split over SPLIT waves (2, or 4 for frames up
-// to 2^20 pixels) whose sums then add in LDS in part order, so small frames still fill the chip.
+// with no barrier at all.  On planes up to 2^20 pixels each row group's offsets are split over SPLIT
+// waves whose sums then add in LDS in part order, so small frames still fill the chip.
 // exp is v_exp_f32 (exp2) with log2(e) folded into h2 on the host.
 #include "common.h"
 
@@ -392,10 +392,14 @@ constexpr int waves_for(int C, bool pair, int split)
 }
 
 // Offset parts per row group: each workgroup runs SPLIT waves per row group, each on a consecutive
-// run of the window's offsets, so a frame puts SPLIT times the waves on the chip (frames up to 2^20
-// pixels, which would leave SIMDs idle otherwise, take four).  The part sums add in part order, the
-// same for every tile shape and both launch forms, which therefore stay bit-identical.
-inline int offset_split(int W, int H) { return (size_t)W * H <= (1u << 20) ? 4 : 2; }
+// run of the window's offset rows, so a frame puts SPLIT times the waves on the chip.  Planes up to
+// 2^20 pixels, which would leave SIMDs idle otherwise, take four (one channel) or two (two
+// channels, whose workgroups hold at most four waves); larger ones fill the chip with one (on an
+// MI355X: 640x360 spatial luma 12.3k frames/s against 6.3k at one part; 1920x1080 4:2:0 at
+// temporal radius 2, 643 frames/s against 596 at one part everywhere, 610 with four for chroma and
+// ≈620 with two for luma).  The part sums add in part order, the same for every tile shape and both
+// launch forms, which therefore stay bit-identical.
+inline int offset_split(int W, int H, int C) { return (size_t)W * H <= (1u << 20) ? (C == 1 ? 4 : 2) : 1; }
 
 // LDS of a workgroup: its tile(s), or the part sums while they combine, whichever is larger.
 size_t tile_bytes(int C, int P, int S, bool pair, int rows, int split)
@@ -449,9 +453,11 @@ hipError_t dispatch_rows(const NlmWindow &w, const float *centre, const float *f
                          const float *cf, const float *cb, float *accum, float *wsum, float *maxw, bool pair, bool conf,
                          hipStream_t s)
 {
-    if (offset_split(w.W, w.H) == 4)
-        return dispatch_split<C, P, ROWS, 4>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
-    return dispatch_split<C, P, ROWS, 2>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    switch (offset_split(w.W, w.H, C)) {
+    case 4: return dispatch_split<C, P, ROWS, 4>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    case 2: return dispatch_split<C, P, ROWS, 2>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    default: return dispatch_split<C, P, ROWS, 1>(w, centre, fwd, bwd, lut, cf, cb, accum, wsum, maxw, pair, conf, s);
+    }
 }
 
 template <int C, int P>
@@ -506,9 +512,11 @@ template <int C, int P, int ROWS>
 hipError_t dispatch_window(const NlmWindow &w, const NlmFrames &f, const float *centre, const float *lut,
                            float self_weight, float *out, hipStream_t s)
 {
-    if (offset_split(w.W, w.H) == 4)
-        return dispatch_window_split<C, P, ROWS, 4>(w, f, centre, lut, self_weight, out, s);
-    return dispatch_window_split<C, P, ROWS, 2>(w, f, centre, lut, self_weight, out, s);
+    switch (offset_split(w.W, w.H, C)) {
+    case 4: return dispatch_window_split<C, P, ROWS, 4>(w, f, centre, lut, self_weight, out, s);
+    case 2: return dispatch_window_split<C, P, ROWS, 2>(w, f, centre, lut, self_weight, out, s);
+    default: return dispatch_window_split<C, P, ROWS, 1>(w, f, centre, lut, self_weight, out, s);
+    }
 }
 
 template <int C>
