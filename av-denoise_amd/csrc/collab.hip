@@ -1137,8 +1137,8 @@ constexpr size_t kLdsBudget = 64 * 1024;
 // grids of GF frames (KM / GF volumes).
 __host__ __device__ constexpr size_t trajectory_static_lds(int C, int NR, int KM, int GF)
 {
-    return (size_t)kWaves * (16 * 4 + 2 * 64 * 4 + NR * (KM / GF) * 8 * 4 * C * 4 + NR * (KM / GF) * 4 +
-                             NR * 16 * (KM / GF - 1) * 4 + NR * 16 * 4 + NR * KM * 4 + 4);
+    return (size_t)kWaves * (16 * 4 + NR * (KM / GF) * 8 * 4 * C * 4 + NR * (KM / GF) * 4 +
+                             NR * 16 * (KM / GF - 1) * 4 + NR * 16 + NR * KM * 4 + 4);
 }
 
 // x / d for x < 2^16 and a divisor fixed for the kernel: one high multiply by m = ceil(2^32 / d)
@@ -1210,9 +1210,8 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     __shared__ uint32_t s_patch[kWaves][NR][VOLS][8][4 * C];  // every volume's anchor patch
     __shared__ uint32_t s_energy[kWaves][NR][VOLS];
     __shared__ uint32_t s_held[kWaves][NR][16][VOLS - 1];  // earlier volumes' picks, by neighbour frame
-    __shared__ uint32_t s_hcnt[kWaves][NR][16];
+    __shared__ uint8_t s_hcnt[kWaves][NR][16];  // held picks per frame (at most VOLS - 1)
     __shared__ uint32_t s_final[kWaves][NR][KM];  // the grid's members, volume by volume
-    __shared__ uint32_t s_cd[kWaves][64], s_cp[kWaves][64];  // decoded bests across a search again
     constexpr int refine = (K - 1) / 2;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int hr = lane / LPR, hl = lane % LPR, hbase = hr * LPR;
@@ -1467,13 +1466,14 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
     };
     constexpr int KP = (K + 2) / 2;  // column pairs per rectangle
     // One task loop serves the first search (the whole list, nothing held) and each search again
-    // (a volume's clashing entries, held positions skipped), so the slide is inlined once.  Across a
-    // search again, the decoded bests wait in s_cd / s_cp.
+    // (a volume's clashing entries, held positions skipped), so the slide is inlined once.  After a
+    // search again every frame of the volume decodes its best again (only the clashing ones
+    // changed; keeping the others' decoded bests in LDS across the loop cost the two-channel kernel
+    // a workgroup per CU).
     const uint16_t *tlist = s_list;
     int tn = n_valid, v = 0;
     bool held = false;
     bool filled = true;
-    uint64_t clashes = 0;
     for (;;) {
         for (int task = lane; task < tn * KP; task += 64) {
             const int k = task / KP, jp = task - k * KP;
@@ -1511,13 +1511,8 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
             };
             uint32_t cd = ~0u, cp = 0;
             if (held) {
-                // Back from searching this volume again: the clashing frames decode their new bests.
-                held = false;
-                if (hl < frames_n) {
-                    cd = s_cd[wave][lane];
-                    cp = s_cp[wave][lane];
-                    if ((clashes >> lane) & 1u) decode(cd, cp);
-                }
+                held = false;  // back from searching this volume again
+                if (hl < frames_n) decode(cd, cp);
             } else {
                 bool clash = false;
                 if (hl < frames_n) {
@@ -1538,12 +1533,8 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                         if (inside && ref0 + hr < refs) scored -= 1u;
                     }
                 }
-                clashes = __ballot(clash);
+                const uint64_t clashes = __ballot(clash);
                 if (clashes) {
-                    if (hl < frames_n) {
-                        s_cd[wave][lane] = cd;
-                        s_cp[wave][lane] = cp;
-                    }
                     if (clash) s_fbest[(hr * volumes + v) * frames_n + hl] = kNoKey;
                     // This volume's entries of the clashing (reference, frame) pairs.
                     int n_redo = 0;
@@ -1599,7 +1590,7 @@ void k_trajectory_codes(avd_collab_pass p, const uint16_t *__restrict__ ring,
                     const int tf = (int)(pick >> 26) - 1;
                     const uint32_t at = s_hcnt[wave][hr][tf];
                     s_held[wave][hr][tf][at] = pick;
-                    s_hcnt[wave][hr][tf] = at + 1;
+                    s_hcnt[wave][hr][tf] = (uint8_t)(at + 1);
                 }
             }
             wave_sync();
