@@ -75,10 +75,12 @@ def test_nlm_single_matches_the_oracle(W, H, C, P, S):
         assert np.allclose(got, want, rtol=1e-4, atol=1e-6), np.max(np.abs(got - want))
 
 
+@pytest.mark.parametrize("S", [3, 1, 0])
 @pytest.mark.parametrize("conf", [False, True])
 @pytest.mark.parametrize("C", [1, 2])
-def test_nlm_pair_matches_the_oracle(conf, C):
-    W, H, P, S = 66, 50, 3, 3
+def test_nlm_pair_matches_the_oracle(conf, C, S):
+    # Search radius 0 and 1 leave some of a workgroup's offset parts without a row of offsets.
+    W, H, P = 66, 50, 3
     rng = np.random.default_rng(3 + C)
     frames = [rng.uniform(0, 1, W * H * C).astype(np.float32) for _ in range(3)]
     h2 = float(O.h2_inv_norm(0.8, P))
