@@ -71,7 +71,7 @@ void validate_format(avd_sample_format f)
             throw Failure(AVD_ERR_INVALID_GEOMETRY, "U16 depth must be between 9 and 16, got " + std::to_string(f.depth));
         return;
     }
-    throw Failure(AVD_ERR_INVALID_GEOMETRY, "F32 planes are not supported by this build; use U8 or U16 planes");
+    throw Failure(AVD_ERR_INVALID_GEOMETRY, "F32 planes are not supported by this library; use U8 or U16 planes");
 }
 
 static void check_planes(const avd_device_plane *planes, uint32_t count, uint32_t expected, uint32_t W, uint32_t H,
@@ -351,7 +351,7 @@ static void check_estimation(const avd_motion_search &m, uint32_t temporal_radiu
 }
 
 // MotionCompensationMode::validate (motion/search.rs:226-290, MotionEstimation::validate :143-155),
-// then what this build runs of it: two pyramid levels at most, Direct estimation, no escalation.
+// then what this library runs of it: two pyramid levels at most, Direct estimation, no escalation.
 static void validate_motion_search(const avd_motion_search &m, uint32_t temporal_radius)
 {
     auto invalid = [](const std::string &what) { throw Failure(AVD_ERR_INVALID_OPTIONS, "motion-compensation " + what); };
@@ -471,14 +471,14 @@ NlmPass::NlmPass(const avd_nlmeans_options &o, uint32_t W, uint32_t H, int C, av
     confidence_ = hq_ && o.temporal_confidence && r_ > 0;
     thsad_scale_ = hq_ ? o.thsad_scale : 1.0f;
 
-    // NlmParams::validate (nlmeans/params.rs:276-340) plus this build's kernel limits.
+    // NlmParams::validate (nlmeans/params.rs:276-340) plus this library's kernel limits.
     if (P_ > 16) throw Failure(AVD_ERR_INVALID_OPTIONS, "patch_radius=" + std::to_string(P_) + " exceeds 16");
     if (S_ > 8) throw Failure(AVD_ERR_INVALID_OPTIONS, "search_radius=" + std::to_string(S_) + " exceeds 8");
     if (r_ > 8) throw Failure(AVD_ERR_INVALID_OPTIONS, "temporal_radius=" + std::to_string(r_) + " exceeds 8");
     if (!(strength > 0.0f) || !std::isfinite(strength))
         throw Failure(AVD_ERR_INVALID_OPTIONS, "strength must be finite and above 0");
     if (!nlm_supported(C, (int)P_, (int)S_))
-        throw Failure(AVD_ERR_INVALID_OPTIONS, "patch_radius above 8 (the separable path) is not supported by this build");
+        throw Failure(AVD_ERR_INVALID_OPTIONS, "patch_radius above 8 (the separable path) is not supported by this library");
     if (hq_ && !has_sigma_)
         throw Failure(AVD_ERR_INVALID_OPTIONS,
                       "HQ noise estimation is not built yet; pass a fixed sigma (HqParams::sigma_override)");
@@ -812,7 +812,7 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     if (o.transform_form == 1 && o.pooled_threshold)
         throw Failure(AVD_ERR_INVALID_OPTIONS, "the mixed transform form thresholds hard: set pooled_threshold = 0");
     mixed_ = o.transform_form == 1;
-    // The reference's grid is stride 4 with groups of 8 (collab/mod.rs:16-20); this build also runs
+    // The reference's grid is stride 4 with groups of 8 (collab/mod.rs:16-20); this library also runs
     // any stride up to 8 and groups of 16 (volume grids of 4 frames x 4 volumes, 2 x 8).
     if (o.step < 1 || o.step > 8) throw Failure(AVD_ERR_INVALID_OPTIONS, "step must be in 1..=8");
     if (o.group_size != 8 && o.group_size != 16)
@@ -820,7 +820,7 @@ Nl4dPass::Nl4dPass(const avd_nl4d_options &o, uint32_t W, uint32_t H, int C, avd
     if (W < 8 || H < 8) throw Failure(AVD_ERR_INVALID_GEOMETRY, "frames must be at least 8x8 for the patch grid");
     if (W >= 8192 || H >= 8192) throw Failure(AVD_ERR_INVALID_GEOMETRY, "frames must be below 8192 on each axis");
     if (in.kind == AVD_FORMAT_U16 && in.depth > 12)
-        throw Failure(AVD_ERR_INVALID_GEOMETRY, "grouping on more than 12-bit samples is not supported by this build");
+        throw Failure(AVD_ERR_INVALID_GEOMETRY, "grouping on more than 12-bit samples is not supported by this library");
 
     sigma_ = o.sigma;
     thsad_scale_ = o.thsad_scale;

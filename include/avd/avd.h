@@ -122,7 +122,7 @@ typedef struct avd_nlmeans_options {
     float sigma_scale;
 } avd_nlmeans_options;
 
-/* Nl4dOptions.  `step` and `group_size` are this build's reference-grid stride and group size;
+/* Nl4dOptions.  `step` and `group_size` are this library's reference-grid stride and group size;
  * the reference fixes them at 4 and 8 (collab/mod.rs:16-20).  This build runs any stride in 1..8
  * and groups of 8 or 16 members (16: volume grids of 4 frames x 4 volumes, or 2 x 8 at temporal
  * radius 1; stacks of up to 16 otherwise). */

@@ -1,9 +1,8 @@
 /*
  * avd_oracle.c -- CPU restatement of av-denoise's denoising core.
  *
- * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this file: it is the
- * checker the parity tests, the smoke check and the CPU baseline of the benchmark run the HIP
- * path against.  Every function cites the reference file:line it restates.
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this file: the parity
+ * tests, the smoke check and the CPU baseline of the benchmark run the HIP path against it.  Every function cites the reference file:line it restates.
  *
  * Arithmetic contract
  *   - Sample codes, block-match SAD/SSD, member positions, group words and the fixed-point

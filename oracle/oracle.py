@@ -1,7 +1,7 @@
 """CPU oracle for av-denoise's denoising core -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` import
-this module, and only as the checker.  The product (``av-denoise_amd``) never imports it.
+this module, and only to check results against.  The product (``av-denoise_amd``) never imports it.
 
 It wraps ``liboracle.so`` (``avd_oracle.c``) and adds, in plain Python/numpy:
 

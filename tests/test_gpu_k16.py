@@ -2,7 +2,7 @@
 
 The reference groups 8 members on a stride-4 grid (collab/mod.rs:16-20); groups of 16 (volume
 grids of 4 frames x 4 volumes, 2 x 8 at temporal radius 1, stacks of up to 16) at any stride are
-this build's extension, restated in the oracle and pinned there by mathematics
+this library's extension, restated in the oracle and pinned there by mathematics
 (tests/test_oracle_k16.py).  Here the HIP search (matrix-core and vector centre searches, the
 trajectory at one and two references per wave, the f32-ring search), the filter (direct scatter
 and binned) and the engine stream must match the oracle bit for bit; the mixed transform form stays

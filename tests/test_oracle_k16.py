@@ -1,7 +1,7 @@
 """The oracle's 16-member, stride-3 grouping, pinned by mathematics.
 
 The reference fixes the grid stride at 4 and the group at 8 members (collab/mod.rs:16-20); the
-stride-3, 16-member grid is this build's extension of it, so nothing in the reference holds its
+stride-3, 16-member grid is this library's extension of it, so nothing in the reference holds its
 answers.  These checks fix them from first principles instead:
 
 * the group Haar at 16 members (a stack of 16, grids of 4 frames x 4 volumes and 2 x 8) is an
