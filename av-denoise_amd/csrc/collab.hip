@@ -1001,11 +1001,16 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
     }
     const int cy_lo = clampi(pos_y(ref_row(0)) - S, 0, maxy), cy_hi = clampi(pos_y(ref_row(3)) + S, 0, maxy);
 
-    uint32_t L[2][KM];
+    // Each lane keeps its best KL = 8 keys per reference (and counts the keys it was offered): with
+    // sixteen members a reference's best sixteen rarely crowd one lane's two columns (about 2 % of
+    // references put more than eight there on textured 4K content), and a lane that gave up keys
+    // and runs out of kept ones during the merge sends its reference to the exact search.
+    constexpr int KL = 8;
+    uint32_t L[2][KL], offered[2] = {0u, 0u};
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int i = 0; i < KM; ++i) L[r][i] = ~0u;
+        for (int i = 0; i < KL; ++i) L[r][i] = ~0u;
     // The next row's operands load while this row's products and keys are formed (two operand
     // sets in turn, so no copy waits on a load in flight).  cy <= maxy, so every patch row exists.
     auto load_row_ops = [&](int cy, half8_t (&B)[KS][2], uint32_t (&eb)[2]) {
@@ -1035,7 +1040,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
                 const uint32_t ssd = kc_[r] + eb[t] - 2u * cross;
                 const uint32_t d = (selfrow && selfcol[t][r]) ? 0u : ssd + 1u;
                 const uint32_t key = (min(d, kKeyCap) << 10) | (uint32_t)(rowbase + colidx[t][r]);
-                insert_n_u32<KM>(L[r], key | (ok ? 0u : ~0u));
+                insert_n_u32<KL>(L[r], key | (ok ? 0u : ~0u));
+                offered[r] += ok ? 1u : 0u;
             }
         }
     };
@@ -1057,6 +1063,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
     for (int r = 0; r < 2; ++r) {
         const uint32_t inv = (65536u + (uint32_t)rw_[r] - 1u) / (uint32_t)rw_[r];  // idx / rw exact below 1024
         uint32_t last = ~0u;
+        bool short_list = false;  // this lane ran out of kept keys while a later pick could need one
 #pragma unroll
         for (int m = 0; m < KM; ++m) {
             const uint32_t head = L[r][0];
@@ -1068,14 +1075,17 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_mfma(avd_collab_pass p, 
                 if (n == m) word[r] = pack_pos_t((uint32_t)left_[r] + qx, (uint32_t)top_[r] + qy, 0);
                 if (head == best) {
 #pragma unroll
-                    for (int i = 0; i < KM - 1; ++i) L[r][i] = L[r][i + 1];
-                    L[r][KM - 1] = ~0u;
+                    for (int i = 0; i < KL - 1; ++i) L[r][i] = L[r][i + 1];
+                    L[r][KL - 1] = ~0u;
+                    if (m < KM - 1 && L[r][0] == ~0u && offered[r] > (uint32_t)KL) short_list = true;
                 }
             }
         }
         // Keys sort by the capped distance; once a chosen one is capped, their order may not be
-        // the distances' order.
-        redo[r] = last != ~0u && (last >> 10) >= kKeyCap;
+        // the distances' order.  A lane of the reference's row that ran short leaves the picks
+        // unproven.
+        const uint32_t row_short = (uint32_t)(__ballot(short_list) >> (lane & 48)) & 0xFFFFu;
+        redo[r] = (last != ~0u && (last >> 10) >= kKeyCap) || row_short != 0u;
         uint32_t k_use = 1;
         while (k_use * 2 <= (uint32_t)nlive_[r] && k_use * 2 <= p.group_size) k_use *= 2;
         // Lane n < KM writes member n; the group word follows them (lane KM for eight members,
@@ -2740,6 +2750,10 @@ bool centre_on_matrix_cores(const avd_collab_pass &p)
 
 namespace {
 
+// Workgroups of the exact centre search over the redo list (a grid-stride loop over the list's
+// count; a few percent of a 4K frame's references when a lane's short list runs out).
+constexpr unsigned kRedoBlocks = 1024;
+
 template <int KM>
 hipError_t launch_centre_km(const avd_collab_pass &p, const uint16_t *codes, uint32_t *members, hipStream_t s,
                             const CentreScratch *centre)
@@ -2760,12 +2774,12 @@ hipError_t launch_centre_km(const avd_collab_pass &p, const uint16_t *codes, uin
             k_centre_mfma<1, KM><<<gm, blk, 0, s>>>(p, codes, centre->plane,
                                                     (uint32_t)(centre->plane_odd - centre->plane), centre->energy,
                                                     members, centre->redo, nullptr);
-            k_centre_redo<1, KM><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
+            k_centre_redo<1, KM><<<kRedoBlocks, blk, 0, s>>>(p, codes, centre->redo, members);
         } else {
             k_centre_prep<2><<<pg, 256, 0, s>>>(frame, W, H, centre->plane, nullptr, centre->energy);
             k_centre_mfma<2, KM><<<gm, blk, 0, s>>>(p, codes, centre->plane, 0u, centre->energy, members,
                                                     centre->redo, nullptr);
-            k_centre_redo<2, KM><<<64, blk, 0, s>>>(p, codes, centre->redo, members);
+            k_centre_redo<2, KM><<<kRedoBlocks, blk, 0, s>>>(p, codes, centre->redo, members);
         }
         return hipGetLastError();
     }

@@ -31,6 +31,9 @@ def _scene_planes(frames, W, H, C, depth, seed, content="scene"):
         clip = avd.synth.clip(frames, W, H, depth, seed, chroma="444" if C == 2 else "gray", sigma_luma=12.0)
         return [[f[0].numpy()] if C == 1 else [f[1].numpy(), f[2].numpy()] for f in clip]
     rng = np.random.default_rng(seed)
+    if content == "stripes":  # every column constant: a column's shifts tie, so one lane holds the best
+        return [[np.broadcast_to(rng.integers(0, top + 1, (1, W)), (H, W)).copy() for _ in range(C)]
+                for _ in range(frames)]
     return [[rng.integers(0, top + 1, (H, W)) for _ in range(C)] for _ in range(frames)]
 
 
@@ -43,6 +46,8 @@ SEARCH_CASES = [
     ("12-bit, vector centre", 64, 48, 1, 12, 2, 9, 3, 12, "scene", 16, 8),
     ("radius 14, vector centre, wide keys", 64, 48, 1, 10, 2, 14, 3, 0, "scene", 16, 8),
     ("full-range noise, capped keys", 40, 32, 1, 10, 2, 3, 3, 10, "noise", 16, 8),
+    ("vertical stripes, one lane's list runs short", 64, 48, 1, 10, 2, 8, 3, 10, "stripes", 16, 8),
+    ("vertical stripes, chroma", 48, 40, 2, 10, 2, 8, 3, 10, "stripes", 16, 8),
     ("16 covering blocks, one reference per wave", 64, 48, 1, 10, 6, 6, 3, 10, "scene", 32, 24),
     ("short frame", 40, 11, 1, 10, 2, 5, 3, 10, "scene", 16, 8),
     ("tiny frame", 9, 8, 1, 10, 1, 3, 3, 10, "scene", 16, 8),
