@@ -1127,15 +1127,18 @@ __global__ __launch_bounds__(kWaves * 64) void k_centre_redo(avd_collab_pass p, 
 // neighbour frame through the refine rectangles of its covering motion blocks, skipping blocks
 // under c_min, positions inside a rectangle already searched for that frame and positions the
 // group already holds; the best per frame joins the volume.  When every volume fills and the
-// stack holds eight, the members become the volume grid.  Needs temporal_radius >= 1.
+// stack holds the group size, the members become the volume grid.  Needs temporal_radius >= 1.
 //
 // K = 2 * refine + 1.  A wave serves NR references (lanes r * 64 / NR onwards keep reference r's
 // lists).  The rectangles of every volume are built up front (entry (v, t, b), b the covering
-// block in row-major order, padded to a power-of-two square); per volume the non-empty ones of all
-// NR references are compacted into one list and lane task k * K + col slides column col of
-// rectangle k, so the references share the lanes their own few rectangles would leave idle.  Each
-// frame's best (distance, walk index) is an LDS atomic min; lane t of a reference's lanes decodes
-// frame t, and the frames join the volume through the reference's ordered insertion.
+// block in row-major order, padded to a power-of-two square); the non-empty ones of all volumes
+// and all NR references are compacted into one list and lane task k * KP + jp slides column pair
+// jp of rectangle k (two adjacent columns from one set of row loads), so the references and
+// volumes share the lanes their own few rectangles would leave idle.  Each (reference, volume,
+// frame) best (distance, walk index) is an LDS atomic min; the volumes then resolve in order, lane
+// t of a reference's lanes decoding frame t (a best an earlier volume already holds is searched
+// again without the held positions), and the frames join the volume through the reference's
+// ordered insertion.
 // LDS of one trajectory workgroup with NR references per wave: the rectangle tables (dynamic) and
 // the lists, frame slots, per-frame bests, anchor patches, energies and held picks, and the scored
 // counters' per-wave sums (static).
