@@ -223,6 +223,9 @@ SIGNATURES = {
     "avd_site_nlm_pair": (ctypes.c_int, [ctypes.POINTER(NlmWindow), P, P, P, P, P, P, P, P, P]),
     "avd_site_nlm_finish": (ctypes.c_int, [P, P, P, P, f32, u32, u32, P, P]),
     "avd_site_noise_spatial": (ctypes.c_int, [P, u32, u32, u32, u32, P, P, u32, P]),
+    "avd_site_noise_temporal_stats": (ctypes.c_int, [P, u32, u32, u32, u32, u32, P, u32, u32, u32, u32, u32, P, P]),
+    "avd_site_noise_temporal_zero": (ctypes.c_int, [P, u32, P]),
+    "avd_noise_temporal_stats_len": (u32, [u32, u32, u32]),
     "avd_host_denoiser_create": (ctypes.c_int, [ctypes.POINTER(EngineSpec), ctypes.POINTER(P)]),
     "avd_host_denoiser_destroy": (None, [P]),
     "avd_host_denoiser_plane_lengths": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64),

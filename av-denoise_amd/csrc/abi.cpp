@@ -445,6 +445,33 @@ avd_status avd_site_noise_spatial_noalloc(const float *ring_values, uint32_t wid
 
 uint32_t avd_noise_partials_len(uint32_t width, uint32_t height) { return avd::noise_partials_len(width, height); }
 
+avd_status avd_site_noise_temporal_stats(const float *ring_values, uint32_t width, uint32_t height, uint32_t channels,
+                                         uint32_t slot_new, uint32_t slot_prev, const int32_t *mv_field,
+                                         uint32_t mv_offset, uint32_t mv_step, uint32_t mv_blocks_x,
+                                         uint32_t mv_blocks_y, uint32_t luma_fields, float *stats, void *stream)
+{
+    if (!ring_values || !stats) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (width == 0 || height == 0 || width > (1u << 20) || height > (1u << 20) || (channels != 1 && channels != 2))
+        return fail(AVD_ERR_INVALID_GEOMETRY, "temporal statistics need a non-empty frame of 1 or 2 channels");
+    if (mv_field && (mv_step == 0 || mv_blocks_x == 0 || mv_blocks_y == 0))
+        return fail(AVD_ERR_INVALID_OPTIONS, "a motion field needs a non-zero step and block grid");
+    return hip_status(avd::launch_temporal_stats(ring_values, width, height, channels, slot_new, slot_prev, mv_field,
+                                                 mv_offset, mv_step, mv_blocks_x, mv_blocks_y, luma_fields != 0, stats,
+                                                 static_cast<hipStream_t>(stream)),
+                      "noise_temporal_stats");
+}
+
+avd_status avd_site_noise_temporal_zero(float *stats, uint32_t len, void *stream)
+{
+    if (!stats) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    return hip_status(avd::launch_temporal_zero(stats, len, static_cast<hipStream_t>(stream)), "noise_temporal_zero");
+}
+
+uint32_t avd_noise_temporal_stats_len(uint32_t width, uint32_t height, uint32_t channels)
+{
+    return avd::temporal_stats_len(width, height, channels);
+}
+
 float avd_noise_sigma_from_abs_sum(float abs_sum, uint32_t width, uint32_t height)
 {
     // The 3x3 mask has no interior response below 3 samples on an axis (the reference's version

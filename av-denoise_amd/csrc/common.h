@@ -126,6 +126,15 @@ hipError_t launch_noise_spatial(const float *ring, uint32_t width, uint32_t heig
 // (kNoiseScratchFloats floats), resolved on the first call per device.
 constexpr uint32_t kNoiseScratchFloats = AVD_NOISE_SCRATCH_FLOATS;
 hipError_t noise_scratch_address(float **out);
+// Temporal block statistics (noise.hip): one record of 3 C + 26 floats per 16x16 block of frame
+// slot_new against slot_prev of the same ring; mv (may be null) holds whole-pixel vectors of an
+// mv_step grid of mv_bx x mv_by blocks from mv[mv_offset].
+uint32_t temporal_stats_len(uint32_t width, uint32_t height, uint32_t channels);
+hipError_t launch_temporal_stats(const float *ring, uint32_t width, uint32_t height, uint32_t channels,
+                                 uint32_t slot_new, uint32_t slot_prev, const int32_t *mv, uint32_t mv_offset,
+                                 uint32_t mv_step, uint32_t mv_bx, uint32_t mv_by, bool luma_fields, float *stats,
+                                 hipStream_t s);
+hipError_t launch_temporal_zero(float *stats, uint32_t len, hipStream_t s);
 hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,
                                    int C, hipStream_t s);
 

@@ -166,6 +166,45 @@ def noise_spatial_measure(avd, torch, clip, calls=20):
             "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4)}
 
 
+def noise_temporal_measure(avd, torch, clip, calls=20):
+    """The temporal block statistics of the clip's second 4K luma frame against its first (f32
+    samples in a two-frame ring in HBM, quarter fields on), with the zero vector and with a motion
+    field of the engine's 8-pixel grid (the gain check on): average time per call from HIP events
+    on torch's stream, against the HBM roofline of one read of both frames plus the records."""
+    import ctypes
+
+    lib = avd._lib.load()
+    ring = torch.stack([clip[f][0].to(torch.float32) / float((1 << DEPTH) - 1) for f in (0, 1)]).contiguous()
+    n = lib.avd_noise_temporal_stats_len(W, H, 1)
+    stats = torch.empty(n, dtype=torch.float32, device="cuda")
+    mbx, mby = -(-W // 8), -(-H // 8)
+    field = torch.randint(-2, 3, (mbx * mby * 2,), dtype=torch.int32, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for name, mv in (("zero_vector", None), ("motion_field", field)):
+        def call():
+            avd._lib.check(lib.avd_site_noise_temporal_stats(
+                ctypes.c_void_p(ring.data_ptr()), W, H, 1, 1, 0,
+                None if mv is None else ctypes.c_void_p(mv.data_ptr()), 0, 8, mbx, mby, 1,
+                ctypes.c_void_p(stats.data_ptr()), stream))
+
+        for _ in range(3):
+            call()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(calls):
+            call()
+        end.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(end) / calls
+        nbytes = 2 * W * H * 4 + n * 4
+        gbps = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"avg_launch_ms": round(ms, 4), "bytes_per_launch": nbytes, "achieved": round(gbps, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+    return {"kernels": "k_temporal_stats", "workload": "3840x2160 luma frame against the previous one, f32 samples, "
+            "quarter fields on", **out}
+
+
 def run_steps(eng, frames, outs, start, count):
     done = 0
     for i in range(count):
@@ -433,7 +472,8 @@ def main():
         "dominant_stage": dominant,
         "forms": {f: {"frames_per_s": forms[f]["value"], "psnr_db_vs_clean": psnr.get(f)} for f in forms},
     }
-    result["next_rows"] = {"noise_spatial": noise_spatial_measure(avd, torch, clip)}
+    result["next_rows"] = {"noise_spatial": noise_spatial_measure(avd, torch, clip),
+                           "noise_temporal": noise_temporal_measure(avd, torch, clip)}
     if not args.no_nlm:
         result["nlm"] = {name: nlm_leg(avd, torch, name, cfg) for name, cfg in NLM_CONFIGS.items()}
     if "mixed" in forms and "fp32" in forms:

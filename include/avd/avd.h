@@ -467,6 +467,28 @@ uint32_t avd_noise_partials_len(uint32_t width, uint32_t height);
  * height is below 3, where the mask has no interior sample. */
 float avd_noise_sigma_from_abs_sum(float abs_sum, uint32_t width, uint32_t height);
 
+/* Temporal block statistics: one record per 16x16 block of frame `slot_new` of a ring of f32
+ * [frame][height][width][channels] samples, its residual taken against frame `slot_prev`
+ * (run_temporal_noise_stats, noise/temporal.rs:206-257; kernel nlm_temporal_noise_stats,
+ * noise/kernels/stats.rs:142-604).  `stats` holds avd_noise_temporal_stats_len floats: the blocks
+ * row-major, each record 3 channels + 26 floats laid out as temporal_stats_record_len and
+ * texture_lane say (noise/temporal.rs:113-131): every channel's sum_d, every sum_d2, the lag-1 sum,
+ * channel 0's texture, the moved flag, four quarter records of six fields (sum_d, sum_d2, luma sum,
+ * flatness, luma min, luma max; zero when `luma_fields` is 0), then every further channel's
+ * texture.  `mv_field` NULL measures the zero vector; otherwise each block reads the whole-pixel
+ * vector (x, y) of the motion block covering its origin, from the `mv_step` grid of mv_blocks_x by
+ * mv_blocks_y blocks at mv_field[mv_offset..], and follows it when its gain check passes.  The
+ * records are bit-identical to the reference's f32 records. */
+avd_status avd_site_noise_temporal_stats(const float *ring_values, uint32_t width, uint32_t height, uint32_t channels,
+                                         uint32_t slot_new, uint32_t slot_prev, const int32_t *mv_field,
+                                         uint32_t mv_offset, uint32_t mv_step, uint32_t mv_blocks_x,
+                                         uint32_t mv_blocks_y, uint32_t luma_fields, float *stats, void *stream);
+/* zero_temporal_stats_slot (noise/temporal.rs:260-287; kernel nlm_temporal_stats_zero,
+ * noise/kernels/stats.rs:662-677): `len` zeroes from `stats`, by a kernel (no memset node). */
+avd_status avd_site_noise_temporal_zero(float *stats, uint32_t len, void *stream);
+/* temporal_stats_slot_len (noise/temporal.rs:141-144): floats of one frame's records. */
+uint32_t avd_noise_temporal_stats_len(uint32_t width, uint32_t height, uint32_t channels);
+
 #ifdef __cplusplus
 }
 #endif

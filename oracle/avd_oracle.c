@@ -1057,3 +1057,215 @@ void avdo_noise_reduce(const float *partials, uint32_t count, float *results, ui
     results[(size_t)slot * 2] = r0;
     results[(size_t)slot * 2 + 1] = r1;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Temporal block statistics, nlm_temporal_noise_stats (noise/kernels/stats.rs:142-660) with  */
+/* the record layout of noise/temporal.rs:18-131.                                            */
+/* ------------------------------------------------------------------------------------------ */
+
+/* One 16x16 block's pixels are numbered t = local_y * 16 + local_x.  Every f32 sum below runs in the
+ * reference's order: the lane totals add t = 0..255 in turn, the halving trees add slot s + stride
+ * into slot s for stride = n/2 .. 1, the cell means add their pixels row by row and divide. */
+static void tree_sum(float *v, uint32_t n)
+{
+    for (uint32_t stride = n / 2; stride >= 1; stride /= 2)
+        for (uint32_t i = 0; i < stride; ++i) v[i] += v[i + stride];
+}
+
+/* The 16 means of the 4x4 cells of a 16x16 tile (write_cell_means, stats.rs:606-624). */
+static void cell_means16(const float *tile, float *cells)
+{
+    for (uint32_t c = 0; c < 16; ++c) {
+        const uint32_t cx = c % 4, cy = c / 4;
+        float cell = 0.0f;
+        for (uint32_t dy = 0; dy < 4; ++dy)
+            for (uint32_t dx = 0; dx < 4; ++dx) cell += tile[(4 * cy + dy) * 16 + 4 * cx + dx];
+        cells[c] = cell / 16.0f;
+    }
+}
+
+/* The mean squared step between adjacent cells that lie fully inside the block, or 0 with no
+ * such pair (cell_texture, stats.rs:626-656). */
+static float cell_texture16(const float *cells, uint32_t bw, uint32_t bh)
+{
+    const uint32_t cols = bw / 4, rows = bh / 4;
+    float energy = 0.0f;
+    uint32_t pairs = 0;
+    for (uint32_t cy = 0; cy < rows; ++cy)
+        for (uint32_t cx = 0; cx < cols; ++cx) {
+            const float here = cells[cy * 4 + cx];
+            if (cx + 1 < cols) {
+                const float step = here - cells[cy * 4 + cx + 1];
+                energy += step * step;
+                pairs++;
+            }
+            if (cy + 1 < rows) {
+                const float step = here - cells[(cy + 1) * 4 + cx];
+                energy += step * step;
+                pairs++;
+            }
+        }
+    return pairs ? energy / (float)pairs : 0.0f;
+}
+
+uint32_t avdo_temporal_record_len(uint32_t C) { return 2 * C + 3 + 4 * 6 + C - 1; }
+
+/* Every block's record of frame slot_new against slot_prev of an f32 ring [slot][H][W][C].
+ * mv (may be NULL: uncompensated) holds whole-pixel vectors (x, y) of an mv_step grid of
+ * mv_bx x mv_by blocks from mv[mv_offset]; a block reads the one covering its origin. */
+void avdo_temporal_stats(const float *ring, uint32_t W, uint32_t H, uint32_t C, uint32_t slot_new,
+                         uint32_t slot_prev, const int32_t *mv, uint32_t mv_offset, uint32_t mv_step,
+                         uint32_t mv_bx, uint32_t mv_by, int luma_fields, float *stats)
+{
+    const uint32_t B = 16, T = 256, bx_n = (W + B - 1) / B, by_n = (H + B - 1) / B;
+    const uint32_t rec = avdo_temporal_record_len(C);
+    const float *fn = ring + (size_t)slot_new * W * H * C, *fp = ring + (size_t)slot_prev * W * H * C;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (uint32_t b = 0; b < bx_n * by_n; ++b) {
+        const uint32_t ox = (b % bx_n) * B, oy = (b / bx_n) * B;
+        const uint32_t bw = W - ox < B ? W - ox : B, bh = H - oy < B ? H - oy : B;
+        float *out = stats + (size_t)b * rec;
+        int sx = 0, sy = 0, matched = 1;
+        float moved = 0.0f;
+        if (mv) {
+            /* The gain check (stats.rs:257-341): zero-vector and vector residual variances of
+             * channel 0 by halving trees over the 256 pixels. */
+            uint32_t col = ox / mv_step, row = oy / mv_step;
+            if (col > mv_bx - 1) col = mv_bx - 1;
+            if (row > mv_by - 1) row = mv_by - 1;
+            const int32_t vx = mv[mv_offset + (row * mv_bx + col) * 2], vy = mv[mv_offset + (row * mv_bx + col) * 2 + 1];
+            float r[4][256];
+            for (uint32_t t = 0; t < T; ++t) {
+                const uint32_t x = ox + t % B, y = oy + t / B;
+                float zr = 0.0f, vr = 0.0f;
+                if (x < W && y < H) {
+                    const float nl = fn[((size_t)y * W + x) * C];
+                    zr = nl - fp[((size_t)y * W + x) * C];
+                    const int64_t mx = (int64_t)x + vx, my = (int64_t)y + vy;
+                    if (mx >= 0 && mx < (int64_t)W && my >= 0 && my < (int64_t)H)
+                        vr = nl - fp[((size_t)my * W + (size_t)mx) * C];
+                }
+                r[0][t] = zr;
+                r[1][t] = zr * zr;
+                r[2][t] = vr;
+                r[3][t] = vr * vr;
+            }
+            for (int k = 0; k < 4; ++k) tree_sum(r[k], T);
+            const int64_t left = (int64_t)ox + vx, top = (int64_t)oy + vy, right = left + bw, bottom = top + bh;
+            int64_t span_w = (right < (int64_t)W ? right : (int64_t)W) - (left > 0 ? left : 0);
+            int64_t span_h = (bottom < (int64_t)H ? bottom : (int64_t)H) - (top > 0 ? top : 0);
+            if (span_w < 0) span_w = 0;
+            if (span_h < 0) span_h = 0;
+            const float matched_px = (float)(span_w * span_h), block_px = (float)(bw * bh);
+            const float zmean = r[0][0] / block_px;
+            const float zvar = r[1][0] / block_px - zmean * zmean;
+            const float safe = matched_px > 1.0f ? matched_px : 1.0f;
+            const float vmean = r[2][0] / safe;
+            const float vvar = r[3][0] / safe - vmean * vmean;
+            const int has_vector = vx != 0 || vy != 0;
+            const int follows = has_vector && (matched_px == 0.0f || vvar <= 0.5f * zvar);
+            if (follows) {
+                sx = vx;
+                sy = vy;
+                matched = left >= 0 && right <= (int64_t)W && top >= 0 && bottom <= (int64_t)H;
+                if (matched) moved = 1.0f;
+            }
+        }
+        /* Per pixel: the residual of each channel, its square, the lag-1 product of channel 0 with
+         * the next pixel of the row, the new frame's values and the luma fields. */
+        float res[2][256], nv[2][256], lane[5][256], mean_tile[256];
+        float qsum[256], qmin[256], qmax[256], qd[256], qd2[256];
+        for (uint32_t t = 0; t < T; ++t) {
+            const uint32_t lx = t % B, ly = t / B, x = ox + lx, y = oy + ly;
+            const int valid = x < W && y < H && matched;
+            float nl_raw = 0.0f, mean_raw = 0.0f, mn = 1.0e30f, mx = -1.0e30f;
+            for (uint32_t c = 0; c < 2; ++c) res[c][t] = nv[c][t] = 0.0f;
+            if (valid) {
+                const size_t pn = ((size_t)y * W + x) * C;
+                const size_t pp = ((size_t)(y + sy) * W + (size_t)(x + sx)) * C;
+                for (uint32_t c = 0; c < C; ++c) {
+                    res[c][t] = fn[pn + c] - fp[pp + c];
+                    nv[c][t] = fn[pn + c];
+                }
+                nl_raw = fn[pn];
+                mean_raw = 0.5f * (fn[pn] + fp[pp]);
+                mn = mx = fn[pn];
+            }
+            for (uint32_t c = 0; c < C; ++c) {
+                lane[c][t] = res[c][t];
+                lane[C + c][t] = res[c][t] * res[c][t];
+            }
+            const uint32_t slot = ((ly / 8) * 2 + lx / 8) * 64 + (ly % 8) * 8 + lx % 8;
+            qsum[slot] = nl_raw;
+            qmin[slot] = mn;
+            qmax[slot] = mx;
+            qd[slot] = res[0][t];
+            qd2[slot] = res[0][t] * res[0][t];
+            mean_tile[t] = mean_raw;
+        }
+        for (uint32_t t = 0; t < T; ++t) {
+            const uint32_t lx = t % B, y = oy + t / B, x = ox + lx;
+            const int valid = x < W && y < H && matched;
+            lane[2 * C][t] = valid && lx + 1 < bw ? res[0][t] * res[0][t + 1] : 0.0f;
+        }
+        for (uint32_t l = 0; l < 2 * C + 1; ++l) {
+            float total = 0.0f;
+            for (uint32_t t = 0; t < T; ++t) total += lane[l][t];
+            out[l] = total;
+        }
+        float cells[16];
+        for (uint32_t c = 0; c < C; ++c) {
+            cell_means16(nv[c], cells);
+            out[c == 0 ? 2 * C + 1 : 2 * C + 3 + 24 + c - 1] = cell_texture16(cells, bw, bh);
+        }
+        out[2 * C + 2] = moved;
+        float *quarters = out + 2 * C + 3;
+        if (!luma_fields) {
+            for (int k = 0; k < 24; ++k) quarters[k] = 0.0f;
+            continue;
+        }
+        for (uint32_t q = 0; q < 4; ++q) {
+            tree_sum(qsum + 64 * q, 64);
+            tree_sum(qd + 64 * q, 64);
+            tree_sum(qd2 + 64 * q, 64);
+            for (uint32_t stride = 32; stride >= 1; stride /= 2)
+                for (uint32_t i = 0; i < stride; ++i) {
+                    qmin[64 * q + i] = fminf(qmin[64 * q + i], qmin[64 * q + i + stride]);
+                    qmax[64 * q + i] = fmaxf(qmax[64 * q + i], qmax[64 * q + i + stride]);
+                }
+        }
+        /* Flatness (stats.rs:520-585): an 8x8 grid of 2x2 means of the temporal mean, each cell's
+         * squared steps to its right and lower neighbour inside its quarter, 16 scores a quarter
+         * summed by a halving tree, over the quarter's 24 pairs. */
+        float grid[64], pair[64];
+        for (uint32_t c = 0; c < 64; ++c) {
+            const uint32_t cy = c / 8, cx = c % 8;
+            float cell = 0.0f;
+            for (uint32_t dy = 0; dy < 2; ++dy)
+                for (uint32_t dx = 0; dx < 2; ++dx) cell += mean_tile[(2 * cy + dy) * 16 + 2 * cx + dx];
+            grid[c] = cell / 4.0f;
+        }
+        for (uint32_t c = 0; c < 64; ++c) {
+            const uint32_t cy = c / 8, cx = c % 8;
+            float e = 0.0f;
+            if (cx % 4 != 3) e += (grid[c] - grid[c + 1]) * (grid[c] - grid[c + 1]);
+            if (cy % 4 != 3) e += (grid[c] - grid[c + 8]) * (grid[c] - grid[c + 8]);
+            pair[((cy / 4) * 2 + cx / 4) * 16 + (cy % 4) * 4 + cx % 4] = e;
+        }
+        for (uint32_t q = 0; q < 4; ++q) tree_sum(pair + 16 * q, 16);
+        for (uint32_t q = 0; q < 4; ++q) {
+            float *f = quarters + 6 * q;
+            if (!matched) {
+                for (int k = 0; k < 6; ++k) f[k] = 0.0f;
+                continue;
+            }
+            const int full = bw >= (q % 2 + 1) * 8 && bh >= (q / 2 + 1) * 8;
+            f[0] = qd[64 * q];
+            f[1] = qd2[64 * q];
+            f[2] = qsum[64 * q];
+            f[3] = full ? pair[16 * q] / 24.0f : 3.0e38f;
+            f[4] = qmin[64 * q];
+            f[5] = qmax[64 * q];
+        }
+    }
+}

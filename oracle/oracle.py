@@ -915,3 +915,55 @@ def sigma_block_p25_from_partials(partials, channels: int, width: int, height: i
         if sigmas[c]:
             out[c] = lower_quartile(sorted(sigmas[c]))
     return out
+
+
+# ---- temporal block statistics (noise/temporal.rs:18-131, noise/kernels/stats.rs:142-660) --------
+
+TEMPORAL_BLOCK = 16
+TEMPORAL_QUARTER_BASE, TEMPORAL_QUARTER_FIELDS = 3, 6  # noise/temporal.rs:36-38
+BLOCK_TEXTURE_LANE, BLOCK_MOVED_LANE = 1, 2            # noise/temporal.rs:27-31
+QUARTER_SUM_D, QUARTER_SUM_D2, QUARTER_LUMA_SUM, QUARTER_FLATNESS, QUARTER_LUMA_MIN, QUARTER_LUMA_MAX = range(6)
+UNMEASURED_FLATNESS = np.float32(3.0e38)               # noise/temporal.rs:62
+COMPENSATION_GAIN_MAX = np.float32(0.5)                # noise/temporal.rs:44
+
+
+def temporal_record_len(channels: int) -> int:
+    """temporal_stats_record_len (noise/temporal.rs:117-119)."""
+    return 2 * channels + TEMPORAL_QUARTER_BASE + 4 * TEMPORAL_QUARTER_FIELDS + channels - 1
+
+
+def temporal_texture_lane(channels: int, channel: int) -> int:
+    """texture_lane (noise/temporal.rs:125-131)."""
+    if channel == 0:
+        return 2 * channels + BLOCK_TEXTURE_LANE
+    return 2 * channels + TEMPORAL_QUARTER_BASE + 4 * TEMPORAL_QUARTER_FIELDS + channel - 1
+
+
+def temporal_blocks(width: int, height: int) -> tuple[int, int]:
+    """temporal_stats_blocks (noise/temporal.rs:134-139): ragged edges truncated."""
+    return -(-width // TEMPORAL_BLOCK), -(-height // TEMPORAL_BLOCK)
+
+
+def temporal_slot_len(width: int, height: int, channels: int) -> int:
+    """temporal_stats_slot_len (noise/temporal.rs:141-144)."""
+    bx, by = temporal_blocks(width, height)
+    return bx * by * temporal_record_len(channels)
+
+
+def temporal_stats(ring: np.ndarray, width: int, height: int, channels: int, slot_new: int, slot_prev: int,
+                   luma_fields: bool, motion=None) -> np.ndarray:
+    """nlm_temporal_noise_stats (noise/kernels/stats.rs:191-604) over an f32 ring [slot][H][W][C]:
+    one record per 16x16 block, in the reference's summation orders (avdo_temporal_stats).
+    ``motion`` is None or (field i32, offset, step, blocks_x, blocks_y) with vectors (x, y)."""
+    ring = np.ascontiguousarray(ring, np.float32).reshape(-1)
+    out = np.full(temporal_slot_len(width, height, channels), np.nan, np.float32)
+    if motion is None:
+        field, args = None, (0, 1, 1, 1)
+    else:
+        field = np.ascontiguousarray(motion[0], np.int32)
+        args = tuple(int(v) for v in motion[1:])
+    lib().avdo_temporal_stats(_p(ring, F32P), ctypes.c_uint32(width), ctypes.c_uint32(height),
+                              ctypes.c_uint32(channels), ctypes.c_uint32(slot_new), ctypes.c_uint32(slot_prev),
+                              None if field is None else _p(field, I32P), *[ctypes.c_uint32(v) for v in args],
+                              ctypes.c_int(1 if luma_fields else 0), _p(out, F32P))
+    return out
