@@ -33,6 +33,7 @@ SOURCES = {
     "collab.hip": EXACT + MFMA_VGPR,
     "nlm.hip": [],
     "noise.hip": EXACT,
+    "grain.hip": EXACT,
     "engine.cpp": EXACT,
     "host.cpp": EXACT,
     "tune.cpp": EXACT,

@@ -472,6 +472,30 @@ uint32_t avd_noise_temporal_stats_len(uint32_t width, uint32_t height, uint32_t 
     return avd::temporal_stats_len(width, height, channels);
 }
 
+avd_status avd_site_grain_measure(const float *ring_values, const float *denoised, uint32_t width, uint32_t height,
+                                  uint32_t channels, uint32_t channel, uint32_t slot_t, uint32_t slot_next,
+                                  const int32_t *saved_mv, const float *saved_conf, uint32_t blocks_x,
+                                  uint32_t blocks_y, uint32_t step, uint32_t source_entry, uint32_t has_source,
+                                  const float *luma_means, uint32_t has_luma_means, uint32_t means_blocks_x,
+                                  uint32_t luma_band, const float *edges, int32_t *hist, float *partials,
+                                  void *stream)
+{
+    if (!ring_values || !denoised || !saved_mv || !saved_conf || !edges || !hist || !partials ||
+        (means_blocks_x > 0 && !luma_means))
+        return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (width == 0 || height == 0 || width > (1u << 16) || height > (1u << 16) || channels < 1 || channels > 4 ||
+        channel >= channels)
+        return fail(AVD_ERR_INVALID_GEOMETRY, "grain measurement needs a frame of 1 to 4 channels and a channel inside it");
+    if (blocks_x == 0 || blocks_y == 0 || step == 0)
+        return fail(AVD_ERR_INVALID_OPTIONS, "grain measurement needs a non-empty motion block grid and step");
+    avd::GrainMeasure m{ring_values, denoised, saved_mv, saved_conf, edges, luma_means, hist, partials,
+                        width, height, channels, channel, slot_t, slot_next, blocks_x, blocks_y, step,
+                        source_entry, means_blocks_x, has_source != 0, has_luma_means != 0, luma_band != 0};
+    return hip_status(avd::launch_grain_measure(m, static_cast<hipStream_t>(stream)), "grain_measure");
+}
+
+uint32_t avd_grain_partials_len(uint32_t width, uint32_t height) { return avd::grain_partials_len(width, height); }
+
 float avd_noise_sigma_from_abs_sum(float abs_sum, uint32_t width, uint32_t height)
 {
     // The 3x3 mask has no interior response below 3 samples on an axis (the reference's version

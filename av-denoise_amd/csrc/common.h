@@ -135,6 +135,21 @@ hipError_t launch_temporal_stats(const float *ring, uint32_t width, uint32_t hei
                                  uint32_t mv_step, uint32_t mv_bx, uint32_t mv_by, bool luma_fields, float *stats,
                                  hipStream_t s);
 hipError_t launch_temporal_zero(float *stats, uint32_t len, hipStream_t s);
+// Film-grain measurement (grain.hip): every 8x8 cell of channel `channel` of ring slots slot_t /
+// slot_next (f32 [slot][H][W][C]) against the denoised frame (f32 [H][W][C]); counts into the
+// frame's 2 x 16 x 64 i32 histograms, one 48-float partial per cell.
+struct GrainMeasure {
+    const float *ring, *denoised;
+    const int32_t *saved_mv;
+    const float *saved_conf, *edges, *luma_means;
+    int32_t *hist;
+    float *partials;
+    uint32_t width, height, channels, channel, slot_t, slot_next;
+    uint32_t blocks_x, blocks_y, step, source_entry, means_blocks_x;
+    bool has_source, has_luma_means, luma_band;
+};
+uint32_t grain_partials_len(uint32_t width, uint32_t height);
+hipError_t launch_grain_measure(const GrainMeasure &m, hipStream_t s);
 hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,
                                    int C, hipStream_t s);
 

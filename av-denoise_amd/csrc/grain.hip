@@ -1,0 +1,232 @@
+// Film-grain measurement (grain_measure, nl4d/kernels/grain.rs:243-513; constants
+// nl4d/grain/consts.rs:1-45).
+//
+// One wave per 8x8 cell of channel `channel` at a time, lane l on pixel (l % 8, l / 8); a workgroup
+// of four waves walks 64 consecutive cells, the four waves on neighbouring cells.  Source grain is
+// the motion-compensated difference of ring slots slot_next and slot_t over sqrt 2; one-frame grain
+// is the high-pass std of slot_t over the cell.  A cell that
+// passes the flatness, band, clipping, interior, confidence and std gates adds one count to its
+// brightness bin and std bucket of the source histogram (i32 atomics); a usable cell adds one to
+// the one-frame histogram.  Every cell writes its partial: the 46 lag sums of its mean-free source
+// grain over the cell and a halo of 6 columns each side and 3 rows below (zeros when rejected), its
+// pixel count and its strength group.  Every f32 sum follows the reference's order: the cell sums
+// are 64-slot halving trees (wave shuffles pair the same slots), each lag sum adds the 64 pixels row
+// by row; so the histograms and partials are bit-identical to the reference's.
+
+#include "common.h"
+
+namespace avd {
+namespace {
+
+constexpr int kCell = 8, kTileW = 20, kTileH = 11, kHalo = 6, kLags = 46, kPartial = 48;
+constexpr int kBuckets = 64, kBins = 16, kHist = kBins * kBuckets;
+constexpr int kWaves = 4, kCellsPerGroup = 64;  // 16 cells per wave
+
+__device__ inline float tree64(float v)
+{
+#pragma unroll
+    for (int stride = 32; stride >= 1; stride >>= 1) v += __shfl_down(v, stride);
+    return __shfl(v, 0);
+}
+
+__device__ inline float min64(float v)
+{
+#pragma unroll
+    for (int stride = 32; stride >= 1; stride >>= 1) v = fminf(v, __shfl_xor(v, stride));
+    return v;
+}
+
+__device__ inline float max64(float v)
+{
+#pragma unroll
+    for (int stride = 32; stride >= 1; stride >>= 1) v = fmaxf(v, __shfl_xor(v, stride));
+    return v;
+}
+
+// Rust's saturating float-to-u32 cast, for the brightness bins.
+__device__ inline uint32_t sat_u32(float v) { return !(v >= 0.0f) ? 0u : v >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)v; }
+
+__device__ inline int shift_clamp(int v, int s, int limit) { return min(max(v + s, 0), limit - 1); }
+
+__device__ inline uint32_t block_for(int cx, int cy, uint32_t bx, uint32_t by, uint32_t step)
+{
+    return min((uint32_t)(cx * kCell) / step, bx - 1) + min((uint32_t)(cy * kCell) / step, by - 1) * bx;
+}
+
+// The highest bucket whose lower edge is at or below s (bucket_for, grain.rs:193-205).
+__device__ inline uint32_t bucket_for(float s, const float *edges)
+{
+    uint32_t b = 0;
+    for (uint32_t i = 1; i < kBuckets; ++i) b = s >= edges[i] ? i : b;
+    return b;
+}
+
+__device__ inline uint32_t luma_bin(float mean) { return min(sat_u32(mean * (float)kBins), (uint32_t)kBins - 1); }
+
+__global__ __launch_bounds__(64 * kWaves) void k_grain_measure(
+    const float *__restrict__ ring, const float *__restrict__ out_t, const int32_t *__restrict__ saved_mv,
+    const float *__restrict__ saved_conf, const float *__restrict__ edges, const float *__restrict__ luma_means,
+    int *__restrict__ hist, float *__restrict__ partials, int W, int H, int C, int ch, uint32_t slot_t,
+    uint32_t slot_next, uint32_t bx, uint32_t by, uint32_t step, uint32_t entry, int has_source, int has_luma_means,
+    uint32_t means_bx, int luma_band)
+{
+    // The workgroup's counts gather in LDS and reach `hist` as one atomic per non-empty bin: a flat
+    // region sends every cell to the same bin, where per-cell global atomics would serialise.
+    __shared__ int s_hist[2 * kHist];
+    __shared__ float s_tile[kWaves][kTileW * kTileH];
+    const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int i = t; i < 2 * kHist; i += 64 * kWaves) s_hist[i] = 0;
+    __syncthreads();
+
+    const int cells_x = W / kCell, cells_y = H / kCell, cells = cells_x * cells_y;
+    const int lx = lane & 7, ly = lane >> 3;
+    const size_t plane = (size_t)W * H;
+    const float *ft = ring + (size_t)slot_t * plane * C + ch, *fnx = ring + (size_t)slot_next * plane * C + ch;
+    const uint32_t blocks = bx * by, base = entry * blocks * 2;
+    constexpr float kR2 = 0.70710677f;
+    float *tile = s_tile[wave];
+
+    // Wave w takes cells first + w, first + w + 4, ...: the four waves work on neighbouring cells.
+    const int first = (int)blockIdx.x * kCellsPerGroup;
+    for (int k = 0; k < kCellsPerGroup / kWaves; ++k) {
+        const int cell = first + wave + k * kWaves;
+        if (cell >= cells) break;  // uniform per wave; only the final barrier follows the loop
+        // The previous cell's lag sums have read the tile.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int cx = cell % cells_x, cy = cell / cells_x, ox = cx * kCell, oy = cy * kCell;
+        const int x = ox + lx, y = oy + ly;
+        const uint32_t blk = block_for(cx, cy, bx, by, step);
+        const int dx = saved_mv[base + blk * 2], dy = saved_mv[base + blk * 2 + 1];
+
+        const float cur = ft[((size_t)y * W + x) * C];
+        const float nxt = fnx[((size_t)shift_clamp(y, dy, H) * W + shift_clamp(x, dx, W)) * C];
+        const float g = (nxt - cur) * kR2;
+        const float clean = out_t[((size_t)y * W + x) * C + ch];
+        const float origin = ft[((size_t)oy * W + ox) * C];
+        tile[ly * kTileW + lx + kHalo] = g;
+
+        // One-frame high pass (neighbour_squares, grain.rs:251-269): squared steps to the right and
+        // lower neighbours inside the cell, relative to the origin sample.
+        const float fv = cur - origin;
+        const float right = __shfl_down(fv, 1), down = __shfl_down(fv, 8);
+        float hp = 0.0f;
+        if (lx + 1 < kCell) hp += (right - fv) * (right - fv);
+        if (ly + 1 < kCell) hp += (down - fv) * (down - fv);
+
+        const float sum = tree64(g), sum_sq = tree64(g * g), clean_sum = tree64(clean), hp_sum = tree64(hp);
+        const float clean_lo = min64(clean), clean_hi = max64(clean);
+        const float src_lo = min64(fminf(cur, nxt)), src_hi = max64(fmaxf(cur, nxt));
+
+        const float count = 64.0f;
+        const float var = (sum_sq - sum * sum / count) / (count - 1.0f);
+        const float grain_std = sqrtf(fmaxf(var, 0.0f));
+        const float mean = clean_sum / count;
+        const float frame_std = sqrtf(hp_sum / 224.0f);
+        const float range = clean_hi - clean_lo;
+        const bool interior = cx >= 1 && cx + 2 <= cells_x && cy + 2 <= cells_y;
+        const float conf = saved_conf[entry * blocks + blk];
+        const bool band = !luma_band || (mean > 20.0f / 255.0f && mean < 235.0f / 255.0f);
+        const bool usable = has_source && range < 5.0f / 255.0f && band && src_lo >= 4.0f / 255.0f &&
+                            src_hi <= 251.0f / 255.0f;
+        const bool passes = usable && interior && conf >= 0.8f && grain_std > 0.05f / 255.0f;
+        const bool frame_passes = usable && (means_bx == 0 || has_luma_means);
+        uint32_t group = 0;
+        if (passes) {
+            const uint32_t b = bucket_for(grain_std, edges);
+            group = b / 4;
+            if (lane == 0) atomicAdd(s_hist + luma_bin(mean) * kBuckets + b, 1);
+        }
+        if (frame_passes && lane == 0) {
+            const uint32_t b = bucket_for(frame_std, edges);
+            uint32_t bin = luma_bin(mean);
+            if (means_bx > 0) {
+                const uint32_t mb = (uint32_t)(cy * kCell / 16) * means_bx + (uint32_t)(cx * kCell / 16);
+                bin = min(sat_u32(luma_means[mb * 3] / 16.0f), (uint32_t)kBins - 1);
+            }
+            atomicAdd(s_hist + kHist + bin * kBuckets + b, 1);
+        }
+
+        // The accepted cell's halo of source grain, each pixel with its own cell's vector, clamped
+        // into the frame (grain.rs:441-475).  Four unrolled steps: every vector load goes out, then
+        // every sample load.
+        if (passes) {
+            constexpr int kSteps = (kTileW * kTileH + 63) / 64;
+            int hx[kSteps], hy[kSteps], ex[kSteps], ey[kSteps];
+            bool halo[kSteps];
+#pragma unroll
+            for (int j = 0; j < kSteps; ++j) {
+                const int f = lane + 64 * j, tx = f % kTileW, ty = f / kTileW;
+                halo[j] = f < kTileW * kTileH && !(ty < kCell && tx >= kHalo && tx < kHalo + kCell);
+                hx[j] = shift_clamp(0, ox + tx - kHalo, W);
+                hy[j] = shift_clamp(oy + min(ty, kTileH - 1), 0, H);
+                const uint32_t b2 = block_for(hx[j] / kCell, hy[j] / kCell, bx, by, step);
+                ex[j] = saved_mv[base + b2 * 2];
+                ey[j] = saved_mv[base + b2 * 2 + 1];
+            }
+            float vn[kSteps], vt[kSteps];
+#pragma unroll
+            for (int j = 0; j < kSteps; ++j) {
+                vn[j] = fnx[((size_t)shift_clamp(hy[j], ey[j], H) * W + shift_clamp(hx[j], ex[j], W)) * C];
+                vt[j] = ft[((size_t)hy[j] * W + hx[j]) * C];
+            }
+#pragma unroll
+            for (int j = 0; j < kSteps; ++j)
+                if (halo[j]) tile[lane + 64 * j] = (vn[j] - vt[j]) * kR2;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+        float *part = partials + (size_t)cell * kPartial;
+        if (lane < kLags) {
+            // Lane l sums lag l (dy 0: dx 0..6; dy 1..3: dx -6..6) over the cell's pixels in row order.
+            const int row = lane > kHalo ? 1 + (lane - kHalo - 1) / 13 : 0;
+            const int col = lane > kHalo ? (lane - kHalo - 1) % 13 : lane + kHalo;
+            const float gmean = sum / 64.0f;
+            float acc = 0.0f;
+            if (passes) {
+#pragma unroll 16
+                for (int p = 0; p < 64; ++p) {
+                    const int px = p & 7, py = p >> 3;
+                    const float c = tile[py * kTileW + px + kHalo] - gmean;
+                    const float n = tile[(py + row) * kTileW + px + col] - gmean;
+                    acc += c * n;
+                }
+            }
+            part[lane] = acc;
+        } else if (lane == kLags) {
+            part[kLags] = passes ? 64.0f : 0.0f;
+        } else if (lane == kLags + 1) {
+            part[kLags + 1] = (float)group;
+        }
+    }
+
+    __syncthreads();
+    for (int i = t; i < 2 * kHist; i += 64 * kWaves) {
+        const int n = s_hist[i];
+        if (n) atomicAdd(hist + i, n);
+    }
+}
+
+}  // namespace
+
+uint32_t grain_partials_len(uint32_t width, uint32_t height)
+{
+    return (width / kCell) * (height / kCell) * kPartial;
+}
+
+hipError_t launch_grain_measure(const GrainMeasure &m, hipStream_t s)
+{
+    const uint32_t cells = (m.width / kCell) * (m.height / kCell);
+    if (cells == 0) return hipSuccess;
+    const uint32_t groups = (cells + kCellsPerGroup - 1) / kCellsPerGroup;
+    k_grain_measure<<<groups, 64 * kWaves, 0, s>>>(
+        m.ring, m.denoised, m.saved_mv, m.saved_conf, m.edges, m.luma_means, m.hist, m.partials, (int)m.width,
+        (int)m.height, (int)m.channels, (int)m.channel, m.slot_t, m.slot_next, m.blocks_x, m.blocks_y, m.step,
+        m.source_entry, m.has_source ? 1 : 0, m.has_luma_means ? 1 : 0, m.means_blocks_x, m.luma_band ? 1 : 0);
+    return hipGetLastError();
+}
+
+}  // namespace avd

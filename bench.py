@@ -205,6 +205,48 @@ def noise_temporal_measure(avd, torch, clip, calls=20):
             "quarter fields on", **out}
 
 
+def grain_measure_measure(avd, torch, clip, calls=20):
+    """The film-grain measurement of the clip's first 4K luma frame against its second (f32 samples
+    in a two-frame ring in HBM, zero vectors at full confidence on the 8-pixel grid) with a flat
+    denoised frame, so every interior cell is accepted and runs its halo and 46 lag sums (the
+    kernel's heaviest path): average time per call from HIP events on torch's stream, against the
+    HBM roofline of one read of both frames and the denoised frame plus the partials."""
+    import ctypes
+
+    lib = avd._lib.load()
+    ring = torch.stack([clip[f][0].to(torch.float32) / float((1 << DEPTH) - 1) for f in (0, 1)]).contiguous()
+    denoised = torch.full((H, W), 0.5, dtype=torch.float32, device="cuda")
+    bx, by = -(-W // 8), -(-H // 8)
+    mv = torch.zeros(bx * by * 2, dtype=torch.int32, device="cuda")
+    conf = torch.ones(bx * by, dtype=torch.float32, device="cuda")
+    lo, hi = 0.05 / 255.0, 32.0 / 255.0
+    edges = torch.tensor([lo * (hi / lo) ** (i / 64) for i in range(65)], dtype=torch.float32, device="cuda")
+    hist = torch.zeros(2048, dtype=torch.int32, device="cuda")
+    n = lib.avd_grain_partials_len(W, H)
+    partials = torch.empty(n, dtype=torch.float32, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    def call():
+        avd._lib.check(lib.avd_site_grain_measure(p(ring), p(denoised), W, H, 1, 0, 0, 1, p(mv), p(conf), bx, by, 8,
+                                                  0, 1, None, 0, 0, 1, p(edges), p(hist), p(partials), stream))
+
+    for _ in range(3):
+        call()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(calls):
+        call()
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / calls
+    nbytes = 3 * W * H * 4 + n * 4
+    gbps = nbytes / (ms * 1e-3) / 1e9
+    return {"kernels": "k_grain_measure", "workload": "3840x2160 luma, flat denoised frame (every interior cell "
+            "accepted), zero vectors", "avg_launch_ms": round(ms, 4), "bytes_per_launch": nbytes,
+            "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+
+
 def run_steps(eng, frames, outs, start, count):
     done = 0
     for i in range(count):
@@ -473,7 +515,8 @@ def main():
         "forms": {f: {"frames_per_s": forms[f]["value"], "psnr_db_vs_clean": psnr.get(f)} for f in forms},
     }
     result["next_rows"] = {"noise_spatial": noise_spatial_measure(avd, torch, clip),
-                           "noise_temporal": noise_temporal_measure(avd, torch, clip)}
+                           "noise_temporal": noise_temporal_measure(avd, torch, clip),
+                           "grain_measure": grain_measure_measure(avd, torch, clip)}
     if not args.no_nlm:
         result["nlm"] = {name: nlm_leg(avd, torch, name, cfg) for name, cfg in NLM_CONFIGS.items()}
     if "mixed" in forms and "fp32" in forms:

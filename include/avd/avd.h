@@ -489,6 +489,29 @@ avd_status avd_site_noise_temporal_zero(float *stats, uint32_t len, void *stream
 /* temporal_stats_slot_len (noise/temporal.rs:141-144): floats of one frame's records. */
 uint32_t avd_noise_temporal_stats_len(uint32_t width, uint32_t height, uint32_t channels);
 
+/* Film-grain measurement of one completed frame (grain_measure, nl4d/kernels/grain.rs:243-513;
+ * launched per plane by the grain stage, nl4d/grain/chunk.rs).  Every 8x8 cell of channel `channel`
+ * of ring slots `slot_t` / `slot_next` (f32 [slot][height][width][channels]) is judged against the
+ * denoised frame `denoised` (f32 [height][width][channels]): source grain is the difference of the
+ * two slots along the cell's saved vector (`saved_mv`, `saved_conf`: entry `source_entry` of
+ * blocks_x x blocks_y blocks on a `step` grid) over sqrt 2, one-frame grain the cell's high-pass
+ * std.  Accepted cells add one count each into `hist` (the frame's 2 x 16 x 64 i32 histograms,
+ * source grain then one-frame grain, by brightness bin and std bucket of the 65 `edges`); a
+ * one-frame cell bins by the colour grid `luma_means` (3 floats per 16x16 block, means_blocks_x
+ * wide) when means_blocks_x > 0 and `has_luma_means` is set.  `partials` receives
+ * avd_grain_partials_len floats: per cell the 46 lag sums of its mean-free source grain (LAGS
+ * order, nl4d/grain/consts.rs:27), its pixel count and strength group, zeros when rejected.
+ * Histogram counts and partials are bit-identical to the reference's. */
+avd_status avd_site_grain_measure(const float *ring_values, const float *denoised, uint32_t width, uint32_t height,
+                                  uint32_t channels, uint32_t channel, uint32_t slot_t, uint32_t slot_next,
+                                  const int32_t *saved_mv, const float *saved_conf, uint32_t blocks_x,
+                                  uint32_t blocks_y, uint32_t step, uint32_t source_entry, uint32_t has_source,
+                                  const float *luma_means, uint32_t has_luma_means, uint32_t means_blocks_x,
+                                  uint32_t luma_band, const float *edges, int32_t *hist, float *partials,
+                                  void *stream);
+/* Floats of one frame's grain partials: 48 per whole 8x8 cell. */
+uint32_t avd_grain_partials_len(uint32_t width, uint32_t height);
+
 #ifdef __cplusplus
 }
 #endif

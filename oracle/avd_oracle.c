@@ -1269,3 +1269,183 @@ void avdo_temporal_stats(const float *ring, uint32_t W, uint32_t H, uint32_t C, 
         }
     }
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Film-grain measurement, grain_measure (nl4d/kernels/grain.rs:243-513, constants             */
+/* nl4d/grain/consts.rs:1-45).                                                                */
+/* ------------------------------------------------------------------------------------------ */
+
+#define GR_CELL 8
+#define GR_TILE_W 20
+#define GR_TILE_H 11
+#define GR_HALO 6
+#define GR_LAGS 46
+#define GR_PARTIAL 48
+#define GR_BUCKETS 64
+#define GR_BINS 16
+#define GR_HIST (GR_BINS * GR_BUCKETS)
+
+/* Rust's saturating float-to-u32 cast. */
+static uint32_t gr_u32(float v)
+{
+    if (!(v >= 0.0f)) return 0;
+    if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)v;
+}
+
+static uint32_t gr_shift(uint32_t v, int32_t s, uint32_t limit)
+{
+    int64_t m = (int64_t)v + s;
+    if (m < 0) m = 0;
+    if (m > (int64_t)limit - 1) m = (int64_t)limit - 1;
+    return (uint32_t)m;
+}
+
+static uint32_t gr_block(uint32_t cx, uint32_t cy, uint32_t bx, uint32_t by, uint32_t step)
+{
+    uint32_t x = cx * GR_CELL / step, y = cy * GR_CELL / step;
+    if (x > bx - 1) x = bx - 1;
+    if (y > by - 1) y = by - 1;
+    return y * bx + x;
+}
+
+static uint32_t gr_bucket(float s, const float *edges)
+{
+    uint32_t b = 0;
+    for (uint32_t i = 1; i < GR_BUCKETS; ++i)
+        if (s >= edges[i]) b = i;
+    return b;
+}
+
+static uint32_t gr_luma_bin(float mean)
+{
+    const uint32_t b = gr_u32(mean * (float)GR_BINS);
+    return b < GR_BINS - 1 ? b : GR_BINS - 1;
+}
+
+/* The 64-slot halving tree of grain.rs:362-385 (slot s + stride into slot s, stride 32 .. 1). */
+static float gr_tree_sum(float *v)
+{
+    for (uint32_t stride = 32; stride >= 1; stride /= 2)
+        for (uint32_t i = 0; i < stride; ++i) v[i] += v[i + stride];
+    return v[0];
+}
+
+/* Every 8x8 cell of channel `ch` of ring slots t / next ([slot][H][W][C] f32) against the denoised
+ * frame `out_t` ([H][W][C] f32): source-grain and one-frame histograms added into hist[0..2048)
+ * (i32 counts) and the cell's 48-float partial (46 lag sums, pixel count, strength group). */
+void avdo_grain_measure(const float *ring, const float *out_t, uint32_t W, uint32_t H, uint32_t C, uint32_t ch,
+                        uint32_t slot_t, uint32_t slot_next, const int32_t *saved_mv, const float *saved_conf,
+                        uint32_t bx, uint32_t by, uint32_t step, uint32_t entry, int has_source,
+                        const float *luma_means, int has_luma_means, uint32_t means_bx, int luma_band,
+                        const float *edges, int32_t *hist, float *partials)
+{
+    const uint32_t cells_x = W / GR_CELL, cells_y = H / GR_CELL, blocks = bx * by;
+    const float flat_range = 5.0f / 255.0f, luma_low = 20.0f / 255.0f, luma_high = 235.0f / 255.0f;
+    const float clip_low = 4.0f / 255.0f, clip_high = 251.0f / 255.0f, std_min = 0.05f / 255.0f;
+    const float r2 = 0.70710677f;
+#define GR_AT(xx, yy, slot) ring[(((size_t)(slot) * H + (yy)) * W + (xx)) * C + ch]
+    for (uint32_t cy = 0; cy < cells_y; ++cy)
+        for (uint32_t cx = 0; cx < cells_x; ++cx) {
+            const uint32_t ox = cx * GR_CELL, oy = cy * GR_CELL;
+            const uint32_t blk = gr_block(cx, cy, bx, by, step), base = entry * blocks * 2;
+            const int32_t dx = saved_mv[base + blk * 2], dy = saved_mv[base + blk * 2 + 1];
+            float tile[GR_TILE_W * GR_TILE_H], fv[64], s0[64], s1[64], s2[64], s3[64], lo0[64], lo1[64], hi0[64],
+                hi1[64];
+            const float origin = GR_AT(ox, oy, slot_t);
+            for (uint32_t t = 0; t < 64; ++t) {
+                const uint32_t lx = t % 8, ly = t / 8, x = ox + lx, y = oy + ly;
+                const float cur = GR_AT(x, y, slot_t);
+                const float nxt = GR_AT(gr_shift(x, dx, W), gr_shift(y, dy, H), slot_next);
+                const float g = (nxt - cur) * r2;
+                const float clean = out_t[((size_t)y * W + x) * C + ch];
+                tile[ly * GR_TILE_W + lx + GR_HALO] = g;
+                fv[t] = cur - origin;
+                s0[t] = g;
+                s1[t] = g * g;
+                s2[t] = clean;
+                lo0[t] = hi0[t] = clean;
+                lo1[t] = fminf(cur, nxt);
+                hi1[t] = fmaxf(cur, nxt);
+            }
+            for (uint32_t t = 0; t < 64; ++t) {
+                const uint32_t lx = t % 8, ly = t / 8;
+                float hp = 0.0f;
+                if (lx + 1 < 8) {
+                    const float r = fv[t + 1] - fv[t];
+                    hp += r * r;
+                }
+                if (ly + 1 < 8) {
+                    const float d = fv[t + 8] - fv[t];
+                    hp += d * d;
+                }
+                s3[t] = hp;
+            }
+            const float sum = gr_tree_sum(s0), sum_sq = gr_tree_sum(s1), clean_sum = gr_tree_sum(s2),
+                        hp_sum = gr_tree_sum(s3);
+            for (uint32_t stride = 32; stride >= 1; stride /= 2)
+                for (uint32_t i = 0; i < stride; ++i) {
+                    lo0[i] = fminf(lo0[i], lo0[i + stride]);
+                    lo1[i] = fminf(lo1[i], lo1[i + stride]);
+                    hi0[i] = fmaxf(hi0[i], hi0[i + stride]);
+                    hi1[i] = fmaxf(hi1[i], hi1[i + stride]);
+                }
+            const float count = 64.0f;
+            float var = (sum_sq - sum * sum / count) / (count - 1.0f);
+            const float grain_std = sqrtf(var > 0.0f ? var : 0.0f);
+            const float mean = clean_sum / count;
+            const float frame_std = sqrtf(hp_sum / 224.0f);
+            const float range = hi0[0] - lo0[0];
+            const int interior = cx >= 1 && cx + 2 <= cells_x && cy + 2 <= cells_y;
+            const float conf = saved_conf[entry * blocks + blk];
+            const int band = !luma_band || (mean > luma_low && mean < luma_high);
+            const int usable = has_source && range < flat_range && band && lo1[0] >= clip_low && hi1[0] <= clip_high;
+            const int passes = usable && interior && conf >= 0.8f && grain_std > std_min;
+            float group = 0.0f;
+            if (passes) {
+                const uint32_t b = gr_bucket(grain_std, edges);
+                hist[gr_luma_bin(mean) * GR_BUCKETS + b] += 1;
+                group = (float)(b / 4);
+            }
+            if (usable && (means_bx == 0 || has_luma_means)) {
+                const uint32_t b = gr_bucket(frame_std, edges);
+                uint32_t bin = gr_luma_bin(mean);
+                if (means_bx > 0) {
+                    const uint32_t mb = (cy * GR_CELL / 16) * means_bx + cx * GR_CELL / 16;
+                    const uint32_t v = gr_u32(luma_means[mb * 3] / 16.0f);
+                    bin = v < GR_BINS - 1 ? v : GR_BINS - 1;
+                }
+                hist[GR_HIST + bin * GR_BUCKETS + b] += 1;
+            }
+            float *part = partials + ((size_t)cy * cells_x + cx) * GR_PARTIAL;
+            if (passes) {
+                for (uint32_t ty = 0; ty < GR_TILE_H; ++ty)
+                    for (uint32_t tx = 0; tx < GR_TILE_W; ++tx) {
+                        if (ty < GR_CELL && tx >= GR_HALO && tx < GR_HALO + GR_CELL) continue;
+                        const uint32_t x = gr_shift(0, (int32_t)ox + (int32_t)tx - GR_HALO, W);
+                        const uint32_t y = gr_shift(oy + ty, 0, H);
+                        const uint32_t b2 = gr_block(x / GR_CELL, y / GR_CELL, bx, by, step);
+                        const int32_t ex = saved_mv[base + b2 * 2], ey = saved_mv[base + b2 * 2 + 1];
+                        const float nxt = GR_AT(gr_shift(x, ex, W), gr_shift(y, ey, H), slot_next);
+                        tile[ty * GR_TILE_W + tx] = (nxt - GR_AT(x, y, slot_t)) * r2;
+                    }
+            }
+            const float gmean = sum / 64.0f;
+            for (uint32_t l = 0; l < GR_LAGS; ++l) {
+                const uint32_t row = l > GR_HALO ? 1 + (l - GR_HALO - 1) / 13 : 0;
+                const uint32_t col = l > GR_HALO ? (l - GR_HALO - 1) % 13 : l + GR_HALO;
+                float acc = 0.0f;
+                if (passes)
+                    for (uint32_t py = 0; py < 8; ++py)
+                        for (uint32_t px = 0; px < 8; ++px) {
+                            const float c = tile[py * GR_TILE_W + px + GR_HALO] - gmean;
+                            const float n = tile[(py + row) * GR_TILE_W + px + col] - gmean;
+                            acc += c * n;
+                        }
+                part[l] = acc;
+            }
+            part[GR_LAGS] = passes ? 64.0f : 0.0f;
+            part[GR_LAGS + 1] = group;
+        }
+#undef GR_AT
+}

@@ -967,3 +967,45 @@ def temporal_stats(ring: np.ndarray, width: int, height: int, channels: int, slo
                               None if field is None else _p(field, I32P), *[ctypes.c_uint32(v) for v in args],
                               ctypes.c_int(1 if luma_fields else 0), _p(out, F32P))
     return out
+
+
+# ---- film-grain measurement (nl4d/kernels/grain.rs:243-513, nl4d/grain/consts.rs) ------------------
+
+GRAIN_CELL, GRAIN_LAGS, GRAIN_PARTIAL, GRAIN_BUCKETS, GRAIN_BINS = 8, 46, 48, 64, 16
+GRAIN_HIST = GRAIN_BINS * GRAIN_BUCKETS
+GRAIN_STD_MIN, GRAIN_STD_MAX = np.float32(0.05 / 255.0), np.float32(32.0 / 255.0)
+
+
+def grain_lags() -> list[tuple[int, int]]:
+    """LAGS (nl4d/grain/consts.rs:27, lag_table): (dy, dx) for dy 0..3, dx -6..6, the half plane."""
+    return [(dy, dx) for dy in range(4) for dx in range(-6, 7) if dy > 0 or dx >= 0]
+
+
+def grain_bucket_edges() -> np.ndarray:
+    """bucket_edges (nl4d/grain/fit.rs:28-38): 65 edges log-spaced from STD_MIN to STD_MAX, computed
+    in f64 from the f32 constants and stored as f32."""
+    lo, hi = float(np.float32(0.05) / np.float32(255.0)), float(np.float32(32.0) / np.float32(255.0))
+    return np.array([lo * (hi / lo) ** (i / GRAIN_BUCKETS) for i in range(GRAIN_BUCKETS + 1)], np.float32)
+
+
+def grain_measure(ring, out_t, width, height, channels, channel, slot_t, slot_next, saved_mv, saved_conf,
+                  blocks_x, blocks_y, step, entry=0, has_source=True, luma_means=None, has_luma_means=False,
+                  means_blocks_x=0, luma_band=True, edges=None, hist=None):
+    """grain_measure (nl4d/kernels/grain.rs:290-513) over every 8x8 cell (avdo_grain_measure): returns
+    (hist, partials), hist the 2 x 16 x 64 i32 counts (source grain, then one-frame grain) added
+    into ``hist`` when given, partials 48 floats per cell."""
+    ring = np.ascontiguousarray(ring, np.float32).reshape(-1)
+    out_t = np.ascontiguousarray(out_t, np.float32).reshape(-1)
+    mv = np.ascontiguousarray(saved_mv, np.int32).reshape(-1)
+    conf = np.ascontiguousarray(saved_conf, np.float32).reshape(-1)
+    means = np.ascontiguousarray(luma_means if luma_means is not None else np.zeros(3, np.float32), np.float32)
+    edges = np.ascontiguousarray(grain_bucket_edges() if edges is None else edges, np.float32)
+    hist = np.zeros(2 * GRAIN_HIST, np.int32) if hist is None else np.ascontiguousarray(hist, np.int32)
+    partials = np.full((width // GRAIN_CELL) * (height // GRAIN_CELL) * GRAIN_PARTIAL, np.nan, np.float32)
+    u = ctypes.c_uint32
+    lib().avdo_grain_measure(_p(ring, F32P), _p(out_t, F32P), u(width), u(height), u(channels), u(channel),
+                             u(slot_t), u(slot_next), _p(mv, I32P), _p(conf, F32P), u(blocks_x), u(blocks_y),
+                             u(step), u(entry), ctypes.c_int(int(bool(has_source))), _p(means, F32P),
+                             ctypes.c_int(int(bool(has_luma_means))), u(means_blocks_x),
+                             ctypes.c_int(int(bool(luma_band))), _p(edges, F32P), _p(hist, I32P), _p(partials, F32P))
+    return hist, partials
