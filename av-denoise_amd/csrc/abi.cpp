@@ -496,6 +496,16 @@ avd_status avd_site_grain_measure(const float *ring_values, const float *denoise
 
 uint32_t avd_grain_partials_len(uint32_t width, uint32_t height) { return avd::grain_partials_len(width, height); }
 
+avd_status avd_site_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint32_t record_offset,
+                                 void *stream)
+{
+    if (!partials || !chunk) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if ((uint64_t)cells * 48 > 0xFFFFFFFFull)
+        return fail(AVD_ERR_INVALID_GEOMETRY, "more grain cells than 32-bit partial offsets reach");
+    return hip_status(avd::launch_grain_reduce(partials, cells, chunk, record_offset, static_cast<hipStream_t>(stream)),
+                      "grain_reduce");
+}
+
 float avd_noise_sigma_from_abs_sum(float abs_sum, uint32_t width, uint32_t height)
 {
     // The 3x3 mask has no interior response below 3 samples on an axis (the reference's version

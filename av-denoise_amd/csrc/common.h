@@ -150,6 +150,10 @@ struct GrainMeasure {
 };
 uint32_t grain_partials_len(uint32_t width, uint32_t height);
 hipError_t launch_grain_measure(const GrainMeasure &m, hipStream_t s);
+// Every cell's partial added into its strength group's 47-float record of `chunk` (16 records from
+// record_offset, earlier sums kept).
+hipError_t launch_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint32_t record_offset,
+                               hipStream_t s);
 hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,
                                    int C, hipStream_t s);
 

@@ -210,6 +210,66 @@ __global__ __launch_bounds__(64 * kWaves) void k_grain_measure(
     }
 }
 
+// grain_reduce_partials (grain.rs:515-560): workgroup `lane` of 47 takes one record lane; thread t
+// of 128 adds lane `lane` of cells t, t + 128, ... in turn into its strength group's column (the
+// group clamped to 15), then each group's 128 columns fold by a halving tree (the 64-stride step
+// through LDS, the rest as wave shuffles, the same slot pairs) and group g's total is added into
+// chunk[record_offset + 47 g + lane].  The 16 columns of a thread stay in registers: each cell adds
+// its value to its own group's and an exact +0.0 to the others (a column starts at +0.0 and so
+// never holds -0.0, which an added +0.0 would change), keeping the reference's order without an LDS
+// read-modify-write on the chain.
+constexpr int kReduceThreads = 128, kGroups = 16, kRecordLen = kLags + 1;
+
+__global__ __launch_bounds__(kReduceThreads) void k_grain_reduce(const float *__restrict__ partials,
+                                                                 float *__restrict__ chunk, uint32_t record_offset,
+                                                                 uint32_t cells)
+{
+    __shared__ float s_col[kGroups][kReduceThreads];
+    const uint32_t lane = blockIdx.x, t = threadIdx.x;
+    float acc[kGroups];
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) acc[g] = 0.0f;
+    auto add = [&](float v, float gv) {
+        const uint32_t group = min(sat_u32(gv), (uint32_t)kGroups - 1);
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) acc[g] += group == (uint32_t)g ? v : 0.0f;
+    };
+    constexpr uint32_t kBatch = 8;
+    uint32_t c = t;
+    for (; c + (kBatch - 1) * kReduceThreads < cells; c += kBatch * kReduceThreads) {
+        float v[kBatch], gv[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const size_t at = (size_t)(c + j * kReduceThreads) * kPartial;
+            v[j] = partials[at + lane];
+            gv[j] = partials[at + kLags + 1];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) add(v[j], gv[j]);
+    }
+    for (; c < cells; c += kReduceThreads) {
+        const size_t at = (size_t)c * kPartial;
+        add(partials[at + lane], partials[at + kLags + 1]);
+    }
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) s_col[g][t] = acc[g];
+    __syncthreads();
+    if (t < 64) {
+        float total[kGroups];
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) {
+            float x = s_col[g][t] + s_col[g][t + 64];
+#pragma unroll
+            for (int stride = 32; stride >= 1; stride >>= 1) x += __shfl_down(x, stride);
+            total[g] = x;
+        }
+        if (t == 0) {
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) chunk[record_offset + g * kRecordLen + lane] += total[g];
+        }
+    }
+}
+
 }  // namespace
 
 uint32_t grain_partials_len(uint32_t width, uint32_t height)
@@ -226,6 +286,14 @@ hipError_t launch_grain_measure(const GrainMeasure &m, hipStream_t s)
         m.ring, m.denoised, m.saved_mv, m.saved_conf, m.edges, m.luma_means, m.hist, m.partials, (int)m.width,
         (int)m.height, (int)m.channels, (int)m.channel, m.slot_t, m.slot_next, m.blocks_x, m.blocks_y, m.step,
         m.source_entry, m.has_source ? 1 : 0, m.has_luma_means ? 1 : 0, m.means_blocks_x, m.luma_band ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint32_t record_offset,
+                               hipStream_t s)
+{
+    if (!partials || !chunk) return hipErrorInvalidValue;
+    k_grain_reduce<<<kRecordLen, kReduceThreads, 0, s>>>(partials, chunk, record_offset, cells);
     return hipGetLastError();
 }
 

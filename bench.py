@@ -242,9 +242,24 @@ def grain_measure_measure(avd, torch, clip, calls=20):
     ms = start.elapsed_time(end) / calls
     nbytes = 3 * W * H * 4 + n * 4
     gbps = nbytes / (ms * 1e-3) / 1e9
+    chunk = torch.zeros(16 * 47, dtype=torch.float32, device="cuda")
+
+    def reduce_call():
+        avd._lib.check(lib.avd_site_grain_reduce(p(partials), n // 48, p(chunk), 0, stream))
+
+    for _ in range(3):
+        reduce_call()
+    start.record()
+    for _ in range(calls):
+        reduce_call()
+    end.record()
+    torch.cuda.synchronize()
+    reduce_ms = start.elapsed_time(end) / calls
     return {"kernels": "k_grain_measure", "workload": "3840x2160 luma, flat denoised frame (every interior cell "
             "accepted), zero vectors", "avg_launch_ms": round(ms, 4), "bytes_per_launch": nbytes,
-            "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+            "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
+            "reduce": {"kernels": "k_grain_reduce", "avg_launch_ms": round(reduce_ms, 4), "bytes_per_launch": n * 4,
+                       "achieved": round(n * 4 / (reduce_ms * 1e-3) / 1e9, 1), "unit": "GB/s"}}
 
 
 def run_steps(eng, frames, outs, start, count):

@@ -511,6 +511,13 @@ avd_status avd_site_grain_measure(const float *ring_values, const float *denoise
                                   void *stream);
 /* Floats of one frame's grain partials: 48 per whole 8x8 cell. */
 uint32_t avd_grain_partials_len(uint32_t width, uint32_t height);
+/* grain_reduce_partials (nl4d/kernels/grain.rs:515-560; launched per frame by the grain stage,
+ * nl4d/grain/export.rs:389-400): each of `cells` partials (48 floats from avd_site_grain_measure)
+ * is added into its strength group's record of `chunk` (16 records of 47 floats from
+ * `record_offset`: the 46 lag sums and the pixel count), keeping the records' earlier sums.  The
+ * sums are bit-identical to the reference's. */
+avd_status avd_site_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint32_t record_offset,
+                                 void *stream);
 
 #ifdef __cplusplus
 }

@@ -1449,3 +1449,27 @@ void avdo_grain_measure(const float *ring, const float *out_t, uint32_t W, uint3
         }
 #undef GR_AT
 }
+
+/* grain_reduce_partials (nl4d/kernels/grain.rs:515-560): for each of the 47 record lanes, thread t
+ * of 128 adds lane `lane` of cells t, t + 128, ... in turn into its strength group's column (the
+ * group clamped to 15), each group's 128 columns then fold by a halving tree, and group g's total
+ * is added into chunk[record_offset + 47 g + lane]. */
+void avdo_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint32_t record_offset)
+{
+    for (uint32_t lane = 0; lane < 47; ++lane) {
+        float col[16][128];
+        memset(col, 0, sizeof col);
+        for (uint32_t t = 0; t < 128; ++t)
+            for (uint32_t c = t; c < cells; c += 128) {
+                const float gv = partials[(size_t)c * 48 + 47];
+                uint32_t g = gr_u32(gv);
+                if (g > 15) g = 15;
+                col[g][t] += partials[(size_t)c * 48 + lane];
+            }
+        for (uint32_t g = 0; g < 16; ++g) {
+            for (uint32_t stride = 64; stride >= 1; stride /= 2)
+                for (uint32_t i = 0; i < stride; ++i) col[g][i] += col[g][i + stride];
+            chunk[record_offset + g * 47 + lane] += col[g][0];
+        }
+    }
+}

@@ -1009,3 +1009,16 @@ def grain_measure(ring, out_t, width, height, channels, channel, slot_t, slot_ne
                              ctypes.c_int(int(bool(has_luma_means))), u(means_blocks_x),
                              ctypes.c_int(int(bool(luma_band))), _p(edges, F32P), _p(hist, I32P), _p(partials, F32P))
     return hist, partials
+
+
+def grain_reduce(partials, chunk=None, record_offset=0):
+    """grain_reduce_partials (nl4d/kernels/grain.rs:515-560): every cell's partial added into its
+    strength group's 47-float record of ``chunk`` (16 records from ``record_offset``, earlier sums
+    kept), in the reference's order (avdo_grain_reduce)."""
+    partials = np.ascontiguousarray(partials, np.float32).reshape(-1)
+    cells = partials.size // GRAIN_PARTIAL
+    if chunk is None:
+        chunk = np.zeros(record_offset + 16 * 47, np.float32)
+    chunk = np.ascontiguousarray(chunk, np.float32).copy()
+    lib().avdo_grain_reduce(_p(partials, F32P), ctypes.c_uint32(cells), _p(chunk, F32P), ctypes.c_uint32(record_offset))
+    return chunk

@@ -283,3 +283,84 @@ def test_site_grain_measure_is_bit_exact(case):
     bad = np.flatnonzero(part.view(np.uint32) != want_part.view(np.uint32))
     assert bad.size == 0, (name, bad[:8], part[bad[:8]], want_part[bad[:8]])
     assert want_hist[:1024].sum() > 0, name  # the case measures something
+
+
+# ---- the partial reduce (grain_reduce_partials, nl4d/kernels/grain.rs:515-560) ----------------------
+
+def random_partials(cells, seed, integer=False):
+    rng = np.random.default_rng(seed)
+    p = np.zeros((cells, O.GRAIN_PARTIAL), np.float32)
+    p[:, :47] = rng.integers(-50, 51, (cells, 47)) if integer else rng.standard_normal((cells, 47)) * 1e-3
+    # strength groups 0..15 plus values the clamp must handle: past the last group, negative, fractional
+    p[:, 47] = rng.choice([0, 1, 3, 7, 15, 16, 99, -2, 4.7], cells)
+    return p
+
+
+def test_reduce_sums_each_group_and_keeps_earlier_sums():
+    cells = 1000
+    p = random_partials(cells, 3, integer=True)  # integer partials: every f32 sum is exact
+    chunk = np.arange(10 + 16 * 47, dtype=np.float32)
+    got = O.grain_reduce(p, chunk, record_offset=10)
+    groups = np.minimum(np.maximum(np.floor(p[:, 47]), 0), 15).astype(int)  # Rust's saturating cast
+    want = chunk.astype(np.float64).copy()
+    for g in range(16):
+        want[10 + 47 * g:10 + 47 * (g + 1)] += p[groups == g, :47].astype(np.float64).sum(0)
+    assert np.array_equal(got, want.astype(np.float32))
+    assert np.array_equal(got[:10], chunk[:10])
+
+
+def test_reduce_of_measured_partials_counts_the_accepted_pixels():
+    W, H = 96, 64
+    ring, out_t = scene(W, H, 2 / 255)
+    hist, part = measure(ring, out_t, W, H)
+    rec = O.grain_reduce(part).reshape(16, 47)
+    # The count lane totals 64 pixels per accepted cell, in the group of its std bucket.
+    assert rec[:, 46].sum() == 64 * hist[:1024].sum()
+    by_group = hist[:1024].reshape(16, 16, 4).sum((0, 2))
+    assert np.array_equal(rec[:, 46], 64 * by_group.astype(np.float32))
+    lag0 = rec[:, 0].sum() / rec[:, 46].sum()
+    assert abs(lag0 / (2 / 255) ** 2 - 1) < 0.1
+
+
+def _site_reduce(part, chunk, record_offset):
+    import torch
+
+    from gpu_helpers import dev, host, require_gpu
+
+    require_gpu()
+    d_part, d_chunk = dev(part), dev(chunk)
+    L.check(L.load().avd_site_grain_reduce(ctypes.c_void_p(d_part.data_ptr()), part.size // O.GRAIN_PARTIAL,
+                                           ctypes.c_void_p(d_chunk.data_ptr()), record_offset, None))
+    return host(d_chunk)
+
+
+REDUCE_CASES = [("cells1", 1, 0), ("cells127", 127, 5), ("cells1000", 1000, 0), ("cells12345", 12345, 47),
+                ("uhd_cells", 480 * 270, 16 * 47)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", REDUCE_CASES, ids=[c[0] for c in REDUCE_CASES])
+def test_site_grain_reduce_is_bit_exact(case):
+    name, cells, offset = case
+    print(f"grain reduce case {name}", flush=True)
+    part = random_partials(cells, cells).reshape(-1)
+    chunk = np.random.default_rng(1).standard_normal(offset + 16 * 47).astype(np.float32)
+    got = _site_reduce(part, chunk, offset)
+    want = O.grain_reduce(part, chunk, offset)
+    bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+    assert bad.size == 0, (name, bad[:8], got[bad[:8]], want[bad[:8]])
+
+
+@pytest.mark.gpu
+def test_site_measure_then_reduce_matches_the_oracle_at_4k():
+    W, H = 3840, 2160
+    print("grain measure + reduce 3840x2160", flush=True)
+    ring, out_t = scene(W, H, 1.5 / 255, shift=(1, 0), seed=5, texture=0.002)
+    g = grid(W, H, 8, (1, 0))
+    _, part = _site(ring, out_t, W, H, 1, 0, g)
+    chunk = np.zeros(3 * 16 * 47, np.float32)
+    got = _site_reduce(part, chunk, 16 * 47)
+    _, want_part = O.grain_measure(ring, out_t, W, H, 1, 0, 0, 1, g[0], g[1], g[2], g[3], g[4])
+    want = O.grain_reduce(want_part, chunk, 16 * 47)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert want[16 * 47:2 * 16 * 47].reshape(16, 47)[:, 46].sum() > 0
