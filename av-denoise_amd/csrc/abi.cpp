@@ -496,6 +496,16 @@ avd_status avd_site_grain_measure(const float *ring_values, const float *denoise
 
 uint32_t avd_grain_partials_len(uint32_t width, uint32_t height) { return avd::grain_partials_len(width, height); }
 
+avd_status avd_site_grain_fit(const double *records, uint32_t count, double *corrected, double *weights,
+                              int32_t *quantised, uint32_t *shifts, int32_t *solved, void *stream)
+{
+    if (!records || !corrected || !weights || !quantised || !shifts || !solved)
+        return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    return hip_status(avd::launch_grain_fit(records, count, corrected, weights, quantised, shifts, solved,
+                                            static_cast<hipStream_t>(stream)),
+                      "grain_fit");
+}
+
 avd_status avd_site_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint32_t record_offset,
                                  void *stream)
 {

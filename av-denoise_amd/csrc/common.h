@@ -154,6 +154,10 @@ hipError_t launch_grain_measure(const GrainMeasure &m, hipStream_t s);
 // record_offset, earlier sums kept).
 hipError_t launch_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint32_t record_offset,
                                hipStream_t s);
+// The f64 AR(3) fit of `count` 47-value records: corrected records, 24 weights, their AV1 integers
+// and shift, and whether each system solved.
+hipError_t launch_grain_fit(const double *records, uint32_t count, double *corrected, double *weights,
+                            int32_t *quantised, uint32_t *shifts, int32_t *solved, hipStream_t s);
 hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,
                                    int C, hipStream_t s);
 

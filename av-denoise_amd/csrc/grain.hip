@@ -270,6 +270,148 @@ __global__ __launch_bounds__(kReduceThreads) void k_grain_reduce(const float *__
     }
 }
 
+// The AR(3) fit in f64 (undo_mean_removal, yule_walker, solve and quantise_ar, nl4d/grain/fit.rs:
+// 79-300): one wave per 47-value record.  Each of the 20 mean-removal rounds has lane l < 46
+// compute lag l's cell-mean term over the 15 x 15 cell offsets in the reference's order and lane 46
+// the zero lag's; elimination gives row r to lane r (the factors
+// first, then each row's update), the pivot is the last of equal maxima (as Iterator::max_by
+// keeps), back substitution and quantisation run in lane 0.  Every f64 operation is the
+// reference's, in its order, so the results are bit-identical.
+__device__ inline int fit_lane(int dy, int dx)
+{
+    if (dy < 0 || (dy == 0 && dx < 0)) {
+        dy = -dy;
+        dx = -dx;
+    }
+    if (dy > 3 || dx > 6 || dx < -6) return -1;
+    return dy == 0 ? dx : 7 + (dy - 1) * 13 + dx + 6;
+}
+
+__device__ inline double cell_term(const double *rho, int dy, int dx)
+{
+    double total = 0.0;
+    for (int oy = -7; oy <= 7; ++oy)
+        for (int ox = -7; ox <= 7; ++ox) {
+            const int lane = fit_lane(dy + oy, dx + ox);
+            if (lane < 0) continue;
+            const double pairs = (double)((8 - abs(oy)) * (8 - abs(ox)));
+            total += pairs / 4096.0 * rho[lane];
+        }
+    return total;
+}
+
+__global__ __launch_bounds__(64) void k_grain_fit(const double *__restrict__ records, double *__restrict__ corrected,
+                                                  double *__restrict__ weights, int32_t *__restrict__ quantised,
+                                                  uint32_t *__restrict__ shifts, int32_t *__restrict__ solved)
+{
+    __shared__ double s_rho[kLags], s_next[kLags], s_m[24][25], s_factor[24], s_sol[24];
+    __shared__ int s_oy[24], s_ox[24], s_ok;
+    __shared__ double s_zero_term;
+    const int lane = (int)threadIdx.x;
+    const size_t r = blockIdx.x;
+    const double *rec = records + r * kRecordLen;
+    double *cor = corrected + r * kRecordLen;
+    auto sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    };
+    if (lane < 24) {  // AR_OFFSETS: dy -3..0, dx -3..3, the causal half
+        const int k = lane < 21 ? lane : lane - 21, dy = lane < 21 ? -3 + k / 7 : 0;
+        s_oy[lane] = dy;
+        s_ox[lane] = lane < 21 ? k % 7 - 3 : -3 + k;
+    }
+    const double zero_lag = rec[0];
+    const double measured = lane < kLags ? rec[lane] / zero_lag : 0.0;
+    if (lane < kLags) s_rho[lane] = lane == 0 ? 1.0 : measured;
+    sync();
+    if (zero_lag > 0.0) {
+        const int dy = lane < 7 ? 0 : lane < 20 ? 1 : lane < 33 ? 2 : 3;
+        const int dx = lane < 7 ? lane : (lane - 7) % 13 - 6;
+        for (int round = 0; round < 20; ++round) {
+            // Lane 46 computes the zero lag's term once for every lane.
+            const double term = lane < kLags ? cell_term(s_rho, dy, dx) : lane == kLags ? cell_term(s_rho, 0, 0) : 0.0;
+            if (lane == kLags) s_zero_term = term;
+            sync();
+            if (lane < kLags) s_next[lane] = measured * (1.0 - s_zero_term) + term;
+            sync();
+            if (lane < kLags) s_rho[lane] = s_next[lane];
+            sync();
+        }
+        if (lane < kLags) cor[lane] = s_rho[lane] * zero_lag;
+        if (lane == kLags) cor[kLags] = rec[kLags];
+    } else if (lane < kRecordLen) {
+        cor[lane] = rec[lane];
+    }
+    // The corrected record, read back as the solve's input.
+    __shared__ double s_cor[kRecordLen];
+    if (lane < kLags) s_cor[lane] = zero_lag > 0.0 ? s_rho[lane] * zero_lag : rec[lane];
+    if (lane == kLags) s_cor[kLags] = rec[kLags];
+    sync();
+    const double pixels = s_cor[kLags];
+    if (lane == 0) s_ok = pixels > 0.0;
+    if (lane < 24 && pixels > 0.0) {
+        for (int j = 0; j < 24; ++j) s_m[lane][j] = s_cor[fit_lane(s_oy[lane] - s_oy[j], s_ox[lane] - s_ox[j])] / pixels;
+        s_m[lane][24] = s_cor[fit_lane(s_oy[lane], s_ox[lane])] / pixels;
+    }
+    sync();
+    for (int col = 0; col < 24 && s_ok; ++col) {
+        if (lane == 0) {
+            int pivot = col;
+            for (int i = col + 1; i < 24; ++i)
+                if (fabs(s_m[i][col]) >= fabs(s_m[pivot][col])) pivot = i;
+            if (fabs(s_m[pivot][col]) < 1e-18) s_ok = 0;
+            else if (pivot != col)
+                for (int j = 0; j < 25; ++j) {
+                    const double t = s_m[col][j];
+                    s_m[col][j] = s_m[pivot][j];
+                    s_m[pivot][j] = t;
+                }
+        }
+        sync();
+        if (!s_ok) break;
+        if (lane > col && lane < 24) s_factor[lane] = s_m[lane][col] / s_m[col][col];
+        sync();
+        if (lane > col && lane < 24)
+            for (int j = col; j < 25; ++j) s_m[lane][j] -= s_factor[lane] * s_m[col][j];
+        sync();
+    }
+    double *w = weights + r * 24;
+    int32_t *q = quantised + r * 24;
+    if (lane == 0) {
+        bool ok = s_ok != 0;
+        if (ok) {
+            for (int i = 23; i >= 0; --i) {
+                double v = s_m[i][24];
+                for (int j = i + 1; j < 24; ++j) v -= s_m[i][j] * s_sol[j];
+                s_sol[i] = v / s_m[i][i];
+            }
+            for (int i = 0; i < 24; ++i) ok = ok && isfinite(s_sol[i]);
+        }
+        uint32_t shift = 6;
+        if (ok) {
+            for (uint32_t cand = 9; cand >= 7; --cand) {
+                bool fits = true;
+                for (int i = 0; i < 24; ++i) {
+                    const double v = round(s_sol[i] * (double)(1u << cand));
+                    fits = fits && v >= -128.0 && v <= 127.0;
+                }
+                if (fits) {
+                    shift = cand;
+                    break;
+                }
+            }
+        }
+        for (int i = 0; i < 24; ++i) {
+            const double v = ok ? fmin(fmax(round(s_sol[i] * (double)(1u << shift)), -128.0), 127.0) : 0.0;
+            w[i] = ok ? s_sol[i] : 0.0;
+            q[i] = (int32_t)v;
+        }
+        shifts[r] = ok ? shift : 0u;
+        solved[r] = ok ? 1 : 0;
+    }
+}
+
 }  // namespace
 
 uint32_t grain_partials_len(uint32_t width, uint32_t height)
@@ -294,6 +436,15 @@ hipError_t launch_grain_reduce(const float *partials, uint32_t cells, float *chu
 {
     if (!partials || !chunk) return hipErrorInvalidValue;
     k_grain_reduce<<<kRecordLen, kReduceThreads, 0, s>>>(partials, chunk, record_offset, cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_grain_fit(const double *records, uint32_t count, double *corrected, double *weights,
+                            int32_t *quantised, uint32_t *shifts, int32_t *solved, hipStream_t s)
+{
+    if (!records || !corrected || !weights || !quantised || !shifts || !solved) return hipErrorInvalidValue;
+    if (count == 0) return hipSuccess;
+    k_grain_fit<<<count, 64, 0, s>>>(records, corrected, weights, quantised, shifts, solved);
     return hipGetLastError();
 }
 

@@ -255,11 +255,29 @@ def grain_measure_measure(avd, torch, clip, calls=20):
     end.record()
     torch.cuda.synchronize()
     reduce_ms = start.elapsed_time(end) / calls
+    records = chunk.view(16, 47).to(torch.float64).contiguous()
+    fit_out = [torch.empty(16, 47, dtype=torch.float64, device="cuda"),
+               torch.empty(16, 24, dtype=torch.float64, device="cuda"),
+               torch.empty(16, 24, dtype=torch.int32, device="cuda"),
+               torch.empty(16, dtype=torch.int32, device="cuda"), torch.empty(16, dtype=torch.int32, device="cuda")]
+
+    def fit_call():
+        avd._lib.check(lib.avd_site_grain_fit(p(records), 16, *[p(t) for t in fit_out], stream))
+
+    for _ in range(3):
+        fit_call()
+    start.record()
+    for _ in range(calls):
+        fit_call()
+    end.record()
+    torch.cuda.synchronize()
+    fit_ms = start.elapsed_time(end) / calls
     return {"kernels": "k_grain_measure", "workload": "3840x2160 luma, flat denoised frame (every interior cell "
             "accepted), zero vectors", "avg_launch_ms": round(ms, 4), "bytes_per_launch": nbytes,
             "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
             "reduce": {"kernels": "k_grain_reduce", "avg_launch_ms": round(reduce_ms, 4), "bytes_per_launch": n * 4,
-                       "achieved": round(n * 4 / (reduce_ms * 1e-3) / 1e9, 1), "unit": "GB/s"}}
+                       "achieved": round(n * 4 / (reduce_ms * 1e-3) / 1e9, 1), "unit": "GB/s"},
+            "fit": {"kernels": "k_grain_fit", "records": 16, "dtype": "f64", "avg_launch_ms": round(fit_ms, 4)}}
 
 
 def run_steps(eng, frames, outs, start, count):

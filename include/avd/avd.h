@@ -518,6 +518,17 @@ uint32_t avd_grain_partials_len(uint32_t width, uint32_t height);
  * sums are bit-identical to the reference's. */
 avd_status avd_site_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint32_t record_offset,
                                  void *stream);
+/* The AR(3) grain fit in f64, on the GPU (undo_mean_removal, yule_walker, solve and quantise_ar,
+ * nl4d/grain/fit.rs:120-300; called per pooled record by nl4d/grain/segment.rs:262-265 and
+ * nl4d/grain/chroma.rs:159-160).  Each of `count` records holds 47 f64 values (the 46 lag sums in
+ * LAGS order and the pixel count); per record it writes the record with the cell-mean bias undone
+ * (corrected[47]), the 24 Yule-Walker weights in AR_OFFSETS order (weights[24]), their AV1
+ * integers (quantised[24], at the finest shift 6..9 that fits a signed byte, clamped) and the shift
+ * (shifts[1]), and solved[1] = 1, or 0 with zero weights where the reference returns None (an
+ * empty record, a pivot below 1e-18 or a non-finite solution).  Bit-identical to the reference's
+ * f64 arithmetic. */
+avd_status avd_site_grain_fit(const double *records, uint32_t count, double *corrected, double *weights,
+                              int32_t *quantised, uint32_t *shifts, int32_t *solved, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1473,3 +1473,143 @@ void avdo_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint
         }
     }
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* The AR(3) grain fit in f64: undo_mean_removal, yule_walker, solve and quantise_ar           */
+/* (nl4d/grain/fit.rs:79-300).                                                                */
+/* ------------------------------------------------------------------------------------------ */
+
+/* measured_lane (fit.rs:79-95): the record lane of a lag folded onto the measured half plane. */
+static int gf_lane(int dy, int dx)
+{
+    if (dy < 0 || (dy == 0 && dx < 0)) {
+        dy = -dy;
+        dx = -dx;
+    }
+    if (dy > 3 || dx > 6 || dx < -6) return -1;
+    return dy == 0 ? dx : 7 + (dy - 1) * 13 + dx + 6;
+}
+
+/* cell_mean_term (fit.rs:97-118). */
+static double gf_cell_term(const double *rho, int dy, int dx)
+{
+    double total = 0.0;
+    for (int oy = -7; oy <= 7; ++oy)
+        for (int ox = -7; ox <= 7; ++ox) {
+            const int lane = gf_lane(dy + oy, dx + ox);
+            if (lane < 0) continue;
+            const double pairs = (double)((8 - abs(oy)) * (8 - abs(ox)));
+            total += pairs / 4096.0 * rho[lane];
+        }
+    return total;
+}
+
+static const int GF_LAG_Y[46] = {0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2,
+                                 2, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3, 3, 3, 3, 3, 3};
+static int gf_lag_x(int l) { return l < 7 ? l : (l - 7) % 13 - 6; }
+
+/* quantise_ar (fit.rs:272-305): the finest shift 9..7 at which every weight rounds (half away from
+ * zero) into a signed byte, else 6, and the weights rounded at it and clamped. */
+void avdo_grain_quantise(const double *sol, int32_t *q, uint32_t *shift)
+{
+    uint32_t s = 6;
+    for (uint32_t cand = 9; cand >= 7; --cand) {
+        int fits = 1;
+        for (int i = 0; i < 24; ++i) {
+            const double v = round(sol[i] * (double)(1u << cand));
+            if (v < -128.0 || v > 127.0) fits = 0;
+        }
+        if (fits) {
+            s = cand;
+            break;
+        }
+    }
+    for (int i = 0; i < 24; ++i) {
+        double v = round(sol[i] * (double)(1u << s));
+        if (v < -128.0) v = -128.0;
+        if (v > 127.0) v = 127.0;
+        q[i] = (int32_t)v;
+    }
+    *shift = s;
+}
+
+/* yule_walker and solve (fit.rs:160-216): AV1's 24 lag-3 weights from a 47-value record by
+ * Gaussian elimination with partial pivoting (the last of equal maxima, as Iterator::max_by keeps);
+ * 0 for an empty record, a pivot below 1e-18 or a non-finite solution. */
+int avdo_grain_yule_walker(const double *cor, double *sol)
+{
+    int oy[24], ox[24], k = 0;
+    for (int dy = -3; dy <= 0; ++dy)
+        for (int dx = -3; dx <= 3; ++dx)
+            if (dy < 0 || dx < 0) {
+                oy[k] = dy;
+                ox[k] = dx;
+                ++k;
+            }
+    const double pixels = cor[46];
+    if (!(pixels > 0.0)) return 0;
+    double m[24][25];
+    for (int i = 0; i < 24; ++i) {
+        for (int j = 0; j < 24; ++j) m[i][j] = cor[gf_lane(oy[i] - oy[j], ox[i] - ox[j])] / pixels;
+        m[i][24] = cor[gf_lane(oy[i], ox[i])] / pixels;
+    }
+    for (int col = 0; col < 24; ++col) {
+        int pivot = col;
+        for (int i = col + 1; i < 24; ++i)
+            if (fabs(m[i][col]) >= fabs(m[pivot][col])) pivot = i;
+        if (fabs(m[pivot][col]) < 1e-18) return 0;
+        for (int j = 0; j < 25; ++j) {
+            const double t = m[col][j];
+            m[col][j] = m[pivot][j];
+            m[pivot][j] = t;
+        }
+        for (int i = col + 1; i < 24; ++i) {
+            const double factor = m[i][col] / m[col][col];
+            for (int j = col; j < 25; ++j) m[i][j] -= factor * m[col][j];
+        }
+    }
+    for (int i = 23; i >= 0; --i) {
+        double v = m[i][24];
+        for (int j = i + 1; j < 24; ++j) v -= m[i][j] * sol[j];
+        sol[i] = v / m[i][i];
+    }
+    for (int i = 0; i < 24; ++i)
+        if (!isfinite(sol[i])) return 0;
+    return 1;
+}
+
+/* Per record (47 f64: 46 lag sums, pixel count): the corrected record, the 24 solved weights in
+ * AR_OFFSETS order, their AV1 integers and shift, and whether the system solved. */
+void avdo_grain_fit(const double *records, uint32_t n, double *corrected, double *weights, int32_t *quantised,
+                    uint32_t *shift, int32_t *solved)
+{
+    for (uint32_t r = 0; r < n; ++r) {
+        const double *rec = records + (size_t)r * 47;
+        double *cor = corrected + (size_t)r * 47, *w = weights + (size_t)r * 24;
+        int32_t *q = quantised + (size_t)r * 24;
+        for (int l = 0; l < 47; ++l) cor[l] = rec[l];
+        const double zero_lag = rec[0];
+        if (zero_lag > 0.0) {
+            double measured[46], rho[46], next[46];
+            for (int l = 0; l < 46; ++l) measured[l] = rho[l] = rec[l] / zero_lag;
+            rho[0] = 1.0;
+            for (int round = 0; round < 20; ++round) {
+                const double zt = gf_cell_term(rho, 0, 0);
+                for (int l = 0; l < 46; ++l) next[l] = measured[l] * (1.0 - zt) + gf_cell_term(rho, GF_LAG_Y[l], gf_lag_x(l));
+                for (int l = 0; l < 46; ++l) rho[l] = next[l];
+            }
+            for (int l = 0; l < 46; ++l) cor[l] = rho[l] * zero_lag;
+        }
+        for (int i = 0; i < 24; ++i) {
+            w[i] = 0.0;
+            q[i] = 0;
+        }
+        shift[r] = 0;
+        solved[r] = 0;
+        double sol[24];
+        if (!avdo_grain_yule_walker(cor, sol)) continue;
+        for (int i = 0; i < 24; ++i) w[i] = sol[i];
+        avdo_grain_quantise(sol, q, shift + r);
+        solved[r] = 1;
+    }
+}

@@ -1022,3 +1022,42 @@ def grain_reduce(partials, chunk=None, record_offset=0):
     chunk = np.ascontiguousarray(chunk, np.float32).copy()
     lib().avdo_grain_reduce(_p(partials, F32P), ctypes.c_uint32(cells), _p(chunk, F32P), ctypes.c_uint32(record_offset))
     return chunk
+
+
+def grain_fit(records):
+    """The AR(3) fit of each 47-value f64 record (undo_mean_removal, yule_walker and quantise_ar,
+    nl4d/grain/fit.rs:120-300; avdo_grain_fit): returns (corrected records, weights [n][24] in
+    AR_OFFSETS order, AV1 integers [n][24], shifts [n], solved [n])."""
+    rec = np.ascontiguousarray(records, np.float64).reshape(-1, 47)
+    n = rec.shape[0]
+    corrected = np.zeros((n, 47), np.float64)
+    weights = np.zeros((n, 24), np.float64)
+    quantised = np.zeros((n, 24), np.int32)
+    shift = np.zeros(n, np.uint32)
+    solved = np.zeros(n, np.int32)
+    lib().avdo_grain_fit(_p(rec, F64P), ctypes.c_uint32(n), _p(corrected, F64P), _p(weights, F64P),
+                         _p(quantised, I32P), _p(shift, U32P), _p(solved, I32P))
+    return corrected, weights, quantised, shift, solved.astype(bool)
+
+
+def ar_offsets() -> list[tuple[int, int]]:
+    """AR_OFFSETS (nl4d/grain/consts.rs:30, ar_offset_table): AV1's 24 causal lag-3 offsets (dy, dx)."""
+    return [(dy, dx) for dy in range(-3, 1) for dx in range(-3, 4) if dy < 0 or dx < 0]
+
+
+def grain_quantise(weights):
+    """quantise_ar (nl4d/grain/fit.rs:272-305) of 24 weights: (AV1 integers, shift)."""
+    w = np.ascontiguousarray(weights, np.float64).reshape(24)
+    q = np.zeros(24, np.int32)
+    shift = np.zeros(1, np.uint32)
+    lib().avdo_grain_quantise(_p(w, F64P), _p(q, I32P), _p(shift, U32P))
+    return q, int(shift[0])
+
+
+def grain_yule_walker(record):
+    """yule_walker and solve (nl4d/grain/fit.rs:160-216) of a 47-value f64 record, without the
+    mean-removal correction: the 24 weights, or None."""
+    rec = np.ascontiguousarray(record, np.float64).reshape(47)
+    sol = np.zeros(24, np.float64)
+    ok = lib().avdo_grain_yule_walker(_p(rec, F64P), _p(sol, F64P))
+    return sol if ok else None
