@@ -231,6 +231,10 @@ SIGNATURES = {
     "avd_grain_partials_len": (u32, [u32, u32]),
     "avd_site_grain_reduce": (ctypes.c_int, [P, u32, P, u32, P]),
     "avd_site_grain_fit": (ctypes.c_int, [P, u32, P, P, P, P, P, P]),
+    "avd_site_grain_save_vectors": (ctypes.c_int, [P, P, P, P, u32, u32, u32, u32, P]),
+    "avd_site_grain_project": (ctypes.c_int, [P, P, P, P, u32, u32, u32, u32, u32, u32, u32, P]),
+    "avd_grain_persist_partials_len": (u32, [u32, u32]),
+    "avd_site_grain_persist": (ctypes.c_int, [P, P, P, u32, u32, u32, u32, P]),
     "avd_host_denoiser_create": (ctypes.c_int, [ctypes.POINTER(EngineSpec), ctypes.POINTER(P)]),
     "avd_host_denoiser_destroy": (None, [P]),
     "avd_host_denoiser_plane_lengths": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64),

@@ -496,6 +496,45 @@ avd_status avd_site_grain_measure(const float *ring_values, const float *denoise
 
 uint32_t avd_grain_partials_len(uint32_t width, uint32_t height) { return avd::grain_partials_len(width, height); }
 
+avd_status avd_site_grain_save_vectors(const int32_t *mv_field, const float *confidence, int32_t *saved_mv,
+                                       float *saved_conf, uint32_t mv_offset, uint32_t conf_offset, uint32_t entry,
+                                       uint32_t blocks, void *stream)
+{
+    if (!mv_field || !confidence || !saved_mv || !saved_conf) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    return hip_status(avd::launch_grain_save_vectors(mv_field, confidence, saved_mv, saved_conf, mv_offset,
+                                                     conf_offset, entry, blocks, static_cast<hipStream_t>(stream)),
+                      "grain_save_vectors");
+}
+
+avd_status avd_site_grain_project(const float *ring_values, const float *denoised, const uint32_t *patterns,
+                                  float *history, uint32_t width, uint32_t height, uint32_t channels,
+                                  uint32_t channel, uint32_t frame_slot, uint32_t history_slot, uint32_t luma_band,
+                                  void *stream)
+{
+    if (!ring_values || !denoised || !patterns || !history) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (width == 0 || height == 0 || width > (1u << 16) || height > (1u << 16) || channels < 1 || channels > 4 ||
+        channel >= channels)
+        return fail(AVD_ERR_INVALID_GEOMETRY, "grain projection needs a frame of 1 to 4 channels and a channel inside it");
+    if (history_slot >= 9) return fail(AVD_ERR_INVALID_OPTIONS, "the grain history holds 9 slots");
+    return hip_status(avd::launch_grain_project(ring_values, denoised, patterns, history, width, height, channels,
+                                                channel, frame_slot, history_slot, luma_band != 0,
+                                                static_cast<hipStream_t>(stream)),
+                      "grain_project");
+}
+
+uint32_t avd_grain_persist_partials_len(uint32_t cells, uint32_t lags) { return avd::grain_persist_blocks(cells) * lags * 3; }
+
+avd_status avd_site_grain_persist(const float *history, float *partials, float *record, uint32_t record_offset,
+                                  uint32_t now_slot, uint32_t cells, uint32_t lags, void *stream)
+{
+    if (!history || !partials || !record) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
+    if (now_slot >= 9 || lags > 8) return fail(AVD_ERR_INVALID_OPTIONS, "the grain history holds 9 slots, 8 lags at most");
+    if ((uint64_t)cells * 9 * 9 > 0xFFFFFFFFull) return fail(AVD_ERR_INVALID_GEOMETRY, "too many grain cells");
+    return hip_status(avd::launch_grain_persist(history, partials, record, record_offset, now_slot, cells, lags,
+                                                static_cast<hipStream_t>(stream)),
+                      "grain_persist");
+}
+
 avd_status avd_site_grain_fit(const double *records, uint32_t count, double *corrected, double *weights,
                               int32_t *quantised, uint32_t *shifts, int32_t *solved, void *stream)
 {

@@ -156,6 +156,17 @@ hipError_t launch_grain_reduce(const float *partials, uint32_t cells, float *chu
                                hipStream_t s);
 // The f64 AR(3) fit of `count` 47-value records: corrected records, 24 weights, their AV1 integers
 // and shift, and whether each system solved.
+// Grain persistence: the vector save, the per-cell projections into a history slot, and the
+// lag-wise persistence sums of the newest slot against the earlier ones, added into `record`.
+hipError_t launch_grain_save_vectors(const int32_t *mv_field, const float *confidence, int32_t *saved_mv,
+                                    float *saved_conf, uint32_t mv_offset, uint32_t conf_offset, uint32_t entry,
+                                    uint32_t blocks, hipStream_t s);
+hipError_t launch_grain_project(const float *ring, const float *out, const uint32_t *patterns, float *history,
+                                uint32_t width, uint32_t height, uint32_t channels, uint32_t channel,
+                                uint32_t frame_slot, uint32_t history_slot, bool luma_band, hipStream_t s);
+uint32_t grain_persist_blocks(uint32_t cells);
+hipError_t launch_grain_persist(const float *history, float *partials, float *record, uint32_t record_offset,
+                                uint32_t now_slot, uint32_t cells, uint32_t lags, hipStream_t s);
 hipError_t launch_grain_fit(const double *records, uint32_t count, double *corrected, double *weights,
                             int32_t *quantised, uint32_t *shifts, int32_t *solved, hipStream_t s);
 hipError_t launch_collab_normalise(unsigned long long *aggregate, int32_t *accum_second, float *out, uint32_t pixels,

@@ -412,6 +412,139 @@ __global__ __launch_bounds__(64) void k_grain_fit(const double *__restrict__ rec
     }
 }
 
+// grain_save_vectors (grain.rs:50-75): one neighbour's vectors and confidences into saved entry
+// `entry`, a grid-stride copy.
+__global__ void k_grain_save_vectors(const int32_t *__restrict__ mv_field, const float *__restrict__ confidence,
+                                     int32_t *__restrict__ saved_mv, float *__restrict__ saved_conf,
+                                     uint32_t mv_offset, uint32_t conf_offset, uint32_t entry, uint32_t blocks)
+{
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < blocks; b += gridDim.x * blockDim.x) {
+        const uint32_t s = entry * blocks + b;
+        saved_mv[s * 2] = mv_field[mv_offset + b * 2];
+        saved_mv[s * 2 + 1] = mv_field[mv_offset + b * 2 + 1];
+        saved_conf[s] = confidence[conf_offset + b];
+    }
+}
+
+// grain_project (grain_persistence.rs:54-153): one wave per 8x8 cell, four cells per workgroup.  The
+// eight pattern projections (pixel y * 8 + x added where the pattern bit is set, taken away where
+// clear) and the denoised mean are 64-slot shuffle trees, the extremes butterflies; lane 0 writes
+// the cell's history entry (8 projections and the flat flag).
+constexpr int kProjections = 8, kEntry = kProjections + 1;
+
+__global__ __launch_bounds__(256) void k_grain_project(const float *__restrict__ ring, const float *__restrict__ out,
+                                                       const uint32_t *__restrict__ patterns,
+                                                       float *__restrict__ history, int W, int H, int C, int ch,
+                                                       uint32_t frame_slot, uint32_t history_slot, int luma_band)
+{
+    const int lane = (int)threadIdx.x & 63, cells_x = W / kCell, cells = cells_x * (H / kCell);
+    const int cell = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    if (cell >= cells) return;  // a whole wave; the kernel has no barrier
+    const int x = cell % cells_x * kCell + (lane & 7), y = cell / cells_x * kCell + (lane >> 3);
+    const float sample = ring[(((size_t)frame_slot * H + y) * W + x) * C + ch];
+    const float clean = out[((size_t)y * W + x) * C + ch];
+    float proj[kProjections];
+#pragma unroll
+    for (int p = 0; p < kProjections; ++p) {
+        const uint32_t bit = (patterns[p * 2 + (lane >> 5)] >> (lane & 31)) & 1u;
+        proj[p] = tree64(((float)bit * 2.0f - 1.0f) * sample);
+    }
+    const float clean_sum = tree64(clean);
+    const float clean_lo = min64(clean), clean_hi = max64(clean), src_lo = min64(sample), src_hi = max64(sample);
+    if (lane == 0) {
+        float *e = history + ((size_t)history_slot * cells + cell) * kEntry;
+#pragma unroll
+        for (int p = 0; p < kProjections; ++p) e[p] = proj[p];
+        const float mean = clean_sum / 64.0f, range = clean_hi - clean_lo;
+        const bool band = !luma_band || (mean > 20.0f / 255.0f && mean < 235.0f / 255.0f);
+        const bool flat = range < 5.0f / 255.0f && band && src_lo >= 4.0f / 255.0f && src_hi <= 251.0f / 255.0f;
+        e[kProjections] = flat ? 1.0f : 0.0f;
+    }
+}
+
+// grain_persist (grain_persistence.rs:155-230): workgroup (block, n) of 256 threads covers 1024
+// cells at lag n + 1; thread t walks cells t, t + 256, t + 512, t + 768 of the block in turn, then
+// each lane (count, auto, cross) folds by a 256-slot halving tree (the 128- and 64-stride steps
+// through LDS, the rest as shuffles).
+constexpr int kPersistThreads = 256, kPersistCells = 1024, kHistoryFrames = 9;
+
+__global__ __launch_bounds__(kPersistThreads) void k_grain_persist(const float *__restrict__ history,
+                                                                   float *__restrict__ partials, uint32_t now_slot,
+                                                                   uint32_t cells, uint32_t blocks)
+{
+    __shared__ float s[3][kPersistThreads];
+    const uint32_t t = threadIdx.x, block = blockIdx.x, lag = blockIdx.y;
+    const uint32_t earlier = (now_slot + kHistoryFrames - 1 - lag) % kHistoryFrames;
+    float count = 0.0f, auto_sum = 0.0f, cross_sum = 0.0f;
+#pragma unroll
+    for (uint32_t step = 0; step < kPersistCells / kPersistThreads; ++step) {
+        const uint32_t cell = block * kPersistCells + t + step * kPersistThreads;
+        if (cell < cells) {
+            const float *now = history + ((size_t)now_slot * cells + cell) * kEntry;
+            const float *ear = history + ((size_t)earlier * cells + cell) * kEntry;
+            if (now[kProjections] * ear[kProjections] > 0.5f) {
+                float sq = 0.0f, pr = 0.0f;
+#pragma unroll
+                for (int p = 0; p < kProjections; ++p) {
+                    sq += now[p] * now[p] + ear[p] * ear[p];
+                    pr += now[p] * ear[p];
+                }
+                count += 1.0f;
+                auto_sum += sq * (1.0f / 1024.0f);
+                cross_sum += pr * (1.0f / 512.0f);
+            }
+        }
+    }
+    s[0][t] = count;
+    s[1][t] = auto_sum;
+    s[2][t] = cross_sum;
+    __syncthreads();
+    if (t < 64) {
+        float v[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            v[l] = (s[l][t] + s[l][t + 128]) + (s[l][t + 64] + s[l][t + 192]);
+#pragma unroll
+            for (int stride = 32; stride >= 1; stride >>= 1) v[l] += __shfl_down(v[l], stride);
+        }
+        if (t == 0) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l) partials[((size_t)lag * blocks + block) * 3 + l] = v[l];
+        }
+    }
+}
+
+// grain_persist_reduce (grain_persistence.rs:232-279): workgroup n of 128 threads; thread t adds
+// blocks t, t + 128, ... in turn per lane, then a 128-slot tree (the 64-stride step through LDS, the
+// rest as shuffles), added into record[record_offset + 3 n + lane].
+__global__ __launch_bounds__(kReduceThreads) void k_grain_persist_reduce(const float *__restrict__ partials,
+                                                                         float *__restrict__ record,
+                                                                         uint32_t record_offset, uint32_t blocks)
+{
+    __shared__ float s[3][kReduceThreads];
+    const uint32_t t = threadIdx.x, lag = blockIdx.x;
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t b = t; b < blocks; b += kReduceThreads)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) acc[l] += partials[((size_t)lag * blocks + b) * 3 + l];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) s[l][t] = acc[l];
+    __syncthreads();
+    if (t < 64) {
+        float v[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            v[l] = s[l][t] + s[l][t + 64];
+#pragma unroll
+            for (int stride = 32; stride >= 1; stride >>= 1) v[l] += __shfl_down(v[l], stride);
+        }
+        if (t == 0) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l) record[record_offset + lag * 3 + l] += v[l];
+        }
+    }
+}
+
 }  // namespace
 
 uint32_t grain_partials_len(uint32_t width, uint32_t height)
@@ -445,6 +578,41 @@ hipError_t launch_grain_fit(const double *records, uint32_t count, double *corre
     if (!records || !corrected || !weights || !quantised || !shifts || !solved) return hipErrorInvalidValue;
     if (count == 0) return hipSuccess;
     k_grain_fit<<<count, 64, 0, s>>>(records, corrected, weights, quantised, shifts, solved);
+    return hipGetLastError();
+}
+
+hipError_t launch_grain_save_vectors(const int32_t *mv_field, const float *confidence, int32_t *saved_mv,
+                                    float *saved_conf, uint32_t mv_offset, uint32_t conf_offset, uint32_t entry,
+                                    uint32_t blocks, hipStream_t s)
+{
+    if (!mv_field || !confidence || !saved_mv || !saved_conf) return hipErrorInvalidValue;
+    if (blocks == 0) return hipSuccess;
+    k_grain_save_vectors<<<std::min<uint32_t>((blocks + 255) / 256, 1024), 256, 0, s>>>(
+        mv_field, confidence, saved_mv, saved_conf, mv_offset, conf_offset, entry, blocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_grain_project(const float *ring, const float *out, const uint32_t *patterns, float *history,
+                                uint32_t width, uint32_t height, uint32_t channels, uint32_t channel,
+                                uint32_t frame_slot, uint32_t history_slot, bool luma_band, hipStream_t s)
+{
+    const uint32_t cells = (width / kCell) * (height / kCell);
+    if (cells == 0) return hipSuccess;
+    k_grain_project<<<(cells + 3) / 4, 256, 0, s>>>(ring, out, patterns, history, (int)width, (int)height,
+                                                   (int)channels, (int)channel, frame_slot, history_slot,
+                                                   luma_band ? 1 : 0);
+    return hipGetLastError();
+}
+
+uint32_t grain_persist_blocks(uint32_t cells) { return (cells + kPersistCells - 1) / kPersistCells; }
+
+hipError_t launch_grain_persist(const float *history, float *partials, float *record, uint32_t record_offset,
+                                uint32_t now_slot, uint32_t cells, uint32_t lags, hipStream_t s)
+{
+    const uint32_t blocks = grain_persist_blocks(cells);
+    if (blocks == 0 || lags == 0) return hipSuccess;
+    k_grain_persist<<<dim3(blocks, lags), kPersistThreads, 0, s>>>(history, partials, now_slot, cells, blocks);
+    k_grain_persist_reduce<<<lags, kReduceThreads, 0, s>>>(partials, record, record_offset, blocks);
     return hipGetLastError();
 }
 

@@ -272,12 +272,45 @@ def grain_measure_measure(avd, torch, clip, calls=20):
     end.record()
     torch.cuda.synchronize()
     fit_ms = start.elapsed_time(end) / calls
+    # Persistence: the frame's cell projections into one of 9 history slots, then the newest slot
+    # against the 8 before it.
+    patterns = [0x9966669999666699, 0x9669699669969669, 0xA55AA55A5AA55AA5, 0xAA55AA55AA55AA55,
+                0xA55A5AA5A55A5AA5, 0x9966996666996699, 0xAA5555AA55AAAA55, 0x9669966996699669]
+    words = torch.tensor([w - (1 << 32) if w >= (1 << 31) else w for q in patterns
+                          for w in (q & 0xFFFFFFFF, q >> 32)], dtype=torch.int32, device="cuda")
+    cells = (W // 8) * (H // 8)
+    history = torch.zeros(9 * cells * 9, dtype=torch.float32, device="cuda")
+    persist_partials = torch.empty(lib.avd_grain_persist_partials_len(cells, 8), dtype=torch.float32, device="cuda")
+    persist_record = torch.zeros(24, dtype=torch.float32, device="cuda")
+
+    def project_call(slot):
+        avd._lib.check(lib.avd_site_grain_project(p(ring), p(denoised), p(words), p(history), W, H, 1, 0, 0, slot, 1,
+                                                  stream))
+
+    def persist_call():
+        avd._lib.check(lib.avd_site_grain_persist(p(history), p(persist_partials), p(persist_record), 0, 0, cells, 8,
+                                                  stream))
+
+    for slot in range(9):
+        project_call(slot)
+    timings = {}
+    for name, fn in (("project", lambda: project_call(0)), ("persist", persist_call)):
+        start.record()
+        for _ in range(calls):
+            fn()
+        end.record()
+        torch.cuda.synchronize()
+        timings[name] = start.elapsed_time(end) / calls
     return {"kernels": "k_grain_measure", "workload": "3840x2160 luma, flat denoised frame (every interior cell "
             "accepted), zero vectors", "avg_launch_ms": round(ms, 4), "bytes_per_launch": nbytes,
             "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
             "reduce": {"kernels": "k_grain_reduce", "avg_launch_ms": round(reduce_ms, 4), "bytes_per_launch": n * 4,
                        "achieved": round(n * 4 / (reduce_ms * 1e-3) / 1e9, 1), "unit": "GB/s"},
-            "fit": {"kernels": "k_grain_fit", "records": 16, "dtype": "f64", "avg_launch_ms": round(fit_ms, 4)}}
+            "fit": {"kernels": "k_grain_fit", "records": 16, "dtype": "f64", "avg_launch_ms": round(fit_ms, 4)},
+            "project": {"kernels": "k_grain_project", "avg_launch_ms": round(timings["project"], 4),
+                        "bytes_per_launch": 2 * W * H * 4 + cells * 9 * 4},
+            "persist": {"kernels": "k_grain_persist + k_grain_persist_reduce", "lags": 8,
+                        "avg_launch_ms": round(timings["persist"], 4), "bytes_per_launch": 9 * cells * 9 * 4}}
 
 
 def run_steps(eng, frames, outs, start, count):

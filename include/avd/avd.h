@@ -518,6 +518,29 @@ uint32_t avd_grain_partials_len(uint32_t width, uint32_t height);
  * sums are bit-identical to the reference's. */
 avd_status avd_site_grain_reduce(const float *partials, uint32_t cells, float *chunk, uint32_t record_offset,
                                  void *stream);
+/* grain_save_vectors (nl4d/kernels/grain.rs:50-75; launched by nl4d/grain/export.rs:264-290): one
+ * neighbour's `blocks` motion vectors (mv_field[mv_offset..], x then y) and confidences
+ * (confidence[conf_offset..]) into saved entry `entry` of saved_mv / saved_conf. */
+avd_status avd_site_grain_save_vectors(const int32_t *mv_field, const float *confidence, int32_t *saved_mv,
+                                       float *saved_conf, uint32_t mv_offset, uint32_t conf_offset, uint32_t entry,
+                                       uint32_t blocks, void *stream);
+/* grain_project (nl4d/kernels/grain_persistence.rs:54-153): per whole 8x8 cell of channel `channel`
+ * of ring slot `frame_slot`, the 8 projections onto `patterns` (PATTERNS, nl4d/grain/consts.rs:76-85,
+ * as two u32 words each, the low word first) and the flat flag of the denoised cell, into history
+ * slot `history_slot` (of 9): 9 floats per cell at (history_slot * cells + cell) * 9. */
+avd_status avd_site_grain_project(const float *ring_values, const float *denoised, const uint32_t *patterns,
+                                  float *history, uint32_t width, uint32_t height, uint32_t channels,
+                                  uint32_t channel, uint32_t frame_slot, uint32_t history_slot, uint32_t luma_band,
+                                  void *stream);
+/* Floats of grain_persist's partials for `cells` cells at `lags` lags: 3 per block of 1024 cells per lag. */
+uint32_t avd_grain_persist_partials_len(uint32_t cells, uint32_t lags);
+/* grain_persist then grain_persist_reduce (nl4d/kernels/grain_persistence.rs:155-279; launched by
+ * nl4d/grain/persistence.rs:245-300): history slot `now_slot` against each of the `lags` (at most 8)
+ * slots before it, per lag the cell count, auto sum and cross sum of the cells flat in both, added
+ * into record[record_offset + 3 lag ..]; `partials` holds avd_grain_persist_partials_len floats.
+ * Bit-identical to the reference's f32 sums. */
+avd_status avd_site_grain_persist(const float *history, float *partials, float *record, uint32_t record_offset,
+                                  uint32_t now_slot, uint32_t cells, uint32_t lags, void *stream);
 /* The AR(3) grain fit in f64, on the GPU (undo_mean_removal, yule_walker, solve and quantise_ar,
  * nl4d/grain/fit.rs:120-300; called per pooled record by nl4d/grain/segment.rs:262-265 and
  * nl4d/grain/chroma.rs:159-160).  Each of `count` records holds 47 f64 values (the 46 lag sums in

@@ -1613,3 +1613,120 @@ void avdo_grain_fit(const double *records, uint32_t n, double *corrected, double
         solved[r] = 1;
     }
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Grain persistence and the vector save (nl4d/kernels/grain_persistence.rs:54-279,           */
+/* nl4d/kernels/grain.rs:50-75).                                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+/* grain_save_vectors: one neighbour's vectors and confidences into saved entry `entry`. */
+void avdo_grain_save_vectors(const int32_t *mv_field, const float *confidence, int32_t *saved_mv, float *saved_conf,
+                             uint32_t mv_offset, uint32_t conf_offset, uint32_t entry, uint32_t blocks)
+{
+    for (uint32_t b = 0; b < blocks; ++b) {
+        const uint32_t s = entry * blocks + b;
+        saved_mv[s * 2] = mv_field[mv_offset + b * 2];
+        saved_mv[s * 2 + 1] = mv_field[mv_offset + b * 2 + 1];
+        saved_conf[s] = confidence[conf_offset + b];
+    }
+}
+
+/* grain_project (grain_persistence.rs:54-153): per 8x8 cell the 8 pattern projections of channel
+ * `ch` of ring slot `frame_slot` (pixel y * 8 + x added where the pattern bit is set, taken away
+ * where clear; `patterns` two u32 words per pattern, low first) and the flat flag of the denoised
+ * cell, into history entry (history_slot * cells + cell) * 9; 64-slot halving trees. */
+void avdo_grain_project(const float *ring, const float *out, const uint32_t *patterns, float *history, uint32_t W,
+                        uint32_t H, uint32_t C, uint32_t ch, uint32_t frame_slot, uint32_t history_slot, int luma_band)
+{
+    const uint32_t cells_x = W / 8, cells = cells_x * (H / 8);
+    for (uint32_t cell = 0; cell < cells; ++cell) {
+        const uint32_t ox = cell % cells_x * 8, oy = cell / cells_x * 8;
+        float s[9][64], lo0[64], lo1[64], hi0[64], hi1[64];
+        for (uint32_t t = 0; t < 64; ++t) {
+            const uint32_t x = ox + t % 8, y = oy + t / 8;
+            const float sample = ring[(((size_t)frame_slot * H + y) * W + x) * C + ch];
+            const float clean = out[((size_t)y * W + x) * C + ch];
+            for (uint32_t p = 0; p < 8; ++p) {
+                const uint32_t bit = (patterns[p * 2 + t / 32] >> (t % 32)) & 1u;
+                s[p][t] = ((float)bit * 2.0f - 1.0f) * sample;
+            }
+            s[8][t] = clean;
+            lo0[t] = hi0[t] = clean;
+            lo1[t] = hi1[t] = sample;
+        }
+        for (uint32_t k = 0; k < 9; ++k) gr_tree_sum(s[k]);
+        for (uint32_t stride = 32; stride >= 1; stride /= 2)
+            for (uint32_t i = 0; i < stride; ++i) {
+                lo0[i] = fminf(lo0[i], lo0[i + stride]);
+                lo1[i] = fminf(lo1[i], lo1[i + stride]);
+                hi0[i] = fmaxf(hi0[i], hi0[i + stride]);
+                hi1[i] = fmaxf(hi1[i], hi1[i + stride]);
+            }
+        float *e = history + ((size_t)history_slot * cells + cell) * 9;
+        for (uint32_t p = 0; p < 8; ++p) e[p] = s[p][0];
+        const float mean = s[8][0] / 64.0f, range = hi0[0] - lo0[0];
+        const int band = !luma_band || (mean > 20.0f / 255.0f && mean < 235.0f / 255.0f);
+        const int flat = range < 5.0f / 255.0f && band && lo1[0] >= 4.0f / 255.0f && hi1[0] <= 251.0f / 255.0f;
+        e[8] = flat ? 1.0f : 0.0f;
+    }
+}
+
+/* grain_persist (grain_persistence.rs:155-230): per block of 1024 cells and lag n + 1, the count,
+ * auto sum and cross sum of the cells flagged flat in both slots; thread t of 256 walks cells
+ * t, t + 256, t + 512, t + 768 of the block, then a 256-slot halving tree per lane. */
+void avdo_grain_persist(const float *history, float *partials, uint32_t now_slot, uint32_t cells, uint32_t lags)
+{
+    const uint32_t blocks = (cells + 1023) / 1024;
+    const float auto_scale = 1.0f / 1024.0f, cross_scale = 1.0f / 512.0f;
+    for (uint32_t lag = 0; lag < lags; ++lag)
+        for (uint32_t b = 0; b < blocks; ++b) {
+            const uint32_t earlier = (now_slot + 9 - 1 - lag) % 9;
+            float sc[3][256];
+            for (uint32_t t = 0; t < 256; ++t) {
+                float count = 0.0f, a = 0.0f, x = 0.0f;
+                for (uint32_t step = 0; step < 4; ++step) {
+                    const uint32_t cell = b * 1024 + t + step * 256;
+                    if (cell >= cells) continue;
+                    const float *now = history + ((size_t)now_slot * cells + cell) * 9;
+                    const float *ear = history + ((size_t)earlier * cells + cell) * 9;
+                    if (now[8] * ear[8] > 0.5f) {
+                        float sq = 0.0f, pr = 0.0f;
+                        for (uint32_t p = 0; p < 8; ++p) {
+                            sq += now[p] * now[p] + ear[p] * ear[p];
+                            pr += now[p] * ear[p];
+                        }
+                        count += 1.0f;
+                        a += sq * auto_scale;
+                        x += pr * cross_scale;
+                    }
+                }
+                sc[0][t] = count;
+                sc[1][t] = a;
+                sc[2][t] = x;
+            }
+            for (uint32_t stride = 128; stride >= 1; stride /= 2)
+                for (uint32_t i = 0; i < stride; ++i)
+                    for (uint32_t l = 0; l < 3; ++l) sc[l][i] += sc[l][i + stride];
+            for (uint32_t l = 0; l < 3; ++l) partials[((size_t)lag * blocks + b) * 3 + l] = sc[l][0];
+        }
+}
+
+/* grain_persist_reduce (grain_persistence.rs:232-279): per lag, thread t of 128 adds blocks t,
+ * t + 128, ... in turn, a 128-slot tree per lane, added into record[record_offset + 3 lag + lane]. */
+void avdo_grain_persist_reduce(const float *partials, float *record, uint32_t record_offset, uint32_t blocks,
+                               uint32_t lags)
+{
+    for (uint32_t lag = 0; lag < lags; ++lag) {
+        float sc[3][128];
+        for (uint32_t t = 0; t < 128; ++t)
+            for (uint32_t l = 0; l < 3; ++l) {
+                float acc = 0.0f;
+                for (uint32_t b = t; b < blocks; b += 128) acc += partials[((size_t)lag * blocks + b) * 3 + l];
+                sc[l][t] = acc;
+            }
+        for (uint32_t stride = 64; stride >= 1; stride /= 2)
+            for (uint32_t i = 0; i < stride; ++i)
+                for (uint32_t l = 0; l < 3; ++l) sc[l][i] += sc[l][i + stride];
+        for (uint32_t l = 0; l < 3; ++l) record[record_offset + lag * 3 + l] += sc[l][0];
+    }
+}

@@ -1061,3 +1061,51 @@ def grain_yule_walker(record):
     sol = np.zeros(24, np.float64)
     ok = lib().avdo_grain_yule_walker(_p(rec, F64P), _p(sol, F64P))
     return sol if ok else None
+
+
+# ---- grain persistence (nl4d/kernels/grain_persistence.rs, nl4d/grain/consts.rs:46-85) -----------
+
+GRAIN_PATTERNS = [0x99666699_99666699, 0x96696996_69969669, 0xA55AA55A_5AA55AA5, 0xAA55AA55_AA55AA55,
+                  0xA55A5AA5_A55A5AA5, 0x99669966_66996699, 0xAA5555AA_55AAAA55, 0x96699669_96699669]
+HISTORY_FRAMES, HISTORY_ENTRY = 9, 9
+
+
+def grain_pattern_words() -> np.ndarray:
+    """PATTERNS (nl4d/grain/consts.rs:76-85) as two u32 words each, the low word first."""
+    return np.array([w for p in GRAIN_PATTERNS for w in (p & 0xFFFFFFFF, p >> 32)], np.uint32)
+
+
+def grain_save_vectors(mv_field, confidence, saved_mv, saved_conf, mv_offset, conf_offset, entry, blocks):
+    """grain_save_vectors (nl4d/kernels/grain.rs:50-75): copies into saved_mv / saved_conf in place."""
+    lib().avdo_grain_save_vectors(_p(np.ascontiguousarray(mv_field, np.int32), I32P),
+                                  _p(np.ascontiguousarray(confidence, np.float32), F32P), _p(saved_mv, I32P),
+                                  _p(saved_conf, F32P), ctypes.c_uint32(mv_offset), ctypes.c_uint32(conf_offset),
+                                  ctypes.c_uint32(entry), ctypes.c_uint32(blocks))
+
+
+def grain_project(ring, out, width, height, channels, channel, frame_slot, history, history_slot, luma_band=True):
+    """grain_project (nl4d/kernels/grain_persistence.rs:54-153): writes history slot ``history_slot``
+    (9 floats per cell: 8 projections, the flat flag) into ``history`` in place."""
+    ring = np.ascontiguousarray(ring, np.float32).reshape(-1)
+    out = np.ascontiguousarray(out, np.float32).reshape(-1)
+    u = ctypes.c_uint32
+    lib().avdo_grain_project(_p(ring, F32P), _p(out, F32P), _p(grain_pattern_words(), U32P), _p(history, F32P),
+                             u(width), u(height), u(channels), u(channel), u(frame_slot), u(history_slot),
+                             ctypes.c_int(int(bool(luma_band))))
+
+
+def grain_persist(history, now_slot, cells, lags):
+    """grain_persist (nl4d/kernels/grain_persistence.rs:155-230): [lags][blocks][3] partials."""
+    blocks = -(-cells // 1024)
+    partials = np.full(lags * blocks * 3, np.nan, np.float32)
+    lib().avdo_grain_persist(_p(np.ascontiguousarray(history, np.float32), F32P), _p(partials, F32P),
+                             ctypes.c_uint32(now_slot), ctypes.c_uint32(cells), ctypes.c_uint32(lags))
+    return partials
+
+
+def grain_persist_reduce(partials, record, record_offset, blocks, lags):
+    """grain_persist_reduce (nl4d/kernels/grain_persistence.rs:232-279): adds into ``record``."""
+    record = np.ascontiguousarray(record, np.float32).copy()
+    lib().avdo_grain_persist_reduce(_p(np.ascontiguousarray(partials, np.float32), F32P), _p(record, F32P),
+                                    ctypes.c_uint32(record_offset), ctypes.c_uint32(blocks), ctypes.c_uint32(lags))
+    return record
