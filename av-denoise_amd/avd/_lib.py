@@ -231,6 +231,11 @@ SIGNATURES = {
     "avd_grain_partials_len": (u32, [u32, u32]),
     "avd_site_grain_reduce": (ctypes.c_int, [P, u32, P, u32, P]),
     "avd_site_grain_fit": (ctypes.c_int, [P, u32, P, P, P, P, P, P]),
+    "avd_grain_bucket_edges": (None, [P]),
+    "avd_grain_hist_median": (ctypes.c_double, [P, P, P]),
+    "avd_grain_finest_scaling_shift": (u32, [ctypes.c_double]),
+    "avd_grain_scaling_points": (ctypes.c_int, [P, P, u32, ctypes.c_double, P, P]),
+    "avd_grain_quantise_ar": (None, [P, P, P]),
     "avd_site_grain_save_vectors": (ctypes.c_int, [P, P, P, P, u32, u32, u32, u32, P]),
     "avd_site_grain_project": (ctypes.c_int, [P, P, P, P, u32, u32, u32, u32, u32, u32, u32, P]),
     "avd_grain_persist_partials_len": (u32, [u32, u32]),

@@ -34,6 +34,7 @@ SOURCES = {
     "nlm.hip": [],
     "noise.hip": EXACT,
     "grain.hip": EXACT,
+    "grain_host.cpp": EXACT,
     "engine.cpp": EXACT,
     "host.cpp": EXACT,
     "tune.cpp": EXACT,

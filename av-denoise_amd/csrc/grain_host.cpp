@@ -1,0 +1,99 @@
+// Host side of the grain table (nl4d/grain/fit.rs): the std bucket edges, the histogram median,
+// AV1 scaling points and the AR weights' quantisation.  Plain f64 arithmetic in the reference's
+// order; these run once per grain segment on a few hundred numbers, so they stay on the host as
+// they do in the reference.
+
+#include <cmath>
+#include <cstdint>
+
+#include "avd/avd.h"
+
+namespace {
+
+constexpr int kBuckets = 64;
+constexpr float kStdMin = 0.05f / 255.0f, kStdMax = 32.0f / 255.0f;
+
+// raw_scaling (fit.rs:323-326): the scaling value one sigma needs before the scaling shift.
+double raw_scaling(double sigma, double sigma_template) { return sigma * 255.0 / sigma_template; }
+
+}  // namespace
+
+extern "C" {
+
+void avd_grain_bucket_edges(float *edges)
+{
+    if (!edges) return;
+    const double ratio = (double)kStdMax / (double)kStdMin;
+    for (int i = 0; i <= kBuckets; ++i) {
+        const double fraction = (double)i / (double)kBuckets;
+        edges[i] = (float)((double)kStdMin * std::pow(ratio, fraction));
+    }
+}
+
+double avd_grain_hist_median(const uint32_t *counts, const float *edges, int32_t *found)
+{
+    if (found) *found = 0;
+    if (!counts || !edges) return 0.0;
+    uint64_t total = 0;
+    for (int b = 0; b < kBuckets; ++b) total += counts[b];
+    if (total == 0) return 0.0;
+    const double target = 0.5 * (double)total;
+    double below = 0.0;
+    for (int b = 0; b < kBuckets; ++b) {
+        const double count = (double)counts[b];
+        if (count > 0.0 && below + count >= target) {
+            const double fraction = (target - below) / count;
+            const double low = (double)edges[b], high = (double)edges[b + 1];
+            if (found) *found = 1;
+            return low * std::pow(high / low, fraction);
+        }
+        below += count;
+    }
+    return 0.0;
+}
+
+uint32_t avd_grain_finest_scaling_shift(double peak)
+{
+    for (uint32_t shift : {11u, 10u, 9u})
+        if (peak * (double)(1u << shift) <= 255.0) return shift;
+    return 8;
+}
+
+avd_status avd_grain_scaling_points(const uint8_t *luma, const double *sigma, uint32_t count, double sigma_template,
+                                    uint8_t *scaled, uint32_t *shift)
+{
+    if ((count && (!luma || !sigma || !scaled)) || !shift) return AVD_ERR_NULL_ARGUMENT;
+    (void)luma;  // the points keep their luma; only the scaling values are computed
+    double peak = 0.0;
+    for (uint32_t i = 0; i < count; ++i) peak = std::fmax(peak, raw_scaling(sigma[i], sigma_template));
+    const uint32_t s = avd_grain_finest_scaling_shift(peak);
+    const double factor = (double)(1u << s);
+    for (uint32_t i = 0; i < count; ++i) {
+        const double v = std::fmin(std::round(raw_scaling(sigma[i], sigma_template) * factor), 255.0);
+        scaled[i] = v > 0.0 ? (uint8_t)v : 0;  // Rust's saturating cast: negative and NaN give 0
+    }
+    *shift = s;
+    return AVD_OK;
+}
+
+void avd_grain_quantise_ar(const double *weights, int32_t *quantised, uint32_t *shift)
+{
+    if (!weights || !quantised || !shift) return;
+    uint32_t s = 6;
+    for (uint32_t cand : {9u, 8u, 7u}) {
+        bool fits = true;
+        for (int i = 0; i < 24; ++i) {
+            const double v = std::round(weights[i] * (double)(1u << cand));
+            fits = fits && v >= -128.0 && v <= 127.0;
+        }
+        if (fits) {
+            s = cand;
+            break;
+        }
+    }
+    for (int i = 0; i < 24; ++i)
+        quantised[i] = (int32_t)std::fmin(std::fmax(std::round(weights[i] * (double)(1u << s)), -128.0), 127.0);
+    *shift = s;
+}
+
+}  // extern "C"

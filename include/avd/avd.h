@@ -541,6 +541,21 @@ uint32_t avd_grain_persist_partials_len(uint32_t cells, uint32_t lags);
  * Bit-identical to the reference's f32 sums. */
 avd_status avd_site_grain_persist(const float *history, float *partials, float *record, uint32_t record_offset,
                                   uint32_t now_slot, uint32_t cells, uint32_t lags, void *stream);
+/* Host side of the grain table (nl4d/grain/fit.rs), f64 arithmetic in the reference's order.
+ * avd_grain_bucket_edges: the 65 std bucket edges, log-spaced from 0.05/255 to 32/255
+ * (bucket_edges, fit.rs:28-38).  avd_grain_hist_median: the median std of a 64-bucket histogram,
+ * interpolated geometrically inside its bucket; *found = 0 for an empty histogram (hist_median,
+ * fit.rs:51-74).  avd_grain_finest_scaling_shift: the finest shift 11..9 keeping `peak` within a
+ * byte, else 8 (fit.rs:335-346).  avd_grain_scaling_points: each point's AV1 scaling value
+ * sigma * 255 / sigma_template at the finest shift for the largest, rounded and capped at 255
+ * (scaling_points, fit.rs:310-360).  avd_grain_quantise_ar: the 24 weights' AV1 integers at the
+ * finest AR shift 9..7 that fits a signed byte, else 6, clamped (quantise_ar, fit.rs:272-305). */
+void avd_grain_bucket_edges(float *edges);
+double avd_grain_hist_median(const uint32_t *counts, const float *edges, int32_t *found);
+uint32_t avd_grain_finest_scaling_shift(double peak);
+avd_status avd_grain_scaling_points(const uint8_t *luma, const double *sigma, uint32_t count, double sigma_template,
+                                    uint8_t *scaled, uint32_t *shift);
+void avd_grain_quantise_ar(const double *weights, int32_t *quantised, uint32_t *shift);
 /* The AR(3) grain fit in f64, on the GPU (undo_mean_removal, yule_walker, solve and quantise_ar,
  * nl4d/grain/fit.rs:120-300; called per pooled record by nl4d/grain/segment.rs:262-265 and
  * nl4d/grain/chroma.rs:159-160).  Each of `count` records holds 47 f64 values (the 46 lag sums in
