@@ -76,6 +76,34 @@ avd_status avd_grain_scaling_points(const uint8_t *luma, const double *sigma, ui
     return AVD_OK;
 }
 
+int32_t avd_grain_pooled_record(const uint32_t *source_hist, const double *group_records, const float *edges,
+                                double *record)
+{
+    if (!source_hist || !group_records || !edges || !record) return 0;
+    // overall_median (segment.rs:170-180): the 16 brightness rows merged, then their median.
+    uint32_t merged[kBuckets] = {};
+    for (int bin = 0; bin < 16; ++bin)
+        for (int b = 0; b < kBuckets; ++b) merged[b] += source_hist[bin * kBuckets + b];
+    int32_t found = 0;
+    const double median = avd_grain_hist_median(merged, edges, &found);
+    if (!found) return 0;
+    const double band_low = 0.5 * median, band_high = 2.0 * median;
+    for (int l = 0; l < 47; ++l) record[l] = 0.0;
+    for (int g = 0; g < 16; ++g) {
+        const double group_low = (double)edges[g * 4], group_high = (double)edges[(g + 1) * 4];
+        if (group_high < band_low || group_low > band_high) continue;
+        for (int l = 0; l < 47; ++l) record[l] += group_records[g * 47 + l];
+    }
+    return record[46] < 50000.0 ? 0 : 1;
+}
+
+void avd_grain_take_out_fine_noise(double *corrected, double floor, double fine_noise)
+{
+    if (!corrected) return;
+    const double share = std::fmin(fine_noise, 0.95) * floor;
+    corrected[0] -= share * corrected[0];
+}
+
 void avd_grain_quantise_ar(const double *weights, int32_t *quantised, uint32_t *shift)
 {
     if (!weights || !quantised || !shift) return;

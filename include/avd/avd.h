@@ -556,6 +556,15 @@ uint32_t avd_grain_finest_scaling_shift(double peak);
 avd_status avd_grain_scaling_points(const uint8_t *luma, const double *sigma, uint32_t count, double sigma_template,
                                     uint8_t *scaled, uint32_t *shift);
 void avd_grain_quantise_ar(const double *weights, int32_t *quantised, uint32_t *shift);
+/* pooled_record (nl4d/grain/segment.rs:170-180, 286-311): the strength groups whose std range
+ * (edges[4 g] .. edges[4 g + 4]) meets half to twice the median of the source histogram (16 x 64
+ * counts, brightness rows merged) summed into one 47-value record (46 lag sums, pixel count) from
+ * `group_records` (16 records of 47); returns 1, or 0 for an empty histogram or fewer than 50000
+ * pixels.  take_out_fine_noise (segment.rs:313-318): min(fine_noise, 0.95) * floor of the zero lag
+ * taken off a corrected record. */
+int32_t avd_grain_pooled_record(const uint32_t *source_hist, const double *group_records, const float *edges,
+                                double *record);
+void avd_grain_take_out_fine_noise(double *corrected, double floor, double fine_noise);
 /* The AR(3) grain fit in f64, on the GPU (undo_mean_removal, yule_walker, solve and quantise_ar,
  * nl4d/grain/fit.rs:120-300; called per pooled record by nl4d/grain/segment.rs:262-265 and
  * nl4d/grain/chroma.rs:159-160).  Each of `count` records holds 47 f64 values (the 46 lag sums in

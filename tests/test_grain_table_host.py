@@ -103,3 +103,40 @@ def test_quantise_ar_matches_the_oracle_and_the_known_answers():
         lib().avd_grain_quantise_ar(ctypes.c_void_p(w.ctypes.data), ctypes.c_void_p(q.ctypes.data), ctypes.byref(s))
         want_q, want_s = O.grain_quantise(w)
         assert s.value == want_s and np.array_equal(q, want_q), scale
+
+
+def pooled(hist, records):
+    h = np.ascontiguousarray(hist, np.uint32)
+    r = np.ascontiguousarray(records, np.float64)
+    e = edges()
+    out = np.full(47, -1.0)
+    ok = lib().avd_grain_pooled_record(ctypes.c_void_p(h.ctypes.data), ctypes.c_void_p(r.ctypes.data),
+                                       ctypes.c_void_p(e.ctypes.data), ctypes.c_void_p(out.ctypes.data))
+    return out if ok else None
+
+
+def test_pooled_record_sums_the_groups_near_the_median():
+    # Every block in bucket 30 of brightness bins 3 and 9: the median is bucket 30's geometric
+    # middle, and half to twice it, a 4x span of 1.106x buckets, reaches buckets 23.6 to 37.4, so
+    # strength groups 5 to 9 (buckets 20-23 ... 36-39) pool.
+    hist = np.zeros((16, 64), np.uint32)
+    hist[3, 30], hist[9, 30] = 100, 200
+    records = np.zeros((16, 47))
+    records[:, :46] = np.arange(16)[:, None] + 1.0
+    records[:, 46] = 20000.0
+    rec = pooled(hist, records)
+    assert rec is not None
+    assert np.array_equal(rec[:46], np.full(46, float(sum(range(6, 11)))))
+    assert rec[46] == 5 * 20000.0
+    records[:, 46] = 9000.0  # 45000 pixels in the band: below the 50000 a texture needs
+    assert pooled(hist, records) is None
+    assert pooled(np.zeros((16, 64)), records) is None
+
+
+def test_take_out_fine_noise_caps_the_floor_share():
+    c = np.array([4.0] + [1.0] * 46)
+    lib().avd_grain_take_out_fine_noise(ctypes.c_void_p(c.ctypes.data), 0.5, 0.5)
+    assert c[0] == 4.0 - 0.25 * 4.0 and np.all(c[1:] == 1.0)
+    c[0] = 4.0
+    lib().avd_grain_take_out_fine_noise(ctypes.c_void_p(c.ctypes.data), 1.0, 3.0)  # at most 0.95 of the floor
+    assert c[0] == 4.0 - 0.95 * 4.0
