@@ -238,6 +238,7 @@ SIGNATURES = {
     "avd_grain_quantise_ar": (None, [P, P, P]),
     "avd_grain_pooled_record": (ctypes.c_int32, [P, P, P, P]),
     "avd_grain_take_out_fine_noise": (None, [P, ctypes.c_double, ctypes.c_double]),
+    "avd_grain_floor_fraction": (ctypes.c_double, [P]),
     "avd_site_grain_save_vectors": (ctypes.c_int, [P, P, P, P, u32, u32, u32, u32, P]),
     "avd_site_grain_project": (ctypes.c_int, [P, P, P, P, u32, u32, u32, u32, u32, u32, u32, P]),
     "avd_grain_persist_partials_len": (u32, [u32, u32]),

@@ -104,6 +104,39 @@ void avd_grain_take_out_fine_noise(double *corrected, double floor, double fine_
     corrected[0] -= share * corrected[0];
 }
 
+double avd_grain_floor_fraction(const double *weights)
+{
+    if (!weights) return 0.0;
+    // ar_spectrum / ar_power (fit.rs:236-270): unit noise through the AR filter on a 64 x 64 grid
+    // of frequencies, row by row; AR_OFFSETS order dy -3..0, dx -3..3, the causal half.
+    int oy[24], ox[24], k = 0;
+    for (int dy = -3; dy <= 0; ++dy)
+        for (int dx = -3; dx <= 3; ++dx)
+            if (dy < 0 || dx < 0) {
+                oy[k] = dy;
+                ox[k] = dx;
+                ++k;
+            }
+    const double step = 6.283185307179586 / 64.0;
+    double lowest = INFINITY, total = 0.0;
+    for (int row = 0; row < 64; ++row)
+        for (int col = 0; col < 64; ++col) {
+            const double fy = (double)row * step, fx = (double)col * step;
+            double re = 1.0, im = 0.0;
+            for (int i = 0; i < 24; ++i) {
+                const double phase = fy * (double)oy[i] + fx * (double)ox[i];
+                re -= weights[i] * std::cos(phase);
+                im -= weights[i] * std::sin(phase);
+            }
+            const double power = 1.0 / (re * re + im * im);
+            lowest = std::fmin(lowest, power);
+            total += power;
+        }
+    const double fraction = lowest / (total / 4096.0);
+    if (!std::isfinite(fraction)) return 0.0;
+    return std::fmin(std::fmax(fraction, 0.0), 1.0);
+}
+
 void avd_grain_quantise_ar(const double *weights, int32_t *quantised, uint32_t *shift)
 {
     if (!weights || !quantised || !shift) return;

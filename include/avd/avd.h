@@ -565,6 +565,10 @@ void avd_grain_quantise_ar(const double *weights, int32_t *quantised, uint32_t *
 int32_t avd_grain_pooled_record(const uint32_t *source_hist, const double *group_records, const float *edges,
                                 double *record);
 void avd_grain_take_out_fine_noise(double *corrected, double floor, double fine_noise);
+/* floor_fraction (nl4d/grain/fit.rs:218-270): the share of the AR filter's power spectrum (24
+ * weights in AR_OFFSETS order, 64 x 64 frequencies) that a flat white floor accounts for, its lowest
+ * point over its mean, clamped to 0..1 (0 when not finite). */
+double avd_grain_floor_fraction(const double *weights);
 /* The AR(3) grain fit in f64, on the GPU (undo_mean_removal, yule_walker, solve and quantise_ar,
  * nl4d/grain/fit.rs:120-300; called per pooled record by nl4d/grain/segment.rs:262-265 and
  * nl4d/grain/chroma.rs:159-160).  Each of `count` records holds 47 f64 values (the 46 lag sums in

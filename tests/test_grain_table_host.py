@@ -140,3 +140,29 @@ def test_take_out_fine_noise_caps_the_floor_share():
     c[0] = 4.0
     lib().avd_grain_take_out_fine_noise(ctypes.c_void_p(c.ctypes.data), 1.0, 3.0)  # at most 0.95 of the floor
     assert c[0] == 4.0 - 0.95 * 4.0
+
+
+def floor_fraction(weights):
+    w = np.ascontiguousarray(weights, np.float64)
+    return lib().avd_grain_floor_fraction(ctypes.c_void_p(w.ctypes.data))
+
+
+def test_floor_fraction_of_white_and_coarse_grain():
+    # White grain (no weights) has a flat spectrum: the floor is all of it.  Strong positive
+    # neighbour weights make coarse grain with almost no power at the finest frequencies
+    # (nl4d/grain/fit.rs:218-224).  The closed form: a left weight a alone has power
+    # 1 / (1 - 2a cos fx + a^2), lowest (1 + a)^-2 at fx = pi, and over the 64 frequencies a mean of
+    # (1 + a^64) / ((1 - a^2)(1 - a^64)).
+    assert floor_fraction(np.zeros(24)) == 1.0
+    offsets = O.ar_offsets()
+    for a in (0.2, 0.5, 0.8):
+        w = np.zeros(24)
+        w[offsets.index((0, -1))] = a
+        want = (1 - a * a) / (1 + a) ** 2 * (1 - a ** 64) / (1 + a ** 64)
+        assert abs(floor_fraction(w) / want - 1) < 1e-9, a
+    coarse = np.zeros(24)
+    coarse[offsets.index((0, -1))], coarse[offsets.index((-1, 0))] = 0.45, 0.45
+    single = np.zeros(24)
+    single[offsets.index((0, -1))] = 0.45
+    assert floor_fraction(coarse) < floor_fraction(single) < floor_fraction(np.zeros(24))  # coarser, smaller floor
+    assert 0.0 <= floor_fraction(np.full(24, 1e6)) <= 1.0
