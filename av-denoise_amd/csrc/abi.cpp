@@ -453,6 +453,8 @@ avd_status avd_site_noise_temporal_stats(const float *ring_values, uint32_t widt
     if (!ring_values || !stats) return fail(AVD_ERR_NULL_ARGUMENT, "null argument");
     if (width == 0 || height == 0 || width > (1u << 20) || height > (1u << 20) || (channels != 1 && channels != 2))
         return fail(AVD_ERR_INVALID_GEOMETRY, "temporal statistics need a non-empty frame of 1 or 2 channels");
+    if ((uint64_t)((width + 15) / 16) * ((height + 15) / 16) * (3 * channels + 26) > 0xFFFFFFFFull)
+        return fail(AVD_ERR_INVALID_GEOMETRY, "the frame's temporal records exceed 32-bit offsets");
     if (mv_field && (mv_step == 0 || mv_blocks_x == 0 || mv_blocks_y == 0))
         return fail(AVD_ERR_INVALID_OPTIONS, "a motion field needs a non-zero step and block grid");
     return hip_status(avd::launch_temporal_stats(ring_values, width, height, channels, slot_new, slot_prev, mv_field,

@@ -460,7 +460,8 @@ hipError_t launch_temporal_stats(const float *ring, uint32_t width, uint32_t hei
                                  hipStream_t s)
 {
     if (!ring || !stats || width == 0 || height == 0 || width > (1u << 20) || height > (1u << 20) ||
-        (channels != 1 && channels != 2) || (mv && (mv_step == 0 || mv_bx == 0 || mv_by == 0)))
+        (channels != 1 && channels != 2) || (mv && (mv_step == 0 || mv_bx == 0 || mv_by == 0)) ||
+        (uint64_t)((width + kTB - 1) / kTB) * ((height + kTB - 1) / kTB) * (3 * channels + 26) > 0xFFFFFFFFull)
         return hipErrorInvalidValue;
     const dim3 grid(((width + kTB - 1) / kTB) * ((height + kTB - 1) / kTB));
     const int W = (int)width, H = (int)height;

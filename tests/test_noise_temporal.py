@@ -396,6 +396,9 @@ def test_site_rejects_bad_arguments_without_a_gpu():
     assert lib.avd_site_noise_temporal_stats(buf, 64, 48, 1, 1, 0, buf, 0, 0, 8, 6, 1, buf, None) == \
         avd.InvalidOptions.status
     assert lib.avd_site_noise_temporal_zero(None, 4, None) == avd.NullArgument.status
+    # 2^20 x 2^20 is within the side limit but its records pass 32-bit offsets.
+    assert lib.avd_site_noise_temporal_stats(buf, 1 << 20, 1 << 20, 1, 1, 0, None, 0, 8, 8, 6, 1, buf, None) == \
+        avd.InvalidGeometry.status
 
 
 # ---- the HIP kernel against the oracle ------------------------------------------------------------
